@@ -1,0 +1,183 @@
+/*
+ * pann.h -- C ABI of the MI355X beam-search library (libpann.so).
+ *
+ * Plain C types only.  Each entry point names the reference interface it
+ * replaces (paths relative to the ParlayANN source tree).  Errors are status
+ * codes (the reference prints and calls abort(), e.g. beamSearch.h:51-57,
+ * :612-616, graph.h:60-63); pann_last_error() gives the message.
+ *
+ * Device pointers: arguments documented "device" must be HIP device memory of
+ * the index's device; pointers documented "host" are ordinary host memory.
+ * `stream` is a hipStream_t (NULL = default stream).
+ */
+#ifndef PANN_H
+#define PANN_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  PANN_OK = 0,
+  PANN_ERR_ARG = 1,         /* invalid argument (the reference aborts) */
+  PANN_ERR_DEVICE = 2,      /* HIP runtime error */
+  PANN_ERR_OOM = 3,         /* device allocation failed */
+  PANN_ERR_CAPACITY = 4,    /* a query outgrew a fixed capacity (visited set, visited list) */
+  PANN_ERR_SHORT_BEAM = 5,  /* beam returned fewer than k entries (beamSearch.h:557-562) */
+  PANN_ERR_UNSUPPORTED = 6  /* shape or option this build does not implement */
+} pann_status;
+
+/* element type of the stored points (point_range.h:43-141 template argument) */
+typedef enum { PANN_DT_F32 = 0, PANN_DT_U8 = 1, PANN_DT_I8 = 2 } pann_dtype;
+
+/* Euclidian_Point (euclidian_point.h:92-248) / Mips_Point (mips_point.h:68-144) */
+typedef enum { PANN_METRIC_L2 = 0, PANN_METRIC_MIPS = 1 } pann_metric;
+
+/* which copy of the points the beam search gathers */
+typedef enum {
+  PANN_STORE_NATIVE = 0, /* the points as given (f32 / u8 / i8) */
+  PANN_STORE_BF16 = 1,   /* f32 points rounded to bfloat16 (RNE) */
+  PANN_STORE_FP8 = 2     /* f32 points as OCP FP8 e4m3 with per-dimension absmax scales */
+} pann_store;
+
+/* FP32 summation order of the native f32 store */
+typedef enum {
+  PANN_SUM_INORDER = 0, /* one lane per row, in-order sum: bit-identical to the reference */
+  PANN_SUM_TREE = 1     /* 8 lanes per row, DPP tree reduction (faster, not bit-identical) */
+} pann_summation;
+
+typedef struct pann_index_s *pann_index_t;
+
+/* An index over n points of `dim` elements and a fixed-degree graph.
+ * graph: n x (max_degree + 1) uint32, slot 0 = degree, then the neighbours
+ * (Graph<unsigned>, graph.h:130-255; file format graph.h:151-237). */
+typedef struct {
+  uint32_t n, dim, max_degree;
+  int32_t dtype;          /* pann_dtype */
+  int32_t metric;         /* pann_metric */
+  const void *points;     /* host, n x dim row-major */
+  const uint32_t *graph;  /* host, n x (max_degree + 1) */
+  int32_t device;         /* HIP device ordinal */
+} pann_index_desc;
+
+/* QueryParams (types.h:233-268): k, beamSize, limit, degree_limit,
+ * rerank_factor, batch_factor.  Same field meaning and defaults
+ * (rerank_factor 100, batch_factor 0.125 -- types.h:237-238). */
+typedef struct {
+  int64_t k;
+  int64_t beam_size;
+  int64_t limit;
+  int64_t degree_limit;
+  double rerank_factor;
+  double batch_factor;
+} pann_qparams;
+
+typedef struct {
+  int32_t store;       /* pann_store gathered by the search */
+  int32_t summation;   /* pann_summation for the native f32 store */
+  int32_t rerank;      /* 1: rerank branch of beam_search_rerank (beamSearch.h:569-587)
+                          against the f32 points; 0: plain branch (:588-596) */
+  int32_t hash_slots_log2; /* LDS visited-set size (0 = automatic) */
+  int32_t async;       /* 1: do not synchronise; call pann_search_status() later */
+  int32_t waves_per_cu;/* persistent waves per CU (0 = automatic) */
+} pann_search_opts;
+
+/* Creates the device index: uploads the points (rows padded to 128 bytes)
+ * and the graph.  Replaces PointRange(char*) + Graph(char*) loading
+ * (point_range.h:74-117, graph.h:151-208) for the device path. */
+pann_status pann_open(const pann_index_desc *desc, pann_index_t *out);
+
+/* Builds a reduced-precision copy of f32 points on the device
+ * (PANN_STORE_BF16 or PANN_STORE_FP8).  The reference's analog is the
+ * quantizing PointRange constructor (point_range.h:55-66 with
+ * Quantized_Mips_Point / Euclidian_Point<uint8_t> parameters). */
+pann_status pann_build_store(pann_index_t idx, int32_t store);
+
+/* Copies the FP8 per-dimension scales (dim floats) to host memory. */
+pann_status pann_get_fp8_scales(pann_index_t idx, float *scales_host);
+
+/* Batched beam search, device buffers.  Replaces qsearchAll
+ * (beamSearch.h:599-646) over beam_search_rerank (:528-597) over
+ * filtered_beam_search (:33-228): one search per query from `starts`.
+ *  queries:   device, nq rows of dim elements in the index dtype
+ *             (f32 queries for BF16/FP8 stores)
+ *  k_pq, beam_pq: optional device int32[nq] per-query k / beamSize (NULL =
+ *             uniform qp->k / qp->beam_size)
+ *  starts:    host, nstarts unique ids (1 <= nstarts <= min(beam, 64))
+ *  out_ids, out_dists: device, nq x out_stride (first k used per row)
+ *  out_visited, out_cmps: optional device uint32[nq]: visited count and
+ *             distance comparisons (stats<>, stats.h:121-155)            */
+pann_status pann_search(pann_index_t idx, const void *queries, uint32_t nq,
+                        const pann_qparams *qp, const int32_t *k_pq, const int32_t *beam_pq,
+                        const uint32_t *starts, uint32_t nstarts, const pann_search_opts *opts,
+                        uint32_t *out_ids, float *out_dists, uint32_t out_stride,
+                        uint32_t *out_visited, uint32_t *out_cmps, void *stream);
+
+/* Same search with host buffers (copies in and out, synchronises).  This is
+ * what a C++ qsearchAll caller binds directly. */
+pann_status pann_search_host(pann_index_t idx, const void *queries, uint32_t nq,
+                             const pann_qparams *qp, const uint32_t *starts, uint32_t nstarts,
+                             const pann_search_opts *opts, uint32_t *out_ids, float *out_dists,
+                             uint32_t *out_visited, uint32_t *out_cmps);
+
+/* Status of the last asynchronous pann_search on `stream` (synchronises). */
+pann_status pann_search_status(pann_index_t idx, void *stream);
+
+/* Duration in milliseconds of the last pann_search kernel, measured with
+ * HIP events on the launch stream (synchronises). */
+pann_status pann_last_kernel_ms(pann_index_t idx, float *ms);
+
+/* Build-mode search of Vamana batch insertion: QueryParams(0, L, n,
+ * max_degree, ..., batch_factor) from start point `start` for each id in
+ * self_ids (the query row is the point itself, same_as skips it), writing
+ * the visited list sorted by (dist, id) -- the candidate set robustPrune
+ * consumes (vamana/index.h:247-267).
+ *  self_ids: device uint32[nq]; vis_ids/vis_dists: device nq x vis_cap;
+ *  vis_count: device uint32[nq]                                          */
+pann_status pann_search_visited(pann_index_t idx, const uint32_t *self_ids, uint32_t nq,
+                                uint32_t beam, double batch_factor, uint32_t start,
+                                uint32_t vis_cap, uint32_t *vis_ids, float *vis_dists,
+                                uint32_t *vis_count, void *stream);
+
+/* knn_index::robustPrune (vamana/index.h:63-137) for a batch of points on
+ * the device: for point p = pts[i], the candidates cand_ids[i*cand_stride
+ * .. + cand_count[i]) with distances cand_dists (to p; NULL = computed, the
+ * id-only overload :124-137).  add = 0: candidates only; 1: plus p's current
+ * out-neighbours (:72-77); 2: the reverse-edge rule of batch_insert
+ * (:292-303): if count + degree <= R the candidates followed by the current
+ * neighbours not among them (add_neighbors_without_repeats, :140-146), else
+ * robustPrune with add; 3: no pruning, p's current out-neighbours sorted by
+ * (dist, id) (the final G[i].sort of build_index, :180-185; cand_count must
+ * be 0).  Writes <= R ids to out_ids[i*R ..] and the count to out_count[i].
+ * Points must be distinct.  All pointers device.                          */
+pann_status pann_robust_prune_batch(pann_index_t idx, const uint32_t *pts, uint32_t np,
+                                    const uint32_t *cand_ids, const float *cand_dists,
+                                    const uint32_t *cand_count, uint32_t cand_stride,
+                                    double alpha, uint32_t R, int32_t add, uint32_t *out_ids,
+                                    uint32_t *out_count, void *stream);
+
+/* Replaces the out-neighbour lists of points pts[0..np) on the device
+ * (edgeRange::update_neighbors, graph.h:77-88).  lists: device np x R,
+ * counts: device uint32[np]. */
+pann_status pann_update_neighbors(pann_index_t idx, const uint32_t *pts, uint32_t np,
+                                  const uint32_t *lists, const uint32_t *counts, uint32_t R,
+                                  void *stream);
+
+/* Copies the device graph back to host memory as n x (max_degree + 1). */
+pann_status pann_download_graph(pann_index_t idx, uint32_t *graph_host);
+
+pann_status pann_close(pann_index_t idx);
+
+/* Message of the last error on this thread. */
+const char *pann_last_error(void);
+
+/* Device properties the library sizes its launches by. */
+pann_status pann_device_info(int32_t device, int32_t *num_cus, int64_t *lds_per_cu,
+                             int64_t *hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PANN_H */

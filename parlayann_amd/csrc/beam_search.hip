@@ -1,0 +1,446 @@
+// beam_search.hip -- wave-per-query beam search over a fixed-degree graph.
+//
+// Restates filtered_beam_search (beamSearch.h:33-228, use_filtering = false)
+// for one query per 64-lane wave, replaying the reference's sequential rules
+// exactly:
+//   * one visit per step: the closest unvisited entry of the (possibly stale)
+//     frontier (beamSearch.h:116-127);
+//   * neighbours G[cur][0 .. min(deg, degree_limit)) not seen before and not
+//     the query's own row are distance-compared (:151-160, :176-182); the seen
+//     set is exact (LDS hash with a per-wave global overflow table), like the
+//     reference's hashset before it ever grows (hashset.h:12-55);
+//   * candidates with d >= cutoff (last frontier distance when the frontier is
+//     full) are dropped (:173-181);
+//   * the merge is deferred while |candidates| < batch_factor * beam and more
+//     unvisited entries remain (:185-192), |candidates| counted before dedup;
+//   * merge = sort by (dist, id), union with the frontier, trim to beam
+//     (:197-222), done here as a rank merge in LDS.
+// Results: the first k frontier entries (beam_search_rerank :588-596), or with
+// a full-precision FP32 store the rerank branch (:569-587): FP32 distances of
+// the first min(k * rerank_factor, |frontier|) entries, sorted, first k.
+//
+// Memory: one wave per workgroup, persistent grid pulling query indices from
+// a device counter.  LDS per wave: two frontier buffers (u64 keys + visited
+// flags), candidate buffers, the gather list, the query row (FP32 paths) and
+// the visited hash.
+#include "kernels.h"
+#include "dist.h"
+
+#include <float.h>
+
+namespace pann {
+
+struct WaveLds {
+  uint64_t *F0, *F1;
+  uint8_t *FV0, *FV1;
+  uint64_t* C;
+  uint64_t* Cs;
+  uint32_t* NL;
+  float* DL;
+  float* QL;
+  uint32_t* H;
+};
+
+__host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
+
+__host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap, uint32_t dim,
+                                                   uint32_t hslots_log2) {
+  uint32_t b = 0;
+  b += 2 * align16(fcap * 8);
+  b += 2 * align16(fcap);
+  b += 2 * align16(ccap * 8);
+  b += align16(64 * 4) * 2;
+  b += align16(dim * 4);
+  b += (1u << hslots_log2) * 4;
+  return b;
+}
+
+__device__ inline WaveLds carve(char* p, const SearchArgs& a) {
+  WaveLds L;
+  uint32_t o = 0;
+  L.F0 = (uint64_t*)(p + o); o += align16(a.fcap * 8);
+  L.F1 = (uint64_t*)(p + o); o += align16(a.fcap * 8);
+  L.FV0 = (uint8_t*)(p + o); o += align16(a.fcap);
+  L.FV1 = (uint8_t*)(p + o); o += align16(a.fcap);
+  L.C = (uint64_t*)(p + o); o += align16(a.ccap * 8);
+  L.Cs = (uint64_t*)(p + o); o += align16(a.ccap * 8);
+  L.NL = (uint32_t*)(p + o); o += align16(64 * 4);
+  L.DL = (float*)(p + o); o += align16(64 * 4);
+  L.QL = (float*)(p + o); o += align16(a.dim * 4);
+  L.H = (uint32_t*)(p + o);
+  return L;
+}
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t key, uint32_t log2n) {
+  return (key * 0x9E3779B1u) >> (32u - log2n);
+}
+
+// Exact insert-or-test in an open-addressing table (linear probing, CAS).
+// Returns true when `key` was absent and has been inserted.
+__device__ __forceinline__ bool lds_insert(uint32_t* H, uint32_t log2n, uint32_t key) {
+  const uint32_t mask = (1u << log2n) - 1u;
+  uint32_t h = hash_slot(key, log2n);
+  for (;;) {
+    uint32_t old = atomicCAS(&H[h], PANN_SENTINEL, key);
+    if (old == PANN_SENTINEL) return true;
+    if (old == key) return false;
+    h = (h + 1u) & mask;
+  }
+}
+
+__device__ __forceinline__ bool lds_contains(const uint32_t* H, uint32_t log2n, uint32_t key) {
+  const uint32_t mask = (1u << log2n) - 1u;
+  uint32_t h = hash_slot(key, log2n);
+  for (;;) {
+    uint32_t v = H[h];
+    if (v == key) return true;
+    if (v == PANN_SENTINEL) return false;
+    h = (h + 1u) & mask;
+  }
+}
+
+__device__ __forceinline__ bool global_insert(uint32_t* G, uint32_t log2n, uint32_t key) {
+  const uint32_t mask = (1u << log2n) - 1u;
+  uint32_t h = hash_slot(key, log2n);
+  for (;;) {
+    uint32_t old = atomicCAS(&G[h], PANN_SENTINEL, key);
+    if (old == PANN_SENTINEL) return true;
+    if (old == key) return false;
+    h = (h + 1u) & mask;
+  }
+}
+
+// number of entries of sorted A[0..n) that are < x
+__device__ __forceinline__ int lower_bound_u64(const uint64_t* A, int n, uint64_t x) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (A[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int ST, int MT, int NCH, bool SEQ>
+__device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
+                                          const WaveLds& L, int ncmp, int lane) {
+  if constexpr (SEQ) {
+    eval_seq_f32<MT>(L.QL, a.pts, a.pts_stride, (int)a.dim, L.NL, ncmp, L.DL, lane);
+  } else {
+    eval_group<ST, MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, L.NL, ncmp, L.DL, lane);
+  }
+}
+
+template <int ST, int MT, int NCH, bool SEQ>
+__device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, int lane,
+                           uint32_t* ghash) {
+  using QT = typename QueryElem<ST>::T;
+  const int B = a.beam_pq ? a.beam_pq[q] : a.beam;
+  const int k = a.k_pq ? a.k_pq[q] : a.k;
+  const uint32_t self = a.self_ids ? a.self_ids[q] : PANN_SENTINEL;
+  // |C| < batch_factor * B  <=>  |C| < ceil(batch_factor * B) for integer |C|
+  const int thr = (int)ceil(a.batch_factor * (double)B);
+  const bool defer_ok = a.limit >= 2 * (int64_t)B;
+  const int dl = a.degree_limit < (int)a.max_degree ? a.degree_limit : (int)a.max_degree;
+  const uint32_t hlog2 = a.hslots_log2;
+  const uint32_t hslots = 1u << hlog2;
+  if (B < 1 || B > (int)a.fcap || k < 0 || k > B || k > (int)a.out_stride ||
+      (int)a.nstarts > B) {  // per-query values outside what this launch was sized for
+    if (lane == 0) atomicOr(a.status, PANN_ST_BAD_PARAMS);
+    return;
+  }
+
+  // ---- query into registers (group mode) and LDS (sequential / rerank) ----
+  GroupDist<ST, MT, NCH> D;
+  const uint64_t qr = a.self_ids ? (uint64_t)self : (uint64_t)q;  // build mode: the point's own row
+  const char* qrow = a.queries + qr * a.q_stride;
+  if constexpr (!SEQ) D.load_query((const QT*)qrow, a.scales, (int)a.dim, lane & 7);
+  if (SEQ || a.full) {
+    const float* fq = SEQ ? (const float*)qrow : (const float*)(a.fqueries + qr * a.fq_stride);
+    for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
+  }
+  // ---- clear the visited hash ----
+  for (uint32_t i = lane * 4; i < hslots; i += PANN_WAVE * 4)
+    *(uint4*)(L.H + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
+  wave_lds_sync();
+
+  bool ovf = false;
+  uint32_t lds_count = 0, g_count = 0;
+  const uint32_t glog2 = a.ghash_log2;
+  uint32_t status = 0;
+
+  // ---- starting points (beamSearch.h:71-93) ----
+  const int ns = (int)a.nstarts;
+  if (lane < ns) {
+    uint32_t s = a.starts[lane];
+    L.NL[lane] = s;
+    lds_insert(L.H, hlog2, s);
+  }
+  lds_count = (uint32_t)ns;
+  wave_lds_sync();
+  eval_rows<ST, MT, NCH, SEQ>(D, a, L, ns, lane);
+  wave_lds_sync();
+  uint64_t skey = 0;
+  if (lane < ns) {
+    skey = make_key(L.DL[lane], L.NL[lane]);
+    L.C[lane] = skey;
+  }
+  wave_lds_sync();
+  int cur_buf = 0;
+  if (lane < ns) {
+    int r = 0;
+    for (int i = 0; i < ns; i++) r += (L.C[i] < skey);
+    L.F0[r] = skey;
+    L.FV0[r] = 0;
+  }
+  wave_lds_sync();
+  int f = ns;
+  uint32_t nvis = 0;
+  uint32_t cmps = (uint32_t)ns;
+  int c = 0;
+
+  // ---- main loop (beamSearch.h:116-223) ----
+  for (;;) {
+    uint64_t* F = cur_buf ? L.F1 : L.F0;
+    uint8_t* FV = cur_buf ? L.FV1 : L.FV0;
+    int u = -1;
+    for (int t = 0; t < f; t += PANN_WAVE) {
+      int i = t + lane;
+      uint64_t m = ballot(i < f && FV[i] == 0);
+      if (m) {
+        u = t + (int)__builtin_ctzll(m);
+        break;
+      }
+    }
+    if (u < 0 || (int64_t)nvis >= a.limit) break;
+    const uint64_t cur_key = F[u];
+    const uint32_t cur = key_id(cur_key);
+    if (lane == 0) FV[u] = 1;
+    if (a.vis_out && lane == 0) {
+      if (nvis < a.vis_cap) a.vis_out[(uint64_t)q * a.vis_cap + nvis] = cur_key;
+      else status |= PANN_ST_VISITED_OVERFLOW;
+    }
+    nvis++;
+    const bool full = (f == B);
+    const float cutoff = full ? key_dist(F[f - 1]) : FLT_MAX;
+
+    // neighbours of cur (graph rows are padded with the sentinel past deg)
+    uint32_t nb = lane < dl ? a.nbrs[(uint64_t)cur * a.nbr_stride + lane] : PANN_SENTINEL;
+    const bool valid = nb != PANN_SENTINEL;
+    const int nvalid = __popcll(ballot(valid));
+    if (!ovf && lds_count + (uint32_t)nvalid > (hslots >> 1)) {
+      // LDS table reached half load: from here on new ids go to this wave's
+      // global table (cleared on first use in this query).
+      ovf = true;
+      uint32_t* G = ghash;
+      for (uint32_t i = lane * 4; i < (1u << glog2); i += PANN_WAVE * 4)
+        *(uint4*)(G + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+      __builtin_amdgcn_wave_barrier();
+    }
+    bool isnew = false;
+    if (valid) {
+      if (!ovf) {
+        isnew = lds_insert(L.H, hlog2, nb);
+      } else if (!lds_contains(L.H, hlog2, nb)) {
+        if (g_count + PANN_WAVE < (1u << glog2)) isnew = global_insert(ghash, glog2, nb);
+        else isnew = true;  // table exhausted: flagged below
+      }
+    }
+    const uint64_t mins = ballot(isnew);
+    if (!ovf) lds_count += (uint32_t)__popcll(mins);
+    else {
+      g_count += (uint32_t)__popcll(mins);
+      if (g_count + PANN_WAVE >= (1u << glog2)) status |= PANN_ST_HASH_CAPACITY;
+    }
+    if (nb == self) isnew = false;  // Points[a].same_as(p) (:156)
+    const uint64_t mnew = ballot(isnew);
+    const int ncmp = __popcll(mnew);
+    cmps += (uint32_t)ncmp;
+    if (isnew) L.NL[mbcnt(mnew)] = nb;
+    wave_lds_sync();
+    eval_rows<ST, MT, NCH, SEQ>(D, a, L, ncmp, lane);
+    wave_lds_sync();
+    bool keep = false;
+    uint64_t key = 0;
+    if (lane < ncmp) {
+      float d = L.DL[lane];
+      keep = !(d >= cutoff);
+      key = make_key(d, L.NL[lane]);
+    }
+    const uint64_t mk = ballot(keep);
+    if (keep) L.C[c + (int)mbcnt(mk)] = key;
+    c += __popcll(mk);
+    wave_lds_sync();
+    // is there an unvisited entry left in the (stale) frontier?
+    bool more = false;
+    for (int t = 0; t < f; t += PANN_WAVE) {
+      int i = t + lane;
+      if (ballot(i < f && FV[i] == 0)) {
+        more = true;
+        break;
+      }
+    }
+    if (c == 0 || (defer_ok && c < thr && more)) continue;
+
+    // ---- merge: rank-sort C, then place C and F by rank into the other buffer
+    for (int j = lane; j < c; j += PANN_WAVE) {
+      uint64_t x = L.C[j];
+      int r = 0;
+      for (int i = 0; i < c; i++) r += (L.C[i] < x);
+      L.Cs[r] = x;
+    }
+    wave_lds_sync();
+    uint64_t* F2 = cur_buf ? L.F0 : L.F1;
+    uint8_t* FV2 = cur_buf ? L.FV0 : L.FV1;
+    for (int j = lane; j < c; j += PANN_WAVE) {
+      uint64_t x = L.Cs[j];
+      int pos = j + lower_bound_u64(F, f, x);
+      if (pos < B) {
+        F2[pos] = x;
+        FV2[pos] = 0;
+      }
+    }
+    for (int i = lane; i < f; i += PANN_WAVE) {
+      uint64_t x = F[i];
+      int pos = i + lower_bound_u64(L.Cs, c, x);
+      if (pos < B) {
+        F2[pos] = x;
+        FV2[pos] = FV[i];
+      }
+    }
+    f = f + c < B ? f + c : B;
+    c = 0;
+    cur_buf ^= 1;
+    wave_lds_sync();
+  }
+
+  // ---- results (beam_search_rerank :557-596) ----
+  const uint64_t* F = cur_buf ? L.F1 : L.F0;
+  uint32_t* oi = a.out_ids + (uint64_t)q * a.out_stride;
+  float* od = a.out_dists + (uint64_t)q * a.out_stride;
+  if (f < k) status |= PANN_ST_SHORT_BEAM;
+  if (a.full) {
+    double nc = (double)k * a.rerank_factor;
+    int nchk = nc < (double)f ? (int)nc : f;
+    uint64_t* R = cur_buf ? L.F0 : L.F1;
+    for (int t = 0; t < nchk; t += PANN_WAVE) {
+      int m = nchk - t < PANN_WAVE ? nchk - t : PANN_WAVE;
+      if (lane < m) L.NL[lane] = key_id(F[t + lane]);
+      wave_lds_sync();
+      eval_seq_f32<MT>(L.QL, a.full, a.full_stride, (int)a.dim, L.NL, m, L.DL, lane);
+      wave_lds_sync();
+      if (lane < m) R[t + lane] = make_key(L.DL[lane], L.NL[lane]);
+      wave_lds_sync();
+    }
+    for (int j = lane; j < nchk; j += PANN_WAVE) {
+      uint64_t x = R[j];
+      int r = 0;
+      for (int i = 0; i < nchk; i++) r += (R[i] < x);
+      if (r < k) {
+        oi[r] = key_id(x);
+        od[r] = key_dist(x);
+      }
+    }
+  } else {
+    for (int j = lane; j < k && j < f; j += PANN_WAVE) {
+      oi[j] = key_id(F[j]);
+      od[j] = key_dist(F[j]);
+    }
+  }
+  if (lane == 0) {
+    if (a.out_visited) a.out_visited[q] = nvis;
+    if (a.out_cmps) a.out_cmps[q] = cmps;
+    if (a.out_nfront) a.out_nfront[q] = (uint32_t)f;
+    if (status) atomicOr(a.status, status);
+  }
+  wave_lds_sync();
+}
+
+template <int ST, int MT, int NCH, bool SEQ>
+__global__ __launch_bounds__(PANN_WAVE) void k_beam_search(SearchArgs a) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x;
+  WaveLds L = carve(smem, a);
+  uint32_t* ghash = a.ghash + ((uint64_t)blockIdx.x << a.ghash_log2);
+  for (;;) {
+    uint32_t q = 0;
+    if (lane == 0) q = atomicAdd(a.work_counter, 1u);
+    q = readfirstlane(q);
+    if (q >= a.nq) break;
+    search_one<ST, MT, NCH, SEQ>(a, L, q, lane, ghash);
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch
+// ---------------------------------------------------------------------------
+uint32_t search_lds_bytes(const SearchArgs& a) {
+  return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2);
+}
+
+template <int ST, int MT, int NCH, bool SEQ>
+static hipError_t launch_t(const SearchArgs& a, uint32_t grid, hipStream_t s) {
+  uint32_t lds = search_lds_bytes(a);
+  auto kern = k_beam_search<ST, MT, NCH, SEQ>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(PANN_WAVE), lds, s, a);
+  return hipGetLastError();
+}
+
+static int nch_for(int row_bytes) {
+  int c = (row_bytes + 127) / 128;
+  if (c <= 1) return 1;
+  if (c <= 2) return 2;
+  if (c <= 3) return 3;
+  if (c <= 4) return 4;
+  if (c <= 8) return 8;
+  return -1;
+}
+
+#define PANN_NCH_CASES(ST, MT, SEQ)                              \
+  switch (nch) {                                                 \
+    case 1: return launch_t<ST, MT, 1, SEQ>(a, grid, s);         \
+    case 2: return launch_t<ST, MT, 2, SEQ>(a, grid, s);         \
+    case 3: return launch_t<ST, MT, 3, SEQ>(a, grid, s);         \
+    case 4: return launch_t<ST, MT, 4, SEQ>(a, grid, s);         \
+    case 8: return launch_t<ST, MT, 8, SEQ>(a, grid, s);         \
+    default: return hipErrorInvalidValue;                        \
+  }
+
+hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool sequential,
+                              uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, a.work_counter, 1u);
+  if (sequential) {
+    if (store != ST_F32) return hipErrorInvalidValue;
+    if (metric == M_L2) return launch_t<ST_F32, M_L2, 1, true>(a, grid, s);
+    return launch_t<ST_F32, M_IP, 1, true>(a, grid, s);
+  }
+  int nch = nch_for((int)a.row_bytes);
+  if (store == ST_F32) {
+    if (metric == M_L2) { PANN_NCH_CASES(ST_F32, M_L2, false) }
+    else { PANN_NCH_CASES(ST_F32, M_IP, false) }
+  } else if (store == ST_U8) {
+    if (metric == M_L2) { PANN_NCH_CASES(ST_U8, M_L2, false) }
+    else { PANN_NCH_CASES(ST_U8, M_IP, false) }
+  } else if (store == ST_I8) {
+    if (metric == M_L2) { PANN_NCH_CASES(ST_I8, M_L2, false) }
+    else { PANN_NCH_CASES(ST_I8, M_IP, false) }
+  } else if (store == ST_BF16) {
+    if (metric == M_L2) { PANN_NCH_CASES(ST_BF16, M_L2, false) }
+    else { PANN_NCH_CASES(ST_BF16, M_IP, false) }
+  } else if (store == ST_FP8) {
+    if (metric == M_L2) { PANN_NCH_CASES(ST_FP8, M_L2, false) }
+    else { PANN_NCH_CASES(ST_FP8, M_IP, false) }
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace pann

@@ -1,0 +1,326 @@
+// dist.h -- distance policies of the beam-search gather (per-hop GEMV work).
+//
+// Two evaluation shapes:
+//  * group: 8 lanes per row, each lane owns 16-byte chunks at byte offsets
+//    sub*16 + c*128 (c < NCH), so every 8 lanes read one full 128-byte line
+//    per instruction; partial sums are reduced with DPP inside the 8-lane
+//    group.  Integer (uint8/int8) sums are exact in any order, so group mode is
+//    bit-exact for them (euclidian_point.h:45-81, mips_point.h:44-58).
+//  * sequential: one lane per row, terms summed strictly in index order with
+//    separately rounded multiply and add -- the arithmetic of the reference's
+//    float loop (euclidian_point.h:83-90, mips_point.h:60-66), so FP32 results
+//    are bit-identical to it.
+#pragma once
+#include "device_common.h"
+
+namespace pann {
+
+enum StoreKind : int { ST_F32 = 0, ST_U8 = 1, ST_I8 = 2, ST_BF16 = 3, ST_FP8 = 4 };
+enum MetricKind : int { M_L2 = 0, M_IP = 1 };
+
+__host__ __device__ constexpr int store_elem_bytes(int st) {
+  return st == ST_F32 ? 4 : (st == ST_BF16 ? 2 : 1);
+}
+
+// ---------------------------------------------------------------------------
+// OCP FP8 E4M3 (e4m3fn: bias 7, no infinities, max finite 448) to f32.
+// Exact: every e4m3 value is representable in f32.  Normal numbers are
+// rebuilt by moving the exponent/mantissa fields into f32 position and
+// re-biasing (+120); subnormals (exponent field 0) are m * 2^-9.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float fp8e4m3_to_f32(uint32_t x) {
+  uint32_t mag = x & 0x7Fu;
+  uint32_t sgn = (x & 0x80u) << 24;
+  float v;
+  if (mag >= 8u) v = __uint_as_float(((mag << 20) + (120u << 23)));
+  else v = (float)mag * 0.001953125f;  // m * 2^-9
+  return __uint_as_float(__float_as_uint(v) | sgn);
+}
+
+__device__ __forceinline__ float bf16lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+
+// ---------------------------------------------------------------------------
+// Group (8 lanes per row) policies.  Q holds this lane's slice of the query.
+// ---------------------------------------------------------------------------
+template <int ST, int MT, int NCH>
+struct GroupDist;
+
+// FP32 rows: 4 floats per 16-byte chunk.
+template <int MT, int NCH>
+struct GroupDist<ST_F32, MT, NCH> {
+  float q[NCH][4];
+  __device__ void load_query(const float* qv, const float* /*scales*/, int dim, int sub) {
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        int j = (c * 128 + sub * 16) / 4 + e;
+        q[c][e] = j < dim ? qv[j] : 0.0f;
+      }
+  }
+  __device__ float partial(const uint4* raw) const {
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      float p[4] = {__uint_as_float(raw[c].x), __uint_as_float(raw[c].y),
+                    __uint_as_float(raw[c].z), __uint_as_float(raw[c].w)};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        if (MT == M_L2) {
+          float t = q[c][e] - p[e];
+          acc = fmaf(t, t, acc);
+        } else {
+          acc = fmaf(q[c][e], p[e], acc);
+        }
+      }
+    }
+    return acc;
+  }
+  __device__ float finish(float s) const { return MT == M_L2 ? s : -s; }
+};
+
+// BF16 rows: 8 values per chunk, f32 arithmetic.
+template <int MT, int NCH>
+struct GroupDist<ST_BF16, MT, NCH> {
+  float q[NCH][8];
+  __device__ void load_query(const float* qv, const float* /*scales*/, int dim, int sub) {
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        int j = (c * 128 + sub * 16) / 2 + e;
+        q[c][e] = j < dim ? qv[j] : 0.0f;
+      }
+  }
+  __device__ float partial(const uint4* raw) const {
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      uint32_t w[4] = {raw[c].x, raw[c].y, raw[c].z, raw[c].w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        float p0 = bf16lo(w[e]), p1 = bf16hi(w[e]);
+        if (MT == M_L2) {
+          float t0 = q[c][2 * e] - p0, t1 = q[c][2 * e + 1] - p1;
+          acc = fmaf(t0, t0, acc);
+          acc = fmaf(t1, t1, acc);
+        } else {
+          acc = fmaf(q[c][2 * e], p0, acc);
+          acc = fmaf(q[c][2 * e + 1], p1, acc);
+        }
+      }
+    }
+    return acc;
+  }
+  __device__ float finish(float s) const { return MT == M_L2 ? s : -s; }
+};
+
+// FP8 e4m3 rows with per-dimension scales s_j: x_j ~ s_j * fp8_j.
+// IP folds the scale into the query (q~_j = q_j * s_j); L2 keeps it.
+template <int MT, int NCH>
+struct GroupDist<ST_FP8, MT, NCH> {
+  float q[NCH][16];
+  float s[MT == M_L2 ? NCH : 1][16];
+  __device__ void load_query(const float* qv, const float* scales, int dim, int sub) {
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        int j = c * 128 + sub * 16 + e;
+        float qq = j < dim ? qv[j] : 0.0f;
+        float sc = j < dim ? scales[j] : 0.0f;
+        if (MT == M_L2) {
+          q[c][e] = qq;
+          s[c][e] = sc;
+        } else {
+          q[c][e] = qq * sc;
+        }
+      }
+  }
+  __device__ float partial(const uint4* raw) const {
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      uint32_t w[4] = {raw[c].x, raw[c].y, raw[c].z, raw[c].w};
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        float p = fp8e4m3_to_f32((w[e >> 2] >> (8 * (e & 3))) & 0xFFu);
+        if (MT == M_L2) {
+          float t = q[c][e] - s[c][e] * p;
+          acc = fmaf(t, t, acc);
+        } else {
+          acc = fmaf(q[c][e], p, acc);
+        }
+      }
+    }
+    return acc;
+  }
+  __device__ float finish(float v) const { return MT == M_L2 ? v : -v; }
+};
+
+// uint8 rows: exact int32 via v_dot4_u32_u8.
+//   L2: sum (q-p)^2 = sum q^2 + sum p^2 - 2 sum q.p   (mod 2^32, exact result)
+template <int MT, int NCH>
+struct GroupDist<ST_U8, MT, NCH> {
+  uint32_t q[NCH][4];
+  uint32_t qq;  // this lane's share of sum q^2
+  __device__ void load_query(const uint8_t* qv, const float*, int dim, int sub) {
+    qq = 0u;
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+#pragma unroll
+      for (int w = 0; w < 4; w++) {
+        uint32_t v = 0u;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          int j = c * 128 + sub * 16 + 4 * w + b;
+          uint32_t x = j < dim ? (uint32_t)qv[j] : 0u;
+          v |= x << (8 * b);
+        }
+        q[c][w] = v;
+        qq = __builtin_amdgcn_udot4(v, v, qq, false);
+      }
+  }
+  __device__ int partial(const uint4* raw) const {
+    uint32_t pp = MT == M_L2 ? qq : 0u, pq = 0u;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      uint32_t w[4] = {raw[c].x, raw[c].y, raw[c].z, raw[c].w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        if (MT == M_L2) pp = __builtin_amdgcn_udot4(w[e], w[e], pp, false);
+        pq = __builtin_amdgcn_udot4(w[e], q[c][e], pq, false);
+      }
+    }
+    return MT == M_L2 ? (int)(pp - 2u * pq) : (int)pq;
+  }
+  __device__ float finish(int s) const { return MT == M_L2 ? (float)s : -((float)s); }
+};
+
+// int8 rows: exact int32 via v_dot4_i32_i8.
+template <int MT, int NCH>
+struct GroupDist<ST_I8, MT, NCH> {
+  int32_t q[NCH][4];
+  int32_t qq;
+  __device__ void load_query(const int8_t* qv, const float*, int dim, int sub) {
+    qq = 0;
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+#pragma unroll
+      for (int w = 0; w < 4; w++) {
+        uint32_t v = 0u;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          int j = c * 128 + sub * 16 + 4 * w + b;
+          uint32_t x = j < dim ? (uint32_t)(uint8_t)qv[j] : 0u;
+          v |= x << (8 * b);
+        }
+        q[c][w] = (int32_t)v;
+        qq = __builtin_amdgcn_sdot4((int)v, (int)v, qq, false);
+      }
+  }
+  __device__ int partial(const uint4* raw) const {
+    int pp = MT == M_L2 ? qq : 0, pq = 0;
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      int w[4] = {(int)raw[c].x, (int)raw[c].y, (int)raw[c].z, (int)raw[c].w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        if (MT == M_L2) pp = __builtin_amdgcn_sdot4(w[e], w[e], pp, false);
+        pq = __builtin_amdgcn_sdot4(w[e], q[c][e], pq, false);
+      }
+    }
+    return MT == M_L2 ? (int)((uint32_t)pp - 2u * (uint32_t)pq) : pq;
+  }
+  __device__ float finish(int s) const { return MT == M_L2 ? (float)s : -((float)s); }
+};
+
+template <int ST>
+struct QueryElem {
+  using T = float;
+};
+template <>
+struct QueryElem<ST_U8> {
+  using T = uint8_t;
+};
+template <>
+struct QueryElem<ST_I8> {
+  using T = int8_t;
+};
+
+// Evaluates the distances of rows NL[0..ncmp) into DL[0..ncmp) (both LDS).
+// Rows are gathered 8 at a time (one per 8-lane group), UP rounds of loads in
+// flight per lane.
+template <int ST, int MT, int NCH>
+__device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, const char* base,
+                                           uint64_t stride, int row_bytes, const uint32_t* NL,
+                                           int ncmp, float* DL, int lane) {
+  constexpr int UP = NCH <= 1 ? 8 : (NCH <= 2 ? 4 : (NCH <= 4 ? 2 : 1));
+  const int sub = lane & 7, grp = lane >> 3;
+  for (int r0 = 0; r0 < ncmp; r0 += 8 * UP) {
+    uint4 raw[UP][NCH];
+#pragma unroll
+    for (int u = 0; u < UP; u++) {
+      int r = r0 + u * 8 + grp;
+      const char* row = base + (uint64_t)(r < ncmp ? NL[r] : 0u) * stride;
+#pragma unroll
+      for (int c = 0; c < NCH; c++) {
+        int off = c * 128 + sub * 16;
+        raw[u][c] = (r < ncmp && off < row_bytes) ? ld16(row + off) : make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UP; u++) {
+      int r = r0 + u * 8 + grp;
+      auto s = group8_sum(D.partial(raw[u]));
+      if (sub == 0 && r < ncmp) DL[r] = D.finish(s);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sequential FP32 (one lane per row, in-order sum of separately rounded terms;
+// the reference's float loop).  The query sits in LDS (QL, dim floats).
+// ---------------------------------------------------------------------------
+template <int MT>
+__device__ __forceinline__ float seq_f32_row(const float* QL, const float* row, int dim) {
+  float acc = 0.0f;
+  int j = 0;
+  for (; j + 4 <= dim; j += 4) {
+    float4 p = *(const float4*)(row + j);
+    float4 q = *(const float4*)(QL + j);
+    if (MT == M_L2) {
+      float t;
+      t = __fsub_rn(q.x, p.x); acc = __fadd_rn(acc, __fmul_rn(t, t));
+      t = __fsub_rn(q.y, p.y); acc = __fadd_rn(acc, __fmul_rn(t, t));
+      t = __fsub_rn(q.z, p.z); acc = __fadd_rn(acc, __fmul_rn(t, t));
+      t = __fsub_rn(q.w, p.w); acc = __fadd_rn(acc, __fmul_rn(t, t));
+    } else {
+      acc = __fadd_rn(acc, __fmul_rn(q.x, p.x));
+      acc = __fadd_rn(acc, __fmul_rn(q.y, p.y));
+      acc = __fadd_rn(acc, __fmul_rn(q.z, p.z));
+      acc = __fadd_rn(acc, __fmul_rn(q.w, p.w));
+    }
+  }
+  for (; j < dim; j++) {
+    float p = row[j], q = QL[j];
+    if (MT == M_L2) {
+      float t = __fsub_rn(q, p);
+      acc = __fadd_rn(acc, __fmul_rn(t, t));
+    } else {
+      acc = __fadd_rn(acc, __fmul_rn(q, p));
+    }
+  }
+  return MT == M_L2 ? acc : -acc;
+}
+
+template <int MT>
+__device__ __forceinline__ void eval_seq_f32(const float* QL, const char* base, uint64_t stride,
+                                             int dim, const uint32_t* NL, int ncmp, float* DL,
+                                             int lane) {
+  for (int r = lane; r < ncmp; r += PANN_WAVE)
+    DL[r] = seq_f32_row<MT>(QL, (const float*)(base + (uint64_t)NL[r] * stride), dim);
+}
+
+}  // namespace pann

@@ -1,0 +1,112 @@
+// kernels.h -- argument blocks and launchers of the CDNA4 kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pann {
+
+// per-launch status bits (OR-ed by the kernels)
+enum : uint32_t {
+  PANN_ST_SHORT_BEAM = 1u,        // a frontier ended with fewer than k entries
+  PANN_ST_HASH_CAPACITY = 2u,     // a query outgrew its visited-set capacity
+  PANN_ST_VISITED_OVERFLOW = 4u,  // build mode: visited list longer than vis_cap
+  PANN_ST_BAD_PARAMS = 8u,        // a per-query k / beam outside the launch's sizing
+};
+
+struct SearchArgs {
+  // search store: n rows of row_bytes valid bytes, pts_stride apart (multiple of 128)
+  const char* pts;
+  uint64_t pts_stride;
+  uint32_t row_bytes;
+  const float* scales;  // FP8 per-dimension scales (nullptr otherwise)
+  // FP32 store for the rerank branch (nullptr: plain branch)
+  const char* full;
+  uint64_t full_stride;
+  // graph: n rows of nbr_stride uint32, entries past the degree = sentinel
+  const uint32_t* nbrs;
+  uint32_t nbr_stride;
+  uint32_t max_degree;
+  uint32_t n, dim;
+  // queries (search format) and FP32 queries for the rerank
+  const char* queries;
+  uint64_t q_stride;
+  const char* fqueries;
+  uint64_t fq_stride;
+  uint32_t nq;
+  // start points (device), 1 <= nstarts <= min(beam, 64), unique
+  const uint32_t* starts;
+  uint32_t nstarts;
+  // QueryParams (types.h:233-268); per-query k / beam override the uniform ones
+  int32_t k, beam;
+  const int32_t* k_pq;
+  const int32_t* beam_pq;
+  int64_t limit;
+  int32_t degree_limit;
+  double batch_factor;
+  double rerank_factor;
+  // build mode: the query's own id (same_as) and the visited list (u64 keys)
+  const uint32_t* self_ids;
+  uint64_t* vis_out;
+  uint32_t vis_cap;
+  // outputs
+  uint32_t* out_ids;
+  float* out_dists;
+  uint32_t out_stride;
+  uint32_t* out_visited;
+  uint32_t* out_cmps;
+  uint32_t* out_nfront;
+  uint32_t* status;
+  // scratch
+  uint32_t* work_counter;
+  uint32_t* ghash;  // grid x 2^ghash_log2 slots
+  uint32_t ghash_log2;
+  // LDS layout
+  uint32_t fcap, ccap, hslots_log2;
+};
+
+uint32_t search_lds_bytes(const SearchArgs& a);
+hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool sequential,
+                              uint32_t grid, hipStream_t s);
+
+}  // namespace pann
+
+namespace pann {
+hipError_t launch_bf16_store(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                             uint16_t* Y, uint64_t stride_h, hipStream_t s);
+hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                            uint32_t* absmax_bits, float* scales, uint8_t* Y, uint64_t stride_b,
+                            hipStream_t s);
+hipError_t launch_fp8_codec_probe(float* decode_out, const float* enc_in, uint8_t* enc_out,
+                                  uint32_t n, hipStream_t s);
+}  // namespace pann
+
+namespace pann {
+struct PruneArgs {
+  const char* pts;
+  uint64_t stride;
+  uint32_t dim;
+  const uint32_t* nbrs;
+  uint32_t nbr_stride, max_degree;
+  const uint32_t* points;  // np point ids
+  uint32_t np;
+  const uint32_t* cand_ids;
+  const float* cand_dists;  // nullptr: compute d(c, p)
+  const uint32_t* cand_count;
+  uint32_t cand_stride;
+  double alpha;
+  uint32_t R;
+  int32_t add;
+  uint32_t* out_ids;
+  uint32_t* out_count;
+  uint32_t sort_cap;  // power of two >= cand_stride + max_degree
+};
+
+uint32_t prune_lds_bytes(uint32_t sort_cap, uint32_t dim, int dtype);
+hipError_t launch_robust_prune(const PruneArgs& a, int dtype, int metric, hipStream_t s);
+hipError_t launch_sort_visited(const uint64_t* vis, uint32_t nq, uint32_t cap,
+                               const uint32_t* counts, uint32_t* ids, float* dists,
+                               hipStream_t s);
+hipError_t launch_update_neighbors(uint32_t* nbrs, uint32_t nbr_stride, uint32_t max_degree,
+                                   const uint32_t* points, uint32_t np, const uint32_t* lists,
+                                   const uint32_t* counts, uint32_t R, hipStream_t s);
+}  // namespace pann

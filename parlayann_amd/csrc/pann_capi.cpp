@@ -1,0 +1,637 @@
+// pann_capi.cpp -- the C ABI of libpann.so (declared in include/pann.h).
+//
+// Host side of the device path: validates arguments where the reference
+// aborts, lays the index out in HBM (rows padded to 128 bytes, graph rows
+// padded with a sentinel past the degree), sizes the persistent launch and
+// reports kernel status bits as pann_status codes.
+#include "../../include/pann.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "dist.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+pann_status fail(pann_status st, const std::string& msg) {
+  g_err = msg;
+  return st;
+}
+
+pann_status hip_fail(hipError_t e, const char* where) {
+  g_err = std::string(where) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? PANN_ERR_OOM : PANN_ERR_DEVICE;
+}
+
+#define HIP_TRY(expr, where)                 \
+  do {                                       \
+    hipError_t _e = (expr);                  \
+    if (_e != hipSuccess) return hip_fail(_e, where); \
+  } while (0)
+
+uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
+uint32_t ceil_log2(uint64_t x) {
+  uint32_t l = 0;
+  while ((1ull << l) < x) l++;
+  return l;
+}
+size_t elem_bytes(int dt) { return dt == PANN_DT_F32 ? 4 : 1; }
+
+}  // namespace
+
+struct pann_index_s {
+  int device = 0;
+  uint32_t n = 0, dim = 0, max_degree = 0;
+  int dtype = 0, metric = 0;
+  char* d_pts = nullptr;
+  uint64_t pts_stride = 0;
+  uint32_t row_bytes = 0;
+  char* d_bf16 = nullptr;
+  uint64_t bf16_stride = 0;
+  char* d_fp8 = nullptr;
+  uint64_t fp8_stride = 0;
+  float* d_scales = nullptr;
+  uint32_t* d_absmax = nullptr;
+  uint32_t* d_nbrs = nullptr;
+  uint32_t nbr_stride = 0;
+  uint32_t* d_counter = nullptr;
+  uint32_t* d_status = nullptr;
+  uint32_t* d_starts = nullptr;
+  uint32_t* d_ghash = nullptr;
+  size_t ghash_words = 0;
+  uint64_t* d_vis = nullptr;
+  size_t vis_words = 0;
+  uint32_t* d_dummy = nullptr;
+  int num_cus = 0;
+  int64_t lds_per_cu = 0;
+  int64_t max_lds_block = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  std::mutex mu;
+};
+
+namespace {
+
+pann_status ensure_ghash(pann_index_t idx, size_t words) {
+  if (idx->ghash_words >= words) return PANN_OK;
+  if (idx->d_ghash) (void)hipFree(idx->d_ghash);
+  idx->d_ghash = nullptr;
+  idx->ghash_words = 0;
+  HIP_TRY(hipMalloc(&idx->d_ghash, words * 4), "hipMalloc(visited overflow table)");
+  idx->ghash_words = words;
+  return PANN_OK;
+}
+
+pann_status ensure_vis(pann_index_t idx, size_t words) {
+  if (idx->vis_words >= words) return PANN_OK;
+  if (idx->d_vis) (void)hipFree(idx->d_vis);
+  idx->d_vis = nullptr;
+  idx->vis_words = 0;
+  HIP_TRY(hipMalloc(&idx->d_vis, words * 8), "hipMalloc(visited lists)");
+  idx->vis_words = words;
+  return PANN_OK;
+}
+
+__global__ void k_zero_status(uint32_t* p) { p[threadIdx.x] = 0u; }
+
+}  // namespace
+
+extern "C" {
+
+const char* pann_last_error(void) { return g_err.c_str(); }
+
+pann_status pann_device_info(int32_t device, int32_t* num_cus, int64_t* lds_per_cu,
+                             int64_t* hbm_bytes) {
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, device), "hipGetDeviceProperties");
+  if (num_cus) *num_cus = p.multiProcessorCount;
+  if (lds_per_cu) *lds_per_cu = (int64_t)p.maxSharedMemoryPerMultiProcessor;
+  if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;
+  return PANN_OK;
+}
+
+pann_status pann_open(const pann_index_desc* desc, pann_index_t* out) {
+  if (!desc || !out) return fail(PANN_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (desc->n == 0 || desc->dim == 0 || desc->max_degree == 0)
+    return fail(PANN_ERR_ARG, "empty index (n, dim and max_degree must be positive)");
+  if (desc->dtype < PANN_DT_F32 || desc->dtype > PANN_DT_I8)
+    return fail(PANN_ERR_ARG, "unknown dtype");
+  if (desc->metric != PANN_METRIC_L2 && desc->metric != PANN_METRIC_MIPS)
+    return fail(PANN_ERR_ARG, "unknown metric");
+  if (desc->max_degree > 64)
+    return fail(PANN_ERR_UNSUPPORTED, "max_degree > 64 is not supported by this build");
+  if (!desc->points || !desc->graph) return fail(PANN_ERR_ARG, "null points or graph");
+  const uint32_t n = desc->n, md = desc->max_degree;
+  // graph.h:60-63 / :240-243 abort on bad degrees and ids; here they are errors
+  for (uint64_t i = 0; i < n; i++) {
+    const uint32_t* row = desc->graph + i * (md + 1);
+    if (row[0] > md) return fail(PANN_ERR_ARG, "graph row " + std::to_string(i) + " exceeds max degree");
+    for (uint32_t j = 0; j < row[0]; j++)
+      if (row[1 + j] >= n)
+        return fail(PANN_ERR_ARG, "graph row " + std::to_string(i) + " has an out-of-range id");
+  }
+  HIP_TRY(hipSetDevice(desc->device), "hipSetDevice");
+  auto idx = new pann_index_s();
+  idx->device = desc->device;
+  idx->n = n;
+  idx->dim = desc->dim;
+  idx->max_degree = md;
+  idx->dtype = desc->dtype;
+  idx->metric = desc->metric;
+  hipDeviceProp_t p;
+  hipError_t e = hipGetDeviceProperties(&p, desc->device);
+  if (e != hipSuccess) {
+    delete idx;
+    return hip_fail(e, "hipGetDeviceProperties");
+  }
+  idx->num_cus = p.multiProcessorCount;
+  idx->lds_per_cu = (int64_t)p.maxSharedMemoryPerMultiProcessor;
+  idx->max_lds_block = (int64_t)p.sharedMemPerBlockOptin > 0 ? (int64_t)p.sharedMemPerBlockOptin
+                                                             : (int64_t)p.sharedMemPerBlock;
+  if (idx->lds_per_cu <= 0) idx->lds_per_cu = 160 * 1024;
+  const size_t eb = elem_bytes(desc->dtype);
+  idx->row_bytes = (uint32_t)(desc->dim * eb);
+  idx->pts_stride = round_up(idx->row_bytes, 128);
+  idx->nbr_stride = (uint32_t)round_up(md, 16);
+  auto cleanup = [&](pann_status st) {
+    pann_close(idx);
+    return st;
+  };
+  pann_status st;
+  e = hipMalloc(&idx->d_pts, (size_t)n * idx->pts_stride);
+  if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(points)"));
+  e = hipMemset(idx->d_pts, 0, (size_t)n * idx->pts_stride);
+  if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemset(points)"));
+  e = hipMemcpy2D(idx->d_pts, idx->pts_stride, desc->points, idx->row_bytes, idx->row_bytes, n,
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemcpy2D(points)"));
+  e = hipMalloc(&idx->d_nbrs, (size_t)n * idx->nbr_stride * 4);
+  if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(graph)"));
+  {
+    const uint64_t chunk = 1u << 20;
+    std::vector<uint32_t> buf;
+    for (uint64_t r0 = 0; r0 < n; r0 += chunk) {
+      uint64_t r1 = std::min<uint64_t>(n, r0 + chunk);
+      buf.assign((r1 - r0) * idx->nbr_stride, PANN_SENTINEL);
+      for (uint64_t i = r0; i < r1; i++) {
+        const uint32_t* row = desc->graph + i * (md + 1);
+        std::memcpy(&buf[(i - r0) * idx->nbr_stride], row + 1, row[0] * 4);
+      }
+      e = hipMemcpy(idx->d_nbrs + r0 * idx->nbr_stride, buf.data(), buf.size() * 4,
+                    hipMemcpyHostToDevice);
+      if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemcpy(graph)"));
+    }
+  }
+  e = hipMalloc(&idx->d_counter, 256);
+  if (e == hipSuccess) e = hipMalloc(&idx->d_status, 256);
+  if (e == hipSuccess) e = hipMalloc(&idx->d_starts, 64 * 4);
+  if (e == hipSuccess) e = hipMalloc(&idx->d_dummy, 256);
+  if (e == hipSuccess) e = hipEventCreate(&idx->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&idx->ev1);
+  if (e != hipSuccess) return cleanup(hip_fail(e, "scratch allocation"));
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return cleanup(hip_fail(e, "hipDeviceSynchronize"));
+  (void)st;
+  *out = idx;
+  return PANN_OK;
+}
+
+pann_status pann_build_store(pann_index_t idx, int32_t store) {
+  if (!idx) return fail(PANN_ERR_ARG, "null index");
+  if (store != PANN_STORE_BF16 && store != PANN_STORE_FP8)
+    return fail(PANN_ERR_ARG, "store must be PANN_STORE_BF16 or PANN_STORE_FP8");
+  if (idx->dtype != PANN_DT_F32)
+    return fail(PANN_ERR_ARG, "reduced-precision stores need float points");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  const uint64_t fstride = idx->pts_stride / 4;
+  if (store == PANN_STORE_BF16) {
+    if (!idx->d_bf16) {
+      idx->bf16_stride = round_up((uint64_t)idx->dim * 2, 128);
+      HIP_TRY(hipMalloc(&idx->d_bf16, (size_t)idx->n * idx->bf16_stride), "hipMalloc(bf16)");
+      HIP_TRY(hipMemset(idx->d_bf16, 0, (size_t)idx->n * idx->bf16_stride), "hipMemset(bf16)");
+    }
+    HIP_TRY(pann::launch_bf16_store((const float*)idx->d_pts, fstride, idx->n, idx->dim,
+                                    (uint16_t*)idx->d_bf16, idx->bf16_stride / 2, nullptr),
+            "bf16 store kernel");
+  } else {
+    if (!idx->d_fp8) {
+      idx->fp8_stride = round_up((uint64_t)idx->dim, 128);
+      HIP_TRY(hipMalloc(&idx->d_fp8, (size_t)idx->n * idx->fp8_stride), "hipMalloc(fp8)");
+      HIP_TRY(hipMemset(idx->d_fp8, 0, (size_t)idx->n * idx->fp8_stride), "hipMemset(fp8)");
+      HIP_TRY(hipMalloc(&idx->d_scales, idx->dim * 4), "hipMalloc(scales)");
+      HIP_TRY(hipMalloc(&idx->d_absmax, idx->dim * 4), "hipMalloc(absmax)");
+    }
+    HIP_TRY(pann::launch_fp8_store((const float*)idx->d_pts, fstride, idx->n, idx->dim,
+                                   idx->d_absmax, idx->d_scales, (uint8_t*)idx->d_fp8,
+                                   idx->fp8_stride, nullptr),
+            "fp8 store kernel");
+  }
+  HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  return PANN_OK;
+}
+
+pann_status pann_get_fp8_scales(pann_index_t idx, float* scales_host) {
+  if (!idx || !scales_host) return fail(PANN_ERR_ARG, "null argument");
+  if (!idx->d_scales) return fail(PANN_ERR_ARG, "FP8 store not built");
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  HIP_TRY(hipMemcpy(scales_host, idx->d_scales, idx->dim * 4, hipMemcpyDeviceToHost),
+          "hipMemcpy(scales)");
+  return PANN_OK;
+}
+
+namespace {
+
+// Fills the launch block shared by pann_search and pann_search_visited.
+pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint32_t* starts,
+                           uint32_t nstarts, const pann_search_opts* opts, pann::SearchArgs& a,
+                           int& store_kind, bool& sequential, uint32_t& grid,
+                           hipStream_t stream) {
+  const int64_t B = qp->beam_size;
+  if (qp->k < 0 || B < 1) return fail(PANN_ERR_ARG, "beam_size must be >= 1 and k >= 0");
+  if (qp->k > B)  // beamSearch.h:612-616
+    return fail(PANN_ERR_ARG, "beam search parameter Q = " + std::to_string(B) +
+                                  " same size or smaller than k = " + std::to_string(qp->k));
+  if (B > 4096) return fail(PANN_ERR_UNSUPPORTED, "beam_size > 4096 is not supported");
+  if (qp->degree_limit < 1) return fail(PANN_ERR_ARG, "degree_limit must be >= 1");
+  if (!(qp->batch_factor >= 0.0) || !(qp->rerank_factor >= 0.0))
+    return fail(PANN_ERR_ARG, "batch_factor and rerank_factor must be >= 0");
+  if (nstarts == 0)  // beamSearch.h:51-53
+    return fail(PANN_ERR_ARG, "beam search expects at least one start point");
+  if (nstarts > B || nstarts > 64)  // :54-57
+    return fail(PANN_ERR_ARG, "beam search has more starting points than beam size (or > 64)");
+  {
+    std::unordered_set<uint32_t> seen;
+    for (uint32_t i = 0; i < nstarts; i++) {
+      if (starts[i] >= idx->n) return fail(PANN_ERR_ARG, "start point out of range");
+      if (!seen.insert(starts[i]).second) return fail(PANN_ERR_ARG, "duplicate start points");
+    }
+  }
+  pann_search_opts o{};
+  if (opts) o = *opts;
+  store_kind = pann::ST_F32;
+  const char* pts = idx->d_pts;
+  uint64_t stride = idx->pts_stride;
+  uint32_t row_bytes = idx->row_bytes;
+  sequential = false;
+  if (o.store == PANN_STORE_NATIVE) {
+    store_kind = idx->dtype == PANN_DT_F32 ? pann::ST_F32
+                 : idx->dtype == PANN_DT_U8 ? pann::ST_U8 : pann::ST_I8;
+    sequential = idx->dtype == PANN_DT_F32 && o.summation == PANN_SUM_INORDER;
+  } else if (o.store == PANN_STORE_BF16) {
+    if (!idx->d_bf16) return fail(PANN_ERR_ARG, "BF16 store not built (pann_build_store)");
+    store_kind = pann::ST_BF16;
+    pts = idx->d_bf16;
+    stride = idx->bf16_stride;
+    row_bytes = idx->dim * 2;
+  } else if (o.store == PANN_STORE_FP8) {
+    if (!idx->d_fp8) return fail(PANN_ERR_ARG, "FP8 store not built (pann_build_store)");
+    store_kind = pann::ST_FP8;
+    pts = idx->d_fp8;
+    stride = idx->fp8_stride;
+    row_bytes = idx->dim;
+  } else {
+    return fail(PANN_ERR_ARG, "unknown store");
+  }
+  if (o.rerank && idx->dtype != PANN_DT_F32)
+    return fail(PANN_ERR_ARG, "rerank needs float points");
+  if (!sequential && row_bytes > 8 * 128)
+    return fail(PANN_ERR_UNSUPPORTED, "rows wider than 1024 bytes need the in-order f32 path");
+
+  std::memset(&a, 0, sizeof(a));
+  a.pts = pts;
+  a.pts_stride = stride;
+  a.row_bytes = row_bytes;
+  a.scales = idx->d_scales;
+  a.full = o.rerank ? idx->d_pts : nullptr;
+  a.full_stride = idx->pts_stride;
+  a.nbrs = idx->d_nbrs;
+  a.nbr_stride = idx->nbr_stride;
+  a.max_degree = idx->max_degree;
+  a.n = idx->n;
+  a.dim = idx->dim;
+  a.k = (int32_t)qp->k;
+  a.beam = (int32_t)B;
+  a.limit = qp->limit;
+  a.degree_limit = (int32_t)std::min<int64_t>(qp->degree_limit, idx->max_degree);
+  a.batch_factor = qp->batch_factor;
+  a.rerank_factor = qp->rerank_factor;
+  a.status = idx->d_status;
+  a.work_counter = idx->d_counter;
+  a.nstarts = nstarts;
+  a.starts = idx->d_starts;
+
+  // LDS layout
+  const int64_t thr = (int64_t)std::ceil(qp->batch_factor * (double)B);
+  const int64_t defer_cap = (qp->limit >= 2 * B) ? std::max<int64_t>(thr - 1, 0) : 0;
+  a.fcap = (uint32_t)round_up((uint64_t)B, 16);
+  a.ccap = (uint32_t)round_up((uint64_t)(defer_cap + a.degree_limit + 1), 16);
+  a.ccap = std::max<uint32_t>(a.ccap, 64);  // also holds the start points
+  uint32_t hlog2 = (uint32_t)o.hash_slots_log2;
+  if (hlog2 == 0) {
+    // expected distinct ids seen ~ visits * new-per-visit; sized for <= 1/2 load
+    double est = (double)B * 1.3 * (double)a.degree_limit * 0.45 * 2.0;
+    hlog2 = std::min<uint32_t>(std::max<uint32_t>(ceil_log2((uint64_t)est), 10), 13);
+  }
+  if (hlog2 < 8 || hlog2 > 15) return fail(PANN_ERR_ARG, "hash_slots_log2 must be in [8, 15]");
+  a.hslots_log2 = hlog2;
+  const uint32_t lds = pann::search_lds_bytes(a);
+  if ((int64_t)lds > idx->max_lds_block || (int64_t)lds > idx->lds_per_cu)
+    return fail(PANN_ERR_UNSUPPORTED, "beam too large for LDS (" + std::to_string(lds) + " bytes)");
+  int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
+  wpc = (int)std::min<int64_t>(wpc, idx->lds_per_cu / lds);
+  if (wpc < 1) wpc = 1;
+  grid = (uint32_t)(idx->num_cus * wpc);
+  // per-wave global overflow table: twice the reference's initial hashset
+  // size 2*(10+B)*degree_limit (beamSearch.h:66), at least 4x the LDS table
+  uint32_t glog2 = std::max<uint32_t>(ceil_log2((uint64_t)2 * (10 + B) * a.degree_limit) + 1,
+                                      hlog2 + 2);
+  a.ghash_log2 = glog2;
+  pann_status st = ensure_ghash(idx, (size_t)grid << glog2);
+  if (st != PANN_OK) return st;
+  a.ghash = idx->d_ghash;
+  HIP_TRY(hipMemcpyAsync(idx->d_starts, starts, nstarts * 4, hipMemcpyHostToDevice, stream),
+          "hipMemcpyAsync(starts)");
+  hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(1), 0, stream, idx->d_status);
+  HIP_TRY(hipGetLastError(), "status reset");
+  return PANN_OK;
+}
+
+pann_status check_status(pann_index_t idx, hipStream_t stream) {
+  HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  uint32_t st = 0;
+  HIP_TRY(hipMemcpy(&st, idx->d_status, 4, hipMemcpyDeviceToHost), "hipMemcpy(status)");
+  if (st & pann::PANN_ST_BAD_PARAMS)  // beamSearch.h:612-616 (k > beam) per query
+    return fail(PANN_ERR_ARG, "a per-query k or beam is invalid (k > beam, beam > qp->beam_size, "
+                              "or k > out_stride)");
+  if (st & pann::PANN_ST_SHORT_BEAM)  // beamSearch.h:557-562
+    return fail(PANN_ERR_SHORT_BEAM, "beam search returned fewer than k elements");
+  if (st & pann::PANN_ST_HASH_CAPACITY)
+    return fail(PANN_ERR_CAPACITY, "a query outgrew the visited-set capacity");
+  if (st & pann::PANN_ST_VISITED_OVERFLOW)
+    return fail(PANN_ERR_CAPACITY, "a visited list outgrew vis_cap");
+  return PANN_OK;
+}
+
+}  // namespace
+
+pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
+                        const pann_qparams* qp, const int32_t* k_pq, const int32_t* beam_pq,
+                        const uint32_t* starts, uint32_t nstarts, const pann_search_opts* opts,
+                        uint32_t* out_ids, float* out_dists, uint32_t out_stride,
+                        uint32_t* out_visited, uint32_t* out_cmps, void* stream_) {
+  if (!idx || !qp || !starts) return fail(PANN_ERR_ARG, "null argument");
+  if (nq == 0) return PANN_OK;
+  if (!queries || !out_ids || !out_dists) return fail(PANN_ERR_ARG, "null buffer");
+  if (out_stride < (uint32_t)std::max<int64_t>(qp->k, 0))
+    return fail(PANN_ERR_ARG, "out_stride smaller than k");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  hipStream_t stream = (hipStream_t)stream_;
+  pann::SearchArgs a;
+  int store_kind;
+  bool sequential;
+  uint32_t grid;
+  pann_status st = prepare_search(idx, qp, starts, nstarts, opts, a, store_kind, sequential, grid,
+                                  stream);
+  if (st != PANN_OK) return st;
+  const bool f32_queries = idx->dtype == PANN_DT_F32;
+  a.queries = (const char*)queries;
+  a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
+  a.fqueries = (const char*)queries;
+  a.fq_stride = (uint64_t)idx->dim * 4;
+  a.nq = nq;
+  a.k_pq = k_pq;
+  a.beam_pq = beam_pq;
+  a.out_ids = out_ids;
+  a.out_dists = out_dists;
+  a.out_stride = out_stride;
+  a.out_visited = out_visited;
+  a.out_cmps = out_cmps;
+  grid = std::min<uint32_t>(grid, nq);
+  HIP_TRY(hipEventRecord(idx->ev0, stream), "hipEventRecord");
+  HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
+          "beam search launch");
+  HIP_TRY(hipEventRecord(idx->ev1, stream), "hipEventRecord");
+  idx->timed = true;
+  if (opts && opts->async) return PANN_OK;
+  return check_status(idx, stream);
+}
+
+pann_status pann_search_status(pann_index_t idx, void* stream) {
+  if (!idx) return fail(PANN_ERR_ARG, "null index");
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  return check_status(idx, (hipStream_t)stream);
+}
+
+pann_status pann_last_kernel_ms(pann_index_t idx, float* ms) {
+  if (!idx || !ms) return fail(PANN_ERR_ARG, "null argument");
+  if (!idx->timed) return fail(PANN_ERR_ARG, "no search has run");
+  HIP_TRY(hipEventSynchronize(idx->ev1), "hipEventSynchronize");
+  HIP_TRY(hipEventElapsedTime(ms, idx->ev0, idx->ev1), "hipEventElapsedTime");
+  return PANN_OK;
+}
+
+pann_status pann_search_host(pann_index_t idx, const void* queries, uint32_t nq,
+                             const pann_qparams* qp, const uint32_t* starts, uint32_t nstarts,
+                             const pann_search_opts* opts, uint32_t* out_ids, float* out_dists,
+                             uint32_t* out_visited, uint32_t* out_cmps) {
+  if (!idx || !qp || !queries || !out_ids || !out_dists) return fail(PANN_ERR_ARG, "null argument");
+  if (nq == 0) return PANN_OK;
+  if (qp->k < 1) return fail(PANN_ERR_ARG, "k must be >= 1");
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  const size_t qbytes = (size_t)nq * idx->dim * (idx->dtype == PANN_DT_F32 ? 4 : 1);
+  const size_t k = (size_t)qp->k;
+  void* dq = nullptr;
+  uint32_t *di = nullptr, *dv = nullptr, *dc = nullptr;
+  float* dd = nullptr;
+  auto release = [&]() {
+    if (dq) (void)hipFree(dq);
+    if (di) (void)hipFree(di);
+    if (dd) (void)hipFree(dd);
+    if (dv) (void)hipFree(dv);
+    if (dc) (void)hipFree(dc);
+  };
+  hipError_t e = hipMalloc(&dq, qbytes);
+  if (e == hipSuccess) e = hipMalloc(&di, nq * k * 4);
+  if (e == hipSuccess) e = hipMalloc(&dd, nq * k * 4);
+  if (e == hipSuccess) e = hipMalloc(&dv, nq * 4);
+  if (e == hipSuccess) e = hipMalloc(&dc, nq * 4);
+  if (e == hipSuccess) e = hipMemcpy(dq, queries, qbytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    release();
+    return hip_fail(e, "search_host staging");
+  }
+  pann_search_opts o{};
+  if (opts) o = *opts;
+  o.async = 0;
+  pann_status st = pann_search(idx, dq, nq, qp, nullptr, nullptr, starts, nstarts, &o, di, dd,
+                               (uint32_t)k, dv, dc, nullptr);
+  if (st == PANN_OK) {
+    e = hipMemcpy(out_ids, di, nq * k * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out_dists, dd, nq * k * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && out_visited) e = hipMemcpy(out_visited, dv, nq * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && out_cmps) e = hipMemcpy(out_cmps, dc, nq * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) st = hip_fail(e, "search_host copy-out");
+  }
+  release();
+  return st;
+}
+
+pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint32_t nq,
+                                uint32_t beam, double batch_factor, uint32_t start,
+                                uint32_t vis_cap, uint32_t* vis_ids, float* vis_dists,
+                                uint32_t* vis_count, void* stream_) {
+  if (!idx || !self_ids || !vis_ids || !vis_dists || !vis_count)
+    return fail(PANN_ERR_ARG, "null argument");
+  if (nq == 0) return PANN_OK;
+  if (vis_cap == 0 || vis_cap > 4096) return fail(PANN_ERR_ARG, "vis_cap must be in [1, 4096]");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  hipStream_t stream = (hipStream_t)stream_;
+  // QueryParams QP(0, L, n, max_degree, rerank, batch_factor) (vamana/index.h:251-252)
+  pann_qparams qp{0, (int64_t)beam, (int64_t)idx->n, (int64_t)idx->max_degree, 100.0,
+                  batch_factor};
+  pann_search_opts o{};
+  o.store = PANN_STORE_NATIVE;
+  o.summation = PANN_SUM_INORDER;
+  pann::SearchArgs a;
+  int store_kind;
+  bool sequential;
+  uint32_t grid;
+  pann_status st = prepare_search(idx, &qp, &start, 1, &o, a, store_kind, sequential, grid, stream);
+  if (st != PANN_OK) return st;
+  st = ensure_vis(idx, (size_t)nq * vis_cap);
+  if (st != PANN_OK) return st;
+  a.queries = idx->d_pts;  // each query is the point's own row
+  a.q_stride = idx->pts_stride;
+  a.fqueries = idx->d_pts;
+  a.fq_stride = idx->pts_stride;
+  a.nq = nq;
+  a.self_ids = self_ids;
+  a.vis_out = idx->d_vis;
+  a.vis_cap = vis_cap;
+  a.out_ids = idx->d_dummy;
+  a.out_dists = (float*)idx->d_dummy;
+  a.out_stride = 0;
+  a.out_visited = vis_count;
+  // with self_ids set the kernel reads query q from row self_ids[q]
+  grid = std::min<uint32_t>(grid, nq);
+  HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
+          "build-mode search launch");
+  HIP_TRY(pann::launch_sort_visited(idx->d_vis, nq, vis_cap, vis_count, vis_ids, vis_dists, stream),
+          "visited sort launch");
+  return check_status(idx, stream);
+}
+
+pann_status pann_robust_prune_batch(pann_index_t idx, const uint32_t* pts, uint32_t np,
+                                    const uint32_t* cand_ids, const float* cand_dists,
+                                    const uint32_t* cand_count, uint32_t cand_stride,
+                                    double alpha, uint32_t R, int32_t add, uint32_t* out_ids,
+                                    uint32_t* out_count, void* stream) {
+  if (!idx || !pts || !cand_ids || !cand_count || !out_ids || !out_count)
+    return fail(PANN_ERR_ARG, "null argument");
+  if (np == 0) return PANN_OK;
+  if (R == 0 || R > idx->max_degree) return fail(PANN_ERR_ARG, "R must be in [1, max_degree]");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  pann::PruneArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.pts = idx->d_pts;
+  a.stride = idx->pts_stride;
+  a.dim = idx->dim;
+  a.nbrs = idx->d_nbrs;
+  a.nbr_stride = idx->nbr_stride;
+  a.max_degree = idx->max_degree;
+  a.points = pts;
+  a.np = np;
+  a.cand_ids = cand_ids;
+  a.cand_dists = cand_dists;
+  a.cand_count = cand_count;
+  a.cand_stride = cand_stride;
+  a.alpha = alpha;
+  a.R = R;
+  a.add = add;
+  a.out_ids = out_ids;
+  a.out_count = out_count;
+  uint32_t cap = 2;
+  while (cap < cand_stride + idx->max_degree) cap <<= 1;
+  a.sort_cap = cap;
+  uint32_t lds = pann::prune_lds_bytes(cap, idx->dim, idx->dtype == PANN_DT_F32 ? pann::ST_F32
+                                                      : idx->dtype == PANN_DT_U8 ? pann::ST_U8
+                                                                                 : pann::ST_I8);
+  if ((int64_t)lds > idx->max_lds_block) return fail(PANN_ERR_UNSUPPORTED, "candidate set too large for LDS");
+  int dk = idx->dtype == PANN_DT_F32 ? pann::ST_F32 : idx->dtype == PANN_DT_U8 ? pann::ST_U8 : pann::ST_I8;
+  HIP_TRY(pann::launch_robust_prune(a, dk, idx->metric, (hipStream_t)stream), "robust prune launch");
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  return PANN_OK;
+}
+
+pann_status pann_update_neighbors(pann_index_t idx, const uint32_t* pts, uint32_t np,
+                                  const uint32_t* lists, const uint32_t* counts, uint32_t R,
+                                  void* stream) {
+  if (!idx || !pts || !lists || !counts) return fail(PANN_ERR_ARG, "null argument");
+  if (np == 0) return PANN_OK;
+  if (R == 0 || R > idx->max_degree) return fail(PANN_ERR_ARG, "R must be in [1, max_degree]");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  HIP_TRY(pann::launch_update_neighbors(idx->d_nbrs, idx->nbr_stride, idx->max_degree, pts, np,
+                                        lists, counts, R, (hipStream_t)stream),
+          "update neighbours launch");
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  return PANN_OK;
+}
+
+pann_status pann_download_graph(pann_index_t idx, uint32_t* graph_host) {
+  if (!idx || !graph_host) return fail(PANN_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  const uint64_t chunk = 1u << 20;
+  std::vector<uint32_t> buf;
+  const uint32_t md = idx->max_degree;
+  for (uint64_t r0 = 0; r0 < idx->n; r0 += chunk) {
+    uint64_t r1 = std::min<uint64_t>(idx->n, r0 + chunk);
+    buf.resize((r1 - r0) * idx->nbr_stride);
+    HIP_TRY(hipMemcpy(buf.data(), idx->d_nbrs + r0 * idx->nbr_stride, buf.size() * 4,
+                      hipMemcpyDeviceToHost),
+            "hipMemcpy(graph)");
+    for (uint64_t i = r0; i < r1; i++) {
+      const uint32_t* src = &buf[(i - r0) * idx->nbr_stride];
+      uint32_t* dst = graph_host + i * (md + 1);
+      uint32_t d = 0;
+      while (d < md && src[d] != PANN_SENTINEL) d++;
+      dst[0] = d;
+      std::memcpy(dst + 1, src, d * 4);
+      std::memset(dst + 1 + d, 0, (md - d) * 4);
+    }
+  }
+  return PANN_OK;
+}
+
+pann_status pann_close(pann_index_t idx) {
+  if (!idx) return PANN_OK;
+  (void)hipSetDevice(idx->device);
+  (void)hipDeviceSynchronize();
+  void* ptrs[] = {idx->d_pts, idx->d_bf16, idx->d_fp8, idx->d_scales, idx->d_absmax, idx->d_nbrs,
+                  idx->d_counter, idx->d_status, idx->d_starts, idx->d_ghash, idx->d_vis,
+                  idx->d_dummy};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (idx->ev0) (void)hipEventDestroy(idx->ev0);
+  if (idx->ev1) (void)hipEventDestroy(idx->ev1);
+  delete idx;
+  return PANN_OK;
+}
+
+}  // extern "C"

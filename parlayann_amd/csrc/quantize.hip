@@ -1,0 +1,129 @@
+// quantize.hip -- reduced-precision copies of an FP32 point store (K8).
+//
+// BF16: round-to-nearest-even of each f32 (NaN inputs are not expected).
+// FP8:  OCP e4m3fn with a per-dimension scale s_j = max_i |x_ij| / 448
+//       (448 = largest finite e4m3), code = RNE(x_ij / s_j) saturated to
+//       +-448; subnormals (|v| < 2^-6) are kept (step 2^-9).
+// The reference's analog is its scalar quantization of points
+// (euclidian_point.h:188-241, mips_point.h:425-501), which uses one global
+// scale; this store scales per dimension instead.
+#include "kernels.h"
+#include "device_common.h"
+#include "dist.h"
+
+namespace pann {
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ uint8_t f32_to_e4m3_rne(float v) {
+  uint32_t sgn = (__float_as_uint(v) >> 24) & 0x80u;
+  float a = fabsf(v);
+  uint32_t code;
+  if (!(a < 448.0f)) {
+    code = 0x7Eu;  // saturate (also catches +inf)
+  } else if (a < 0.015625f) {  // below 2^-6: subnormal grid m * 2^-9, m in [0, 8]
+    code = (uint32_t)rintf(a * 512.0f);  // exact scaling, RNE; m == 8 is the first normal
+  } else {
+    uint32_t b = __float_as_uint(a);
+    b += 0x7FFFFu + ((b >> 20) & 1u);      // RNE to 3 mantissa bits
+    code = (b >> 20) - ((127u - 7u) << 3); // re-bias the exponent 127 -> 7
+    if (code > 0x7Eu) code = 0x7Eu;
+  }
+  return (uint8_t)(code | sgn);
+}
+
+__global__ void k_absmax_cols(const float* __restrict__ X, uint64_t stride_f, uint32_t n,
+                              uint32_t dim, uint32_t* __restrict__ out_bits) {
+  // each block: a slab of rows, one thread per column (looping)
+  uint32_t r0 = blockIdx.x * 256u;
+  uint32_t r1 = r0 + 256u < n ? r0 + 256u : n;
+  for (uint32_t j = threadIdx.x; j < dim; j += blockDim.x) {
+    float m = 0.0f;
+    for (uint32_t r = r0; r < r1; r++) m = fmaxf(m, fabsf(X[(uint64_t)r * stride_f + j]));
+    atomicMax(&out_bits[j], __float_as_uint(m));  // non-negative floats order as uints
+  }
+}
+
+__global__ void k_scales_from_absmax(const uint32_t* __restrict__ bits, uint32_t dim,
+                                     float* __restrict__ scales) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < dim) {
+    float m = __uint_as_float(bits[j]);
+    scales[j] = m > 0.0f ? m / 448.0f : 1.0f;
+  }
+}
+
+__global__ void k_encode_bf16(const float* __restrict__ X, uint64_t stride_f, uint32_t n,
+                              uint32_t dim, uint16_t* __restrict__ Y, uint64_t stride_h) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t total = (uint64_t)n * dim;
+  if (i >= total) return;
+  uint64_t r = i / dim, j = i % dim;
+  Y[r * stride_h + j] = f32_to_bf16_rne(X[r * stride_f + j]);
+}
+
+__global__ void k_encode_fp8(const float* __restrict__ X, uint64_t stride_f, uint32_t n,
+                             uint32_t dim, const float* __restrict__ scales,
+                             uint8_t* __restrict__ Y, uint64_t stride_b) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t total = (uint64_t)n * dim;
+  if (i >= total) return;
+  uint64_t r = i / dim, j = i % dim;
+  Y[r * stride_b + j] = f32_to_e4m3_rne(X[r * stride_f + j] / scales[j]);
+}
+
+__global__ void k_zero_u32_quant(uint32_t* p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// Decodes all 256 e4m3 codes with the search kernels' decoder (for tests).
+__global__ void k_fp8_decode_all(float* out) {
+  uint32_t i = threadIdx.x;
+  if (i < 256u) out[i] = fp8e4m3_to_f32(i);
+}
+
+// Encodes n floats with the store encoder (for tests).
+__global__ void k_fp8_encode_n(const float* in, uint8_t* out, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = f32_to_e4m3_rne(in[i]);
+}
+
+hipError_t launch_bf16_store(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                             uint16_t* Y, uint64_t stride_h, hipStream_t s) {
+  uint64_t total = (uint64_t)n * dim;
+  uint32_t blocks = (uint32_t)((total + 255) / 256);
+  hipLaunchKernelGGL(k_encode_bf16, dim3(blocks), dim3(256), 0, s, X, stride_f, n, dim, Y,
+                     stride_h);
+  return hipGetLastError();
+}
+
+hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                            uint32_t* absmax_bits, float* scales, uint8_t* Y, uint64_t stride_b,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u32_quant, dim3((dim + 255) / 256), dim3(256), 0, s, absmax_bits, dim);
+  hipLaunchKernelGGL(k_absmax_cols, dim3((n + 255) / 256), dim3(128), 0, s, X, stride_f, n, dim,
+                     absmax_bits);
+  hipLaunchKernelGGL(k_scales_from_absmax, dim3((dim + 255) / 256), dim3(256), 0, s, absmax_bits,
+                     dim, scales);
+  uint64_t total = (uint64_t)n * dim;
+  uint32_t blocks = (uint32_t)((total + 255) / 256);
+  hipLaunchKernelGGL(k_encode_fp8, dim3(blocks), dim3(256), 0, s, X, stride_f, n, dim, scales, Y,
+                     stride_b);
+  return hipGetLastError();
+}
+
+}  // namespace pann
+
+namespace pann {
+hipError_t launch_fp8_codec_probe(float* decode_out, const float* enc_in, uint8_t* enc_out,
+                                  uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fp8_decode_all, dim3(1), dim3(256), 0, s, decode_out);
+  if (n) hipLaunchKernelGGL(k_fp8_encode_n, dim3((n + 255) / 256), dim3(256), 0, s, enc_in, enc_out, n);
+  return hipGetLastError();
+}
+}  // namespace pann

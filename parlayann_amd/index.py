@@ -1,0 +1,282 @@
+"""Device index and the reference's Python search surface.
+
+``DeviceIndex`` owns one libpann index (points + graph in HBM) and exposes
+the batched search of qsearchAll / beam_search_rerank (beamSearch.h:528-646)
+over host (numpy) or device (torch) buffers.
+
+``load_index`` / ``GraphIndex`` mirror python/wrapper.py ``load_index`` and
+python/graph_index.cpp ``GraphIndex`` (batch_search :192-216, single_search
+:218-235, batch_search_from_string :237-257, check_recall :259-305), with the
+intended QueryParams mapping (k = knn, beamSize = beam_width, limit =
+visit_limit if > 0 else n, degree_limit = max degree) instead of the stale
+``QueryParams(knn, beam_width, 1.35, ...)`` call (graph_index.cpp:198).
+``quant=True`` searches the FP8 e4m3 store and reranks in FP32 (the
+reference's quant path uses its own uint8/int8 quantizers, :141-186).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+from . import formats
+
+_DT = {np.dtype(np.float32): L.DT_F32, np.dtype(np.uint8): L.DT_U8, np.dtype(np.int8): L.DT_I8}
+_METRIC = {"Euclidian": L.METRIC_L2, "l2": L.METRIC_L2, "L2": L.METRIC_L2,
+           "mips": L.METRIC_MIPS, "ip": L.METRIC_MIPS, "MIPS": L.METRIC_MIPS}
+_STORE = {"native": L.STORE_NATIVE, "bf16": L.STORE_BF16, "fp8": L.STORE_FP8}
+_SUM = {"inorder": L.SUM_INORDER, "tree": L.SUM_TREE}
+
+
+def _ptr(a) -> Optional[int]:
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor
+
+
+class DeviceIndex:
+    """Points (n x dim, float32/uint8/int8) and a fixed-stride adjacency
+    (n x (max_degree + 1) uint32, slot 0 = degree; graph.h:46-64) in HBM."""
+
+    def __init__(self, points: np.ndarray, adj: np.ndarray, metric: str = "l2", device: int = 0):
+        points = np.ascontiguousarray(points)
+        adj = np.ascontiguousarray(adj, dtype=np.uint32)
+        if points.dtype not in _DT:
+            raise TypeError(f"unsupported point dtype {points.dtype}")
+        if adj.ndim != 2 or adj.shape[0] != points.shape[0]:
+            raise ValueError("graph size and point size do not match")  # graph_index.cpp:113-116
+        self.n, self.dim = points.shape
+        self.max_degree = adj.shape[1] - 1
+        self.dtype = points.dtype
+        self.metric = metric
+        self.device = device
+        desc = L.IndexDesc(self.n, self.dim, self.max_degree, _DT[points.dtype], _METRIC[metric],
+                           points.ctypes.data, adj.ctypes.data, device)
+        h = C.c_void_p()
+        L.check(L.lib().pann_open(C.byref(desc), C.byref(h)))
+        self._h = h
+        self._stores = {"native"}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().pann_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stores --------------------------------------------------------------
+    def build_store(self, kind: str):
+        """Reduced-precision copy of float points: 'bf16' or 'fp8' (e4m3,
+        per-dimension absmax scales)."""
+        if kind not in self._stores:
+            L.check(L.lib().pann_build_store(self._h, _STORE[kind]))
+            self._stores.add(kind)
+
+    def fp8_scales(self) -> np.ndarray:
+        s = np.zeros(self.dim, np.float32)
+        L.check(L.lib().pann_get_fp8_scales(self._h, s.ctypes.data))
+        return s
+
+    def download_graph(self) -> np.ndarray:
+        adj = np.zeros((self.n, self.max_degree + 1), np.uint32)
+        L.check(L.lib().pann_download_graph(self._h, adj.ctypes.data))
+        return adj
+
+    # -- search --------------------------------------------------------------
+    def _qparams(self, k, beam, limit, degree_limit, rerank_factor, batch_factor):
+        return L.QParams(int(k), int(beam), int(self.n if limit is None else limit),
+                         int(self.max_degree if degree_limit is None else degree_limit),
+                         float(rerank_factor), float(batch_factor))
+
+    @staticmethod
+    def _opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu):
+        return L.SearchOpts(_STORE[store], _SUM[summation], int(bool(rerank)),
+                            int(hash_slots_log2), int(bool(async_)), int(waves_per_cu))
+
+    def search(self, queries, k: int, beam: int, *, limit=None, degree_limit=None,
+               batch_factor: float = 0.125, rerank_factor: float = 100.0,
+               store: str = "native", summation: str = "inorder", rerank: bool = False,
+               starts: Sequence[int] = (0,), hash_slots_log2: int = 0, waves_per_cu: int = 0):
+        """Host-buffer batched search (qsearchAll, beamSearch.h:599-646).
+        Returns dict(ids uint32[nq,k], dists float32[nq,k], visited, dist_cmps)."""
+        qdt = np.float32 if (self.dtype == np.float32) else self.dtype
+        Q = np.ascontiguousarray(queries, dtype=qdt)
+        if Q.ndim != 2 or Q.shape[1] != self.dim:
+            raise ValueError(f"queries must be (nq, {self.dim})")
+        nq = Q.shape[0]
+        ids = np.zeros((nq, k), np.uint32)
+        dists = np.zeros((nq, k), np.float32)
+        vis = np.zeros(nq, np.uint32)
+        cmps = np.zeros(nq, np.uint32)
+        st = np.ascontiguousarray(starts, dtype=np.uint32)
+        qp = self._qparams(k, beam, limit, degree_limit, rerank_factor, batch_factor)
+        op = self._opts(store, summation, rerank, hash_slots_log2, False, waves_per_cu)
+        L.check(L.lib().pann_search_host(self._h, Q.ctypes.data, nq, C.byref(qp), st.ctypes.data,
+                                         len(st), C.byref(op), ids.ctypes.data,
+                                         dists.ctypes.data, vis.ctypes.data, cmps.ctypes.data))
+        return {"ids": ids, "dists": dists, "visited": vis, "dist_cmps": cmps}
+
+    def search_device(self, queries, k: int, beam: int, out_ids, out_dists, *, out_visited=None,
+                      out_cmps=None, k_per_query=None, beam_per_query=None, limit=None,
+                      degree_limit=None, batch_factor: float = 0.125,
+                      rerank_factor: float = 100.0, store: str = "native",
+                      summation: str = "inorder", rerank: bool = False,
+                      starts: Sequence[int] = (0,), hash_slots_log2: int = 0,
+                      waves_per_cu: int = 0, stream=None, async_: bool = False):
+        """Device-buffer search: queries/outputs are torch tensors on the
+        index's device.  `beam` is the maximum beam when beam_per_query is
+        given.  Launches on `stream` (torch stream or raw handle)."""
+        nq = int(queries.shape[0])
+        st = np.ascontiguousarray(starts, dtype=np.uint32)
+        qp = self._qparams(k, beam, limit, degree_limit, rerank_factor, batch_factor)
+        op = self._opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu)
+        if stream is not None and hasattr(stream, "cuda_stream"):
+            stream = stream.cuda_stream
+        L.check(L.lib().pann_search(self._h, _ptr(queries), nq, C.byref(qp), _ptr(k_per_query),
+                                    _ptr(beam_per_query), st.ctypes.data, len(st), C.byref(op),
+                                    _ptr(out_ids), _ptr(out_dists), int(out_ids.shape[1]),
+                                    _ptr(out_visited), _ptr(out_cmps), stream))
+
+    def search_status(self, stream=None):
+        if stream is not None and hasattr(stream, "cuda_stream"):
+            stream = stream.cuda_stream
+        L.check(L.lib().pann_search_status(self._h, stream))
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float(0)
+        L.check(L.lib().pann_last_kernel_ms(self._h, C.byref(ms)))
+        return float(ms.value)
+
+
+def device_info(device: int = 0):
+    cus, lds, hbm = C.c_int32(), C.c_int64(), C.c_int64()
+    L.check(L.lib().pann_device_info(device, C.byref(cus), C.byref(lds), C.byref(hbm)))
+    return {"num_cus": cus.value, "lds_per_cu": lds.value, "hbm_bytes": hbm.value}
+
+
+# ---------------------------------------------------------------------------
+# the reference's Python surface (python/wrapper.py, python/graph_index.cpp)
+# ---------------------------------------------------------------------------
+
+_NP_DT = {"float": np.float32, "uint8": np.uint8, "int8": np.int8}
+
+
+class GraphIndex:
+    """GraphIndex<T, Point> (graph_index.cpp:48-118) over a `.bin` vector file
+    and a graph file, searched on the GPU."""
+
+    def __init__(self, metric: str, dtype: str, data_path: str, index_path: str,
+                 hnsw: bool = False, device: int = 0):
+        if hnsw:
+            from .hnsw import HNSWIndex  # layered model file (HNSW.hpp:631-744)
+            self._hnsw = HNSWIndex.load(index_path, data_path, metric, dtype, device=device)
+            self._dev = self._hnsw.base
+            self.points = self._hnsw.points
+            self.metric, self.dtype = metric, _NP_DT[dtype]
+            return
+        self._hnsw = None
+        if metric not in ("Euclidian", "mips"):
+            raise ValueError("Invalid metric")
+        if dtype not in _NP_DT:
+            raise ValueError("Invalid data type")
+        self.metric, self.dtype = metric, _NP_DT[dtype]
+        pts = formats.read_vectors(data_path, self.dtype)
+        if self.dtype == np.float32 and metric == "mips":
+            # graph_index.cpp:92-95: float MIPS points are normalised on load
+            pts = _normalize(pts)
+        self.points = pts
+        adj = formats.read_graph(index_path)
+        self._dev = DeviceIndex(pts, adj, "l2" if metric == "Euclidian" else "mips", device)
+
+    @property
+    def device_index(self) -> DeviceIndex:
+        return self._dev
+
+    def _prep_queries(self, queries, quant):
+        Q = np.ascontiguousarray(queries, dtype=self.dtype)
+        if Q.ndim == 1:
+            Q = Q[None, :]
+        if quant and self.metric == "mips" and self.dtype == np.float32:
+            Q = _normalize(Q)  # graph_index.cpp:171
+        return Q
+
+    def batch_search(self, queries, knn: int, beam_width: int, quant: bool = False,
+                     visit_limit: int = -1):
+        """(ids uint32[nq, knn], dists float32[nq, knn]) (graph_index.cpp:192-216)."""
+        Q = self._prep_queries(queries, quant)
+        limit = visit_limit if visit_limit > 0 else None
+        if self._hnsw is not None:
+            r = self._hnsw.search(Q, knn, beam_width, limit=limit)
+            return r["ids"], r["dists"]
+        use_q = quant and self.dtype == np.float32
+        if use_q:
+            self._dev.build_store("fp8")
+        r = self._dev.search(Q, knn, beam_width, limit=limit,
+                             store="fp8" if use_q else "native", rerank=use_q)
+        return r["ids"], r["dists"]
+
+    search = batch_search
+
+    def single_search(self, q, knn: int, beam_width: int, quant: bool = False,
+                      visit_limit: int = -1):
+        ids, _ = self.batch_search(np.asarray(q)[None, :], knn, beam_width, quant, visit_limit)
+        return ids[0]
+
+    def batch_search_from_string(self, queries: str, knn: int, beam_width: int,
+                                 quant: bool = False, visit_limit: int = -1):
+        Q = formats.read_vectors(queries, self.dtype)
+        return self.batch_search(Q, knn, beam_width, quant, visit_limit)
+
+    def check_recall(self, queries_file: str, graph_file: str, neighbors, k: int) -> float:
+        """Recall with ties at the k-th groundtruth distance counted
+        (graph_index.cpp:259-305); prints and returns it."""
+        from .recall import recall_with_ties
+        Q = formats.read_vectors(queries_file, self.dtype)
+        gt_ids, _ = formats.read_ibin(graph_file)
+        nb = np.asarray(neighbors)
+        if nb.min() < 0 or nb.max() >= len(self.points):
+            raise ValueError("neighbor reported by query out of range")
+        r = recall_with_ties(self.points, Q, gt_ids, nb, k,
+                             "l2" if self.metric == "Euclidian" else "mips")
+        print(f"Recall: {r:.6f}")
+        return r
+
+
+def _normalize(X: np.ndarray) -> np.ndarray:
+    """Mips_Point::normalize (mips_point.h:120-129): double norm, x * (1/norm)."""
+    X = np.asarray(X, dtype=np.float32)
+    norm = np.sqrt(np.einsum("ij,ij->i", X.astype(np.float64), X.astype(np.float64)))
+    norm[norm == 0] = 1.0
+    inv = (1.0 / norm).astype(np.float32)
+    return (X * inv[:, None]).astype(np.float32)
+
+
+def load_index(metric: str, dtype: str, data_dir: str, index_dir: str, hnsw: bool = False):
+    """python/wrapper.py load_index."""
+    if metric not in ("Euclidian", "mips"):
+        raise Exception("Invalid metric")
+    if dtype not in _NP_DT:
+        raise Exception("Invalid data type")
+    return GraphIndex(metric, dtype, data_dir, index_dir, hnsw)
+
+
+def _variant(metric, dtype):
+    def make(data_path, index_path, hnsw=False):
+        return GraphIndex(metric, dtype, data_path, index_path, hnsw)
+    return make
+
+
+FloatEuclidianIndex = _variant("Euclidian", "float")
+FloatMipsIndex = _variant("mips", "float")
+UInt8EuclidianIndex = _variant("Euclidian", "uint8")
+UInt8MipsIndex = _variant("mips", "uint8")
+Int8EuclidianIndex = _variant("Euclidian", "int8")
+Int8MipsIndex = _variant("mips", "int8")

@@ -1,0 +1,124 @@
+"""Vamana graph construction on the device.
+
+build_index / batch_insert (vamana/index.h:150-319) with every per-point step
+on the GPU: the build-mode beam search that yields each insert's visited list
+(:247-264), robustPrune over it (:267), update_neighbors (:271-273), and the
+reverse-edge rule per grouped key (:292-303).  The host keeps the batch
+schedule (:206-240) and the group_by_key of reverse edges (:281-285; values
+kept in insertion order).  The insertion order is an input (the reference
+draws parlay::random_permutation, :212).  Within a batch every search reads
+the graph as it was before the batch, as in the reference, so the result
+does not depend on scheduling.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from .index import DeviceIndex
+
+
+def batch_schedule(n: int, m: int, base: float = 2.0, max_fraction: float = 0.02):
+    """[(floor, ceiling)] of batch_insert's prefix-doubling schedule
+    (vamana/index.h:206-240)."""
+    max_batch = int(max_fraction * float(np.float32(n)))
+    max_batch = min(max_batch, 1000000)
+    if max_batch == 0:
+        max_batch = n
+    out, count, inc = [], 0, 0
+    while count < m:
+        if math.pow(base, inc) <= max_batch:
+            floor = int(math.pow(base, inc)) - 1
+            ceiling = min(int(math.pow(base, inc + 1)) - 1, m)
+            count = ceiling
+        else:
+            floor = count
+            ceiling = min(count + max_batch, m)
+            count += max_batch
+        out.append((floor, ceiling))
+        inc += 1
+    return out
+
+
+def _p(t):
+    return t.data_ptr()
+
+
+def build_vamana(points: np.ndarray, metric: str, R: int, L_build: int, alpha: float,
+                 perms: np.ndarray, num_passes: int = 1, batch_factor: float = 0.125,
+                 sort_neighbors: bool = True, device: int = 0, vis_cap: int | None = None,
+                 index: DeviceIndex | None = None, progress=None) -> np.ndarray:
+    """Returns the n x (R + 1) adjacency (slot 0 = degree)."""
+    import torch
+
+    n = points.shape[0]
+    perms = np.asarray(perms, dtype=np.uint32).reshape(num_passes, n)
+    idx = index or DeviceIndex(points, np.zeros((n, R + 1), np.uint32), metric, device)
+    if idx.max_degree != R:
+        raise ValueError("index max_degree must equal R")
+    h = idx._h
+    lib = L.lib()
+    dev = torch.device("cuda", device)
+    vis_cap = vis_cap or min(4096, max(4 * L_build, 256))
+    i32 = torch.int32
+    for p in range(num_passes):
+        a = alpha if p == num_passes - 1 else 1.0  # build_index :173-178
+        order = torch.from_numpy(perms[p].astype(np.int32)).to(dev)
+        for bi, (lo, hi) in enumerate(batch_schedule(n, n)):
+            nb = hi - lo
+            ids = order[lo:hi].contiguous()
+            vid = torch.empty((nb, vis_cap), dtype=i32, device=dev)
+            vd = torch.empty((nb, vis_cap), dtype=torch.float32, device=dev)
+            vc = torch.empty(nb, dtype=i32, device=dev)
+            L.check(lib.pann_search_visited(h, _p(ids), nb, L_build, batch_factor, 0, vis_cap,
+                                            _p(vid), _p(vd), _p(vc), None))
+            out = torch.empty((nb, R), dtype=i32, device=dev)
+            cnt = torch.empty(nb, dtype=i32, device=dev)
+            L.check(lib.pann_robust_prune_batch(h, _p(ids), nb, _p(vid), _p(vd), _p(vc), vis_cap,
+                                                a, R, 1, _p(out), _p(cnt), None))
+            L.check(lib.pann_update_neighbors(h, _p(ids), nb, _p(out), _p(cnt), R, None))
+            # reverse edges: (ngh, index) pairs grouped by ngh (:281-285)
+            out_h = out.cpu().numpy()
+            cnt_h = cnt.cpu().numpy()
+            ids_h = perms[p][lo:hi].astype(np.int64)
+            mask = np.arange(R)[None, :] < cnt_h[:, None]
+            keys = out_h[mask].astype(np.int64)
+            if keys.size == 0:
+                continue
+            vals = np.repeat(ids_h, cnt_h)
+            o = np.argsort(keys, kind="stable")
+            keys, vals = keys[o], vals[o]
+            ukeys, starts, counts = np.unique(keys, return_index=True, return_counts=True)
+            ng = ukeys.size
+            maxc = int(counts.max())
+            cand = np.zeros((ng, maxc), np.int32)
+            col = np.arange(keys.size) - np.repeat(starts, counts)
+            cand[np.repeat(np.arange(ng), counts), col] = vals
+            gk = torch.from_numpy(ukeys.astype(np.int32)).to(dev)
+            gc = torch.from_numpy(cand).to(dev)
+            gn = torch.from_numpy(counts.astype(np.int32)).to(dev)
+            out2 = torch.empty((ng, R), dtype=i32, device=dev)
+            cnt2 = torch.empty(ng, dtype=i32, device=dev)
+            L.check(lib.pann_robust_prune_batch(h, _p(gk), ng, _p(gc), None, _p(gn), maxc, a, R, 2,
+                                                _p(out2), _p(cnt2), None))
+            L.check(lib.pann_update_neighbors(h, _p(gk), ng, _p(out2), _p(cnt2), R, None))
+            if progress:
+                progress(p, bi, lo, hi)
+    if sort_neighbors:  # :180-185, ties by id
+        chunk = 1 << 18
+        for lo in range(0, n, chunk):
+            hi = min(n, lo + chunk)
+            ids = torch.arange(lo, hi, dtype=i32, device=dev)
+            zero = torch.zeros(hi - lo, dtype=i32, device=dev)
+            out = torch.empty((hi - lo, R), dtype=i32, device=dev)
+            cnt = torch.empty(hi - lo, dtype=i32, device=dev)
+            L.check(lib.pann_robust_prune_batch(h, _p(ids), hi - lo, _p(zero), None, _p(zero), 0,
+                                                1.0, R, 3, _p(out), _p(cnt), None))
+            L.check(lib.pann_update_neighbors(h, _p(ids), hi - lo, _p(out), _p(cnt), R, None))
+    adj = idx.download_graph()
+    if index is None:
+        idx.close()
+    return adj

@@ -1,0 +1,346 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same
+seeded inputs and the same graph file.
+
+Bars (the tolerance is part of each test):
+  * uint8/int8 L2 and IP: bit-exact ids, distances, visited counts and
+    distance-comparison counts;
+  * float32 with in-order summation: bit-exact, like the integer paths;
+  * float32 with the tree (DPP) summation: identical ids except where the
+    two distances differ by at most 1e-6 relative (ties), >= 99% of queries
+    fully identical;
+  * BF16 / FP8 gathers + FP32 rerank: recall@10 against FP64 groundtruth
+    within 0.02 of the plain FP32 search, reranked distances equal to the
+    in-order FP32 distance (relative error vs FP64 <= 1e-6).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = pytest.mark.gpu
+
+if not gpu_available():
+    pytest.skip("no HIP device", allow_module_level=True)
+
+from parlayann_amd import DeviceIndex, datagen  # noqa: E402
+from parlayann_amd.recall import recall_plain  # noqa: E402
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def u8_case(oracle):
+    X, Q = datagen.workload_u8_l2(n=20000, nq=400)
+    perm = datagen.insertion_orders(len(X), 1, 11)
+    adj = oracle.build_vamana(X, "l2", 64, 128, 1.2, perm)
+    return X, Q, adj
+
+
+@pytest.fixture(scope="module")
+def f32_case(oracle):
+    X, Q = datagen.workload_f32_l2(n=20000, nq=400)
+    perm = datagen.insertion_orders(len(X), 1, 12)
+    adj = oracle.build_vamana(X, "l2", 32, 64, 1.2, perm)
+    return X, Q, adj
+
+
+@pytest.fixture(scope="module")
+def ip_case(oracle):
+    X, Q = datagen.workload_aniso_ip(n=20000, nq=400)
+    perm = datagen.insertion_orders(len(X), 1, 13)
+    adj = oracle.build_vamana(X, "ip", 32, 64, 1.0, perm)
+    return X, Q, adj
+
+
+def _assert_exact(g, o, k):
+    assert np.array_equal(g["ids"], o["ids"]), "ids differ"
+    assert np.array_equal(g["dists"], o["dists"]), "dists differ"
+    assert np.array_equal(g["visited"], o["visited"]), "visited counts differ"
+    assert np.array_equal(g["dist_cmps"].astype(np.uint64), o["dist_cmps"]), "cmps differ"
+
+
+@pytest.mark.parametrize("B", [10, 20, 50, 100, 200])
+def test_u8_l2_bit_exact(oracle, u8_case, B):
+    X, Q, adj = u8_case
+    _say(f"u8 l2 B={B}")
+    o = oracle.search_batch(adj, X, "l2", Q, 10, B)
+    assert not o["hash_grew"].any()
+    idx = DeviceIndex(X, adj, "l2")
+    g = idx.search(Q, 10, B)
+    _assert_exact(g, o, 10)
+
+
+def test_u8_ip_and_i8_bit_exact(oracle, u8_case):
+    X, Q, adj = u8_case
+    _say("u8 ip / i8 l2")
+    o = oracle.search_batch(adj, X, "ip", Q, 10, 32)
+    g = DeviceIndex(X, adj, "ip").search(Q, 10, 32)
+    _assert_exact(g, o, 10)
+    Xs = (X.astype(np.int16) - 128).astype(np.int8)
+    Qs = (Q.astype(np.int16) - 128).astype(np.int8)
+    for metric in ("l2", "ip"):
+        o = oracle.search_batch(adj, Xs, metric, Qs, 10, 32)
+        g = DeviceIndex(Xs, adj, metric).search(Qs, 10, 32)
+        _assert_exact(g, o, 10)
+
+
+@pytest.mark.parametrize("metric,B", [("l2", 10), ("l2", 64), ("l2", 128), ("ip", 16),
+                                      ("ip", 64)])
+def test_f32_inorder_bit_exact(oracle, f32_case, ip_case, metric, B):
+    X, Q, adj = f32_case if metric == "l2" else ip_case
+    _say(f"f32 inorder {metric} B={B}")
+    o = oracle.search_batch(adj, X, metric, Q, 10, B)
+    assert not o["hash_grew"].any()
+    g = DeviceIndex(X, adj, metric).search(Q, 10, B, summation="inorder")
+    _assert_exact(g, o, 10)
+
+
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+def test_f32_tree_ids_up_to_ties(oracle, f32_case, ip_case, metric):
+    X, Q, adj = f32_case if metric == "l2" else ip_case
+    _say(f"f32 tree {metric}")
+    o = oracle.search_batch(adj, X, metric, Q, 10, 64)
+    g = DeviceIndex(X, adj, metric).search(Q, 10, 64, summation="tree")
+    same = np.all(g["ids"] == o["ids"], axis=1)
+    assert same.mean() >= 0.99, same.mean()
+    for i in np.nonzero(~same)[0]:
+        # every differing slot holds a distance tied (1e-6 rel) with the oracle's
+        for a, b in zip(g["ids"][i], o["ids"][i]):
+            if a != b:
+                da = oracle.distance(X[a], Q[i], metric)
+                db = oracle.distance(X[b], Q[i], metric)
+                assert abs(da - db) <= 1e-6 * max(abs(da), abs(db)), (i, a, b, da, db)
+    # tree distances agree with the in-order ones to float rounding
+    rel = np.abs(g["dists"] - o["dists"]) / np.maximum(np.abs(o["dists"]), 1e-30)
+    assert rel[same].max() <= 1e-5
+
+
+def test_hash_overflow_path_is_exact(oracle, u8_case, f32_case):
+    """A 256-slot LDS table overflows into the per-wave global table after
+    128 ids; results must not change."""
+    X, Q, adj = u8_case
+    _say("hash overflow")
+    o = oracle.search_batch(adj, X, "l2", Q, 10, 100)
+    g = DeviceIndex(X, adj, "l2").search(Q, 10, 100, hash_slots_log2=8)
+    _assert_exact(g, o, 10)
+    X, Q, adj = f32_case
+    o = oracle.search_batch(adj, X, "l2", Q, 10, 64)
+    g = DeviceIndex(X, adj, "l2").search(Q, 10, 64, hash_slots_log2=8)
+    _assert_exact(g, o, 10)
+
+
+def test_limit_and_degree_limit(oracle, f32_case):
+    X, Q, adj = f32_case
+    _say("limit / degree limit")
+    for lim, dl in [(15, 32), (20, 16), (35, 8)]:
+        B = max(lim, 10)
+        o = oracle.search_batch(adj, X, "l2", Q, 10, B, limit=lim, degree_limit=dl)
+        g = DeviceIndex(X, adj, "l2").search(Q, 10, B, limit=lim, degree_limit=dl)
+        _assert_exact(g, o, 10)
+
+
+def test_per_query_k_and_beam(oracle, u8_case):
+    import torch
+    X, Q, adj = u8_case
+    _say("per-query k / beam")
+    rng = np.random.default_rng(7)
+    nq = len(Q)
+    ks = rng.choice([1, 10, 50], nq).astype(np.int32)
+    bs = np.array([rng.integers(max(16, k), 129) for k in ks], np.int32)
+    idx = DeviceIndex(X, adj, "l2")
+    dev = torch.device("cuda")
+    tq = torch.from_numpy(Q).to(dev)
+    oi = torch.zeros((nq, 50), dtype=torch.int32, device=dev)
+    od = torch.zeros((nq, 50), dtype=torch.float32, device=dev)
+    idx.search_device(tq, 50, int(bs.max()), oi, od, k_per_query=torch.from_numpy(ks).to(dev),
+                      beam_per_query=torch.from_numpy(bs).to(dev))
+    torch.cuda.synchronize()
+    gi, gd = oi.cpu().numpy().view(np.uint32), od.cpu().numpy()
+    for B in np.unique(bs):
+        sel = np.nonzero(bs == B)[0]
+        kmax = int(ks[sel].max())
+        o = oracle.search_batch(adj, X, "l2", Q[sel], kmax, int(B))
+        for j, qi in enumerate(sel):
+            k = ks[qi]
+            assert np.array_equal(gi[qi, :k], o["ids"][j, :k])
+            assert np.array_equal(gd[qi, :k], o["dists"][j, :k])
+
+
+@pytest.mark.parametrize("store", ["bf16", "fp8"])
+def test_reduced_precision_rerank(oracle, ip_case, store):
+    from parlayann_amd.groundtruth import groundtruth
+    X, Q, adj = ip_case
+    _say(f"{store} + rerank")
+    gt, gd = groundtruth(X, Q, 10, "ip")
+    idx = DeviceIndex(X, adj, "ip")
+    idx.build_store(store)
+    plain = idx.search(Q, 10, 64, summation="inorder")
+    red = idx.search(Q, 10, 64, store=store, rerank=True)
+    r_plain = recall_plain(gt, plain["ids"], 10)
+    r_red = recall_plain(gt, red["ids"], 10)
+    assert r_red >= r_plain - 0.02, (r_plain, r_red)
+    # reranked distances are the in-order FP32 distances of the returned ids
+    for i in range(0, len(Q), 37):
+        for a, d in zip(red["ids"][i], red["dists"][i]):
+            assert d == oracle.distance(X[a], Q[i], "ip")
+    # and close to FP64
+    ex = -(X[red["ids"]].astype(np.float64) * Q[:, None, :].astype(np.float64)).sum(-1)
+    rel = np.abs(red["dists"] - ex) / np.maximum(np.abs(ex), 1e-30)
+    assert rel.max() <= 1e-5
+    # sorted by (dist, id)
+    assert np.all(np.diff(red["dists"], axis=1) >= 0)
+
+
+def _e4m3_table():
+    v = np.zeros(256, np.float64)
+    for c in range(256):
+        s = -1.0 if c & 0x80 else 1.0
+        e, m = (c >> 3) & 0xF, c & 7
+        if e == 0xF and m == 7:
+            v[c] = np.nan
+        elif e == 0:
+            v[c] = s * m * 2.0 ** -9
+        else:
+            v[c] = s * (1 + m / 8.0) * 2.0 ** (e - 7)
+    return v
+
+
+def _complete(n):
+    adj = np.zeros((n, n), np.uint32)
+    for i in range(n):
+        nb = [j for j in range(n) if j != i]
+        adj[i, 0] = len(nb)
+        adj[i, 1:] = nb
+    return adj
+
+
+def _e4m3_encode_bruteforce(v, tab):
+    finite = np.where(np.isnan(tab), np.inf, tab)
+    out = np.zeros(v.shape, np.int64)
+    for idx, x in np.ndenumerate(v):
+        dif = np.abs(finite - float(x))
+        best = np.flatnonzero(dif == dif.min())
+        if len(best) > 1:  # ties: even mantissa
+            best = [b for b in best if (b & 1) == 0] or list(best)
+        out[idx] = int(best[0])
+    return out
+
+
+def test_fp8_store_codec():
+    """FP8 store: scales s_j = max|x_j| / 448; codes = nearest e4m3fn value of
+    x/s (ties to even, saturating), decoded exactly.  Checked through the
+    search itself: on a complete graph with beam = n every row's FP8 IP
+    distance comes back, and must equal -sum_j q_j s_j dec(code_ij)
+    computed from a brute-force encoder over the 256-code table."""
+    _say("fp8 codec")
+    rng = np.random.default_rng(3)
+    n, d = 60, 16
+    X = (rng.standard_normal((n, d)) * 10.0 ** rng.uniform(-4, 1, (n, d))).astype(np.float32)
+    Q = rng.standard_normal((8, d)).astype(np.float32)
+    idx = DeviceIndex(X, _complete(n), "ip")
+    idx.build_store("fp8")
+    s = idx.fp8_scales()
+    assert np.array_equal(s, (np.abs(X).max(0) / np.float32(448.0)).astype(np.float32))
+    tab = _e4m3_table()
+    codes = _e4m3_encode_bruteforce((X / s[None, :]).astype(np.float32), tab)
+    assert (codes & 0x7F).min() < 8  # subnormal codes are exercised
+    deq = tab[codes] * s[None, :].astype(np.float64)
+    r = idx.search(Q, n, n, store="fp8")
+    for i in range(len(Q)):
+        exp = -(deq[r["ids"][i]] * Q[i].astype(np.float64)).sum(1)
+        assert np.allclose(r["dists"][i], exp, rtol=1e-5, atol=1e-6 * np.abs(exp).max())
+
+
+def test_bf16_store_codec():
+    import torch
+    _say("bf16 codec")
+    rng = np.random.default_rng(4)
+    n, d = 60, 24
+    X = (rng.standard_normal((n, d)) * 3).astype(np.float32)
+    Q = rng.standard_normal((8, d)).astype(np.float32)
+    idx = DeviceIndex(X, _complete(n), "l2")
+    idx.build_store("bf16")
+    Xb = torch.from_numpy(X).to(torch.bfloat16).to(torch.float64).numpy()  # RNE on the CPU
+    r = idx.search(Q, n, n, store="bf16")
+    for i in range(len(Q)):
+        exp = ((Xb[r["ids"][i]] - Q[i].astype(np.float64)) ** 2).sum(1)
+        assert np.allclose(r["dists"][i], exp, rtol=1e-5)
+
+
+def test_errors(oracle, u8_case):
+    from parlayann_amd._lib import PannError
+    X, Q, adj = u8_case
+    idx = DeviceIndex(X, adj, "l2")
+    with pytest.raises(PannError):
+        idx.search(Q[:4], 11, 10)  # k > beam (beamSearch.h:612-616)
+    with pytest.raises(PannError):
+        idx.search(Q[:4], 5, 10, starts=[])  # no start point
+    with pytest.raises(PannError):
+        idx.search(Q[:4], 5, 10, starts=[1, 1])  # duplicate starts
+    with pytest.raises(PannError):
+        idx.search(Q[:4], 5, 10, starts=[len(X)])  # out of range
+    empty = np.zeros_like(adj)
+    with pytest.raises(PannError):
+        DeviceIndex(X, empty, "l2").search(Q[:4], 5, 10)  # short beam (:557-562)
+
+
+# ------------------------------------------------------------- Vamana build
+@pytest.fixture(scope="module")
+def small_u8():
+    return datagen.workload_u8_l2(n=4000, nq=100)
+
+
+def test_build_mode_visited_and_prune(oracle, u8_case):
+    import torch
+    from parlayann_amd import _lib as L
+    X, Q, adj = u8_case
+    _say("build-mode search + prune")
+    idx = DeviceIndex(X, adj, "l2")
+    dev = torch.device("cuda")
+    ids = np.arange(0, 20000, 257, dtype=np.int32)
+    nb = len(ids)
+    cap = 512
+    tid = torch.from_numpy(ids).to(dev)
+    vid = torch.zeros((nb, cap), dtype=torch.int32, device=dev)
+    vd = torch.zeros((nb, cap), dtype=torch.float32, device=dev)
+    vc = torch.zeros(nb, dtype=torch.int32, device=dev)
+    L.check(L.lib().pann_search_visited(idx._h, tid.data_ptr(), nb, 128, 0.125, 0, cap,
+                                        vid.data_ptr(), vd.data_ptr(), vc.data_ptr(), None))
+    vid_h, vd_h, vc_h = vid.cpu().numpy().view(np.uint32), vd.cpu().numpy(), vc.cpu().numpy()
+    R = 64
+    out = torch.zeros((nb, R), dtype=torch.int32, device=dev)
+    cnt = torch.zeros(nb, dtype=torch.int32, device=dev)
+    L.check(L.lib().pann_robust_prune_batch(idx._h, tid.data_ptr(), nb, vid.data_ptr(),
+                                            vd.data_ptr(), vc.data_ptr(), cap, 1.2, R, 1,
+                                            out.data_ptr(), cnt.data_ptr(), None))
+    out_h, cnt_h = out.cpu().numpy().view(np.uint32), cnt.cpu().numpy()
+    for j, p in enumerate(ids):
+        o = oracle.beam_search(adj, X, "l2", X[p], [0], 0, 128, self_id=int(p))
+        assert vc_h[j] == o["num_visited"]
+        assert np.array_equal(vid_h[j, :vc_h[j]], o["visited_ids"])
+        assert np.array_equal(vd_h[j, :vc_h[j]], o["visited_dists"])
+        pr, _ = oracle.robust_prune(adj, X, "l2", int(p), o["visited_ids"], o["visited_dists"],
+                                    1.2, R)
+        assert np.array_equal(out_h[j, :cnt_h[j]], pr)
+
+
+@pytest.mark.parametrize("kind", ["u8", "f32"])
+def test_gpu_vamana_build_matches_oracle(oracle, kind):
+    from parlayann_amd.vamana import build_vamana
+    _say(f"gpu vamana build {kind}")
+    if kind == "u8":
+        X, _ = datagen.workload_u8_l2(n=4000, nq=10)
+        R, Lb, alpha, metric = 32, 64, 1.2, "l2"
+    else:
+        X, _ = datagen.workload_f32_l2(n=4000, nq=10)
+        R, Lb, alpha, metric = 32, 64, 1.2, "l2"
+    perm = datagen.insertion_orders(len(X), 2, 21)
+    o = oracle.build_vamana(X, metric, R, Lb, alpha, perm, num_passes=2)
+    g = build_vamana(X, metric, R, Lb, alpha, perm, num_passes=2)
+    assert np.array_equal(g, o)
