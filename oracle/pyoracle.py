@@ -237,6 +237,10 @@ def build_vamana(X, metric, R, L, alpha, perms, num_passes=1, batch_factor=0.125
                                    _ptr(perms), int(sort_neighbors), nthreads)
     if rc != 0:
         raise ValueError("oracle build failed")
+    # slots past each degree keep stale ids (update_neighbors writes only the
+    # new list, graph.h:77-88); the graph file never contains them -- zero them
+    tail = adj[:, 1:]
+    tail[np.arange(R)[None, :] >= adj[:, :1].astype(np.int64)] = 0
     return adj
 
 

@@ -285,32 +285,36 @@ __device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, cons
 // ---------------------------------------------------------------------------
 template <int MT>
 __device__ __forceinline__ float seq_f32_row(const float* QL, const float* row, int dim) {
+  // HIP compiles with fp-contract=fast: without this pragma the multiply and
+  // the add below would be fused into one FMA (a single rounding).  Plain
+  // operators are used because the pragma does not reach into helper bodies.
+#pragma clang fp contract(off)
   float acc = 0.0f;
   int j = 0;
   for (; j + 4 <= dim; j += 4) {
     float4 p = *(const float4*)(row + j);
     float4 q = *(const float4*)(QL + j);
+    float t0, t1, t2, t3;
     if (MT == M_L2) {
-      float t;
-      t = __fsub_rn(q.x, p.x); acc = __fadd_rn(acc, __fmul_rn(t, t));
-      t = __fsub_rn(q.y, p.y); acc = __fadd_rn(acc, __fmul_rn(t, t));
-      t = __fsub_rn(q.z, p.z); acc = __fadd_rn(acc, __fmul_rn(t, t));
-      t = __fsub_rn(q.w, p.w); acc = __fadd_rn(acc, __fmul_rn(t, t));
+      t0 = q.x - p.x; t1 = q.y - p.y; t2 = q.z - p.z; t3 = q.w - p.w;
+      t0 = t0 * t0; t1 = t1 * t1; t2 = t2 * t2; t3 = t3 * t3;
     } else {
-      acc = __fadd_rn(acc, __fmul_rn(q.x, p.x));
-      acc = __fadd_rn(acc, __fmul_rn(q.y, p.y));
-      acc = __fadd_rn(acc, __fmul_rn(q.z, p.z));
-      acc = __fadd_rn(acc, __fmul_rn(q.w, p.w));
+      t0 = q.x * p.x; t1 = q.y * p.y; t2 = q.z * p.z; t3 = q.w * p.w;
     }
+    acc = acc + t0;
+    acc = acc + t1;
+    acc = acc + t2;
+    acc = acc + t3;
   }
   for (; j < dim; j++) {
-    float p = row[j], q = QL[j];
+    float t;
     if (MT == M_L2) {
-      float t = __fsub_rn(q, p);
-      acc = __fadd_rn(acc, __fmul_rn(t, t));
+      t = QL[j] - row[j];
+      t = t * t;
     } else {
-      acc = __fadd_rn(acc, __fmul_rn(q, p));
+      t = QL[j] * row[j];
     }
+    acc = acc + t;
   }
   return MT == M_L2 ? acc : -acc;
 }
