@@ -99,15 +99,45 @@ __device__ __forceinline__ bool lds_contains(const uint32_t* H, uint32_t log2n, 
   }
 }
 
-__device__ __forceinline__ bool global_insert(uint32_t* G, uint32_t log2n, uint32_t key) {
+// Per-wave global table of 64-bit slots (tag << 32 | id).  A slot whose tag
+// differs from this query's tag is empty, so the table is never cleared:
+// every query of every launch carries a distinct tag.
+__device__ __forceinline__ bool global_insert(unsigned long long* G, uint32_t log2n, uint32_t tag,
+                                              uint32_t key) {
   const uint32_t mask = (1u << log2n) - 1u;
+  const unsigned long long mine = ((unsigned long long)tag << 32) | key;
   uint32_t h = hash_slot(key, log2n);
+  unsigned long long old = __hip_atomic_load(&G[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (;;) {
-    uint32_t old = atomicCAS(&G[h], PANN_SENTINEL, key);
-    if (old == PANN_SENTINEL) return true;
-    if (old == key) return false;
+    if ((uint32_t)(old >> 32) != tag) {
+      unsigned long long prev = atomicCAS(&G[h], old, mine);
+      if (prev == old) return true;
+      old = prev;  // another lane won this slot: look at it again
+      continue;
+    }
+    if ((uint32_t)old == key) return false;
     h = (h + 1u) & mask;
+    old = __hip_atomic_load(&G[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+// index of the first entry of FV[start..f) that is 0 (unvisited), or -1
+__device__ __forceinline__ int first_unvisited(const uint8_t* FV, int f, int start, int lane) {
+  for (int t = start; t < f; t += PANN_WAVE) {
+    int i = t + lane;
+    uint64_t m = ballot(i < f && FV[i] == 0);
+    if (m) return t + (int)__builtin_ctzll(m);
+  }
+  return -1;
+}
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    int w = __shfl_xor(v, o);
+    v = w < v ? w : v;
+  }
+  return v;
 }
 
 // number of entries of sorted A[0..n) that are < x
@@ -133,7 +163,8 @@ __device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const
 
 template <int ST, int MT, int NCH, bool SEQ>
 __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, int lane,
-                           uint32_t* ghash) {
+                           unsigned long long* ghash) {
+  const uint32_t gtag = a.ghash_tag_base + q + 1u;
   using QT = typename QueryElem<ST>::T;
   const int B = a.beam_pq ? a.beam_pq[q] : a.beam;
   const int k = a.k_pq ? a.k_pq[q] : a.k;
@@ -200,18 +231,15 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   int c = 0;
 
   // ---- main loop (beamSearch.h:116-223) ----
+  // u = the first unvisited frontier entry (unvisited_frontier[offset] of the
+  // reference: entries before it are all visited).  The adjacency row of the
+  // next candidate for the following step is loaded while this step gathers.
+  int u = first_unvisited(L.FV0, f, 0, lane);
+  uint32_t pref_id = PANN_SENTINEL;
+  uint32_t nb_pref = PANN_SENTINEL;
   for (;;) {
     uint64_t* F = cur_buf ? L.F1 : L.F0;
     uint8_t* FV = cur_buf ? L.FV1 : L.FV0;
-    int u = -1;
-    for (int t = 0; t < f; t += PANN_WAVE) {
-      int i = t + lane;
-      uint64_t m = ballot(i < f && FV[i] == 0);
-      if (m) {
-        u = t + (int)__builtin_ctzll(m);
-        break;
-      }
-    }
     if (u < 0 || (int64_t)nvis >= a.limit) break;
     const uint64_t cur_key = F[u];
     const uint32_t cur = key_id(cur_key);
@@ -225,25 +253,24 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     const float cutoff = full ? key_dist(F[f - 1]) : FLT_MAX;
 
     // neighbours of cur (graph rows are padded with the sentinel past deg)
-    uint32_t nb = lane < dl ? a.nbrs[(uint64_t)cur * a.nbr_stride + lane] : PANN_SENTINEL;
+    uint32_t nb = cur == pref_id ? nb_pref
+                                 : (lane < dl ? a.nbrs[(uint64_t)cur * a.nbr_stride + lane]
+                                              : PANN_SENTINEL);
+    // next unvisited entry of this (stale) frontier; prefetch its adjacency
+    const int nu = first_unvisited(FV, f, u + 1, lane);
+    pref_id = nu >= 0 ? key_id(F[nu]) : PANN_SENTINEL;
+    nb_pref = (nu >= 0 && lane < dl) ? a.nbrs[(uint64_t)pref_id * a.nbr_stride + lane]
+                                     : PANN_SENTINEL;
     const bool valid = nb != PANN_SENTINEL;
     const int nvalid = __popcll(ballot(valid));
-    if (!ovf && lds_count + (uint32_t)nvalid > (hslots >> 1)) {
-      // LDS table reached half load: from here on new ids go to this wave's
-      // global table (cleared on first use in this query).
-      ovf = true;
-      uint32_t* G = ghash;
-      for (uint32_t i = lane * 4; i < (1u << glog2); i += PANN_WAVE * 4)
-        *(uint4*)(G + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-      __builtin_amdgcn_wave_barrier();
-    }
+    // LDS table at half load: from here on new ids go to this wave's global table
+    if (!ovf && lds_count + (uint32_t)nvalid > (hslots >> 1)) ovf = true;
     bool isnew = false;
     if (valid) {
       if (!ovf) {
         isnew = lds_insert(L.H, hlog2, nb);
       } else if (!lds_contains(L.H, hlog2, nb)) {
-        if (g_count + PANN_WAVE < (1u << glog2)) isnew = global_insert(ghash, glog2, nb);
+        if (g_count + PANN_WAVE < (1u << glog2)) isnew = global_insert(ghash, glog2, gtag, nb);
         else isnew = true;  // table exhausted: flagged below
       }
     }
@@ -272,16 +299,12 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     if (keep) L.C[c + (int)mbcnt(mk)] = key;
     c += __popcll(mk);
     wave_lds_sync();
-    // is there an unvisited entry left in the (stale) frontier?
-    bool more = false;
-    for (int t = 0; t < f; t += PANN_WAVE) {
-      int i = t + lane;
-      if (ballot(i < f && FV[i] == 0)) {
-        more = true;
-        break;
-      }
+    // defer while the candidates are few and the stale frontier still has
+    // unvisited entries (:185-192)
+    if (c == 0 || (defer_ok && c < thr && nu >= 0)) {
+      u = nu;
+      continue;
     }
-    if (c == 0 || (defer_ok && c < thr && more)) continue;
 
     // ---- merge: rank-sort C, then place C and F by rank into the other buffer
     for (int j = lane; j < c; j += PANN_WAVE) {
@@ -293,12 +316,14 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     wave_lds_sync();
     uint64_t* F2 = cur_buf ? L.F0 : L.F1;
     uint8_t* FV2 = cur_buf ? L.FV0 : L.FV1;
+    int first = 0x7FFFFFFF;  // lowest new position of an unvisited entry
     for (int j = lane; j < c; j += PANN_WAVE) {
       uint64_t x = L.Cs[j];
       int pos = j + lower_bound_u64(F, f, x);
       if (pos < B) {
         F2[pos] = x;
         FV2[pos] = 0;
+        first = pos < first ? pos : first;
       }
     }
     for (int i = lane; i < f; i += PANN_WAVE) {
@@ -306,12 +331,16 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
       int pos = i + lower_bound_u64(L.Cs, c, x);
       if (pos < B) {
         F2[pos] = x;
-        FV2[pos] = FV[i];
+        uint8_t v = FV[i];
+        FV2[pos] = v;
+        if (!v) first = pos < first ? pos : first;
       }
     }
     f = f + c < B ? f + c : B;
     c = 0;
     cur_buf ^= 1;
+    u = wave_min_i32(first);
+    if (u == 0x7FFFFFFF) u = -1;
     wave_lds_sync();
   }
 
@@ -362,7 +391,7 @@ __global__ __launch_bounds__(PANN_WAVE) void k_beam_search(SearchArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int lane = threadIdx.x;
   WaveLds L = carve(smem, a);
-  uint32_t* ghash = a.ghash + ((uint64_t)blockIdx.x << a.ghash_log2);
+  unsigned long long* ghash = a.ghash + ((uint64_t)blockIdx.x << a.ghash_log2);
   for (;;) {
     uint32_t q = 0;
     if (lane == 0) q = atomicAdd(a.work_counter, 1u);

@@ -37,6 +37,8 @@ __device__ __forceinline__ float fp8e4m3_to_f32(uint32_t x) {
   return __uint_as_float(__float_as_uint(v) | sgn);
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ float bf16lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
@@ -143,9 +145,19 @@ struct GroupDist<ST_FP8, MT, NCH> {
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
       uint32_t w[4] = {raw[c].x, raw[c].y, raw[c].z, raw[c].w};
+      float pv[16];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {  // v_cvt_pk_f32_fp8: two e4m3 bytes per instruction
+        f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[e], false);
+        f32x2 hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[e], true);
+        pv[4 * e + 0] = lo[0];
+        pv[4 * e + 1] = lo[1];
+        pv[4 * e + 2] = hi[0];
+        pv[4 * e + 3] = hi[1];
+      }
 #pragma unroll
       for (int e = 0; e < 16; e++) {
-        float p = fp8e4m3_to_f32((w[e >> 2] >> (8 * (e & 3))) & 0xFFu);
+        float p = pv[e];
         if (MT == M_L2) {
           float t = q[c][e] - s[c][e] * p;
           acc = fmaf(t, t, acc);

@@ -58,8 +58,9 @@ struct SearchArgs {
   uint32_t* status;
   // scratch
   uint32_t* work_counter;
-  uint32_t* ghash;  // grid x 2^ghash_log2 slots
+  unsigned long long* ghash;  // grid x 2^ghash_log2 slots of (tag << 32 | id)
   uint32_t ghash_log2;
+  uint32_t ghash_tag_base;    // query q of this launch uses tag base + q + 1
   // LDS layout
   uint32_t fcap, ccap, hslots_log2;
 };

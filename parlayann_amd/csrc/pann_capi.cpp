@@ -67,8 +67,9 @@ struct pann_index_s {
   uint32_t* d_counter = nullptr;
   uint32_t* d_status = nullptr;
   uint32_t* d_starts = nullptr;
-  uint32_t* d_ghash = nullptr;
+  unsigned long long* d_ghash = nullptr;
   size_t ghash_words = 0;
+  uint32_t ghash_tag = 0;  // tags handed out so far (table zeroed when they wrap)
   uint64_t* d_vis = nullptr;
   size_t vis_words = 0;
   uint32_t* d_dummy = nullptr;
@@ -82,13 +83,22 @@ struct pann_index_s {
 
 namespace {
 
-pann_status ensure_ghash(pann_index_t idx, size_t words) {
-  if (idx->ghash_words >= words) return PANN_OK;
-  if (idx->d_ghash) (void)hipFree(idx->d_ghash);
-  idx->d_ghash = nullptr;
-  idx->ghash_words = 0;
-  HIP_TRY(hipMalloc(&idx->d_ghash, words * 4), "hipMalloc(visited overflow table)");
-  idx->ghash_words = words;
+// The overflow table holds (tag << 32 | id) slots; tag 0 never occurs, so a
+// zeroed table is empty for every query.
+pann_status ensure_ghash(pann_index_t idx, size_t words, uint32_t nq) {
+  if (idx->ghash_words < words) {
+    if (idx->d_ghash) (void)hipFree(idx->d_ghash);
+    idx->d_ghash = nullptr;
+    idx->ghash_words = 0;
+    HIP_TRY(hipMalloc(&idx->d_ghash, words * 8), "hipMalloc(visited overflow table)");
+    HIP_TRY(hipMemset(idx->d_ghash, 0, words * 8), "hipMemset(visited overflow table)");
+    idx->ghash_words = words;
+    idx->ghash_tag = 0;
+  }
+  if ((uint64_t)idx->ghash_tag + nq + 1 >= 0xFFFFFFFFull) {  // tags about to wrap
+    HIP_TRY(hipMemset(idx->d_ghash, 0, idx->ghash_words * 8), "hipMemset(visited overflow table)");
+    idx->ghash_tag = 0;
+  }
   return PANN_OK;
 }
 
@@ -255,9 +265,9 @@ namespace {
 
 // Fills the launch block shared by pann_search and pann_search_visited.
 pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint32_t* starts,
-                           uint32_t nstarts, const pann_search_opts* opts, pann::SearchArgs& a,
-                           int& store_kind, bool& sequential, uint32_t& grid,
-                           hipStream_t stream) {
+                           uint32_t nstarts, const pann_search_opts* opts, uint32_t nq,
+                           pann::SearchArgs& a, int& store_kind, bool& sequential,
+                           uint32_t& grid, hipStream_t stream) {
   const int64_t B = qp->beam_size;
   if (qp->k < 0 || B < 1) return fail(PANN_ERR_ARG, "beam_size must be >= 1 and k >= 0");
   if (qp->k > B)  // beamSearch.h:612-616
@@ -341,8 +351,11 @@ pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint3
   uint32_t hlog2 = (uint32_t)o.hash_slots_log2;
   if (hlog2 == 0) {
     // expected distinct ids seen ~ visits * new-per-visit; sized for <= 1/2 load
+    // A small LDS table keeps more waves per CU; ids past half its load go to
+    // the per-wave global table.  Occupancy wins over LDS hits (measured at
+    // beam 200, degree 64: 2^11 slots 1.5x faster than 2^13).
     double est = (double)B * 1.3 * (double)a.degree_limit * 0.45 * 2.0;
-    hlog2 = std::min<uint32_t>(std::max<uint32_t>(ceil_log2((uint64_t)est), 10), 13);
+    hlog2 = std::min<uint32_t>(std::max<uint32_t>(ceil_log2((uint64_t)est), 10), 11);
   }
   if (hlog2 < 8 || hlog2 > 15) return fail(PANN_ERR_ARG, "hash_slots_log2 must be in [8, 15]");
   a.hslots_log2 = hlog2;
@@ -358,9 +371,11 @@ pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint3
   uint32_t glog2 = std::max<uint32_t>(ceil_log2((uint64_t)2 * (10 + B) * a.degree_limit) + 1,
                                       hlog2 + 2);
   a.ghash_log2 = glog2;
-  pann_status st = ensure_ghash(idx, (size_t)grid << glog2);
+  pann_status st = ensure_ghash(idx, (size_t)grid << glog2, nq);
   if (st != PANN_OK) return st;
   a.ghash = idx->d_ghash;
+  a.ghash_tag_base = idx->ghash_tag;
+  idx->ghash_tag += nq;
   HIP_TRY(hipMemcpyAsync(idx->d_starts, starts, nstarts * 4, hipMemcpyHostToDevice, stream),
           "hipMemcpyAsync(starts)");
   hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(1), 0, stream, idx->d_status);
@@ -403,8 +418,8 @@ pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
   int store_kind;
   bool sequential;
   uint32_t grid;
-  pann_status st = prepare_search(idx, qp, starts, nstarts, opts, a, store_kind, sequential, grid,
-                                  stream);
+  pann_status st = prepare_search(idx, qp, starts, nstarts, opts, nq, a, store_kind, sequential,
+                                  grid, stream);
   if (st != PANN_OK) return st;
   const bool f32_queries = idx->dtype == PANN_DT_F32;
   a.queries = (const char*)queries;
@@ -510,7 +525,8 @@ pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint
   int store_kind;
   bool sequential;
   uint32_t grid;
-  pann_status st = prepare_search(idx, &qp, &start, 1, &o, a, store_kind, sequential, grid, stream);
+  pann_status st = prepare_search(idx, &qp, &start, 1, &o, nq, a, store_kind, sequential, grid,
+                                  stream);
   if (st != PANN_OK) return st;
   st = ensure_vis(idx, (size_t)nq * vis_cap);
   if (st != PANN_OK) return st;

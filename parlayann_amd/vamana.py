@@ -62,7 +62,7 @@ def build_vamana(points: np.ndarray, metric: str, R: int, L_build: int, alpha: f
     h = idx._h
     lib = L.lib()
     dev = torch.device("cuda", device)
-    vis_cap = vis_cap or min(4096, max(4 * L_build, 256))
+    vis_cap = vis_cap or min(4096, max(8 * L_build, 512))
     i32 = torch.int32
     for p in range(num_passes):
         a = alpha if p == num_passes - 1 else 1.0  # build_index :173-178
@@ -70,11 +70,19 @@ def build_vamana(points: np.ndarray, metric: str, R: int, L_build: int, alpha: f
         for bi, (lo, hi) in enumerate(batch_schedule(n, n)):
             nb = hi - lo
             ids = order[lo:hi].contiguous()
-            vid = torch.empty((nb, vis_cap), dtype=i32, device=dev)
-            vd = torch.empty((nb, vis_cap), dtype=torch.float32, device=dev)
-            vc = torch.empty(nb, dtype=i32, device=dev)
-            L.check(lib.pann_search_visited(h, _p(ids), nb, L_build, batch_factor, 0, vis_cap,
-                                            _p(vid), _p(vd), _p(vc), None))
+            while True:
+                vid = torch.empty((nb, vis_cap), dtype=i32, device=dev)
+                vd = torch.empty((nb, vis_cap), dtype=torch.float32, device=dev)
+                vc = torch.empty(nb, dtype=i32, device=dev)
+                st = lib.pann_search_visited(h, _p(ids), nb, L_build, batch_factor, 0, vis_cap,
+                                             _p(vid), _p(vd), _p(vc), None)
+                if st == L.PANN_ERR_CAPACITY and vis_cap < 4096:
+                    # a visited list outgrew the buffer: the batch only read the
+                    # graph, so it is simply searched again with room to spare
+                    vis_cap = min(4096, vis_cap * 2)
+                    continue
+                L.check(st)
+                break
             out = torch.empty((nb, R), dtype=i32, device=dev)
             cnt = torch.empty(nb, dtype=i32, device=dev)
             L.check(lib.pann_robust_prune_batch(h, _p(ids), nb, _p(vid), _p(vd), _p(vc), vis_cap,
