@@ -47,7 +47,7 @@ WORKLOADS = {
     "aniso-ip-10M": dict(kind="aniso", metric="ip", n=10_000_000, nq=100_000, d=96, R=64,
                          L=128, alpha=1.0, k=10, legs=["fp32-tree", "fp32-inorder",
                                                        "bf16-rerank", "fp8-rerank"],
-                         headline="fp8-rerank"),
+                         headline="fp8-rerank", uneven=True),
 }
 
 
@@ -88,14 +88,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="f32-l2-100k", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="aniso-ip-10M", choices=sorted(WORKLOADS))
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--legs", default="")
     ap.add_argument("--n", type=int, default=0, help="override n (smaller runs; not the metric)")
     ap.add_argument("--nq", type=int, default=0)
-    ap.add_argument("--rerank-factor", type=float, default=10.0,
-                    help="rerank min(k * factor, beam) entries (the reference CLI's default 10)")
+    ap.add_argument("--rerank-factor", type=float, default=100.0,
+                    help="rerank min(k * factor, beam) entries (QueryParams default 100, "
+                         "types.h:237: the whole beam for k=10, beam <= 1000)")
+    ap.add_argument("--beam", type=int, default=0,
+                    help="fixed operating beam (skips the recall sweep; used for profiling)")
+    ap.add_argument("--cpu-seconds", type=float, default=6.0,
+                    help="target wall time of one CPU-baseline repetition")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
@@ -114,14 +119,22 @@ def main():
     if args.nq:
         w["nq"] = args.nq
     legs = args.legs.split(",") if args.legs else w["legs"]
-    beams = w.get("beams", BEAMS)
+    beams = [args.beam] if args.beam else w.get("beams", BEAMS)
     dev = torch.device("cuda", 0)
     t0 = time.time()
     X, Q = make_data(w)
     log(f"[bench] data {X.shape} {X.dtype} + {Q.shape} in {time.time() - t0:.1f}s")
     t0 = time.time()
     perm = datagen.insertion_orders(len(X), 1, 4242)
-    adj = build_vamana(X, w["metric"], w["R"], w["L"], w["alpha"], perm)
+    last = [time.time()]
+
+    def progress(p, bi, lo, hi):  # one line per ~10 s of build work
+        if time.time() - last[0] > 10:
+            last[0] = time.time()
+            log(f"[bench] build pass {p}: {hi:,}/{len(X):,} points inserted "
+                f"({time.time() - t0:.0f}s)")
+
+    adj = build_vamana(X, w["metric"], w["R"], w["L"], w["alpha"], perm, progress=progress)
     t_build = time.time() - t0
     log(f"[bench] GPU Vamana build R={w['R']} L={w['L']} a={w['alpha']}: {t_build:.1f}s, "
         f"avg degree {adj[:, 0].mean():.2f}")
@@ -200,6 +213,43 @@ def main():
         log(f"[bench] {leg}: beam {B} recall {rec:.4f} QPS {qps:,.0f} kernel {kms_avg:.3f} ms "
             f"{gbs:,.0f} GB/s visited {vis.mean():.1f} cmps {cmps.mean():.1f}")
 
+    # batched uneven search: per-query k in {1, 10, 100}, beam ~ U[max(16, k), 256],
+    # the whole query set in one persistent launch
+    if w.get("uneven"):
+        rng = np.random.default_rng(77)
+        ks = rng.choice([1, 10, 100], nq).astype(np.int32)
+        bs = np.array([rng.integers(max(16, kk), 257) for kk in ks], np.int32)
+        tk, tb = torch.from_numpy(ks).to(dev), torch.from_numpy(bs).to(dev)
+        ui = torch.zeros((nq, 100), dtype=torch.int32, device=dev)
+        ud = torch.zeros((nq, 100), dtype=torch.float32, device=dev)
+        for leg in [l for l in ("fp32-tree", "fp8-rerank") if l in legs]:
+            opts = LEG_OPTS[leg]
+            run = lambda a=False: idx.search_device(tq, 100, 256, ui, ud, out_visited=ov,
+                                                    out_cmps=oc, k_per_query=tk,
+                                                    beam_per_query=tb, async_=a,
+                                                    rerank_factor=args.rerank_factor, **opts)
+            run()
+            torch.cuda.synchronize()
+            ids = ui.cpu().numpy().view(np.uint32)
+            sel = ks >= 10
+            rec = recall_plain(gt[sel], ids[sel][:, :10], 10)
+            for _ in range(args.warmup):
+                run(True)
+            torch.cuda.synchronize()
+            kms = []
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                run(True)
+                kms.append(idx.last_kernel_ms())
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            idx.search_status()
+            results["uneven-" + leg] = dict(
+                qps=nq * args.steps / dt, ms_per_step=dt * 1e3 / args.steps,
+                kernel_ms=float(np.mean(kms)), recall_at_10_k_ge_10=round(rec, 5),
+                k_choices=[1, 10, 100], beam_range=[16, 256])
+            log(f"[bench] uneven {leg}: QPS {nq * args.steps / dt:,.0f} recall@10(k>=10) {rec:.4f}")
+
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle
@@ -207,16 +257,23 @@ def main():
         B = results[base_leg]["beam"]
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         threads = min(threads, 64)
+        # bounded sample: size one repetition to about --cpu-seconds
+        m = min(nq, 1000)
+        t0 = time.perf_counter()
+        pyoracle.search_batch(adj, X, w["metric"], Q[:m], k, B, nthreads=threads)
+        rate = m / (time.perf_counter() - t0)
+        m = int(min(nq, max(1000, rate * args.cpu_seconds)))
         times = []
         for rep in range(5):
             _ = np.random.default_rng(rep).permutation(5_000_000)  # cache flush
             t0 = time.perf_counter()
-            pyoracle.search_batch(adj, X, w["metric"], Q, k, B, nthreads=threads)
+            pyoracle.search_batch(adj, X, w["metric"], Q[:m], k, B, nthreads=threads)
             times.append(time.perf_counter() - t0)
         med = float(np.median(times))
-        cpu = {"value": nq / med, "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"all {nq} queries at beam {B} (the in-order FP32 operating point), "
-                         f"5 repetitions, median; oracle restatement of beamSearch.h"}
+        cpu = {"value": m / med, "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": f"first {m} of the {nq} queries at beam {B} (the in-order FP32 "
+                         f"operating point), 5 repetitions, median; oracle restatement of "
+                         f"beamSearch.h"}
         log(f"[bench] host CPU ({threads} threads): {cpu['value']:,.0f} QPS")
 
     head = w["headline"] if w["headline"] in results else next(iter(results))

@@ -158,6 +158,16 @@ pann_status pann_robust_prune_batch(pann_index_t idx, const uint32_t *pts, uint3
                                     double alpha, uint32_t R, int32_t add, uint32_t *out_ids,
                                     uint32_t *out_count, void *stream);
 
+/* The same robustPrune for groups in CSR form: point pts[i] takes the
+ * id-only candidates cand_ids[offsets[i] .. offsets[i+1]) (distances are
+ * computed).  This is the shape of batch_insert's reverse-edge step, whose
+ * groups come from group_by_key (vamana/index.h:281-303); max_count bounds
+ * every group's size.  All pointers device.                               */
+pann_status pann_robust_prune_groups(pann_index_t idx, const uint32_t *pts, uint32_t np,
+                                     const uint32_t *offsets, const uint32_t *cand_ids,
+                                     uint32_t max_count, double alpha, uint32_t R, int32_t add,
+                                     uint32_t *out_ids, uint32_t *out_count, void *stream);
+
 /* Replaces the out-neighbour lists of points pts[0..np) on the device
  * (edgeRange::update_neighbors, graph.h:77-88).  lists: device np x R,
  * counts: device uint32[np]. */

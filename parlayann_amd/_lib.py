@@ -18,7 +18,8 @@ SUM_INORDER, SUM_TREE = 0, 1
 # every symbol include/pann.h declares
 EXPORTS = ["pann_open", "pann_build_store", "pann_get_fp8_scales", "pann_search",
            "pann_search_host", "pann_search_status", "pann_last_kernel_ms",
-           "pann_search_visited", "pann_robust_prune_batch", "pann_update_neighbors",
+           "pann_search_visited", "pann_robust_prune_batch", "pann_robust_prune_groups",
+           "pann_update_neighbors",
            "pann_download_graph", "pann_close", "pann_last_error", "pann_device_info"]
 
 
@@ -72,6 +73,8 @@ def lib():
                                 C.c_uint32, vp, vp, vp, vp],
         "pann_robust_prune_batch": [vp, vp, C.c_uint32, vp, vp, vp, C.c_uint32, C.c_double,
                                     C.c_uint32, C.c_int32, vp, vp, vp],
+        "pann_robust_prune_groups": [vp, vp, C.c_uint32, vp, vp, C.c_uint32, C.c_double,
+                                     C.c_uint32, C.c_int32, vp, vp, vp],
         "pann_update_neighbors": [vp, vp, C.c_uint32, vp, vp, C.c_uint32, vp],
         "pann_download_graph": [vp, vp],
         "pann_close": [vp],

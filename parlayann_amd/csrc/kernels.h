@@ -94,6 +94,9 @@ struct PruneArgs {
   const float* cand_dists;  // nullptr: compute d(c, p)
   const uint32_t* cand_count;
   uint32_t cand_stride;
+  const uint32_t* cand_offsets;  // CSR groups (np + 1 offsets) instead of count/stride
+  uint64_t* gkeys;               // per-block global sort scratch (grid x sort_cap)
+  uint8_t* gflags;
   double alpha;
   uint32_t R;
   int32_t add;
@@ -103,7 +106,8 @@ struct PruneArgs {
 };
 
 uint32_t prune_lds_bytes(uint32_t sort_cap, uint32_t dim, int dtype);
-hipError_t launch_robust_prune(const PruneArgs& a, int dtype, int metric, hipStream_t s);
+hipError_t launch_robust_prune(const PruneArgs& a, int dtype, int metric, bool keys_global,
+                               uint32_t grid, hipStream_t s);
 hipError_t launch_sort_visited(const uint64_t* vis, uint32_t nq, uint32_t cap,
                                const uint32_t* counts, uint32_t* ids, float* dists,
                                hipStream_t s);

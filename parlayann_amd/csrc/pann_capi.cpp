@@ -73,6 +73,9 @@ struct pann_index_s {
   uint64_t* d_vis = nullptr;
   size_t vis_words = 0;
   uint32_t* d_dummy = nullptr;
+  uint64_t* d_pkeys = nullptr;   // prune sort scratch for groups too large for LDS
+  uint8_t* d_pflags = nullptr;
+  size_t pscratch_entries = 0;
   int num_cus = 0;
   int64_t lds_per_cu = 0;
   int64_t max_lds_block = 0;
@@ -551,17 +554,14 @@ pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint
   return check_status(idx, stream);
 }
 
-pann_status pann_robust_prune_batch(pann_index_t idx, const uint32_t* pts, uint32_t np,
-                                    const uint32_t* cand_ids, const float* cand_dists,
-                                    const uint32_t* cand_count, uint32_t cand_stride,
-                                    double alpha, uint32_t R, int32_t add, uint32_t* out_ids,
-                                    uint32_t* out_count, void* stream) {
-  if (!idx || !pts || !cand_ids || !cand_count || !out_ids || !out_count)
-    return fail(PANN_ERR_ARG, "null argument");
-  if (np == 0) return PANN_OK;
+namespace {
+
+pann_status run_prune(pann_index_t idx, const uint32_t* pts, uint32_t np, const uint32_t* cand_ids,
+                      const float* cand_dists, const uint32_t* cand_count, uint32_t cand_stride,
+                      const uint32_t* offsets, uint32_t max_count, double alpha, uint32_t R,
+                      int32_t add, uint32_t* out_ids, uint32_t* out_count, hipStream_t stream) {
   if (R == 0 || R > idx->max_degree) return fail(PANN_ERR_ARG, "R must be in [1, max_degree]");
-  std::lock_guard<std::mutex> lk(idx->mu);
-  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  if (add < 0 || add > 3) return fail(PANN_ERR_ARG, "add must be 0, 1, 2 or 3");
   pann::PruneArgs a;
   std::memset(&a, 0, sizeof(a));
   a.pts = idx->d_pts;
@@ -576,22 +576,70 @@ pann_status pann_robust_prune_batch(pann_index_t idx, const uint32_t* pts, uint3
   a.cand_dists = cand_dists;
   a.cand_count = cand_count;
   a.cand_stride = cand_stride;
+  a.cand_offsets = offsets;
   a.alpha = alpha;
   a.R = R;
   a.add = add;
   a.out_ids = out_ids;
   a.out_count = out_count;
-  uint32_t cap = 2;
-  while (cap < cand_stride + idx->max_degree) cap <<= 1;
-  a.sort_cap = cap;
-  uint32_t lds = pann::prune_lds_bytes(cap, idx->dim, idx->dtype == PANN_DT_F32 ? pann::ST_F32
-                                                      : idx->dtype == PANN_DT_U8 ? pann::ST_U8
-                                                                                 : pann::ST_I8);
-  if ((int64_t)lds > idx->max_lds_block) return fail(PANN_ERR_UNSUPPORTED, "candidate set too large for LDS");
-  int dk = idx->dtype == PANN_DT_F32 ? pann::ST_F32 : idx->dtype == PANN_DT_U8 ? pann::ST_U8 : pann::ST_I8;
-  HIP_TRY(pann::launch_robust_prune(a, dk, idx->metric, (hipStream_t)stream), "robust prune launch");
-  HIP_TRY(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  uint64_t cap = 2;
+  while (cap < (uint64_t)max_count + idx->max_degree) cap <<= 1;
+  if (cap > (1u << 24)) return fail(PANN_ERR_UNSUPPORTED, "candidate set too large");
+  a.sort_cap = (uint32_t)cap;
+  const int dk = idx->dtype == PANN_DT_F32 ? pann::ST_F32
+                 : idx->dtype == PANN_DT_U8 ? pann::ST_U8 : pann::ST_I8;
+  const uint32_t lds = pann::prune_lds_bytes(a.sort_cap, idx->dim, dk);
+  const bool keys_global = (int64_t)lds > std::min<int64_t>(idx->max_lds_block, 64 * 1024);
+  uint32_t grid = std::min<uint32_t>(np, 1u << 20);
+  if (keys_global) {
+    grid = std::min<uint32_t>(np, (uint32_t)idx->num_cus * 4);
+    size_t need = (size_t)grid * cap;
+    if (idx->pscratch_entries < need) {
+      if (idx->d_pkeys) (void)hipFree(idx->d_pkeys);
+      if (idx->d_pflags) (void)hipFree(idx->d_pflags);
+      idx->d_pkeys = nullptr;
+      idx->d_pflags = nullptr;
+      idx->pscratch_entries = 0;
+      HIP_TRY(hipMalloc(&idx->d_pkeys, need * 8), "hipMalloc(prune scratch)");
+      HIP_TRY(hipMalloc(&idx->d_pflags, need), "hipMalloc(prune scratch)");
+      idx->pscratch_entries = need;
+    }
+    a.gkeys = idx->d_pkeys;
+    a.gflags = idx->d_pflags;
+  }
+  HIP_TRY(pann::launch_robust_prune(a, dk, idx->metric, keys_global, grid, stream),
+          "robust prune launch");
+  HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
   return PANN_OK;
+}
+
+}  // namespace
+
+pann_status pann_robust_prune_batch(pann_index_t idx, const uint32_t* pts, uint32_t np,
+                                    const uint32_t* cand_ids, const float* cand_dists,
+                                    const uint32_t* cand_count, uint32_t cand_stride,
+                                    double alpha, uint32_t R, int32_t add, uint32_t* out_ids,
+                                    uint32_t* out_count, void* stream) {
+  if (!idx || !pts || !cand_ids || !cand_count || !out_ids || !out_count)
+    return fail(PANN_ERR_ARG, "null argument");
+  if (np == 0) return PANN_OK;
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  return run_prune(idx, pts, np, cand_ids, cand_dists, cand_count, cand_stride, nullptr,
+                   cand_stride, alpha, R, add, out_ids, out_count, (hipStream_t)stream);
+}
+
+pann_status pann_robust_prune_groups(pann_index_t idx, const uint32_t* pts, uint32_t np,
+                                     const uint32_t* offsets, const uint32_t* cand_ids,
+                                     uint32_t max_count, double alpha, uint32_t R, int32_t add,
+                                     uint32_t* out_ids, uint32_t* out_count, void* stream) {
+  if (!idx || !pts || !offsets || !cand_ids || !out_ids || !out_count)
+    return fail(PANN_ERR_ARG, "null argument");
+  if (np == 0) return PANN_OK;
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  return run_prune(idx, pts, np, cand_ids, nullptr, nullptr, 0, offsets, max_count, alpha, R,
+                   add, out_ids, out_count, (hipStream_t)stream);
 }
 
 pann_status pann_update_neighbors(pann_index_t idx, const uint32_t* pts, uint32_t np,
@@ -641,7 +689,7 @@ pann_status pann_close(pann_index_t idx) {
   (void)hipDeviceSynchronize();
   void* ptrs[] = {idx->d_pts, idx->d_bf16, idx->d_fp8, idx->d_scales, idx->d_absmax, idx->d_nbrs,
                   idx->d_counter, idx->d_status, idx->d_starts, idx->d_ghash, idx->d_vis,
-                  idx->d_dummy};
+                  idx->d_dummy, idx->d_pkeys, idx->d_pflags};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (idx->ev0) (void)hipEventDestroy(idx->ev0);

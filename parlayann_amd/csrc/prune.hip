@@ -38,7 +38,22 @@ __device__ __forceinline__ float row_dist(const char* qrow_lds, const char* row,
   }
 }
 
+// Orders this wave's accesses to its sort buffer (LDS, or global scratch
+// that only this wave touches: workgroup scope suffices, the wave's CU L1
+// serves its own stores).
+template <bool KG>
+__device__ __forceinline__ void wave_sync_k() {
+  if constexpr (KG) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    wave_lds_sync();
+  }
+}
+
 // ascending bitonic sort of K[0..np) (np a power of two) by one wave
+template <bool KG>
 __device__ void wave_bitonic_sort(uint64_t* K, int np, int lane) {
   for (int size = 2; size <= np; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -52,31 +67,34 @@ __device__ void wave_bitonic_sort(uint64_t* K, int np, int lane) {
           K[hi] = a;
         }
       }
-      wave_lds_sync();
+      wave_sync_k<KG>();
     }
   }
 }
 
 
 
-template <int DT, int MT>
-__global__ __launch_bounds__(PANN_WAVE) void k_robust_prune(PruneArgs a) {
-  extern __shared__ __attribute__((aligned(16))) char smem[];
-  const int lane = threadIdx.x;
-  const uint32_t i = blockIdx.x;
-  if (i >= a.np) return;
+template <int DT, int MT, bool KG>
+__device__ void prune_one(const PruneArgs& a, uint32_t i, int lane, char* QP, char* QS,
+                          uint64_t* K, uint8_t* PR) {
   const int esz = store_elem_bytes(DT);
-  uint64_t* K = (uint64_t*)smem;                               // sort_cap keys
-  char* QP = smem + a.sort_cap * 8;                            // row of p
-  char* QS = QP + ((a.dim * esz + 15) & ~15u);                 // row of p*
-  uint8_t* PR = (uint8_t*)(QS + ((a.dim * esz + 15) & ~15u));  // pruned flags
   const uint32_t p = a.points[i];
+
   const char* prow = a.pts + (uint64_t)p * a.stride;
   for (uint32_t j = lane; j < a.dim * esz; j += PANN_WAVE) QP[j] = prow[j];
-  wave_lds_sync();
+  wave_sync_k<KG>();
 
-  uint32_t nc = a.cand_count[i];
-  const uint32_t* cid = a.cand_ids + (uint64_t)i * a.cand_stride;
+  uint32_t nc;
+  const uint32_t* cid;
+  const float* cdist = nullptr;
+  if (a.cand_offsets) {  // CSR groups
+    nc = a.cand_offsets[i + 1] - a.cand_offsets[i];
+    cid = a.cand_ids + a.cand_offsets[i];
+  } else {
+    nc = a.cand_count[i];
+    cid = a.cand_ids + (uint64_t)i * a.cand_stride;
+    if (a.cand_dists) cdist = a.cand_dists + (uint64_t)i * a.cand_stride;
+  }
   if (a.add == 2) {
     // reverse-edge rule of batch_insert (vamana/index.h:292-303): when the
     // candidates fit next to the current list, append without repeats
@@ -110,13 +128,13 @@ __global__ __launch_bounds__(PANN_WAVE) void k_robust_prune(PruneArgs a) {
         w += (uint32_t)__popcll(m);
       }
       if (lane == 0) a.out_count[i] = w;
+      wave_sync_k<KG>();
       return;
     }
   }
   for (uint32_t t = lane; t < nc; t += PANN_WAVE) {
     uint32_t c = cid[t];
-    float d = a.cand_dists ? a.cand_dists[(uint64_t)i * a.cand_stride + t]
-                           : row_dist<DT, MT>(QP, a.pts + (uint64_t)c * a.stride, (int)a.dim);
+    float d = cdist ? cdist[t] : row_dist<DT, MT>(QP, a.pts + (uint64_t)c * a.stride, (int)a.dim);
     K[t] = make_key(d, c);
   }
   uint32_t total = nc;
@@ -138,12 +156,12 @@ __global__ __launch_bounds__(PANN_WAVE) void k_robust_prune(PruneArgs a) {
   while ((uint32_t)np2 < total) np2 <<= 1;
   if (np2 < 2) np2 = 2;
   for (uint32_t t = total + lane; t < (uint32_t)np2; t += PANN_WAVE) K[t] = ~0ull;
-  wave_lds_sync();
-  wave_bitonic_sort(K, np2, lane);  // :83
+  wave_sync_k<KG>();
+  wave_bitonic_sort<KG>(K, np2, lane);  // :83
   // unique by id (:86-88): equal ids sit next to each other (same distance)
   for (uint32_t t = lane; t < total; t += PANN_WAVE)
     PR[t] = (t > 0 && key_id(K[t]) == key_id(K[t - 1])) ? 1 : 0;
-  wave_lds_sync();
+  wave_sync_k<KG>();
 
   uint32_t nout = 0;
   uint32_t* out = a.out_ids + (uint64_t)i * a.R;
@@ -155,7 +173,7 @@ __global__ __launch_bounds__(PANN_WAVE) void k_robust_prune(PruneArgs a) {
     nout++;
     const char* srow = a.pts + (uint64_t)ps * a.stride;
     for (uint32_t j = lane; j < a.dim * esz; j += PANN_WAVE) QS[j] = srow[j];
-    wave_lds_sync();
+    wave_sync_k<KG>();
     for (uint32_t t = idx + 1 + lane; t < total; t += PANN_WAVE) {
       if (!PR[t]) {
         uint64_t kk = K[t];
@@ -163,11 +181,29 @@ __global__ __launch_bounds__(PANN_WAVE) void k_robust_prune(PruneArgs a) {
         if (a.add != 3 && a.alpha * (double)d_sp <= (double)key_dist(kk)) PR[t] = 1;
       }
     }
-    wave_lds_sync();
+    wave_sync_k<KG>();
   }
   if (lane == 0) a.out_count[i] = nout;
 }
 
+
+// KG: sort keys and pruned flags live in a per-block global scratch (groups too
+// large for LDS); otherwise in LDS.  Blocks loop over points.
+template <int DT, int MT, bool KG>
+__global__ __launch_bounds__(PANN_WAVE) void k_robust_prune(PruneArgs a) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int lane = threadIdx.x;
+  const int esz = store_elem_bytes(DT);
+  char* QP = smem;                                             // row of p
+  char* QS = QP + ((a.dim * esz + 15) & ~15u);                 // row of p*
+  char* tail = QS + ((a.dim * esz + 15) & ~15u);
+  uint64_t* K = KG ? a.gkeys + (uint64_t)blockIdx.x * a.sort_cap : (uint64_t*)tail;
+  uint8_t* PR = KG ? a.gflags + (uint64_t)blockIdx.x * a.sort_cap
+                   : (uint8_t*)(tail + a.sort_cap * 8);
+  for (uint32_t i = blockIdx.x; i < a.np; i += gridDim.x) {
+    prune_one<DT, MT, KG>(a, i, lane, QP, QS, K, PR);
+  }
+}
 // Sorts each query's visited keys and splits them into ids / dists.
 __global__ __launch_bounds__(PANN_WAVE) void k_sort_visited(const uint64_t* vis, uint32_t nq,
                                                            uint32_t cap, const uint32_t* counts,
@@ -183,7 +219,7 @@ __global__ __launch_bounds__(PANN_WAVE) void k_sort_visited(const uint64_t* vis,
   for (uint32_t t = lane; t < (uint32_t)np2; t += PANN_WAVE)
     K[t] = t < n ? vis[(uint64_t)q * cap + t] : ~0ull;
   wave_lds_sync();
-  wave_bitonic_sort(K, np2, lane);
+  wave_bitonic_sort<false>(K, np2, lane);
   for (uint32_t t = lane; t < n; t += PANN_WAVE) {
     ids[(uint64_t)q * cap + t] = key_id(K[t]);
     dists[(uint64_t)q * cap + t] = key_dist(K[t]);
@@ -204,20 +240,21 @@ __global__ void k_update_neighbors(uint32_t* nbrs, uint32_t nbr_stride, uint32_t
 
 uint32_t prune_lds_bytes(uint32_t sort_cap, uint32_t dim, int dtype) {
   uint32_t row = (dim * (uint32_t)store_elem_bytes(dtype) + 15u) & ~15u;
-  return sort_cap * 8u + 2u * row + sort_cap;
+  return 2u * row + (sort_cap ? sort_cap * 9u : 0u);
 }
 
-hipError_t launch_robust_prune(const PruneArgs& a, int dtype, int metric, hipStream_t s) {
-  uint32_t lds = prune_lds_bytes(a.sort_cap, a.dim, dtype);
+hipError_t launch_robust_prune(const PruneArgs& a, int dtype, int metric, bool keys_global,
+                               uint32_t grid, hipStream_t s) {
+  uint32_t lds = prune_lds_bytes(keys_global ? 0 : a.sort_cap, a.dim, dtype);
   if (a.np == 0) return hipSuccess;
-#define PANN_PRUNE(DT, MT)                                                                  \
-  {                                                                                         \
-    auto kern = k_robust_prune<DT, MT>;                                                     \
-    hipError_t e = hipFuncSetAttribute((const void*)kern,                                   \
+#define PANN_PRUNE(DT, MT)                                                                    \
+  {                                                                                           \
+    auto kern = keys_global ? k_robust_prune<DT, MT, true> : k_robust_prune<DT, MT, false>;   \
+    hipError_t e = hipFuncSetAttribute((const void*)kern,                                     \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
-    if (e != hipSuccess) return e;                                                          \
-    hipLaunchKernelGGL(kern, dim3(a.np), dim3(PANN_WAVE), lds, s, a);                       \
-    return hipGetLastError();                                                               \
+    if (e != hipSuccess) return e;                                                            \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(PANN_WAVE), lds, s, a);                         \
+    return hipGetLastError();                                                                 \
   }
   if (dtype == ST_F32) {
     if (metric == M_L2) PANN_PRUNE(ST_F32, M_L2) else PANN_PRUNE(ST_F32, M_IP)

@@ -22,9 +22,11 @@ def _exact_f64(points, ids, q, metric):
 
 
 def groundtruth(points: np.ndarray, queries: np.ndarray, k: int, metric: str = "l2",
-                margin: int = 32, block: int = 256, use_gpu: bool | None = None):
+                margin: int = 32, block: int = 0, use_gpu: bool | None = None):
     """Returns (ids uint32[nq,k], dists float64[nq,k])."""
     n = points.shape[0]
+    if block <= 0:  # keep one block's distance matrix near 2 GB
+        block = int(max(1, min(256, (2 << 30) // (8 * max(n, 1)))))
     kk = min(n, k + margin)
     nq = queries.shape[0]
     cand = np.empty((nq, kk), dtype=np.int64)
@@ -56,14 +58,18 @@ def groundtruth(points: np.ndarray, queries: np.ndarray, k: int, metric: str = "
             D = xx[None, :] - 2.0 * (Qb @ X.T) if metric == "l2" else -(Qb @ X.T)
             part = np.argpartition(D, kk - 1, axis=1)[:, :kk]
             cand[s:s + block] = part
+    # exact float64 re-score of every candidate, then (dist, id) order
     ids = np.empty((nq, k), np.uint32)
     dists = np.empty((nq, k), np.float64)
-    for i in range(nq):
-        c = cand[i]
-        d = _exact_f64(points, c, queries[i], metric)
-        order = np.lexsort((c, d))[:k]
-        ids[i] = c[order]
-        dists[i] = d[order]
+    for s in range(0, nq, 2048):
+        e = min(nq, s + 2048)
+        c = cand[s:e]
+        P = points[c].astype(np.float64)                      # b x kk x d
+        qd = np.asarray(queries[s:e], dtype=np.float64)[:, None, :]
+        d = ((P - qd) ** 2).sum(-1) if metric == "l2" else -(P * qd).sum(-1)
+        o = np.array([np.lexsort((c[i], d[i]))[:k] for i in range(e - s)])
+        ids[s:e] = np.take_along_axis(c, o, 1)
+        dists[s:e] = np.take_along_axis(d, o, 1)
     return ids, dists
 
 

@@ -100,19 +100,25 @@ def build_vamana(points: np.ndarray, metric: str, R: int, L_build: int, alpha: f
             o = np.argsort(keys, kind="stable")
             keys, vals = keys[o], vals[o]
             ukeys, starts, counts = np.unique(keys, return_index=True, return_counts=True)
-            ng = ukeys.size
-            maxc = int(counts.max())
-            cand = np.zeros((ng, maxc), np.int32)
-            col = np.arange(keys.size) - np.repeat(starts, counts)
-            cand[np.repeat(np.arange(ng), counts), col] = vals
-            gk = torch.from_numpy(ukeys.astype(np.int32)).to(dev)
-            gc = torch.from_numpy(cand).to(dev)
-            gn = torch.from_numpy(counts.astype(np.int32)).to(dev)
-            out2 = torch.empty((ng, R), dtype=i32, device=dev)
-            cnt2 = torch.empty(ng, dtype=i32, device=dev)
-            L.check(lib.pann_robust_prune_batch(h, _p(gk), ng, _p(gc), None, _p(gn), maxc, a, R, 2,
-                                                _p(out2), _p(cnt2), None))
-            L.check(lib.pann_update_neighbors(h, _p(gk), ng, _p(out2), _p(cnt2), R, None))
+            # CSR groups, values in insertion order; small and large groups in
+            # separate launches (large ones sort in global scratch)
+            for sel in (counts <= 1024, counts > 1024):
+                if not sel.any():
+                    continue
+                gk_h = ukeys[sel].astype(np.int32)
+                gcnt = counts[sel]
+                offs = np.concatenate([[0], np.cumsum(gcnt)]).astype(np.int32)
+                vals_g = vals[np.repeat(sel, counts)]  # groups are contiguous runs
+                ng = gk_h.size
+                gk = torch.from_numpy(gk_h).to(dev)
+                go = torch.from_numpy(offs).to(dev)
+                gv = torch.from_numpy(vals_g.astype(np.int32)).to(dev)
+                out2 = torch.empty((ng, R), dtype=i32, device=dev)
+                cnt2 = torch.empty(ng, dtype=i32, device=dev)
+                L.check(lib.pann_robust_prune_groups(h, _p(gk), ng, _p(go), _p(gv),
+                                                     int(gcnt.max()), a, R, 2, _p(out2),
+                                                     _p(cnt2), None))
+                L.check(lib.pann_update_neighbors(h, _p(gk), ng, _p(out2), _p(cnt2), R, None))
             if progress:
                 progress(p, bi, lo, hi)
     if sort_neighbors:  # :180-185, ties by id
