@@ -99,6 +99,10 @@ def main():
                          "types.h:237: the whole beam for k=10, beam <= 1000)")
     ap.add_argument("--beam", type=int, default=0,
                     help="fixed operating beam (skips the recall sweep; used for profiling)")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (calibrates rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE against a copy of known size)")
     ap.add_argument("--cpu-seconds", type=float, default=6.0,
                     help="target wall time of one CPU-baseline repetition")
     args = ap.parse_args()
@@ -142,6 +146,12 @@ def main():
     gt, _ = groundtruth(X, Q, 100, w["metric"])
     log(f"[bench] FP64 groundtruth (k=100) in {time.time() - t0:.1f}s")
 
+    if args.calib_copy:
+        a_ = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+        b_ = torch.empty_like(a_)
+        b_.copy_(a_)
+        torch.cuda.synchronize()
+        del a_, b_
     idx = DeviceIndex(X, adj, w["metric"])
     for leg in legs:
         st = LEG_OPTS[leg]["store"]
@@ -215,7 +225,7 @@ def main():
 
     # batched uneven search: per-query k in {1, 10, 100}, beam ~ U[max(16, k), 256],
     # the whole query set in one persistent launch
-    if w.get("uneven"):
+    if w.get("uneven") and not args.no_uneven:
         rng = np.random.default_rng(77)
         ks = rng.choice([1, 10, 100], nq).astype(np.int32)
         bs = np.array([rng.integers(max(16, kk), 257) for kk in ks], np.int32)

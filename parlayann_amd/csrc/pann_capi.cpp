@@ -95,11 +95,13 @@ pann_status ensure_ghash(pann_index_t idx, size_t words, uint32_t nq) {
     idx->ghash_words = 0;
     HIP_TRY(hipMalloc(&idx->d_ghash, words * 8), "hipMalloc(visited overflow table)");
     HIP_TRY(hipMemset(idx->d_ghash, 0, words * 8), "hipMemset(visited overflow table)");
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
     idx->ghash_words = words;
     idx->ghash_tag = 0;
   }
   if ((uint64_t)idx->ghash_tag + nq + 1 >= 0xFFFFFFFFull) {  // tags about to wrap
     HIP_TRY(hipMemset(idx->d_ghash, 0, idx->ghash_words * 8), "hipMemset(visited overflow table)");
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
     idx->ghash_tag = 0;
   }
   return PANN_OK;
@@ -184,11 +186,23 @@ pann_status pann_open(const pann_index_desc* desc, pann_index_t* out) {
   pann_status st;
   e = hipMalloc(&idx->d_pts, (size_t)n * idx->pts_stride);
   if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(points)"));
-  e = hipMemset(idx->d_pts, 0, (size_t)n * idx->pts_stride);
-  if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemset(points)"));
-  e = hipMemcpy2D(idx->d_pts, idx->pts_stride, desc->points, idx->row_bytes, idx->row_bytes, n,
-                  hipMemcpyHostToDevice);
-  if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemcpy2D(points)"));
+  {
+    // rows padded with zeros on the host, uploaded in chunks with one plain
+    // copy each (no separate device-side clear that a copy could overtake)
+    const uint64_t rows = std::max<uint64_t>(1, (64ull << 20) / idx->pts_stride);
+    std::vector<char> buf;
+    for (uint64_t r0 = 0; r0 < n; r0 += rows) {
+      uint64_t r1 = std::min<uint64_t>(n, r0 + rows);
+      buf.assign((r1 - r0) * idx->pts_stride, 0);
+      const char* src = (const char*)desc->points + r0 * idx->row_bytes;
+      for (uint64_t i = r0; i < r1; i++)
+        std::memcpy(&buf[(i - r0) * idx->pts_stride], src + (i - r0) * idx->row_bytes,
+                    idx->row_bytes);
+      e = hipMemcpy(idx->d_pts + r0 * idx->pts_stride, buf.data(), buf.size(),
+                    hipMemcpyHostToDevice);
+      if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemcpy(points)"));
+    }
+  }
   e = hipMalloc(&idx->d_nbrs, (size_t)n * idx->nbr_stride * 4);
   if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(graph)"));
   {
@@ -234,6 +248,7 @@ pann_status pann_build_store(pann_index_t idx, int32_t store) {
       idx->bf16_stride = round_up((uint64_t)idx->dim * 2, 128);
       HIP_TRY(hipMalloc(&idx->d_bf16, (size_t)idx->n * idx->bf16_stride), "hipMalloc(bf16)");
       HIP_TRY(hipMemset(idx->d_bf16, 0, (size_t)idx->n * idx->bf16_stride), "hipMemset(bf16)");
+      HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
     }
     HIP_TRY(pann::launch_bf16_store((const float*)idx->d_pts, fstride, idx->n, idx->dim,
                                     (uint16_t*)idx->d_bf16, idx->bf16_stride / 2, nullptr),
@@ -243,6 +258,7 @@ pann_status pann_build_store(pann_index_t idx, int32_t store) {
       idx->fp8_stride = round_up((uint64_t)idx->dim, 128);
       HIP_TRY(hipMalloc(&idx->d_fp8, (size_t)idx->n * idx->fp8_stride), "hipMalloc(fp8)");
       HIP_TRY(hipMemset(idx->d_fp8, 0, (size_t)idx->n * idx->fp8_stride), "hipMemset(fp8)");
+      HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
       HIP_TRY(hipMalloc(&idx->d_scales, idx->dim * 4), "hipMalloc(scales)");
       HIP_TRY(hipMalloc(&idx->d_absmax, idx->dim * 4), "hipMalloc(absmax)");
     }
