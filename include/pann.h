@@ -82,6 +82,10 @@ typedef struct {
   int32_t hash_slots_log2; /* LDS visited-set size (0 = automatic) */
   int32_t async;       /* 1: do not synchronise; call pann_search_status() later */
   int32_t waves_per_cu;/* persistent waves per CU (0 = automatic) */
+  int32_t exact_counts;/* 1: out_cmps equals the reference's dist_cmps (every id seen is
+                          remembered); 0: only ids that beat the cutoff are remembered and
+                          a rejected id is evaluated again when it recurs -- same ids,
+                          distances and visited counts, out_cmps counts evaluations */
 } pann_search_opts;
 
 /* Creates the device index: uploads the points (rows padded to 128 bytes)

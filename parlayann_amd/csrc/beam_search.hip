@@ -140,6 +140,18 @@ __device__ __forceinline__ int wave_min_i32(int v) {
   return v;
 }
 
+__device__ __forceinline__ bool global_contains(const unsigned long long* G, uint32_t log2n,
+                                                uint32_t tag, uint32_t key) {
+  const uint32_t mask = (1u << log2n) - 1u;
+  uint32_t h = hash_slot(key, log2n);
+  for (;;) {
+    unsigned long long v = __hip_atomic_load(&G[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(v >> 32) != tag) return false;
+    if ((uint32_t)v == key) return true;
+    h = (h + 1u) & mask;
+  }
+}
+
 // number of entries of sorted A[0..n) that are < x
 __device__ __forceinline__ int lower_bound_u64(const uint64_t* A, int n, uint64_t x) {
   int lo = 0, hi = n;
@@ -262,29 +274,41 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     nb_pref = (nu >= 0 && lane < dl) ? a.nbrs[(uint64_t)pref_id * a.nbr_stride + lane]
                                      : PANN_SENTINEL;
     const bool valid = nb != PANN_SENTINEL;
-    const int nvalid = __popcll(ballot(valid));
-    // LDS table at half load: from here on new ids go to this wave's global table
-    if (!ovf && lds_count + (uint32_t)nvalid > (hslots >> 1)) ovf = true;
-    bool isnew = false;
-    if (valid) {
-      if (!ovf) {
-        isnew = lds_insert(L.H, hlog2, nb);
-      } else if (!lds_contains(L.H, hlog2, nb)) {
-        if (g_count + PANN_WAVE < (1u << glog2)) isnew = global_insert(ghash, glog2, gtag, nb);
-        else isnew = true;  // table exhausted: flagged below
+    bool evalme;
+    if (a.exact_seen) {
+      // reference counting: every id enters the seen set at first encounter
+      // (has_been_seen, :156), so dist_cmps equals the reference's
+      const int nvalid = __popcll(ballot(valid));
+      if (!ovf && lds_count + (uint32_t)nvalid > (hslots >> 1)) ovf = true;
+      bool isnew = false;
+      if (valid) {
+        if (!ovf) {
+          isnew = lds_insert(L.H, hlog2, nb);
+        } else if (!lds_contains(L.H, hlog2, nb)) {
+          if (g_count + PANN_WAVE < (1u << glog2)) isnew = global_insert(ghash, glog2, gtag, nb);
+          else isnew = true;  // table exhausted: flagged below
+        }
       }
+      const uint64_t mins = ballot(isnew);
+      if (!ovf) lds_count += (uint32_t)__popcll(mins);
+      else {
+        g_count += (uint32_t)__popcll(mins);
+        if (g_count + PANN_WAVE >= (1u << glog2)) status |= PANN_ST_HASH_CAPACITY;
+      }
+      evalme = isnew && nb != self;  // Points[a].same_as(p) (:156)
+    } else {
+      // accepted-only seen set: an id is remembered once it beats the cutoff.
+      // A rejected id is rejected again whenever it recurs (the cutoff never
+      // rises once the frontier is full), so it is simply evaluated again:
+      // candidates, merges and visits are the reference's; only the count of
+      // distance evaluations exceeds the reference's dist_cmps.
+      evalme = valid && nb != self && !lds_contains(L.H, hlog2, nb) &&
+               !(ovf && global_contains(ghash, glog2, gtag, nb));
     }
-    const uint64_t mins = ballot(isnew);
-    if (!ovf) lds_count += (uint32_t)__popcll(mins);
-    else {
-      g_count += (uint32_t)__popcll(mins);
-      if (g_count + PANN_WAVE >= (1u << glog2)) status |= PANN_ST_HASH_CAPACITY;
-    }
-    if (nb == self) isnew = false;  // Points[a].same_as(p) (:156)
-    const uint64_t mnew = ballot(isnew);
+    const uint64_t mnew = ballot(evalme);
     const int ncmp = __popcll(mnew);
     cmps += (uint32_t)ncmp;
-    if (isnew) L.NL[mbcnt(mnew)] = nb;
+    if (evalme) L.NL[mbcnt(mnew)] = nb;
     wave_lds_sync();
     eval_rows<ST, MT, NCH, SEQ>(D, a, L, ncmp, lane);
     wave_lds_sync();
@@ -294,6 +318,26 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
       float d = L.DL[lane];
       keep = !(d >= cutoff);
       key = make_key(d, L.NL[lane]);
+    }
+    if (!a.exact_seen) {
+      // remember the accepted ids; a duplicate within one adjacency list is
+      // inserted once and dropped the second time (as the reference's seen set)
+      const int nk = __popcll(ballot(keep));
+      if (!ovf && lds_count + (uint32_t)nk > (hslots >> 1)) ovf = true;
+      bool ins = false;
+      if (keep) {
+        const uint32_t id = key_id(key);
+        if (!ovf) ins = lds_insert(L.H, hlog2, id);
+        else if (g_count + PANN_WAVE < (1u << glog2)) ins = global_insert(ghash, glog2, gtag, id);
+        else ins = true;
+      }
+      const uint64_t mi = ballot(ins);
+      if (!ovf) lds_count += (uint32_t)__popcll(mi);
+      else {
+        g_count += (uint32_t)__popcll(mi);
+        if (g_count + PANN_WAVE >= (1u << glog2)) status |= PANN_ST_HASH_CAPACITY;
+      }
+      keep = ins;
     }
     const uint64_t mk = ballot(keep);
     if (keep) L.C[c + (int)mbcnt(mk)] = key;

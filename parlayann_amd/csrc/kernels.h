@@ -61,6 +61,9 @@ struct SearchArgs {
   unsigned long long* ghash;  // grid x 2^ghash_log2 slots of (tag << 32 | id)
   uint32_t ghash_log2;
   uint32_t ghash_tag_base;    // query q of this launch uses tag base + q + 1
+  // 1: the seen set holds every id at first encounter (dist_cmps equal to the
+  // reference's); 0: it holds accepted ids only (same results, fewer inserts)
+  uint32_t exact_seen;
   // LDS layout
   uint32_t fcap, ccap, hslots_log2;
 };

@@ -367,14 +367,17 @@ pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint3
   a.fcap = (uint32_t)round_up((uint64_t)B, 16);
   a.ccap = (uint32_t)round_up((uint64_t)(defer_cap + a.degree_limit + 1), 16);
   a.ccap = std::max<uint32_t>(a.ccap, 64);  // also holds the start points
+  a.exact_seen = o.exact_counts ? 1u : 0u;
   uint32_t hlog2 = (uint32_t)o.hash_slots_log2;
   if (hlog2 == 0) {
-    // expected distinct ids seen ~ visits * new-per-visit; sized for <= 1/2 load
     // A small LDS table keeps more waves per CU; ids past half its load go to
     // the per-wave global table.  Occupancy wins over LDS hits (measured at
-    // beam 200, degree 64: 2^11 slots 1.5x faster than 2^13).
-    double est = (double)B * 1.3 * (double)a.degree_limit * 0.45 * 2.0;
-    hlog2 = std::min<uint32_t>(std::max<uint32_t>(ceil_log2((uint64_t)est), 10), 11);
+    // beam 200, degree 64: 2^11 slots 1.5x faster than 2^13).  The
+    // accepted-only set needs about 5 entries per beam slot.
+    double est = o.exact_counts ? (double)B * 1.3 * (double)a.degree_limit * 0.45 * 2.0
+                                : (double)B * 5.0 * 2.0;
+    hlog2 = std::min<uint32_t>(std::max<uint32_t>(ceil_log2((uint64_t)est), 10),
+                               o.exact_counts ? 11u : 12u);
   }
   if (hlog2 < 8 || hlog2 > 15) return fail(PANN_ERR_ARG, "hash_slots_log2 must be in [8, 15]");
   a.hslots_log2 = hlog2;

@@ -72,21 +72,26 @@ def test_u8_l2_bit_exact(oracle, u8_case, B):
     o = oracle.search_batch(adj, X, "l2", Q, 10, B)
     assert not o["hash_grew"].any()
     idx = DeviceIndex(X, adj, "l2")
-    g = idx.search(Q, 10, B)
+    g = idx.search(Q, 10, B, exact_counts=True)
     _assert_exact(g, o, 10)
+    # the accepted-only seen set: same results, at least as many evaluations
+    f = idx.search(Q, 10, B)
+    assert np.array_equal(f["ids"], o["ids"]) and np.array_equal(f["dists"], o["dists"])
+    assert np.array_equal(f["visited"], o["visited"])
+    assert np.all(f["dist_cmps"].astype(np.uint64) >= o["dist_cmps"])
 
 
 def test_u8_ip_and_i8_bit_exact(oracle, u8_case):
     X, Q, adj = u8_case
     _say("u8 ip / i8 l2")
     o = oracle.search_batch(adj, X, "ip", Q, 10, 32)
-    g = DeviceIndex(X, adj, "ip").search(Q, 10, 32)
+    g = DeviceIndex(X, adj, "ip").search(Q, 10, 32, exact_counts=True)
     _assert_exact(g, o, 10)
     Xs = (X.astype(np.int16) - 128).astype(np.int8)
     Qs = (Q.astype(np.int16) - 128).astype(np.int8)
     for metric in ("l2", "ip"):
         o = oracle.search_batch(adj, Xs, metric, Qs, 10, 32)
-        g = DeviceIndex(Xs, adj, metric).search(Qs, 10, 32)
+        g = DeviceIndex(Xs, adj, metric).search(Qs, 10, 32, exact_counts=True)
         _assert_exact(g, o, 10)
 
 
@@ -97,8 +102,12 @@ def test_f32_inorder_bit_exact(oracle, f32_case, ip_case, metric, B):
     _say(f"f32 inorder {metric} B={B}")
     o = oracle.search_batch(adj, X, metric, Q, 10, B)
     assert not o["hash_grew"].any()
-    g = DeviceIndex(X, adj, metric).search(Q, 10, B, summation="inorder")
+    idx = DeviceIndex(X, adj, metric)
+    g = idx.search(Q, 10, B, summation="inorder", exact_counts=True)
     _assert_exact(g, o, 10)
+    f = idx.search(Q, 10, B, summation="inorder")
+    assert np.array_equal(f["ids"], o["ids"]) and np.array_equal(f["dists"], o["dists"])
+    assert np.array_equal(f["visited"], o["visited"])
 
 
 @pytest.mark.parametrize("metric", ["l2", "ip"])
@@ -127,12 +136,18 @@ def test_hash_overflow_path_is_exact(oracle, u8_case, f32_case):
     X, Q, adj = u8_case
     _say("hash overflow")
     o = oracle.search_batch(adj, X, "l2", Q, 10, 100)
-    g = DeviceIndex(X, adj, "l2").search(Q, 10, 100, hash_slots_log2=8)
+    idx = DeviceIndex(X, adj, "l2")
+    g = idx.search(Q, 10, 100, hash_slots_log2=8, exact_counts=True)
     _assert_exact(g, o, 10)
+    f = idx.search(Q, 10, 100, hash_slots_log2=8)
+    assert np.array_equal(f["ids"], o["ids"]) and np.array_equal(f["visited"], o["visited"])
     X, Q, adj = f32_case
     o = oracle.search_batch(adj, X, "l2", Q, 10, 64)
-    g = DeviceIndex(X, adj, "l2").search(Q, 10, 64, hash_slots_log2=8)
+    idx = DeviceIndex(X, adj, "l2")
+    g = idx.search(Q, 10, 64, hash_slots_log2=8, exact_counts=True)
     _assert_exact(g, o, 10)
+    f = idx.search(Q, 10, 64, hash_slots_log2=8)
+    assert np.array_equal(f["ids"], o["ids"]) and np.array_equal(f["visited"], o["visited"])
 
 
 def test_limit_and_degree_limit(oracle, f32_case):
@@ -141,7 +156,8 @@ def test_limit_and_degree_limit(oracle, f32_case):
     for lim, dl in [(15, 32), (20, 16), (35, 8)]:
         B = max(lim, 10)
         o = oracle.search_batch(adj, X, "l2", Q, 10, B, limit=lim, degree_limit=dl)
-        g = DeviceIndex(X, adj, "l2").search(Q, 10, B, limit=lim, degree_limit=dl)
+        g = DeviceIndex(X, adj, "l2").search(Q, 10, B, limit=lim, degree_limit=dl,
+                                             exact_counts=True)
         _assert_exact(g, o, 10)
 
 
