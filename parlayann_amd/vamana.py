@@ -88,34 +88,38 @@ def build_vamana(points: np.ndarray, metric: str, R: int, L_build: int, alpha: f
             L.check(lib.pann_robust_prune_batch(h, _p(ids), nb, _p(vid), _p(vd), _p(vc), vis_cap,
                                                 a, R, 1, _p(out), _p(cnt), None))
             L.check(lib.pann_update_neighbors(h, _p(ids), nb, _p(out), _p(cnt), R, None))
-            # reverse edges: (ngh, index) pairs grouped by ngh (:281-285)
-            out_h = out.cpu().numpy()
-            cnt_h = cnt.cpu().numpy()
-            ids_h = perms[p][lo:hi].astype(np.int64)
-            mask = np.arange(R)[None, :] < cnt_h[:, None]
-            keys = out_h[mask].astype(np.int64)
-            if keys.size == 0:
+            # reverse edges: (ngh, index) pairs grouped by ngh (:281-285), on the
+            # device: a stable sort by ngh keeps each group's values in
+            # insertion (flattened) order
+            mask = torch.arange(R, device=dev)[None, :] < cnt[:, None]
+            keys = out[mask]
+            if keys.numel() == 0:
                 continue
-            vals = np.repeat(ids_h, cnt_h)
-            o = np.argsort(keys, kind="stable")
-            keys, vals = keys[o], vals[o]
-            ukeys, starts, counts = np.unique(keys, return_index=True, return_counts=True)
-            # CSR groups, values in insertion order; small and large groups in
-            # separate launches (large ones sort in global scratch)
-            for sel in (counts <= 1024, counts > 1024):
-                if not sel.any():
+            vals = ids[:, None].expand(nb, R)[mask]
+            keys_s, order_k = torch.sort(keys, stable=True)
+            vals_s = vals[order_k]
+            ukeys, counts = torch.unique_consecutive(keys_s, return_counts=True)
+            offs = torch.zeros(ukeys.numel() + 1, dtype=torch.int64, device=dev)
+            offs[1:] = torch.cumsum(counts, 0)
+            # small and large groups in separate launches (large ones sort in
+            # global scratch)
+            big = counts > 1024
+            for sel in ((~big).nonzero().flatten(), big.nonzero().flatten()):
+                ng = int(sel.numel())
+                if ng == 0:
                     continue
-                gk_h = ukeys[sel].astype(np.int32)
+                gk = ukeys[sel].to(i32).contiguous()
                 gcnt = counts[sel]
-                offs = np.concatenate([[0], np.cumsum(gcnt)]).astype(np.int32)
-                vals_g = vals[np.repeat(sel, counts)]  # groups are contiguous runs
-                ng = gk_h.size
-                gk = torch.from_numpy(gk_h).to(dev)
-                go = torch.from_numpy(offs).to(dev)
-                gv = torch.from_numpy(vals_g.astype(np.int32)).to(dev)
+                go = torch.zeros(ng + 1, dtype=torch.int64, device=dev)
+                go[1:] = torch.cumsum(gcnt, 0)
+                # gather the selected groups' values into one CSR array
+                starts = offs[:-1][sel]
+                rep_start = torch.repeat_interleave(starts - go[:-1], gcnt)
+                gv = vals_s[torch.arange(int(go[-1]), device=dev) + rep_start].to(i32).contiguous()
+                go32 = go.to(i32).contiguous()
                 out2 = torch.empty((ng, R), dtype=i32, device=dev)
                 cnt2 = torch.empty(ng, dtype=i32, device=dev)
-                L.check(lib.pann_robust_prune_groups(h, _p(gk), ng, _p(go), _p(gv),
+                L.check(lib.pann_robust_prune_groups(h, _p(gk), ng, _p(go32), _p(gv),
                                                      int(gcnt.max()), a, R, 2, _p(out2),
                                                      _p(cnt2), None))
                 L.check(lib.pann_update_neighbors(h, _p(gk), ng, _p(out2), _p(cnt2), R, None))
