@@ -49,7 +49,7 @@ __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap,
   b += 2 * align16(fcap * 8);
   b += 2 * align16(fcap);
   b += 2 * align16(ccap * 8);
-  b += align16(64 * 4) * 2;
+  b += align16(128 * 4) * 2;
   b += align16(dim * 4);
   b += (1u << hslots_log2) * 4;
   return b;
@@ -64,8 +64,8 @@ __device__ inline WaveLds carve(char* p, const SearchArgs& a) {
   L.FV1 = (uint8_t*)(p + o); o += align16(a.fcap);
   L.C = (uint64_t*)(p + o); o += align16(a.ccap * 8);
   L.Cs = (uint64_t*)(p + o); o += align16(a.ccap * 8);
-  L.NL = (uint32_t*)(p + o); o += align16(64 * 4);
-  L.DL = (float*)(p + o); o += align16(64 * 4);
+  L.NL = (uint32_t*)(p + o); o += align16(128 * 4);
+  L.DL = (float*)(p + o); o += align16(128 * 4);
   L.QL = (float*)(p + o); o += align16(a.dim * 4);
   L.H = (uint32_t*)(p + o);
   return L;
@@ -249,6 +249,51 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   int u = first_unvisited(L.FV0, f, 0, lane);
   uint32_t pref_id = PANN_SENTINEL;
   uint32_t nb_pref = PANN_SENTINEL;
+  // accepted-only seen set: remember the ids of d < cutoff in [lo, hi) of
+  // NL/DL and append them to C; a duplicate is inserted once and dropped the
+  // second time, as the reference's seen set drops it
+  auto accept_range = [&](int lo, int hi, float cutoff) {
+    const int r = lo + lane;
+    bool keep = false;
+    uint64_t key = 0;
+    if (r < hi) {
+      const float d = L.DL[r];
+      keep = !(d >= cutoff);
+      key = make_key(d, L.NL[r]);
+    }
+    const int nk = __popcll(ballot(keep));
+    if (!ovf && lds_count + (uint32_t)nk > (hslots >> 1)) ovf = true;
+    bool ins = false;
+    if (keep) {
+      const uint32_t id = key_id(key);
+      if (!ovf) ins = lds_insert(L.H, hlog2, id);
+      else if (g_count + PANN_WAVE < (1u << glog2)) ins = global_insert(ghash, glog2, gtag, id);
+      else ins = true;
+    }
+    const uint64_t mi = ballot(ins);
+    if (!ovf) lds_count += (uint32_t)__popcll(mi);
+    else {
+      g_count += (uint32_t)__popcll(mi);
+      if (g_count + PANN_WAVE >= (1u << glog2)) status |= PANN_ST_HASH_CAPACITY;
+    }
+    if (ins) L.C[c + (int)mbcnt(mi)] = key;
+    c += __popcll(mi);
+  };
+  auto record_visit = [&](uint64_t k) {
+    if (a.vis_out && lane == 0) {
+      if (nvis < a.vis_cap) a.vis_out[(uint64_t)q * a.vis_cap + nvis] = k;
+      else status |= PANN_ST_VISITED_OVERFLOW;
+    }
+    nvis++;
+  };
+  auto adjacency = [&](uint32_t id) -> uint32_t {
+    return lane < dl ? a.nbrs[(uint64_t)id * a.nbr_stride + lane] : PANN_SENTINEL;
+  };
+  auto unseen = [&](uint32_t x) -> bool {
+    return x != PANN_SENTINEL && x != self && !lds_contains(L.H, hlog2, x) &&
+           !(ovf && global_contains(ghash, glog2, gtag, x));
+  };
+
   for (;;) {
     uint64_t* F = cur_buf ? L.F1 : L.F0;
     uint8_t* FV = cur_buf ? L.FV1 : L.FV0;
@@ -256,28 +301,21 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     const uint64_t cur_key = F[u];
     const uint32_t cur = key_id(cur_key);
     if (lane == 0) FV[u] = 1;
-    if (a.vis_out && lane == 0) {
-      if (nvis < a.vis_cap) a.vis_out[(uint64_t)q * a.vis_cap + nvis] = cur_key;
-      else status |= PANN_ST_VISITED_OVERFLOW;
-    }
-    nvis++;
+    record_visit(cur_key);
     const bool full = (f == B);
     const float cutoff = full ? key_dist(F[f - 1]) : FLT_MAX;
 
     // neighbours of cur (graph rows are padded with the sentinel past deg)
-    uint32_t nb = cur == pref_id ? nb_pref
-                                 : (lane < dl ? a.nbrs[(uint64_t)cur * a.nbr_stride + lane]
-                                              : PANN_SENTINEL);
-    // next unvisited entry of this (stale) frontier; prefetch its adjacency
+    const uint32_t nb = cur == pref_id ? nb_pref : adjacency(cur);
+    // next unvisited entry of this (stale) frontier
     const int nu = first_unvisited(FV, f, u + 1, lane);
-    pref_id = nu >= 0 ? key_id(F[nu]) : PANN_SENTINEL;
-    nb_pref = (nu >= 0 && lane < dl) ? a.nbrs[(uint64_t)pref_id * a.nbr_stride + lane]
-                                     : PANN_SENTINEL;
-    const bool valid = nb != PANN_SENTINEL;
-    bool evalme;
+
     if (a.exact_seen) {
       // reference counting: every id enters the seen set at first encounter
       // (has_been_seen, :156), so dist_cmps equals the reference's
+      pref_id = nu >= 0 ? key_id(F[nu]) : PANN_SENTINEL;
+      nb_pref = nu >= 0 ? adjacency(pref_id) : PANN_SENTINEL;
+      const bool valid = nb != PANN_SENTINEL;
       const int nvalid = __popcll(ballot(valid));
       if (!ovf && lds_count + (uint32_t)nvalid > (hslots >> 1)) ovf = true;
       bool isnew = false;
@@ -295,59 +333,78 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
         g_count += (uint32_t)__popcll(mins);
         if (g_count + PANN_WAVE >= (1u << glog2)) status |= PANN_ST_HASH_CAPACITY;
       }
-      evalme = isnew && nb != self;  // Points[a].same_as(p) (:156)
+      const bool evalme = isnew && nb != self;  // Points[a].same_as(p) (:156)
+      const uint64_t mnew = ballot(evalme);
+      const int ncmp = __popcll(mnew);
+      cmps += (uint32_t)ncmp;
+      if (evalme) L.NL[mbcnt(mnew)] = nb;
+      wave_lds_sync();
+      eval_rows<ST, MT, NCH, SEQ>(D, a, L, ncmp, lane);
+      wave_lds_sync();
+      bool keep = false;
+      uint64_t key = 0;
+      if (lane < ncmp) {
+        const float d = L.DL[lane];
+        keep = !(d >= cutoff);
+        key = make_key(d, L.NL[lane]);
+      }
+      const uint64_t mk = ballot(keep);
+      if (keep) L.C[c + (int)mbcnt(mk)] = key;
+      c += __popcll(mk);
+      wave_lds_sync();
+      // defer while the candidates are few and the stale frontier still has
+      // unvisited entries (:185-192)
+      if (c == 0 || (defer_ok && c < thr && nu >= 0)) {
+        u = nu;
+        continue;
+      }
     } else {
-      // accepted-only seen set: an id is remembered once it beats the cutoff.
+      // Accepted-only seen set: an id is remembered once it beats the cutoff.
       // A rejected id is rejected again whenever it recurs (the cutoff never
       // rises once the frontier is full), so it is simply evaluated again:
       // candidates, merges and visits are the reference's; only the count of
       // distance evaluations exceeds the reference's dist_cmps.
-      evalme = valid && nb != self && !lds_contains(L.H, hlog2, nb) &&
-               !(ovf && global_contains(ghash, glog2, gtag, nb));
-    }
-    const uint64_t mnew = ballot(evalme);
-    const int ncmp = __popcll(mnew);
-    cmps += (uint32_t)ncmp;
-    if (evalme) L.NL[mbcnt(mnew)] = nb;
-    wave_lds_sync();
-    eval_rows<ST, MT, NCH, SEQ>(D, a, L, ncmp, lane);
-    wave_lds_sync();
-    bool keep = false;
-    uint64_t key = 0;
-    if (lane < ncmp) {
-      float d = L.DL[lane];
-      keep = !(d >= cutoff);
-      key = make_key(d, L.NL[lane]);
-    }
-    if (!a.exact_seen) {
-      // remember the accepted ids; a duplicate within one adjacency list is
-      // inserted once and dropped the second time (as the reference's seen set)
-      const int nk = __popcll(ballot(keep));
-      if (!ovf && lds_count + (uint32_t)nk > (hslots >> 1)) ovf = true;
-      bool ins = false;
-      if (keep) {
-        const uint32_t id = key_id(key);
-        if (!ovf) ins = lds_insert(L.H, hlog2, id);
-        else if (g_count + PANN_WAVE < (1u << glog2)) ins = global_insert(ghash, glog2, gtag, id);
-        else ins = true;
+      //
+      // Two visits per step: while merges are deferred the next visit is nu
+      // with the same stale frontier and cutoff, so its neighbours are
+      // gathered together with cur's.  Its results are committed only if the
+      // first visit defers; its accepted ids are inserted after cur's, so an
+      // id both lists share is taken once, for cur, as in the reference.
+      const bool have2 = nu >= 0 && (int64_t)nvis < a.limit;
+      const uint64_t key2 = have2 ? F[nu] : 0;
+      const uint32_t id2 = key_id(key2);
+      const uint32_t nb2 = !have2 ? PANN_SENTINEL : (id2 == pref_id ? nb_pref : adjacency(id2));
+      const int nnu = have2 ? first_unvisited(FV, f, nu + 1, lane) : -1;
+      const int nxt = have2 ? nnu : nu;
+      pref_id = nxt >= 0 ? key_id(F[nxt]) : PANN_SENTINEL;
+      nb_pref = nxt >= 0 ? adjacency(pref_id) : PANN_SENTINEL;
+      const bool e1 = unseen(nb);
+      const bool e2 = have2 && unseen(nb2);
+      const uint64_t m1 = ballot(e1), m2 = ballot(e2);
+      const int n1 = __popcll(m1), n2 = __popcll(m2);
+      if (e1) L.NL[mbcnt(m1)] = nb;
+      if (e2) L.NL[n1 + (int)mbcnt(m2)] = nb2;
+      wave_lds_sync();
+      eval_rows<ST, MT, NCH, SEQ>(D, a, L, n1 + n2, lane);
+      wave_lds_sync();
+      accept_range(0, n1, cutoff);
+      cmps += (uint32_t)n1;
+      wave_lds_sync();
+      if (c == 0 || (defer_ok && c < thr && nu >= 0)) {
+        if (!have2) {
+          u = nu;
+          continue;
+        }
+        if (lane == 0) FV[nu] = 1;
+        record_visit(key2);
+        accept_range(n1, n1 + n2, cutoff);
+        cmps += (uint32_t)n2;
+        wave_lds_sync();
+        if (c == 0 || (defer_ok && c < thr && nnu >= 0)) {
+          u = nnu;
+          continue;
+        }
       }
-      const uint64_t mi = ballot(ins);
-      if (!ovf) lds_count += (uint32_t)__popcll(mi);
-      else {
-        g_count += (uint32_t)__popcll(mi);
-        if (g_count + PANN_WAVE >= (1u << glog2)) status |= PANN_ST_HASH_CAPACITY;
-      }
-      keep = ins;
-    }
-    const uint64_t mk = ballot(keep);
-    if (keep) L.C[c + (int)mbcnt(mk)] = key;
-    c += __popcll(mk);
-    wave_lds_sync();
-    // defer while the candidates are few and the stale frontier still has
-    // unvisited entries (:185-192)
-    if (c == 0 || (defer_ok && c < thr && nu >= 0)) {
-      u = nu;
-      continue;
     }
 
     // ---- merge: rank-sort C, then place C and F by rank into the other buffer

@@ -268,7 +268,10 @@ template <int ST, int MT, int NCH>
 __device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, const char* base,
                                            uint64_t stride, int row_bytes, const uint32_t* NL,
                                            int ncmp, float* DL, int lane) {
-  constexpr int UP = NCH <= 1 ? 8 : (NCH <= 2 ? 4 : (NCH <= 4 ? 2 : 1));
+  // rows in flight per lane: every 8-lane group gathers UP rows per round.
+  // Occupancy is bounded by LDS, not VGPRs, so all 64 candidates of a visit
+  // are issued in one round where the row fits 4 chunks.
+  constexpr int UP = NCH <= 4 ? 8 : 4;
   const int sub = lane & 7, grp = lane >> 3;
   for (int r0 = 0; r0 < ncmp; r0 += 8 * UP) {
     uint4 raw[UP][NCH];
