@@ -6,7 +6,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpann.so")
+LIB_PATH = os.environ.get("PANN_LIB_PATH") or os.path.join(HERE, "libpann.so")
 
 PANN_OK, PANN_ERR_ARG, PANN_ERR_DEVICE, PANN_ERR_OOM, PANN_ERR_CAPACITY = 0, 1, 2, 3, 4
 PANN_ERR_SHORT_BEAM, PANN_ERR_UNSUPPORTED = 5, 6

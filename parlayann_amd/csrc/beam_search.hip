@@ -71,32 +71,55 @@ __device__ inline WaveLds carve(char* p, const SearchArgs& a) {
   return L;
 }
 
-__device__ __forceinline__ uint32_t hash_slot(uint32_t key, uint32_t log2n) {
-  return (key * 0x9E3779B1u) >> (32u - log2n);
-}
 
-// Exact insert-or-test in an open-addressing table (linear probing, CAS).
-// Returns true when `key` was absent and has been inserted.
-__device__ __forceinline__ bool lds_insert(uint32_t* H, uint32_t log2n, uint32_t key) {
-  const uint32_t mask = (1u << log2n) - 1u;
-  uint32_t h = hash_slot(key, log2n);
-  for (;;) {
-    uint32_t old = atomicCAS(&H[h], PANN_SENTINEL, key);
-    if (old == PANN_SENTINEL) return true;
-    if (old == key) return false;
-    h = (h + 1u) & mask;
-  }
+// The LDS seen set: open addressing over buckets of 4 slots, probed bucket
+// after bucket; one ds_read_b128 reads a whole bucket, so a lookup almost
+// always resolves in one LDS access even for the slowest lane of the wave.
+// Slots only ever go from empty to a key, so a key inserted into bucket b
+// found every bucket before b full, and they stay full: a lookup that meets
+// a bucket with an empty slot can stop.
+__device__ __forceinline__ uint32_t bucket_of(uint32_t key, uint32_t log2_slots) {
+  return (key * 0x9E3779B1u) >> (32u - (log2_slots - 2u));
 }
 
 __device__ __forceinline__ bool lds_contains(const uint32_t* H, uint32_t log2n, uint32_t key) {
-  const uint32_t mask = (1u << log2n) - 1u;
-  uint32_t h = hash_slot(key, log2n);
+  const uint32_t bmask = (1u << (log2n - 2u)) - 1u;
+  uint32_t b = bucket_of(key, log2n);
   for (;;) {
-    uint32_t v = H[h];
-    if (v == key) return true;
-    if (v == PANN_SENTINEL) return false;
-    h = (h + 1u) & mask;
+    const uint4 v = *(const uint4*)(H + 4u * b);
+    if (v.x == key || v.y == key || v.z == key || v.w == key) return true;
+    if (v.x == PANN_SENTINEL || v.y == PANN_SENTINEL || v.z == PANN_SENTINEL ||
+        v.w == PANN_SENTINEL)
+      return false;
+    b = (b + 1u) & bmask;
   }
+}
+
+// Exact insert-or-test.  Returns true when `key` was absent and is now in.
+__device__ __forceinline__ bool lds_insert(uint32_t* H, uint32_t log2n, uint32_t key) {
+  const uint32_t bmask = (1u << (log2n - 2u)) - 1u;
+  uint32_t b = bucket_of(key, log2n);
+  for (;;) {
+    const uint4 v = *(const uint4*)(H + 4u * b);
+    if (v.x == key || v.y == key || v.z == key || v.w == key) return false;
+    int j = v.x == PANN_SENTINEL ? 0
+            : v.y == PANN_SENTINEL ? 1
+            : v.z == PANN_SENTINEL ? 2
+            : v.w == PANN_SENTINEL ? 3 : -1;
+    if (j < 0) {
+      b = (b + 1u) & bmask;
+      continue;
+    }
+    const uint32_t old = atomicCAS(&H[4u * b + (uint32_t)j], PANN_SENTINEL, key);
+    if (old == PANN_SENTINEL) return true;
+    if (old == key) return false;
+    // another lane took this slot for a different key: read the bucket again
+  }
+}
+
+// Open addressing with linear probing (the global overflow table).
+__device__ __forceinline__ uint32_t hash_slot(uint32_t key, uint32_t log2n) {
+  return (key * 0x9E3779B1u) >> (32u - log2n);
 }
 
 // Per-wave global table of 64-bit slots (tag << 32 | id).  A slot whose tag
@@ -173,7 +196,7 @@ __device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const
   }
 }
 
-template <int ST, int MT, int NCH, bool SEQ>
+template <int ST, int MT, int NCH, bool SEQ, bool EX>
 __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, int lane,
                            unsigned long long* ghash) {
   const uint32_t gtag = a.ghash_tag_base + q + 1u;
@@ -310,7 +333,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     // next unvisited entry of this (stale) frontier
     const int nu = first_unvisited(FV, f, u + 1, lane);
 
-    if (a.exact_seen) {
+    if constexpr (EX) {
       // reference counting: every id enters the seen set at first encounter
       // (has_been_seen, :156), so dist_cmps equals the reference's
       pref_id = nu >= 0 ? key_id(F[nu]) : PANN_SENTINEL;
@@ -487,7 +510,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   wave_lds_sync();
 }
 
-template <int ST, int MT, int NCH, bool SEQ>
+template <int ST, int MT, int NCH, bool SEQ, bool EX>
 __global__ __launch_bounds__(PANN_WAVE) void k_beam_search(SearchArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int lane = threadIdx.x;
@@ -498,7 +521,7 @@ __global__ __launch_bounds__(PANN_WAVE) void k_beam_search(SearchArgs a) {
     if (lane == 0) q = atomicAdd(a.work_counter, 1u);
     q = readfirstlane(q);
     if (q >= a.nq) break;
-    search_one<ST, MT, NCH, SEQ>(a, L, q, lane, ghash);
+    search_one<ST, MT, NCH, SEQ, EX>(a, L, q, lane, ghash);
   }
 }
 
@@ -514,15 +537,23 @@ uint32_t search_lds_bytes(const SearchArgs& a) {
   return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2);
 }
 
-template <int ST, int MT, int NCH, bool SEQ>
-static hipError_t launch_t(const SearchArgs& a, uint32_t grid, hipStream_t s) {
+template <int ST, int MT, int NCH, bool SEQ, bool EX>
+static hipError_t launch_k(const SearchArgs& a, uint32_t grid, hipStream_t s) {
   uint32_t lds = search_lds_bytes(a);
-  auto kern = k_beam_search<ST, MT, NCH, SEQ>;
+  auto kern = k_beam_search<ST, MT, NCH, SEQ, EX>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(PANN_WAVE), lds, s, a);
   return hipGetLastError();
+}
+
+// the exact-counts seen set is a separate instantiation, so the default
+// kernel carries none of its state
+template <int ST, int MT, int NCH, bool SEQ>
+static hipError_t launch_t(const SearchArgs& a, uint32_t grid, hipStream_t s) {
+  return a.exact_seen ? launch_k<ST, MT, NCH, SEQ, true>(a, grid, s)
+                      : launch_k<ST, MT, NCH, SEQ, false>(a, grid, s);
 }
 
 static int nch_for(int row_bytes) {

@@ -268,22 +268,23 @@ template <int ST, int MT, int NCH>
 __device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, const char* base,
                                            uint64_t stride, int row_bytes, const uint32_t* NL,
                                            int ncmp, float* DL, int lane) {
-  // rows in flight per lane: every 8-lane group gathers UP rows per round.
-  // Occupancy is bounded by LDS, not VGPRs, so all 64 candidates of a visit
-  // are issued in one round where the row fits 4 chunks.
-  constexpr int UP = NCH <= 4 ? 8 : 4;
+  // rows in flight per lane: every 8-lane group gathers UP rows per round,
+  // sized so the raw chunks (UP * NCH * 4 VGPRs) keep the kernel at or
+  // under ~128 VGPRs, i.e. no fewer waves per CU than LDS already allows
+  constexpr int UP = NCH <= 2 ? 8 : (NCH <= 4 ? 4 : 2);
   const int sub = lane & 7, grp = lane >> 3;
+  // Branch-free gathers: slots past ncmp re-read row NL[0] (an L1 hit) and are
+  // discarded; chunks past row_bytes read the row's zero padding (rows are
+  // padded to a 128-byte multiple, the query is zero there), which adds
+  // nothing to either metric and no cache lines.
   for (int r0 = 0; r0 < ncmp; r0 += 8 * UP) {
     uint4 raw[UP][NCH];
 #pragma unroll
     for (int u = 0; u < UP; u++) {
-      int r = r0 + u * 8 + grp;
-      const char* row = base + (uint64_t)(r < ncmp ? NL[r] : 0u) * stride;
+      const int r = r0 + u * 8 + grp;
+      const char* row = base + (uint64_t)NL[r < ncmp ? r : 0] * stride;
 #pragma unroll
-      for (int c = 0; c < NCH; c++) {
-        int off = c * 128 + sub * 16;
-        raw[u][c] = (r < ncmp && off < row_bytes) ? ld16(row + off) : make_uint4(0u, 0u, 0u, 0u);
-      }
+      for (int c = 0; c < NCH; c++) raw[u][c] = ld16(row + c * 128 + sub * 16);
     }
 #pragma unroll
     for (int u = 0; u < UP; u++) {
