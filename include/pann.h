@@ -79,13 +79,15 @@ typedef struct {
   int32_t summation;   /* pann_summation for the native f32 store */
   int32_t rerank;      /* 1: rerank branch of beam_search_rerank (beamSearch.h:569-587)
                           against the f32 points; 0: plain branch (:588-596) */
-  int32_t hash_slots_log2; /* LDS visited-set size (0 = automatic) */
+  int32_t hash_slots_log2; /* exact_counts = 1 only: LDS seen-set size, log2 of slots in
+                              [8, 15] (0 = automatic) */
   int32_t async;       /* 1: do not synchronise; call pann_search_status() later */
   int32_t waves_per_cu;/* persistent waves per CU (0 = automatic) */
   int32_t exact_counts;/* 1: out_cmps equals the reference's dist_cmps (every id seen is
-                          remembered); 0: only ids that beat the cutoff are remembered and
-                          a rejected id is evaluated again when it recurs -- same ids,
-                          distances and visited counts, out_cmps counts evaluations */
+                          remembered); 0: every neighbour is evaluated and checked against
+                          the frontier and pending candidates afterwards -- same ids,
+                          distances and visited counts, out_cmps counts evaluations, no
+                          capacity limit */
 } pann_search_opts;
 
 /* Creates the device index: uploads the points (rows padded to 128 bytes)

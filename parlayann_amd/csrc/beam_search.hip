@@ -186,6 +186,21 @@ __device__ __forceinline__ int lower_bound_u64(const uint64_t* A, int n, uint64_
   return lo;
 }
 
+// whether sorted A[0..n) holds x
+__device__ __forceinline__ bool sorted_contains(const uint64_t* A, int n, uint64_t x) {
+  int lo = 0, len = n;
+  while (len > 0) {
+    const int half = len >> 1;
+    if (A[lo + half] < x) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  return lo < n && A[lo] == x;
+}
+
 template <int ST, int MT, int NCH, bool SEQ>
 __device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
                                           const WaveLds& L, int ncmp, int lane) {
@@ -240,7 +255,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   if (lane < ns) {
     uint32_t s = a.starts[lane];
     L.NL[lane] = s;
-    lds_insert(L.H, hlog2, s);
+    if constexpr (EX) lds_insert(L.H, hlog2, s);
   }
   lds_count = (uint32_t)ns;
   wave_lds_sync();
@@ -272,33 +287,21 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   int u = first_unvisited(L.FV0, f, 0, lane);
   uint32_t pref_id = PANN_SENTINEL;
   uint32_t nb_pref = PANN_SENTINEL;
-  // accepted-only seen set: remember the ids of d < cutoff in [lo, hi) of
-  // NL/DL and append them to C; a duplicate is inserted once and dropped the
-  // second time, as the reference's seen set drops it
-  auto accept_range = [&](int lo, int hi, float cutoff) {
+  // Frontier-and-pending seen set: rows [lo, hi) of NL/DL with d < cutoff
+  // join C unless their key is already in the frontier F (binary search on
+  // the exact (dist, id) key: a row's distance is computed the same way every
+  // time) or already pending in C (the small LDS table H, emptied at every
+  // merge).  An id the reference's seen set would also drop is dropped here.
+  auto accept_range = [&](const uint64_t* F, int lo, int hi, float cutoff) {
     const int r = lo + lane;
-    bool keep = false;
+    bool ins = false;
     uint64_t key = 0;
     if (r < hi) {
       const float d = L.DL[r];
-      keep = !(d >= cutoff);
       key = make_key(d, L.NL[r]);
-    }
-    const int nk = __popcll(ballot(keep));
-    if (!ovf && lds_count + (uint32_t)nk > (hslots >> 1)) ovf = true;
-    bool ins = false;
-    if (keep) {
-      const uint32_t id = key_id(key);
-      if (!ovf) ins = lds_insert(L.H, hlog2, id);
-      else if (g_count + PANN_WAVE < (1u << glog2)) ins = global_insert(ghash, glog2, gtag, id);
-      else ins = true;
+      if (!(d >= cutoff) && !sorted_contains(F, f, key)) ins = lds_insert(L.H, hlog2, key_id(key));
     }
     const uint64_t mi = ballot(ins);
-    if (!ovf) lds_count += (uint32_t)__popcll(mi);
-    else {
-      g_count += (uint32_t)__popcll(mi);
-      if (g_count + PANN_WAVE >= (1u << glog2)) status |= PANN_ST_HASH_CAPACITY;
-    }
     if (ins) L.C[c + (int)mbcnt(mi)] = key;
     c += __popcll(mi);
   };
@@ -311,10 +314,6 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   };
   auto adjacency = [&](uint32_t id) -> uint32_t {
     return lane < dl ? a.nbrs[(uint64_t)id * a.nbr_stride + lane] : PANN_SENTINEL;
-  };
-  auto unseen = [&](uint32_t x) -> bool {
-    return x != PANN_SENTINEL && x != self && !lds_contains(L.H, hlog2, x) &&
-           !(ovf && global_contains(ghash, glog2, gtag, x));
   };
 
   for (;;) {
@@ -382,11 +381,17 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
         continue;
       }
     } else {
-      // Accepted-only seen set: an id is remembered once it beats the cutoff.
-      // A rejected id is rejected again whenever it recurs (the cutoff never
-      // rises once the frontier is full), so it is simply evaluated again:
-      // candidates, merges and visits are the reference's; only the count of
-      // distance evaluations exceeds the reference's dist_cmps.
+      // Every neighbour is evaluated; the seen test happens after the
+      // distance (accept_range).  Which ids the reference's seen set would
+      // skip, and this kernel evaluates:
+      //   * ids rejected earlier: the cutoff never rises once the frontier is
+      //     full (and nothing is rejected before), so they are rejected again;
+      //   * ids once accepted and since evicted from the frontier: eviction
+      //     leaves the frontier full of B smaller keys, so they now fail the
+      //     cutoff too;
+      //   * ids in the frontier or pending in C: found by accept_range.
+      // Candidates, merges and visits are therefore the reference's; only the
+      // count of distance evaluations exceeds its dist_cmps.
       //
       // Two visits per step: while merges are deferred the next visit is nu
       // with the same stale frontier and cutoff, so its neighbours are
@@ -401,8 +406,8 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
       const int nxt = have2 ? nnu : nu;
       pref_id = nxt >= 0 ? key_id(F[nxt]) : PANN_SENTINEL;
       nb_pref = nxt >= 0 ? adjacency(pref_id) : PANN_SENTINEL;
-      const bool e1 = unseen(nb);
-      const bool e2 = have2 && unseen(nb2);
+      const bool e1 = nb != PANN_SENTINEL && nb != self;
+      const bool e2 = have2 && nb2 != PANN_SENTINEL && nb2 != self;
       const uint64_t m1 = ballot(e1), m2 = ballot(e2);
       const int n1 = __popcll(m1), n2 = __popcll(m2);
       if (e1) L.NL[mbcnt(m1)] = nb;
@@ -410,7 +415,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
       wave_lds_sync();
       eval_rows<ST, MT, NCH, SEQ>(D, a, L, n1 + n2, lane);
       wave_lds_sync();
-      accept_range(0, n1, cutoff);
+      accept_range(F, 0, n1, cutoff);
       cmps += (uint32_t)n1;
       wave_lds_sync();
       if (c == 0 || (defer_ok && c < thr && nu >= 0)) {
@@ -420,7 +425,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
         }
         if (lane == 0) FV[nu] = 1;
         record_visit(key2);
-        accept_range(n1, n1 + n2, cutoff);
+        accept_range(F, n1, n1 + n2, cutoff);
         cmps += (uint32_t)n2;
         wave_lds_sync();
         if (c == 0 || (defer_ok && c < thr && nnu >= 0)) {
@@ -463,6 +468,10 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     f = f + c < B ? f + c : B;
     c = 0;
     cur_buf ^= 1;
+    if constexpr (!EX) {  // C is empty again: so is its table
+      for (uint32_t i = lane * 4; i < hslots; i += PANN_WAVE * 4)
+        *(uint4*)(L.H + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
+    }
     u = wave_min_i32(first);
     if (u == 0x7FFFFFFF) u = -1;
     wave_lds_sync();
@@ -515,7 +524,7 @@ __global__ __launch_bounds__(PANN_WAVE) void k_beam_search(SearchArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int lane = threadIdx.x;
   WaveLds L = carve(smem, a);
-  unsigned long long* ghash = a.ghash + ((uint64_t)blockIdx.x << a.ghash_log2);
+  unsigned long long* ghash = EX ? a.ghash + ((uint64_t)blockIdx.x << a.ghash_log2) : nullptr;
   for (;;) {
     uint32_t q = 0;
     if (lane == 0) q = atomicAdd(a.work_counter, 1u);

@@ -368,18 +368,20 @@ pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint3
   a.ccap = (uint32_t)round_up((uint64_t)(defer_cap + a.degree_limit + 1), 16);
   a.ccap = std::max<uint32_t>(a.ccap, 64);  // also holds the start points
   a.exact_seen = o.exact_counts ? 1u : 0u;
-  uint32_t hlog2 = (uint32_t)o.hash_slots_log2;
-  if (hlog2 == 0) {
-    // A small LDS table keeps more waves per CU; ids past half its load go to
-    // the per-wave global table.  Occupancy wins over LDS hits (measured at
-    // beam 200, degree 64: 2^11 slots 1.5x faster than 2^13).  The
-    // accepted-only set needs about 5 entries per beam slot.
-    double est = o.exact_counts ? (double)B * 1.3 * (double)a.degree_limit * 0.45 * 2.0
-                                : (double)B * 5.0 * 2.0;
-    hlog2 = std::min<uint32_t>(std::max<uint32_t>(ceil_log2((uint64_t)est), 10),
-                               o.exact_counts ? 11u : 12u);
+  uint32_t hlog2;
+  if (o.exact_counts) {
+    // The reference's seen set (every id met): a small LDS table keeps more
+    // waves per CU, ids past half its load go to the per-wave global table.
+    hlog2 = (uint32_t)o.hash_slots_log2;
+    if (hlog2 == 0) {
+      double est = (double)B * 1.3 * (double)a.degree_limit * 0.45 * 2.0;
+      hlog2 = std::min<uint32_t>(std::max<uint32_t>(ceil_log2((uint64_t)est), 10), 11u);
+    }
+    if (hlog2 < 8 || hlog2 > 15) return fail(PANN_ERR_ARG, "hash_slots_log2 must be in [8, 15]");
+  } else {
+    // The table only holds the pending candidates (at most ccap): half load.
+    hlog2 = std::max<uint32_t>(ceil_log2((uint64_t)2 * a.ccap), 6);
   }
-  if (hlog2 < 8 || hlog2 > 15) return fail(PANN_ERR_ARG, "hash_slots_log2 must be in [8, 15]");
   a.hslots_log2 = hlog2;
   const uint32_t lds = pann::search_lds_bytes(a);
   if ((int64_t)lds > idx->max_lds_block || (int64_t)lds > idx->lds_per_cu)
@@ -388,16 +390,21 @@ pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint3
   wpc = (int)std::min<int64_t>(wpc, idx->lds_per_cu / lds);
   if (wpc < 1) wpc = 1;
   grid = (uint32_t)(idx->num_cus * wpc);
-  // per-wave global overflow table: twice the reference's initial hashset
-  // size 2*(10+B)*degree_limit (beamSearch.h:66), at least 4x the LDS table
-  uint32_t glog2 = std::max<uint32_t>(ceil_log2((uint64_t)2 * (10 + B) * a.degree_limit) + 1,
-                                      hlog2 + 2);
-  a.ghash_log2 = glog2;
-  pann_status st = ensure_ghash(idx, (size_t)grid << glog2, nq);
-  if (st != PANN_OK) return st;
-  a.ghash = idx->d_ghash;
-  a.ghash_tag_base = idx->ghash_tag;
-  idx->ghash_tag += nq;
+  a.ghash = nullptr;
+  a.ghash_log2 = 0;
+  a.ghash_tag_base = 0;
+  if (o.exact_counts) {
+    // per-wave global overflow table: twice the reference's initial hashset
+    // size 2*(10+B)*degree_limit (beamSearch.h:66), at least 4x the LDS table
+    uint32_t glog2 = std::max<uint32_t>(
+        ceil_log2((uint64_t)2 * (10 + B) * a.degree_limit) + 1, hlog2 + 2);
+    a.ghash_log2 = glog2;
+    pann_status st = ensure_ghash(idx, (size_t)grid << glog2, nq);
+    if (st != PANN_OK) return st;
+    a.ghash = idx->d_ghash;
+    a.ghash_tag_base = idx->ghash_tag;
+    idx->ghash_tag += nq;
+  }
   HIP_TRY(hipMemcpyAsync(idx->d_starts, starts, nstarts * 4, hipMemcpyHostToDevice, stream),
           "hipMemcpyAsync(starts)");
   hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(1), 0, stream, idx->d_status);
