@@ -181,6 +181,79 @@ pann_status pann_update_neighbors(pann_index_t idx, const uint32_t *pts, uint32_
                                   const uint32_t *lists, const uint32_t *counts, uint32_t R,
                                   void *stream);
 
+/* ------------------------------------------------------------------------
+ * HNSW (algorithms/HNSW/HNSW.hpp).  A layered index is a pann_index whose
+ * level-0 graph (pann_open, degree 2m) is joined by one graph per upper level
+ * (degree m) over the same node ids; node ids are row indices of the points.
+ * ------------------------------------------------------------------------ */
+
+/* Sets the graph of level `level` >= 1: n x (max_degree + 1) rows in Graph
+ * layout (degree, then node ids; NULL = every list empty).  Replaces the
+ * per-level neighbour lists the model loader reads (HNSW.hpp:715-730). */
+pann_status pann_hnsw_set_level(pann_index_t idx, uint32_t level, uint32_t max_degree,
+                                const uint32_t *graph_host);
+
+/* HNSW::search_layer (HNSW.hpp:1088-1109) for a batch: beam search on graph
+ * `level` with the intended QueryParams(ef, ef, limit, degree_limit) (the
+ * reference passes a literal 1.35 that lands in `limit`).  Query q starts
+ * from starts[q * starts_stride .. + nstarts[q]) (nstarts NULL: one start),
+ * 1 <= count <= ef.  queries: device rows; or NULL with self_ids: query q is
+ * point self_ids[q], which same_as skips (the insert step, :896-910).  Writes
+ * the first min(out_stride, |frontier|) entries, sorted by (dist, id), and
+ * |frontier| to out_count.  limit <= 0 means n.  All pointers device. */
+pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void *queries,
+                                   const uint32_t *self_ids, uint32_t nq, uint32_t ef,
+                                   uint32_t degree_limit, int64_t limit, const uint32_t *starts,
+                                   const uint32_t *nstarts, uint32_t starts_stride,
+                                   const pann_search_opts *opts, uint32_t *out_ids,
+                                   float *out_dists, uint32_t out_stride, uint32_t *out_count,
+                                   uint32_t *out_visited, uint32_t *out_cmps, void *stream);
+
+/* HNSW::search (HNSW.hpp:1568-1613) with search_layer_to (:1545-1566) for a
+ * batch of device queries: beam 1 from the entrance (host array, <= 64 ids)
+ * through levels top_level .. 1, keeping the closest, then layer 0 with beam
+ * ef and `limit` (<= 0: n); each level's degree limit is its max_degree.  The
+ * frontier is sorted by distance and, when longer than k, cut after the last
+ * entry tied with the k-th distance (:1590-1596).  Ids are mapped through
+ * node_ids (device, NULL = node ids).  out_stride >= ef; out_count[q] =
+ * entries written; out_visited / out_cmps sum every level (count_cmps).
+ * opts.rerank re-scores layer 0 in FP32 before the cut. */
+pann_status pann_hnsw_search(pann_index_t idx, const void *queries, uint32_t nq, uint32_t k,
+                             uint32_t ef, int64_t limit, const uint32_t *entrance,
+                             uint32_t nentrance, uint32_t top_level, const uint32_t *node_ids,
+                             const pann_search_opts *opts, uint32_t *out_ids, float *out_dists,
+                             uint32_t out_stride, uint32_t *out_count, uint32_t *out_visited,
+                             uint32_t *out_cmps, void *stream);
+
+/* select_neighbors / prune_heuristic (HNSW.hpp:430-503) for a batch: point
+ * pts[i] takes the candidates cand_ids/cand_dists[i*cand_stride .. +
+ * cand_count[i]) (a search frontier with distances to the point), sorted by
+ * distance; c is dropped when an already selected r has d(c, r) < alpha *
+ * d(c, p) (float compare); at most M are kept.  All pointers device. */
+pann_status pann_hnsw_select_neighbors(pann_index_t idx, uint32_t level, const uint32_t *pts,
+                                       uint32_t np, const uint32_t *cand_ids,
+                                       const float *cand_dists, const uint32_t *cand_count,
+                                       uint32_t cand_stride, float alpha, uint32_t M,
+                                       uint32_t *out_ids, uint32_t *out_count, void *stream);
+
+/* The reverse-edge step of HNSW::insert (HNSW.hpp:935-983): point pts[i]
+ * receives the ids cand_ids[offsets[i] .. offsets[i+1]) on graph `level`;
+ * when its list and the new ids fit in M they are appended in order,
+ * otherwise both are sorted by distance to the point and cut to M.  Writes
+ * the new lists (use pann_hnsw_set_neighbors to apply).  All pointers device. */
+pann_status pann_hnsw_add_reverse(pann_index_t idx, uint32_t level, const uint32_t *pts,
+                                  uint32_t np, const uint32_t *offsets, const uint32_t *cand_ids,
+                                  uint32_t max_count, uint32_t M, uint32_t *out_ids,
+                                  uint32_t *out_count, void *stream);
+
+/* neighbourhood(u, level) = list (HNSW.hpp:928-933): lists np x R, counts np. */
+pann_status pann_hnsw_set_neighbors(pann_index_t idx, uint32_t level, const uint32_t *pts,
+                                    uint32_t np, const uint32_t *lists, const uint32_t *counts,
+                                    uint32_t R, void *stream);
+
+/* Copies graph `level` back as n x (max_degree + 1) (for HNSW::save, :1615-1686). */
+pann_status pann_hnsw_download_level(pann_index_t idx, uint32_t level, uint32_t *graph_host);
+
 /* Copies the device graph back to host memory as n x (max_degree + 1). */
 pann_status pann_download_graph(pann_index_t idx, uint32_t *graph_host);
 

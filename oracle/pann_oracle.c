@@ -950,3 +950,355 @@ void oracle_sort_pairs(uint32_t *ids, float *dists, long n) {
   }
   free(a);
 }
+
+/* ------------------------------------------------------------------------ */
+/* HNSW (algorithms/HNSW/HNSW.hpp): search and batch construction, with the  */
+/* intended QueryParams(ef, ef, limit, threshold_m) of search_layer (:1096;  */
+/* the reference's literal 1.35 lands in `limit`).                           */
+/* ------------------------------------------------------------------------ */
+
+static oracle_graph hnsw_level(const oracle_hnsw *H, int l) {
+  oracle_graph g = {H->adj[l], H->n, H->max_degree[l]};
+  return g;
+}
+
+/* search_layer (:1088-1109): beam search on level l from eps; frontier in S */
+static int hnsw_search_layer(bs_scratch *S, const oracle_hnsw *H, const oracle_points *P,
+                             const void *q, long self_id, const uint32_t *eps, int neps,
+                             long ef, int l, long limit, bs_result *r) {
+  oracle_graph g = hnsw_level(H, l);
+  oracle_qparams QP = {ef, ef, limit > 0 ? limit : (long)H->n, (long)H->max_degree[l], 100.0,
+                       0.125};
+  return bs_core(S, &g, P, q, self_id, eps, neps, &QP, r);
+}
+
+typedef struct {
+  const oracle_hnsw *H;
+  const oracle_points *P;
+  const uint32_t *entrance;
+  int nentrance;
+  const char *queries;
+  size_t qstride;
+  int k, ef;
+  long limit;
+  uint32_t *out_ids;
+  float *out_dists;
+  int *out_count;
+  uint64_t *out_cmps;
+  uint32_t *out_visited;
+  bs_scratch *scratch;
+  atomic_int failed;
+} hs_ctx;
+
+static void hs_body(void *vctx, long i, int tid) {
+  hs_ctx *c = (hs_ctx *)vctx;
+  bs_scratch *S = &c->scratch[tid];
+  const void *q = c->queries + (size_t)i * c->qstride;
+  uint32_t ep[64];
+  int nep = c->nentrance;
+  memcpy(ep, c->entrance, (size_t)nep * sizeof(uint32_t));
+  uint64_t cmps = 0, vis = 0;
+  bs_result r;
+  for (int l = c->H->top; l > 0; l--) { /* search_layer_to (:1545-1566) */
+    if (hnsw_search_layer(S, c->H, c->P, q, -1, ep, nep, 1, l, 0, &r) != 0) {
+      atomic_store(&c->failed, 1);
+      return;
+    }
+    cmps += r.full_dist_cmps;
+    vis += (uint64_t)r.vsize;
+    ep[0] = S->frontier[0].id; /* eps.push_back(W[0].u) */
+    nep = 1;
+  }
+  if (hnsw_search_layer(S, c->H, c->P, q, -1, ep, nep, c->ef, 0, c->limit, &r) != 0) {
+    atomic_store(&c->failed, 1);
+    return;
+  }
+  cmps += r.full_dist_cmps;
+  vis += (uint64_t)r.vsize;
+  long m = r.fsize;
+  uint32_t *oi = c->out_ids + (size_t)i * c->ef;
+  float *od = c->out_dists + (size_t)i * c->ef;
+  for (long t = 0; t < m; t++) {
+    oi[t] = S->frontier[t].id;
+    od[t] = S->frontier[t].dist;
+  }
+  if (m > c->k) { /* sort by distance (already in (dist, id) order), cut with ties (:1590-1596) */
+    long kk = c->k;
+    if (c->k > 0) {
+      float dk = od[c->k - 1];
+      while (kk < m && !(dk < od[kk])) kk++;
+    }
+    m = kk;
+  }
+  c->out_count[i] = (int)m;
+  if (c->out_cmps) c->out_cmps[i] = cmps;
+  if (c->out_visited) c->out_visited[i] = (uint32_t)vis;
+}
+
+int oracle_hnsw_search(const oracle_hnsw *H, const oracle_points *P, const uint32_t *entrance,
+                       int nentrance, const void *queries, size_t nq, int k, int ef, long limit,
+                       int nthreads, uint32_t *out_ids, float *out_dists, int *out_count,
+                       uint64_t *out_cmps, uint32_t *out_visited) {
+  if (nentrance < 1 || nentrance > 64 || ef < 1 || k < 0) return -1;
+  if (nthreads < 1) nthreads = 1;
+  hs_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.H = H;
+  c.P = P;
+  c.entrance = entrance;
+  c.nentrance = nentrance;
+  c.queries = (const char *)queries;
+  c.qstride = (size_t)P->dim * elt_size(P->dtype);
+  c.k = k;
+  c.ef = ef;
+  c.limit = limit;
+  c.out_ids = out_ids;
+  c.out_dists = out_dists;
+  c.out_count = out_count;
+  c.out_cmps = out_cmps;
+  c.out_visited = out_visited;
+  c.scratch = (bs_scratch *)calloc((size_t)nthreads, sizeof(bs_scratch));
+  atomic_init(&c.failed, 0);
+  parallel_for((long)nq, 16, nthreads, hs_body, &c);
+  for (int t = 0; t < nthreads; t++) bs_scratch_free(&c.scratch[t]);
+  free(c.scratch);
+  return atomic_load(&c.failed) ? -1 : 0;
+}
+
+/* ---- construction: HNSW::HNSW (:748-807) and insert (:809-996) ---- */
+
+typedef struct {
+  const oracle_hnsw *H;
+  const oracle_points *P;
+  const uint32_t *levels;
+  float alpha;
+  unsigned efc;
+  uint32_t bb;        /* first node of the batch */
+  uint32_t entrance;
+  int level_ep, lc;
+  uint32_t *eps;      /* batch x efc */
+  int *neps;
+  uint32_t *nbh;      /* batch x max_degree[0] */
+  int *nnbh;
+  bs_scratch *scratch;
+  atomic_int failed;
+} hb_ctx;
+
+/* descent to the node's level with ef = 1 (:895-904) */
+static void hb_descend_body(void *vctx, long i, int tid) {
+  hb_ctx *c = (hb_ctx *)vctx;
+  uint32_t u = c->bb + (uint32_t)i;
+  int lu = (int)c->levels[u];
+  uint32_t *ep = c->eps + (size_t)i * c->efc;
+  ep[0] = c->entrance; /* eps_u = entrance */
+  int nep = 1;
+  bs_result r;
+  for (int l = c->level_ep; l > lu; l--) {
+    if (hnsw_search_layer(&c->scratch[tid], c->H, c->P, row_of(c->P, u), (long)u, ep, nep, 1, l,
+                          0, &r) != 0) {
+      atomic_store(&c->failed, 1);
+      return;
+    }
+    ep[0] = c->scratch[tid].frontier[0].id;
+    nep = 1;
+  }
+  c->neps[i] = nep;
+}
+
+/* prune_heuristic (:430-484) on a frontier sorted by (dist, id) */
+static int hnsw_prune(const oracle_points *P, const idd *cand, long nc, unsigned size, float alpha,
+                      uint32_t *out) {
+  int nsel = 0;
+  for (long t = 0; t < nc && nsel < (int)size; t++) {
+    const float d_cu = cand[t].dist * alpha;
+    int pruned = 0;
+    for (int s = 0; s < nsel; s++) {
+      float d_cr = oracle_distance(row_of(P, out[s]), row_of(P, cand[t].id), P->dim, P->dtype,
+                                   P->metric);
+      if (d_cr < d_cu) {
+        pruned = 1;
+        break;
+      }
+    }
+    if (!pruned) out[nsel++] = cand[t].id;
+  }
+  return nsel;
+}
+
+/* one level of insert: search with efc, select neighbours (:910-925) */
+static void hb_level_body(void *vctx, long i, int tid) {
+  hb_ctx *c = (hb_ctx *)vctx;
+  uint32_t u = c->bb + (uint32_t)i;
+  if (c->lc > (int)c->levels[u]) return;
+  bs_scratch *S = &c->scratch[tid];
+  uint32_t *ep = c->eps + (size_t)i * c->efc;
+  bs_result r;
+  if (hnsw_search_layer(S, c->H, c->P, row_of(c->P, u), (long)u, ep, c->neps[i], c->efc, c->lc, 0,
+                        &r) != 0) {
+    atomic_store(&c->failed, 1);
+    return;
+  }
+  unsigned M = c->H->max_degree[c->lc]; /* get_threshold_m(l_c) */
+  c->nnbh[i] = hnsw_prune(c->P, S->frontier, r.fsize, M, c->alpha,
+                          c->nbh + (size_t)i * c->H->max_degree[0]);
+  for (long t = 0; t < r.fsize; t++) ep[t] = S->frontier[t].id; /* eps_u = res (:940-944) */
+  c->neps[i] = (int)r.fsize;
+}
+
+typedef struct {
+  hb_ctx *b;
+  uint32_t *gkeys, *gvals;
+  long *gstart;
+  idd **buf;
+  long *cap;
+} hr_ctx;
+
+/* reverse edges (:952-983): append when they fit, else the m nearest */
+static void hr_body(void *vctx, long g, int tid) {
+  hr_ctx *c = (hr_ctx *)vctx;
+  const oracle_hnsw *H = c->b->H;
+  const oracle_points *P = c->b->P;
+  int lc = c->b->lc;
+  unsigned md = H->max_degree[lc];
+  uint32_t pv = c->gkeys[g];
+  long s = c->gstart[g], e = c->gstart[g + 1];
+  uint32_t *row = H->adj[lc] + (size_t)pv * (md + 1);
+  long deg = row[0], nadd = e - s;
+  if (deg + nadd > (long)md) {
+    c->buf[tid] = (idd *)grow(c->buf[tid], &c->cap[tid], deg + nadd, sizeof(idd));
+    idd *cand = c->buf[tid];
+    for (long k = 0; k < deg; k++) {
+      cand[k].id = row[1 + k];
+      cand[k].dist = oracle_distance(row_of(P, row[1 + k]), row_of(P, pv), P->dim, P->dtype,
+                                     P->metric);
+    }
+    for (long k = 0; k < nadd; k++) {
+      cand[deg + k].id = c->gvals[s + k];
+      cand[deg + k].dist = oracle_distance(row_of(P, c->gvals[s + k]), row_of(P, pv), P->dim,
+                                           P->dtype, P->metric);
+    }
+    idd_sort(cand, deg + nadd);
+    row[0] = md;
+    for (unsigned k = 0; k < md; k++) row[1 + k] = cand[k].id;
+  } else {
+    for (long k = 0; k < nadd; k++) row[1 + deg + k] = c->gvals[s + k];
+    row[0] = (uint32_t)(deg + nadd);
+  }
+}
+
+int oracle_build_hnsw(const oracle_hnsw *H, const oracle_points *P, const uint32_t *levels,
+                      unsigned efc, float alpha, float batch_base, uint32_t *entrance_out,
+                      int nthreads) {
+  const uint32_t n = (uint32_t)P->n;
+  if (n == 0) return 0;
+  if (nthreads < 1) nthreads = 1;
+  for (uint32_t i = 0; i < n; i++)
+    if ((int)levels[i] > H->top) return -1;
+  uint32_t entrance = 0; /* node 0 starts the index (:763-774) */
+  uint32_t bb = 0, be = 1;
+  uint32_t size_limit = (uint32_t)((double)n * 0.02); /* n*0.02 (:778) */
+  if (size_limit == 0) size_limit = 1; /* the reference never ends a batch of 0 */
+  hb_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.H = H;
+  c.P = P;
+  c.levels = levels;
+  c.alpha = alpha;
+  c.efc = efc;
+  c.scratch = (bs_scratch *)calloc((size_t)nthreads, sizeof(bs_scratch));
+  atomic_init(&c.failed, 0);
+  size_t cap_batch = 0;
+  hr_ctx rc;
+  memset(&rc, 0, sizeof(rc));
+  rc.b = &c;
+  rc.buf = (idd **)calloc((size_t)nthreads, sizeof(idd *));
+  rc.cap = (long *)calloc((size_t)nthreads, sizeof(long));
+  kv *edges = NULL;
+  size_t cap_edges = 0;
+  while (be < n) { /* :777-792 */
+    bb = be;
+    uint32_t b1 = (uint32_t)ceilf((float)bb * batch_base) + 1;
+    be = n;
+    if (b1 < be) be = b1;
+    if (bb + size_limit < be) be = bb + size_limit;
+    size_t sz = be - bb;
+    if (sz > cap_batch) {
+      cap_batch = sz;
+      c.eps = (uint32_t *)realloc(c.eps, sz * efc * sizeof(uint32_t));
+      c.neps = (int *)realloc(c.neps, sz * sizeof(int));
+      c.nbh = (uint32_t *)realloc(c.nbh, sz * H->max_degree[0] * sizeof(uint32_t));
+      c.nnbh = (int *)realloc(c.nnbh, sz * sizeof(int));
+    }
+    c.bb = bb;
+    c.entrance = entrance;
+    c.level_ep = (int)levels[entrance];
+    parallel_for((long)sz, 8, nthreads, hb_descend_body, &c);
+    for (int lc = c.level_ep; lc >= 0; lc--) { /* :907-985 */
+      c.lc = lc;
+      parallel_for((long)sz, 4, nthreads, hb_level_body, &c);
+      if (atomic_load(&c.failed)) break;
+      unsigned md = H->max_degree[lc];
+      size_t ne = 0;
+      for (size_t i = 0; i < sz; i++) { /* forward edges (:928-933) */
+        uint32_t u = bb + (uint32_t)i;
+        if (lc > (int)levels[u]) continue;
+        uint32_t *row = H->adj[lc] + (size_t)u * (md + 1);
+        row[0] = (uint32_t)c.nnbh[i];
+        for (int t = 0; t < c.nnbh[i]; t++) row[1 + t] = c.nbh[i * H->max_degree[0] + t];
+        ne += (size_t)c.nnbh[i];
+      }
+      if (ne > cap_edges) {
+        cap_edges = ne;
+        edges = (kv *)realloc(edges, ne * sizeof(kv));
+        rc.gkeys = (uint32_t *)realloc(rc.gkeys, ne * sizeof(uint32_t));
+        rc.gvals = (uint32_t *)realloc(rc.gvals, ne * sizeof(uint32_t));
+        rc.gstart = (long *)realloc(rc.gstart, (ne + 1) * sizeof(long));
+      }
+      size_t t = 0;
+      for (size_t i = 0; i < sz; i++) { /* edge_add (pv, pu) in batch order */
+        uint32_t u = bb + (uint32_t)i;
+        if (lc > (int)levels[u]) continue;
+        for (int j = 0; j < c.nnbh[i]; j++) {
+          edges[t].key = c.nbh[i * H->max_degree[0] + j];
+          edges[t].val = u;
+          edges[t].pos = (long)t;
+          t++;
+        }
+      }
+      qsort(edges, ne, sizeof(kv), kv_cmp); /* group_by_key, values in insertion order */
+      long ng = 0;
+      for (size_t j = 0; j < ne; j++) {
+        if (j == 0 || edges[j].key != edges[j - 1].key) {
+          rc.gkeys[ng] = edges[j].key;
+          rc.gstart[ng] = (long)j;
+          ng++;
+        }
+        rc.gvals[j] = edges[j].val;
+      }
+      if (rc.gstart) rc.gstart[ng] = (long)ne;
+      parallel_for(ng, 16, nthreads, hr_body, &rc);
+    }
+    if (atomic_load(&c.failed)) break;
+    uint32_t hi = bb; /* max_element: the first node of the highest level (:987-995) */
+    for (uint32_t u = bb + 1; u < be; u++)
+      if (levels[u] > levels[hi]) hi = u;
+    if (levels[hi] > levels[entrance]) entrance = hi;
+  }
+  *entrance_out = entrance;
+  for (int t = 0; t < nthreads; t++) {
+    bs_scratch_free(&c.scratch[t]);
+    free(rc.buf[t]);
+  }
+  free(c.scratch);
+  free(rc.buf);
+  free(rc.cap);
+  free(c.eps);
+  free(c.neps);
+  free(c.nbh);
+  free(c.nnbh);
+  free(edges);
+  free(rc.gkeys);
+  free(rc.gvals);
+  free(rc.gstart);
+  return atomic_load(&c.failed) ? -1 : 0;
+}

@@ -109,6 +109,38 @@ int oracle_build_vamana(const oracle_points *P, uint32_t *adj, unsigned R, unsig
 int oracle_groundtruth_f64(const oracle_points *P, const void *queries, size_t nq,
                            int k, int nthreads, uint32_t *out_ids, double *out_dists);
 
+/* HNSW layered graph (HNSW.hpp): adj[l] = n x (max_degree[l] + 1) uint32 in
+ * Graph layout for levels l = 0..top (degree 2m at 0, m above; node ids). */
+typedef struct {
+  size_t n;
+  int top;
+  uint32_t *const *adj;
+  const unsigned *max_degree;
+} oracle_hnsw;
+
+/* HNSW::search (HNSW.hpp:1568-1613) per query: ef = 1 descent from the
+ * entrance through levels top..1 (search_layer_to, :1545-1566), layer-0 beam
+ * search with ef and `limit` (<= 0: n), each level's degree limit its
+ * max_degree; the frontier cut after the last entry tied with the k-th
+ * distance.  out_ids/out_dists: nq x ef (node ids); out_count: entries kept;
+ * out_cmps / out_visited summed over the levels.  Returns 0 or -1. */
+int oracle_hnsw_search(const oracle_hnsw *H, const oracle_points *P, const uint32_t *entrance,
+                       int nentrance, const void *queries, size_t nq, int k, int ef, long limit,
+                       int nthreads, uint32_t *out_ids, float *out_dists, int *out_count,
+                       uint64_t *out_cmps, uint32_t *out_visited);
+
+/* HNSW::HNSW (HNSW.hpp:748-807) + insert (:809-996): node i is point i, with
+ * the level levels[i] (the reference draws them at random; here they are
+ * inputs), batches [b, min(n, ceil(b * batch_base) + 1, b + 0.02 n)), per
+ * batch a beam-1 descent, then for each level from the entrance's down: a
+ * search with efc from the previous level's results, prune_heuristic to the
+ * level's max degree, forward lists, and reverse edges grouped by target in
+ * insertion order (appended, or the nearest max_degree by (dist, id)).  H->adj
+ * must be zeroed with top >= every level.  Writes the final entrance node.  */
+int oracle_build_hnsw(const oracle_hnsw *H, const oracle_points *P, const uint32_t *levels,
+                      unsigned efc, float alpha, float batch_base, uint32_t *entrance_out,
+                      int nthreads);
+
 /* The (dist, id) sort used everywhere above, exposed for its property test. */
 void oracle_sort_pairs(uint32_t *ids, float *dists, long n);
 

@@ -33,6 +33,11 @@ class _Graph(C.Structure):
     _fields_ = [("adj", C.c_void_p), ("n", C.c_size_t), ("max_degree", C.c_uint)]
 
 
+class _HNSW(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("top", C.c_int), ("adj", C.c_void_p),
+                ("max_degree", C.c_void_p)]
+
+
 class _QP(C.Structure):
     _fields_ = [("k", C.c_long), ("beam_size", C.c_long), ("limit", C.c_long),
                 ("degree_limit", C.c_long), ("rerank_factor", C.c_double),
@@ -91,6 +96,13 @@ def lib():
         L.oracle_groundtruth_f64.argtypes = [C.POINTER(_Points), vp, sz, C.c_int, C.c_int, u32p,
                                              vp]
         L.oracle_sort_pairs.argtypes = [u32p, f32p, C.c_long]
+        L.oracle_hnsw_search.restype = C.c_int
+        L.oracle_hnsw_search.argtypes = [C.POINTER(_HNSW), C.POINTER(_Points), u32p, C.c_int, vp,
+                                         sz, C.c_int, C.c_int, C.c_long, C.c_int, u32p, f32p, vp,
+                                         vp, u32p]
+        L.oracle_build_hnsw.restype = C.c_int
+        L.oracle_build_hnsw.argtypes = [C.POINTER(_HNSW), C.POINTER(_Points), u32p, C.c_uint,
+                                        C.c_float, C.c_float, u32p, C.c_int]
         _lib = L
     return _lib
 
@@ -254,3 +266,56 @@ def groundtruth_f64(X, metric, Q, k, nthreads=None):
     if rc != 0:
         raise ValueError("bad k")
     return ids, d
+
+
+def _hnsw(levels_adj):
+    """oracle_hnsw over a list of per-level (n, deg + 1) uint32 arrays; the
+    second value keeps the pointer tables alive."""
+    for a in levels_adj:
+        assert a.dtype == np.uint32 and a.flags.c_contiguous
+    ptrs = (C.c_void_p * len(levels_adj))(*[a.ctypes.data for a in levels_adj])
+    degs = (C.c_uint * len(levels_adj))(*[a.shape[1] - 1 for a in levels_adj])
+    H = _HNSW(levels_adj[0].shape[0], len(levels_adj) - 1, C.cast(ptrs, C.c_void_p),
+              C.cast(degs, C.c_void_p))
+    return H, (ptrs, degs, levels_adj)
+
+
+def hnsw_search(levels_adj, X, metric, Q, k, ef, entrance, limit=0, nthreads=None):
+    """HNSW::search (HNSW.hpp:1568-1613).  Returns (ids, dists) of shape
+    (nq, ef) holding `count[q]` node ids each, count, cmps, visited."""
+    Q = np.ascontiguousarray(Q, dtype=X.dtype)
+    nq = Q.shape[0]
+    ids = np.zeros((nq, ef), np.uint32)
+    d = np.zeros((nq, ef), np.float32)
+    cnt = np.zeros(nq, np.int32)
+    cmps = np.zeros(nq, np.uint64)
+    vis = np.zeros(nq, np.uint32)
+    ent = np.ascontiguousarray(entrance, dtype=np.uint32)
+    H, keep = _hnsw(levels_adj)
+    P = _points(X, metric)
+    rc = lib().oracle_hnsw_search(C.byref(H), C.byref(P), _ptr(ent), len(ent), _ptr(Q), nq, k, ef,
+                                  limit, nthreads or os.cpu_count() or 1, _ptr(ids), _ptr(d),
+                                  _ptr(cnt), _ptr(cmps), _ptr(vis))
+    if rc != 0:
+        raise ValueError("oracle hnsw search failed")
+    return ids, d, cnt, cmps, vis
+
+
+def build_hnsw(X, metric, m, efc, alpha, levels, batch_base=2.0, nthreads=None):
+    """HNSW::HNSW + insert (HNSW.hpp:748-996) with the given node levels.
+    Returns (per-level adjacency list, entrance node)."""
+    n = X.shape[0]
+    levels = np.ascontiguousarray(levels, dtype=np.uint32)
+    top = int(levels.max()) if n else 0
+    adj = [np.zeros((n, (2 * m if l == 0 else m) + 1), np.uint32) for l in range(top + 1)]
+    H, keep = _hnsw(adj)
+    P = _points(X, metric)
+    ent = np.zeros(1, np.uint32)
+    rc = lib().oracle_build_hnsw(C.byref(H), C.byref(P), _ptr(levels), efc, alpha, batch_base,
+                                 _ptr(ent), nthreads or os.cpu_count() or 1)
+    if rc != 0:
+        raise ValueError("oracle hnsw build failed")
+    for a in adj:  # slots past each degree are never read; keep them zero
+        tail = a[:, 1:]
+        tail[np.arange(a.shape[1] - 1)[None, :] >= a[:, :1].astype(np.int64)] = 0
+    return adj, int(ent[0])

@@ -20,7 +20,10 @@ EXPORTS = ["pann_open", "pann_build_store", "pann_get_fp8_scales", "pann_search"
            "pann_search_host", "pann_search_status", "pann_last_kernel_ms",
            "pann_search_visited", "pann_robust_prune_batch", "pann_robust_prune_groups",
            "pann_update_neighbors",
-           "pann_download_graph", "pann_close", "pann_last_error", "pann_device_info"]
+           "pann_download_graph", "pann_close", "pann_last_error", "pann_device_info",
+           "pann_hnsw_set_level", "pann_hnsw_search_layer", "pann_hnsw_search",
+           "pann_hnsw_select_neighbors", "pann_hnsw_add_reverse", "pann_hnsw_set_neighbors",
+           "pann_hnsw_download_level"]
 
 
 class PannError(RuntimeError):
@@ -77,6 +80,19 @@ def lib():
                                      C.c_uint32, C.c_int32, vp, vp, vp],
         "pann_update_neighbors": [vp, vp, C.c_uint32, vp, vp, C.c_uint32, vp],
         "pann_download_graph": [vp, vp],
+        "pann_hnsw_set_level": [vp, C.c_uint32, C.c_uint32, vp],
+        "pann_hnsw_search_layer": [vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.c_int64, vp, vp, C.c_uint32, C.POINTER(SearchOpts), vp, vp,
+                                   C.c_uint32, vp, vp, vp, vp],
+        "pann_hnsw_search": [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int64, vp,
+                             C.c_uint32, C.c_uint32, vp, C.POINTER(SearchOpts), vp, vp,
+                             C.c_uint32, vp, vp, vp, vp],
+        "pann_hnsw_select_neighbors": [vp, C.c_uint32, vp, C.c_uint32, vp, vp, vp, C.c_uint32,
+                                       C.c_float, C.c_uint32, vp, vp, vp],
+        "pann_hnsw_add_reverse": [vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_uint32,
+                                  C.c_uint32, vp, vp, vp],
+        "pann_hnsw_set_neighbors": [vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_uint32, vp],
+        "pann_hnsw_download_level": [vp, C.c_uint32, vp],
         "pann_close": [vp],
         "pann_device_info": [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                              C.POINTER(C.c_int64)],

@@ -225,8 +225,10 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   const int dl = a.degree_limit < (int)a.max_degree ? a.degree_limit : (int)a.max_degree;
   const uint32_t hlog2 = a.hslots_log2;
   const uint32_t hslots = 1u << hlog2;
-  if (B < 1 || B > (int)a.fcap || k < 0 || k > B || k > (int)a.out_stride ||
-      (int)a.nstarts > B) {  // per-query values outside what this launch was sized for
+  const uint32_t* sp = a.starts_pq ? a.starts_pq + (uint64_t)q * a.starts_stride : a.starts;
+  const int ns = a.starts_pq ? (a.nstarts_pq ? (int)a.nstarts_pq[q] : 1) : (int)a.nstarts;
+  if (B < 1 || B > (int)a.fcap || k < 0 || k > B || k > (int)a.out_stride || ns < 1 ||
+      ns > B) {  // per-query values outside what this launch was sized for
     if (lane == 0) atomicOr(a.status, PANN_ST_BAD_PARAMS);
     return;
   }
@@ -251,27 +253,31 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   uint32_t status = 0;
 
   // ---- starting points (beamSearch.h:71-93) ----
-  const int ns = (int)a.nstarts;
-  if (lane < ns) {
-    uint32_t s = a.starts[lane];
-    L.NL[lane] = s;
-    if constexpr (EX) lds_insert(L.H, hlog2, s);
+  // evaluated 64 at a time into F1, then rank-sorted into F0 (equal keys keep
+  // their order, so repeated starts stay repeated as in the reference)
+  for (int t = 0; t < ns; t += PANN_WAVE) {
+    const int m = ns - t < PANN_WAVE ? ns - t : PANN_WAVE;
+    if (lane < m) {
+      const uint32_t s = sp[t + lane];
+      L.NL[lane] = s;
+      if constexpr (EX) lds_insert(L.H, hlog2, s);
+    }
+    wave_lds_sync();
+    eval_rows<ST, MT, NCH, SEQ>(D, a, L, m, lane);
+    wave_lds_sync();
+    if (lane < m) L.F1[t + lane] = make_key(L.DL[lane], L.NL[lane]);
+    wave_lds_sync();
   }
   lds_count = (uint32_t)ns;
-  wave_lds_sync();
-  eval_rows<ST, MT, NCH, SEQ>(D, a, L, ns, lane);
-  wave_lds_sync();
-  uint64_t skey = 0;
-  if (lane < ns) {
-    skey = make_key(L.DL[lane], L.NL[lane]);
-    L.C[lane] = skey;
-  }
-  wave_lds_sync();
   int cur_buf = 0;
-  if (lane < ns) {
+  for (int j = lane; j < ns; j += PANN_WAVE) {
+    const uint64_t x = L.F1[j];
     int r = 0;
-    for (int i = 0; i < ns; i++) r += (L.C[i] < skey);
-    L.F0[r] = skey;
+    for (int i = 0; i < ns; i++) {
+      const uint64_t y = L.F1[i];
+      r += (y < x) || (y == x && i < j);
+    }
+    L.F0[r] = x;
     L.FV0[r] = 0;
   }
   wave_lds_sync();
@@ -511,8 +517,8 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     }
   }
   if (lane == 0) {
-    if (a.out_visited) a.out_visited[q] = nvis;
-    if (a.out_cmps) a.out_cmps[q] = cmps;
+    if (a.out_visited) a.out_visited[q] = (a.accumulate ? a.out_visited[q] : 0u) + nvis;
+    if (a.out_cmps) a.out_cmps[q] = (a.accumulate ? a.out_cmps[q] : 0u) + cmps;
     if (a.out_nfront) a.out_nfront[q] = (uint32_t)f;
     if (status) atomicOr(a.status, status);
   }

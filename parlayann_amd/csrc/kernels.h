@@ -33,9 +33,16 @@ struct SearchArgs {
   const char* fqueries;
   uint64_t fq_stride;
   uint32_t nq;
-  // start points (device), 1 <= nstarts <= min(beam, 64), unique
+  // start points (device) shared by all queries, 1 <= nstarts <= beam ...
   const uint32_t* starts;
   uint32_t nstarts;
+  // ... or per query: starts_pq[q * starts_stride .. + nstarts_pq[q]) (nstarts_pq
+  // NULL: one start each), 1 <= count <= the query's beam
+  const uint32_t* starts_pq;
+  const uint32_t* nstarts_pq;
+  uint32_t starts_stride;
+  // 1: add this launch's visits and evaluations to out_visited / out_cmps
+  uint32_t accumulate;
   // QueryParams (types.h:233-268); per-query k / beam override the uniform ones
   int32_t k, beam;
   const int32_t* k_pq;
@@ -114,6 +121,10 @@ hipError_t launch_robust_prune(const PruneArgs& a, int dtype, int metric, bool k
 hipError_t launch_sort_visited(const uint64_t* vis, uint32_t nq, uint32_t cap,
                                const uint32_t* counts, uint32_t* ids, float* dists,
                                hipStream_t s);
+// HNSW::search's final sort-and-cut with ties (HNSW.hpp:1590-1596) and the
+// node -> point id map
+hipError_t launch_hnsw_cut(uint32_t* ids, float* dists, uint32_t stride, uint32_t* count,
+                           uint32_t nq, uint32_t k, const uint32_t* node_ids, hipStream_t s);
 hipError_t launch_update_neighbors(uint32_t* nbrs, uint32_t nbr_stride, uint32_t max_degree,
                                    const uint32_t* points, uint32_t np, const uint32_t* lists,
                                    const uint32_t* counts, uint32_t R, hipStream_t s);

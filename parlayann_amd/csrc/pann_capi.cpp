@@ -47,6 +47,38 @@ uint32_t ceil_log2(uint64_t x) {
 }
 size_t elem_bytes(int dt) { return dt == PANN_DT_F32 ? 4 : 1; }
 
+// Sentinel-padded device copy of a host graph in Graph layout (graph.h:46-64).
+pann_status upload_graph(uint32_t n, uint32_t md, uint32_t stride, const uint32_t* graph,
+                         uint32_t* d_nbrs) {
+  const uint64_t chunk = 1u << 20;
+  std::vector<uint32_t> buf;
+  for (uint64_t r0 = 0; r0 < n; r0 += chunk) {
+    uint64_t r1 = std::min<uint64_t>(n, r0 + chunk);
+    buf.assign((r1 - r0) * stride, PANN_SENTINEL);
+    if (graph)
+      for (uint64_t i = r0; i < r1; i++) {
+        const uint32_t* row = graph + i * (md + 1);
+        std::memcpy(&buf[(i - r0) * stride], row + 1, row[0] * 4);
+      }
+    HIP_TRY(hipMemcpy(d_nbrs + r0 * stride, buf.data(), buf.size() * 4, hipMemcpyHostToDevice),
+            "hipMemcpy(graph)");
+  }
+  return PANN_OK;
+}
+
+pann_status check_graph(uint32_t n, uint32_t md, const uint32_t* graph) {
+  // graph.h:60-63 / :240-243 abort on bad degrees and ids; here they are errors
+  for (uint64_t i = 0; i < n; i++) {
+    const uint32_t* row = graph + i * (md + 1);
+    if (row[0] > md) return fail(PANN_ERR_ARG, "graph row " + std::to_string(i) + " exceeds max degree");
+    for (uint32_t j = 0; j < row[0]; j++)
+      if (row[1 + j] >= n)
+        return fail(PANN_ERR_ARG, "graph row " + std::to_string(i) + " has an out-of-range id");
+  }
+  return PANN_OK;
+}
+
+
 }  // namespace
 
 struct pann_index_s {
@@ -64,6 +96,16 @@ struct pann_index_s {
   uint32_t* d_absmax = nullptr;
   uint32_t* d_nbrs = nullptr;
   uint32_t nbr_stride = 0;
+  // graph per level of a layered (HNSW) index; level 0 is d_nbrs above
+  struct Level {
+    uint32_t* d_nbrs = nullptr;
+    uint32_t max_degree = 0, nbr_stride = 0;
+  };
+  std::vector<Level> levels;
+  uint32_t* d_eps = nullptr;  // per-query entry points of the HNSW descent
+  float* d_epsd = nullptr;
+  uint32_t* d_epsn = nullptr;
+  size_t eps_cap = 0;
   uint32_t* d_counter = nullptr;
   uint32_t* d_status = nullptr;
   uint32_t* d_starts = nullptr;
@@ -148,13 +190,9 @@ pann_status pann_open(const pann_index_desc* desc, pann_index_t* out) {
     return fail(PANN_ERR_UNSUPPORTED, "max_degree > 64 is not supported by this build");
   if (!desc->points || !desc->graph) return fail(PANN_ERR_ARG, "null points or graph");
   const uint32_t n = desc->n, md = desc->max_degree;
-  // graph.h:60-63 / :240-243 abort on bad degrees and ids; here they are errors
-  for (uint64_t i = 0; i < n; i++) {
-    const uint32_t* row = desc->graph + i * (md + 1);
-    if (row[0] > md) return fail(PANN_ERR_ARG, "graph row " + std::to_string(i) + " exceeds max degree");
-    for (uint32_t j = 0; j < row[0]; j++)
-      if (row[1 + j] >= n)
-        return fail(PANN_ERR_ARG, "graph row " + std::to_string(i) + " has an out-of-range id");
+  {
+    pann_status cst = check_graph(n, md, desc->graph);
+    if (cst != PANN_OK) return cst;
   }
   HIP_TRY(hipSetDevice(desc->device), "hipSetDevice");
   auto idx = new pann_index_s();
@@ -205,21 +243,12 @@ pann_status pann_open(const pann_index_desc* desc, pann_index_t* out) {
   }
   e = hipMalloc(&idx->d_nbrs, (size_t)n * idx->nbr_stride * 4);
   if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(graph)"));
-  {
-    const uint64_t chunk = 1u << 20;
-    std::vector<uint32_t> buf;
-    for (uint64_t r0 = 0; r0 < n; r0 += chunk) {
-      uint64_t r1 = std::min<uint64_t>(n, r0 + chunk);
-      buf.assign((r1 - r0) * idx->nbr_stride, PANN_SENTINEL);
-      for (uint64_t i = r0; i < r1; i++) {
-        const uint32_t* row = desc->graph + i * (md + 1);
-        std::memcpy(&buf[(i - r0) * idx->nbr_stride], row + 1, row[0] * 4);
-      }
-      e = hipMemcpy(idx->d_nbrs + r0 * idx->nbr_stride, buf.data(), buf.size() * 4,
-                    hipMemcpyHostToDevice);
-      if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemcpy(graph)"));
-    }
-  }
+  st = upload_graph(n, md, idx->nbr_stride, desc->graph, idx->d_nbrs);
+  if (st != PANN_OK) return cleanup(st);
+  idx->levels.resize(1);
+  idx->levels[0].d_nbrs = idx->d_nbrs;
+  idx->levels[0].max_degree = md;
+  idx->levels[0].nbr_stride = idx->nbr_stride;
   e = hipMalloc(&idx->d_counter, 256);
   if (e == hipSuccess) e = hipMalloc(&idx->d_status, 256);
   if (e == hipSuccess) e = hipMalloc(&idx->d_starts, 64 * 4);
@@ -282,11 +311,24 @@ pann_status pann_get_fp8_scales(pann_index_t idx, float* scales_host) {
 
 namespace {
 
-// Fills the launch block shared by pann_search and pann_search_visited.
-pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint32_t* starts,
-                           uint32_t nstarts, const pann_search_opts* opts, uint32_t nq,
-                           pann::SearchArgs& a, int& store_kind, bool& sequential,
-                           uint32_t& grid, hipStream_t stream) {
+// Per-query start sets (device): query q starts from starts[q * stride ..
+// + counts[q]) (counts NULL: one start each).  Counts are checked by the kernel.
+struct StartSets {
+  const uint32_t* starts = nullptr;
+  const uint32_t* counts = nullptr;
+  uint32_t stride = 0;
+};
+
+// Fills the launch block shared by every search entry point: graph `level`,
+// host start points (starts, nstarts) or per-query device start sets (ss).
+pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams* qp,
+                           const uint32_t* starts, uint32_t nstarts, const StartSets* ss,
+                           const pann_search_opts* opts, uint32_t nq, pann::SearchArgs& a,
+                           int& store_kind, bool& sequential, uint32_t& grid,
+                           hipStream_t stream) {
+  if (level >= idx->levels.size() || !idx->levels[level].d_nbrs)
+    return fail(PANN_ERR_ARG, "graph level " + std::to_string(level) + " is not set");
+  const auto& G = idx->levels[level];
   const int64_t B = qp->beam_size;
   if (qp->k < 0 || B < 1) return fail(PANN_ERR_ARG, "beam_size must be >= 1 and k >= 0");
   if (qp->k > B)  // beamSearch.h:612-616
@@ -296,11 +338,11 @@ pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint3
   if (qp->degree_limit < 1) return fail(PANN_ERR_ARG, "degree_limit must be >= 1");
   if (!(qp->batch_factor >= 0.0) || !(qp->rerank_factor >= 0.0))
     return fail(PANN_ERR_ARG, "batch_factor and rerank_factor must be >= 0");
-  if (nstarts == 0)  // beamSearch.h:51-53
-    return fail(PANN_ERR_ARG, "beam search expects at least one start point");
-  if (nstarts > B || nstarts > 64)  // :54-57
-    return fail(PANN_ERR_ARG, "beam search has more starting points than beam size (or > 64)");
-  {
+  if (!ss) {
+    if (nstarts == 0)  // beamSearch.h:51-53
+      return fail(PANN_ERR_ARG, "beam search expects at least one start point");
+    if (nstarts > B || nstarts > 64)  // :54-57
+      return fail(PANN_ERR_ARG, "beam search has more starting points than beam size (or > 64)");
     std::unordered_set<uint32_t> seen;
     for (uint32_t i = 0; i < nstarts; i++) {
       if (starts[i] >= idx->n) return fail(PANN_ERR_ARG, "start point out of range");
@@ -345,21 +387,26 @@ pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint3
   a.scales = idx->d_scales;
   a.full = o.rerank ? idx->d_pts : nullptr;
   a.full_stride = idx->pts_stride;
-  a.nbrs = idx->d_nbrs;
-  a.nbr_stride = idx->nbr_stride;
-  a.max_degree = idx->max_degree;
+  a.nbrs = G.d_nbrs;
+  a.nbr_stride = G.nbr_stride;
+  a.max_degree = G.max_degree;
   a.n = idx->n;
   a.dim = idx->dim;
   a.k = (int32_t)qp->k;
   a.beam = (int32_t)B;
   a.limit = qp->limit;
-  a.degree_limit = (int32_t)std::min<int64_t>(qp->degree_limit, idx->max_degree);
+  a.degree_limit = (int32_t)std::min<int64_t>(qp->degree_limit, G.max_degree);
   a.batch_factor = qp->batch_factor;
   a.rerank_factor = qp->rerank_factor;
   a.status = idx->d_status;
   a.work_counter = idx->d_counter;
-  a.nstarts = nstarts;
+  a.nstarts = ss ? 0u : nstarts;
   a.starts = idx->d_starts;
+  if (ss) {
+    a.starts_pq = ss->starts;
+    a.nstarts_pq = ss->counts;
+    a.starts_stride = ss->stride;
+  }
 
   // LDS layout
   const int64_t thr = (int64_t)std::ceil(qp->batch_factor * (double)B);
@@ -405,21 +452,24 @@ pann_status prepare_search(pann_index_t idx, const pann_qparams* qp, const uint3
     a.ghash_tag_base = idx->ghash_tag;
     idx->ghash_tag += nq;
   }
-  HIP_TRY(hipMemcpyAsync(idx->d_starts, starts, nstarts * 4, hipMemcpyHostToDevice, stream),
-          "hipMemcpyAsync(starts)");
+  if (!ss)
+    HIP_TRY(hipMemcpyAsync(idx->d_starts, starts, nstarts * 4, hipMemcpyHostToDevice, stream),
+            "hipMemcpyAsync(starts)");
   hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(1), 0, stream, idx->d_status);
   HIP_TRY(hipGetLastError(), "status reset");
   return PANN_OK;
 }
 
-pann_status check_status(pann_index_t idx, hipStream_t stream) {
+// allow_short: a frontier shorter than k is a result, not an error (HNSW
+// layers return whatever frontier they reach)
+pann_status check_status(pann_index_t idx, hipStream_t stream, bool allow_short = false) {
   HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
   uint32_t st = 0;
   HIP_TRY(hipMemcpy(&st, idx->d_status, 4, hipMemcpyDeviceToHost), "hipMemcpy(status)");
   if (st & pann::PANN_ST_BAD_PARAMS)  // beamSearch.h:612-616 (k > beam) per query
     return fail(PANN_ERR_ARG, "a per-query k or beam is invalid (k > beam, beam > qp->beam_size, "
                               "or k > out_stride)");
-  if (st & pann::PANN_ST_SHORT_BEAM)  // beamSearch.h:557-562
+  if ((st & pann::PANN_ST_SHORT_BEAM) && !allow_short)  // beamSearch.h:557-562
     return fail(PANN_ERR_SHORT_BEAM, "beam search returned fewer than k elements");
   if (st & pann::PANN_ST_HASH_CAPACITY)
     return fail(PANN_ERR_CAPACITY, "a query outgrew the visited-set capacity");
@@ -447,8 +497,8 @@ pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
   int store_kind;
   bool sequential;
   uint32_t grid;
-  pann_status st = prepare_search(idx, qp, starts, nstarts, opts, nq, a, store_kind, sequential,
-                                  grid, stream);
+  pann_status st = prepare_search(idx, 0, qp, starts, nstarts, nullptr, opts, nq, a, store_kind,
+                                  sequential, grid, stream);
   if (st != PANN_OK) return st;
   const bool f32_queries = idx->dtype == PANN_DT_F32;
   a.queries = (const char*)queries;
@@ -554,8 +604,8 @@ pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint
   int store_kind;
   bool sequential;
   uint32_t grid;
-  pann_status st = prepare_search(idx, &qp, &start, 1, &o, nq, a, store_kind, sequential, grid,
-                                  stream);
+  pann_status st = prepare_search(idx, 0, &qp, &start, 1, nullptr, &o, nq, a, store_kind,
+                                  sequential, grid, stream);
   if (st != PANN_OK) return st;
   st = ensure_vis(idx, (size_t)nq * vis_cap);
   if (st != PANN_OK) return st;
@@ -582,20 +632,24 @@ pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint
 
 namespace {
 
-pann_status run_prune(pann_index_t idx, const uint32_t* pts, uint32_t np, const uint32_t* cand_ids,
-                      const float* cand_dists, const uint32_t* cand_count, uint32_t cand_stride,
-                      const uint32_t* offsets, uint32_t max_count, double alpha, uint32_t R,
-                      int32_t add, uint32_t* out_ids, uint32_t* out_count, hipStream_t stream) {
-  if (R == 0 || R > idx->max_degree) return fail(PANN_ERR_ARG, "R must be in [1, max_degree]");
-  if (add < 0 || add > 3) return fail(PANN_ERR_ARG, "add must be 0, 1, 2 or 3");
+pann_status run_prune(pann_index_t idx, uint32_t level, const uint32_t* pts, uint32_t np,
+                      const uint32_t* cand_ids, const float* cand_dists,
+                      const uint32_t* cand_count, uint32_t cand_stride, const uint32_t* offsets,
+                      uint32_t max_count, double alpha, uint32_t R, int32_t add,
+                      uint32_t* out_ids, uint32_t* out_count, hipStream_t stream) {
+  if (level >= idx->levels.size() || !idx->levels[level].d_nbrs)
+    return fail(PANN_ERR_ARG, "graph level " + std::to_string(level) + " is not set");
+  const auto& G = idx->levels[level];
+  if (R == 0 || R > G.max_degree) return fail(PANN_ERR_ARG, "R must be in [1, max_degree]");
+  if (add < 0 || add > 5) return fail(PANN_ERR_ARG, "unknown prune mode");
   pann::PruneArgs a;
   std::memset(&a, 0, sizeof(a));
   a.pts = idx->d_pts;
   a.stride = idx->pts_stride;
   a.dim = idx->dim;
-  a.nbrs = idx->d_nbrs;
-  a.nbr_stride = idx->nbr_stride;
-  a.max_degree = idx->max_degree;
+  a.nbrs = G.d_nbrs;
+  a.nbr_stride = G.nbr_stride;
+  a.max_degree = G.max_degree;
   a.points = pts;
   a.np = np;
   a.cand_ids = cand_ids;
@@ -609,7 +663,7 @@ pann_status run_prune(pann_index_t idx, const uint32_t* pts, uint32_t np, const 
   a.out_ids = out_ids;
   a.out_count = out_count;
   uint64_t cap = 2;
-  while (cap < (uint64_t)max_count + idx->max_degree) cap <<= 1;
+  while (cap < (uint64_t)max_count + G.max_degree) cap <<= 1;
   if (cap > (1u << 24)) return fail(PANN_ERR_UNSUPPORTED, "candidate set too large");
   a.sort_cap = (uint32_t)cap;
   const int dk = idx->dtype == PANN_DT_F32 ? pann::ST_F32
@@ -651,7 +705,8 @@ pann_status pann_robust_prune_batch(pann_index_t idx, const uint32_t* pts, uint3
   if (np == 0) return PANN_OK;
   std::lock_guard<std::mutex> lk(idx->mu);
   HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
-  return run_prune(idx, pts, np, cand_ids, cand_dists, cand_count, cand_stride, nullptr,
+  if (add > 3) return fail(PANN_ERR_ARG, "add must be 0, 1, 2 or 3");
+  return run_prune(idx, 0, pts, np, cand_ids, cand_dists, cand_count, cand_stride, nullptr,
                    cand_stride, alpha, R, add, out_ids, out_count, (hipStream_t)stream);
 }
 
@@ -664,40 +719,42 @@ pann_status pann_robust_prune_groups(pann_index_t idx, const uint32_t* pts, uint
   if (np == 0) return PANN_OK;
   std::lock_guard<std::mutex> lk(idx->mu);
   HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
-  return run_prune(idx, pts, np, cand_ids, nullptr, nullptr, 0, offsets, max_count, alpha, R,
+  if (add > 3) return fail(PANN_ERR_ARG, "add must be 0, 1, 2 or 3");
+  return run_prune(idx, 0, pts, np, cand_ids, nullptr, nullptr, 0, offsets, max_count, alpha, R,
                    add, out_ids, out_count, (hipStream_t)stream);
 }
 
-pann_status pann_update_neighbors(pann_index_t idx, const uint32_t* pts, uint32_t np,
-                                  const uint32_t* lists, const uint32_t* counts, uint32_t R,
-                                  void* stream) {
-  if (!idx || !pts || !lists || !counts) return fail(PANN_ERR_ARG, "null argument");
-  if (np == 0) return PANN_OK;
-  if (R == 0 || R > idx->max_degree) return fail(PANN_ERR_ARG, "R must be in [1, max_degree]");
-  std::lock_guard<std::mutex> lk(idx->mu);
-  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
-  HIP_TRY(pann::launch_update_neighbors(idx->d_nbrs, idx->nbr_stride, idx->max_degree, pts, np,
-                                        lists, counts, R, (hipStream_t)stream),
+namespace {
+
+pann_status update_level(pann_index_t idx, uint32_t level, const uint32_t* pts, uint32_t np,
+                         const uint32_t* lists, const uint32_t* counts, uint32_t R,
+                         hipStream_t stream) {
+  if (level >= idx->levels.size() || !idx->levels[level].d_nbrs)
+    return fail(PANN_ERR_ARG, "graph level " + std::to_string(level) + " is not set");
+  const auto& G = idx->levels[level];
+  if (R == 0 || R > G.max_degree) return fail(PANN_ERR_ARG, "R must be in [1, max_degree]");
+  HIP_TRY(pann::launch_update_neighbors(G.d_nbrs, G.nbr_stride, G.max_degree, pts, np, lists,
+                                        counts, R, stream),
           "update neighbours launch");
-  HIP_TRY(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
   return PANN_OK;
 }
 
-pann_status pann_download_graph(pann_index_t idx, uint32_t* graph_host) {
-  if (!idx || !graph_host) return fail(PANN_ERR_ARG, "null argument");
-  std::lock_guard<std::mutex> lk(idx->mu);
-  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+pann_status download_level(pann_index_t idx, uint32_t level, uint32_t* graph_host) {
+  if (level >= idx->levels.size() || !idx->levels[level].d_nbrs)
+    return fail(PANN_ERR_ARG, "graph level " + std::to_string(level) + " is not set");
+  const auto& G = idx->levels[level];
   const uint64_t chunk = 1u << 20;
   std::vector<uint32_t> buf;
-  const uint32_t md = idx->max_degree;
+  const uint32_t md = G.max_degree;
   for (uint64_t r0 = 0; r0 < idx->n; r0 += chunk) {
     uint64_t r1 = std::min<uint64_t>(idx->n, r0 + chunk);
-    buf.resize((r1 - r0) * idx->nbr_stride);
-    HIP_TRY(hipMemcpy(buf.data(), idx->d_nbrs + r0 * idx->nbr_stride, buf.size() * 4,
+    buf.resize((r1 - r0) * G.nbr_stride);
+    HIP_TRY(hipMemcpy(buf.data(), G.d_nbrs + r0 * G.nbr_stride, buf.size() * 4,
                       hipMemcpyDeviceToHost),
             "hipMemcpy(graph)");
     for (uint64_t i = r0; i < r1; i++) {
-      const uint32_t* src = &buf[(i - r0) * idx->nbr_stride];
+      const uint32_t* src = &buf[(i - r0) * G.nbr_stride];
       uint32_t* dst = graph_host + i * (md + 1);
       uint32_t d = 0;
       while (d < md && src[d] != PANN_SENTINEL) d++;
@@ -709,6 +766,239 @@ pann_status pann_download_graph(pann_index_t idx, uint32_t* graph_host) {
   return PANN_OK;
 }
 
+}  // namespace
+
+pann_status pann_update_neighbors(pann_index_t idx, const uint32_t* pts, uint32_t np,
+                                  const uint32_t* lists, const uint32_t* counts, uint32_t R,
+                                  void* stream) {
+  if (!idx || !pts || !lists || !counts) return fail(PANN_ERR_ARG, "null argument");
+  if (np == 0) return PANN_OK;
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  return update_level(idx, 0, pts, np, lists, counts, R, (hipStream_t)stream);
+}
+
+pann_status pann_download_graph(pann_index_t idx, uint32_t* graph_host) {
+  if (!idx || !graph_host) return fail(PANN_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  return download_level(idx, 0, graph_host);
+}
+
+/* ------------------------------------------------------------------------ */
+/* HNSW (algorithms/HNSW/HNSW.hpp)                                           */
+/* ------------------------------------------------------------------------ */
+
+pann_status pann_hnsw_set_level(pann_index_t idx, uint32_t level, uint32_t max_degree,
+                                const uint32_t* graph_host) {
+  if (!idx) return fail(PANN_ERR_ARG, "null index");
+  if (level == 0) return fail(PANN_ERR_ARG, "level 0 is the graph given to pann_open");
+  if (level > 64) return fail(PANN_ERR_ARG, "level > 64");
+  if (max_degree == 0 || max_degree > 64)
+    return fail(PANN_ERR_UNSUPPORTED, "level max_degree must be in [1, 64]");
+  if (graph_host) {
+    pann_status cst = check_graph(idx->n, max_degree, graph_host);
+    if (cst != PANN_OK) return cst;
+  }
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  if (idx->levels.size() <= level) idx->levels.resize(level + 1);
+  auto& G = idx->levels[level];
+  if (G.d_nbrs) (void)hipFree(G.d_nbrs);
+  G = pann_index_s::Level{};
+  const uint32_t stride = (uint32_t)round_up(max_degree, 16);
+  uint32_t* d = nullptr;
+  HIP_TRY(hipMalloc(&d, (size_t)idx->n * stride * 4), "hipMalloc(level graph)");
+  pann_status st = upload_graph(idx->n, max_degree, stride, graph_host, d);
+  if (st != PANN_OK) {
+    (void)hipFree(d);
+    return st;
+  }
+  G.d_nbrs = d;
+  G.max_degree = max_degree;
+  G.nbr_stride = stride;
+  return PANN_OK;
+}
+
+pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void* queries,
+                                   const uint32_t* self_ids, uint32_t nq, uint32_t ef,
+                                   uint32_t degree_limit, int64_t limit, const uint32_t* starts,
+                                   const uint32_t* nstarts, uint32_t starts_stride,
+                                   const pann_search_opts* opts, uint32_t* out_ids,
+                                   float* out_dists, uint32_t out_stride, uint32_t* out_count,
+                                   uint32_t* out_visited, uint32_t* out_cmps, void* stream_) {
+  if (!idx || !starts || !out_ids || !out_dists || !out_count)
+    return fail(PANN_ERR_ARG, "null argument");
+  if ((queries == nullptr) == (self_ids == nullptr))
+    return fail(PANN_ERR_ARG, "give exactly one of queries and self_ids");
+  if (nq == 0) return PANN_OK;
+  if (ef == 0 || out_stride == 0) return fail(PANN_ERR_ARG, "ef and out_stride must be >= 1");
+  if (starts_stride == 0 && nstarts) return fail(PANN_ERR_ARG, "starts_stride must be >= 1");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  hipStream_t stream = (hipStream_t)stream_;
+  // the intended QueryParams(ef, ef, limit, threshold_m(l)) of search_layer
+  // (HNSW.hpp:1096); the reference's literal 1.35 lands in `limit`
+  pann_qparams qp{(int64_t)std::min(ef, out_stride), (int64_t)ef,
+                  limit > 0 ? limit : (int64_t)idx->n, (int64_t)degree_limit, 100.0, 0.125};
+  StartSets ss{starts, nstarts, nstarts ? starts_stride : 1u};
+  pann::SearchArgs a;
+  int store_kind;
+  bool sequential;
+  uint32_t grid;
+  pann_status st = prepare_search(idx, level, &qp, nullptr, 0, &ss, opts, nq, a, store_kind,
+                                  sequential, grid, stream);
+  if (st != PANN_OK) return st;
+  if (self_ids) {  // query q is point self_ids[q]; same_as skips it
+    a.queries = idx->d_pts;
+    a.q_stride = idx->pts_stride;
+    a.fqueries = idx->d_pts;
+    a.fq_stride = idx->pts_stride;
+    a.self_ids = self_ids;
+  } else {
+    const bool f32_queries = idx->dtype == PANN_DT_F32;
+    a.queries = (const char*)queries;
+    a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
+    a.fqueries = (const char*)queries;
+    a.fq_stride = (uint64_t)idx->dim * 4;
+  }
+  a.nq = nq;
+  a.out_ids = out_ids;
+  a.out_dists = out_dists;
+  a.out_stride = out_stride;
+  a.out_visited = out_visited;
+  a.out_cmps = out_cmps;
+  a.out_nfront = out_count;
+  grid = std::min<uint32_t>(grid, nq);
+  HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
+          "search_layer launch");
+  if (opts && opts->async) return PANN_OK;
+  return check_status(idx, stream, true);
+}
+
+pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq, uint32_t k,
+                             uint32_t ef, int64_t limit, const uint32_t* entrance,
+                             uint32_t nentrance, uint32_t top_level, const uint32_t* node_ids,
+                             const pann_search_opts* opts, uint32_t* out_ids, float* out_dists,
+                             uint32_t out_stride, uint32_t* out_count, uint32_t* out_visited,
+                             uint32_t* out_cmps, void* stream_) {
+  if (!idx || !queries || !entrance || !out_ids || !out_dists || !out_count)
+    return fail(PANN_ERR_ARG, "null argument");
+  if (nq == 0) return PANN_OK;
+  if (ef == 0) return fail(PANN_ERR_ARG, "ef must be >= 1");
+  if (out_stride < ef) return fail(PANN_ERR_ARG, "out_stride smaller than ef");
+  if (nentrance == 0) return fail(PANN_ERR_ARG, "empty entrance");
+  if (top_level >= idx->levels.size()) return fail(PANN_ERR_ARG, "top_level is not set");
+  for (uint32_t l = 0; l <= top_level; l++)
+    if (!idx->levels[l].d_nbrs)
+      return fail(PANN_ERR_ARG, "graph level " + std::to_string(l) + " is not set");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  hipStream_t stream = (hipStream_t)stream_;
+  if (idx->eps_cap < nq) {
+    if (idx->d_eps) (void)hipFree(idx->d_eps);
+    if (idx->d_epsd) (void)hipFree(idx->d_epsd);
+    idx->d_eps = nullptr;
+    idx->d_epsd = nullptr;
+    idx->eps_cap = 0;
+    HIP_TRY(hipMalloc(&idx->d_eps, (size_t)nq * 4), "hipMalloc(entry points)");
+    HIP_TRY(hipMalloc(&idx->d_epsd, (size_t)nq * 4), "hipMalloc(entry points)");
+    idx->eps_cap = nq;
+  }
+  pann_search_opts o{};
+  if (opts) o = *opts;
+  const bool f32_queries = idx->dtype == PANN_DT_F32;
+  StartSets ss{idx->d_eps, nullptr, 1u};
+  HIP_TRY(hipEventRecord(idx->ev0, stream), "hipEventRecord");
+  // search_layer_to (HNSW.hpp:1545-1566): ef = 1 from the entrance through
+  // levels top .. 1, keeping the closest; then layer 0 with ef (:1582-1586)
+  for (int64_t l = top_level; l >= 0; l--) {
+    const auto& G = idx->levels[(size_t)l];
+    const bool last = l == 0;
+    const uint32_t beam = last ? ef : 1u;
+    pann_qparams qp{(int64_t)beam, (int64_t)beam,
+                    last && limit > 0 ? limit : (int64_t)idx->n, (int64_t)G.max_degree,
+                    100.0, 0.125};
+    pann::SearchArgs a;
+    int store_kind;
+    bool sequential;
+    uint32_t grid;
+    pann_search_opts lo = o;
+    lo.rerank = last ? o.rerank : 0;
+    const bool first = l == (int64_t)top_level;
+    pann_status st = prepare_search(idx, (uint32_t)l, &qp, entrance, nentrance,
+                                    first ? nullptr : &ss, &lo, nq, a, store_kind, sequential,
+                                    grid, stream);
+    if (st != PANN_OK) return st;
+    a.queries = (const char*)queries;
+    a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
+    a.fqueries = (const char*)queries;
+    a.fq_stride = (uint64_t)idx->dim * 4;
+    a.nq = nq;
+    a.out_ids = last ? out_ids : idx->d_eps;  // in place: each wave reads its start first
+    a.out_dists = last ? out_dists : idx->d_epsd;
+    a.out_stride = last ? out_stride : 1u;
+    a.out_nfront = last ? out_count : nullptr;
+    a.out_visited = out_visited;
+    a.out_cmps = out_cmps;  // count_cmps sums every level (:1553, :1097-1098)
+    a.accumulate = first ? 0u : 1u;
+    grid = std::min<uint32_t>(grid, nq);
+    HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
+            "HNSW layer launch");
+  }
+  HIP_TRY(pann::launch_hnsw_cut(out_ids, out_dists, out_stride, out_count, nq, k, node_ids,
+                                stream),
+          "HNSW cut launch");
+  HIP_TRY(hipEventRecord(idx->ev1, stream), "hipEventRecord");
+  idx->timed = true;
+  if (o.async) return PANN_OK;
+  return check_status(idx, stream, true);
+}
+
+pann_status pann_hnsw_select_neighbors(pann_index_t idx, uint32_t level, const uint32_t* pts,
+                                       uint32_t np, const uint32_t* cand_ids,
+                                       const float* cand_dists, const uint32_t* cand_count,
+                                       uint32_t cand_stride, float alpha, uint32_t M,
+                                       uint32_t* out_ids, uint32_t* out_count, void* stream) {
+  if (!idx || !pts || !cand_ids || !cand_dists || !cand_count || !out_ids || !out_count)
+    return fail(PANN_ERR_ARG, "null argument");
+  if (np == 0) return PANN_OK;
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  return run_prune(idx, level, pts, np, cand_ids, cand_dists, cand_count, cand_stride, nullptr,
+                   cand_stride, (double)alpha, M, 4, out_ids, out_count, (hipStream_t)stream);
+}
+
+pann_status pann_hnsw_add_reverse(pann_index_t idx, uint32_t level, const uint32_t* pts,
+                                  uint32_t np, const uint32_t* offsets, const uint32_t* cand_ids,
+                                  uint32_t max_count, uint32_t M, uint32_t* out_ids,
+                                  uint32_t* out_count, void* stream) {
+  if (!idx || !pts || !offsets || !cand_ids || !out_ids || !out_count)
+    return fail(PANN_ERR_ARG, "null argument");
+  if (np == 0) return PANN_OK;
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  return run_prune(idx, level, pts, np, cand_ids, nullptr, nullptr, 0, offsets, max_count, 1.0,
+                   M, 5, out_ids, out_count, (hipStream_t)stream);
+}
+
+pann_status pann_hnsw_set_neighbors(pann_index_t idx, uint32_t level, const uint32_t* pts,
+                                    uint32_t np, const uint32_t* lists, const uint32_t* counts,
+                                    uint32_t R, void* stream) {
+  if (!idx || !pts || !lists || !counts) return fail(PANN_ERR_ARG, "null argument");
+  if (np == 0) return PANN_OK;
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  return update_level(idx, level, pts, np, lists, counts, R, (hipStream_t)stream);
+}
+
+pann_status pann_hnsw_download_level(pann_index_t idx, uint32_t level, uint32_t* graph_host) {
+  if (!idx || !graph_host) return fail(PANN_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  return download_level(idx, level, graph_host);
+}
+
 pann_status pann_close(pann_index_t idx) {
   if (!idx) return PANN_OK;
   (void)hipSetDevice(idx->device);
@@ -717,6 +1007,11 @@ pann_status pann_close(pann_index_t idx) {
                   idx->d_counter, idx->d_status, idx->d_starts, idx->d_ghash, idx->d_vis,
                   idx->d_dummy, idx->d_pkeys, idx->d_pflags};
   for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (size_t l = 1; l < idx->levels.size(); l++)
+    if (idx->levels[l].d_nbrs) (void)hipFree(idx->levels[l].d_nbrs);
+  void* eps[] = {idx->d_eps, idx->d_epsd, idx->d_epsn};
+  for (void* p : eps)
     if (p) (void)hipFree(p);
   if (idx->ev0) (void)hipEventDestroy(idx->ev0);
   if (idx->ev1) (void)hipEventDestroy(idx->ev1);

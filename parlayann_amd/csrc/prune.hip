@@ -10,6 +10,14 @@
 //   comparison is done in double exactly as the reference writes it (:111).
 // Distances are the reference's (euclidian_point.h:45-90, mips_point.h:44-66):
 // exact int32 for uint8/int8, in-order FP32 sums otherwise.
+//
+// The same kernel runs HNSW's two neighbour-selection steps (HNSW.hpp):
+//   add = 4: prune_heuristic (:430-484) on a search frontier: candidates
+//     sorted by distance, then c is dropped when some already selected r has
+//     d(c, r) < alpha * d(c, u), compared in float; no dedup, no self skip;
+//   add = 5: the reverse-edge step of insert (:952-983): the added ids are
+//     appended to the current list when both fit in R, otherwise the union is
+//     sorted by distance to the point and cut to R.
 #include "kernels.h"
 #include "dist.h"
 
@@ -95,6 +103,22 @@ __device__ void prune_one(const PruneArgs& a, uint32_t i, int lane, char* QP, ch
     cid = a.cand_ids + (uint64_t)i * a.cand_stride;
     if (a.cand_dists) cdist = a.cand_dists + (uint64_t)i * a.cand_stride;
   }
+  if (a.add == 5) {
+    const uint32_t* row = a.nbrs + (uint64_t)p * a.nbr_stride;
+    uint32_t deg = 0;
+    for (uint32_t t = 0; t < a.max_degree; t += PANN_WAVE) {
+      uint32_t tt = t + lane;
+      deg += __popcll(ballot(tt < a.max_degree && row[tt] != PANN_SENTINEL));
+    }
+    if (nc + deg <= a.R) {  // nbh_v.insert(end, adds) (:982)
+      uint32_t* out = a.out_ids + (uint64_t)i * a.R;
+      for (uint32_t t = lane; t < deg; t += PANN_WAVE) out[t] = row[t];
+      for (uint32_t t = lane; t < nc; t += PANN_WAVE) out[deg + t] = cid[t];
+      if (lane == 0) a.out_count[i] = deg + nc;
+      wave_sync_k<KG>();
+      return;
+    }
+  }
   if (a.add == 2) {
     // reverse-edge rule of batch_insert (vamana/index.h:292-303): when the
     // candidates fit next to the current list, append without repeats
@@ -138,7 +162,7 @@ __device__ void prune_one(const PruneArgs& a, uint32_t i, int lane, char* QP, ch
     K[t] = make_key(d, c);
   }
   uint32_t total = nc;
-  if (a.add) {  // :72-77 (robustPrune's add, also taken by the reverse-edge rule)
+  if (a.add != 0 && a.add != 4) {  // :72-77 (robustPrune's add, also the reverse-edge rules)
     const uint32_t* row = a.nbrs + (uint64_t)p * a.nbr_stride;
     for (uint32_t t = lane; t < a.max_degree; t += PANN_WAVE) {
       uint32_t c = row[t];
@@ -158,9 +182,20 @@ __device__ void prune_one(const PruneArgs& a, uint32_t i, int lane, char* QP, ch
   for (uint32_t t = total + lane; t < (uint32_t)np2; t += PANN_WAVE) K[t] = ~0ull;
   wave_sync_k<KG>();
   wave_bitonic_sort<KG>(K, np2, lane);  // :83
-  // unique by id (:86-88): equal ids sit next to each other (same distance)
+  if (a.add == 3 || a.add == 5) {  // sort, then cut to R: no pruning
+    uint32_t* out = a.out_ids + (uint64_t)i * a.R;
+    const uint32_t m = total < a.R ? total : a.R;
+    for (uint32_t t = lane; t < m; t += PANN_WAVE) out[t] = key_id(K[t]);
+    if (lane == 0) a.out_count[i] = m;
+    wave_sync_k<KG>();
+    return;
+  }
+  const bool hnsw = a.add == 4;
+  const float alpha_f = (float)a.alpha;
+  // unique by id (:86-88): equal ids sit next to each other (same distance);
+  // prune_heuristic keeps repeats
   for (uint32_t t = lane; t < total; t += PANN_WAVE)
-    PR[t] = (t > 0 && key_id(K[t]) == key_id(K[t - 1])) ? 1 : 0;
+    PR[t] = (!hnsw && t > 0 && key_id(K[t]) == key_id(K[t - 1])) ? 1 : 0;
   wave_sync_k<KG>();
 
   uint32_t nout = 0;
@@ -168,7 +203,7 @@ __device__ void prune_one(const PruneArgs& a, uint32_t i, int lane, char* QP, ch
   for (uint32_t idx = 0; idx < total && nout < a.R; idx++) {  // :95-116
     if (PR[idx]) continue;
     const uint32_t ps = key_id(K[idx]);
-    if (ps == p) continue;
+    if (!hnsw && ps == p) continue;
     if (lane == 0) out[nout] = ps;
     nout++;
     const char* srow = a.pts + (uint64_t)ps * a.stride;
@@ -178,7 +213,12 @@ __device__ void prune_one(const PruneArgs& a, uint32_t i, int lane, char* QP, ch
       if (!PR[t]) {
         uint64_t kk = K[t];
         float d_sp = row_dist<DT, MT>(QS, a.pts + (uint64_t)key_id(kk) * a.stride, (int)a.dim);
-        if (a.add != 3 && a.alpha * (double)d_sp <= (double)key_dist(kk)) PR[t] = 1;
+        if (hnsw) {
+          const float d_cu = key_dist(kk) * alpha_f;  // HNSW.hpp:460-470
+          if (d_sp < d_cu) PR[t] = 1;
+        } else if (a.alpha * (double)d_sp <= (double)key_dist(kk)) {
+          PR[t] = 1;
+        }
       }
     }
     wave_sync_k<KG>();

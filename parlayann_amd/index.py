@@ -221,8 +221,21 @@ class GraphIndex:
         Q = self._prep_queries(queries, quant)
         limit = visit_limit if visit_limit > 0 else None
         if self._hnsw is not None:
-            r = self._hnsw.search(Q, knn, beam_width, limit=limit)
-            return r["ids"], r["dists"]
+            # HNSW::search returns a tie-extended list (HNSW.hpp:1590-1596); the
+            # binding's fixed (nq, knn) output takes its first knn entries and
+            # marks missing ones with id 0xFFFFFFFF, distance +inf
+            use_q = quant and self.dtype == np.float32
+            if use_q:
+                self._hnsw.build_store("fp8")
+            r = self._hnsw.search(Q, knn, max(beam_width, knn), limit=limit or 0,
+                                  store="fp8" if use_q else "native", rerank=use_q)
+            ids = np.full((len(Q), knn), 0xFFFFFFFF, np.uint32)
+            dists = np.full((len(Q), knn), np.inf, np.float32)
+            w = min(knn, r["ids"].shape[1])
+            have = np.arange(w)[None, :] < r["count"][:, None]
+            ids[:, :w][have] = r["ids"][:, :w][have]
+            dists[:, :w][have] = r["dists"][:, :w][have]
+            return ids, dists
         use_q = quant and self.dtype == np.float32
         if use_q:
             self._dev.build_store("fp8")
