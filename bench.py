@@ -48,7 +48,13 @@ WORKLOADS = {
                          L=128, alpha=1.0, k=10, legs=["fp32-tree", "fp32-inorder",
                                                        "bf16-rerank", "fp8-rerank"],
                          headline="fp8-rerank", uneven=True),
+    # serving: HNSW M=32 (layer-0 degree 64) efc=200 over 1M x 200 unit vectors
+    # (500-centre mixture, queries shifted 0.5 sigma), IP, k in {10, 100}
+    "hnsw-ip-1M": dict(kind="hnsw", metric="ip", n=1_000_000, nq=10_000, d=200, m=32, efc=200,
+                       k=10, legs=["fp32-tree", "bf16-rerank", "fp8-rerank"],
+                       headline="fp8-rerank"),
 }
+EFS = [16, 20, 24, 32, 40, 48, 64, 80, 100, 128, 160, 200, 256, 320, 400]
 
 
 def log(*a):
@@ -57,6 +63,10 @@ def log(*a):
 
 def make_data(w):
     from parlayann_amd import datagen
+    if w["kind"] == "hnsw":
+        from parlayann_amd.index import _normalize
+        X, Q = datagen.workload_hnsw_ip(n=w["n"], nq=w["nq"], d=w["d"])
+        return _normalize(X), _normalize(Q)
     if w["kind"] == "f32":
         return datagen.workload_f32_l2(n=w["n"], nq=w["nq"], d=w["d"])
     if w["kind"] == "u8":
@@ -142,6 +152,8 @@ def main():
         w["n"] = args.n
     if args.nq:
         w["nq"] = args.nq
+    if w["kind"] == "hnsw":
+        return run_hnsw(args, w)
     legs = args.legs.split(",") if args.legs else w["legs"]
     beams = [args.beam] if args.beam else w.get("beams", BEAMS)
     dev = torch.device("cuda", 0)
@@ -355,6 +367,182 @@ def main():
         "roofline": {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": h["achieved_gbs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, w, head, h["beam"])},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def run_hnsw(args, w):
+    """HNSW serving workload: calibrate ef per leg, closed-loop batches of all
+    queries (inputs resident), then open-loop Poisson arrivals through the
+    continuous-batching server at 50% and 80% of the closed-loop rate."""
+    import torch
+    from parlayann_amd import hnsw
+    from parlayann_amd.groundtruth import groundtruth
+    from parlayann_amd.recall import recall_plain
+    from parlayann_amd.serve import BatchServer, poisson_arrivals
+
+    legs = args.legs.split(",") if args.legs else w["legs"]
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    X, Q = make_data(w)
+    log(f"[bench] data {X.shape} + {Q.shape} in {time.time() - t0:.1f}s")
+    tag = f"{args.workload}-n{w['n']}-q{w['nq']}"
+    cm = os.path.join(args.cache, tag + ".hnsw") if args.cache else ""
+    cgt = os.path.join(args.cache, tag + "-gt.npy") if args.cache else ""
+    t_build = None
+    if cm and os.path.exists(cm):
+        model = hnsw.read_model(cm)
+        log(f"[bench] model loaded from {cm}")
+    else:
+        t0 = time.time()
+        last = [time.time()]
+
+        def progress(bi, bb, be):
+            if time.time() - last[0] > 10:
+                last[0] = time.time()
+                log(f"[bench] HNSW build: {be:,}/{len(X):,} nodes ({time.time() - t0:.0f}s)")
+
+        model = hnsw.build_hnsw(X, w["metric"], w["m"], w["efc"], 1.0, seed=4242,
+                                progress=progress)
+        t_build = time.time() - t0
+        log(f"[bench] GPU HNSW build m={w['m']} efc={w['efc']}: {t_build:.1f}s, "
+            f"top level {model.top_level}, avg layer-0 degree {model.graphs[0][:, 0].mean():.2f}")
+        if cm:
+            os.makedirs(args.cache, exist_ok=True)
+            hnsw.write_model(cm, model)
+    if cgt and os.path.exists(cgt):
+        gt = np.load(cgt)
+    else:
+        t0 = time.time()
+        gt, _ = groundtruth(X, Q, 100, w["metric"])
+        log(f"[bench] FP64 groundtruth (k=100) in {time.time() - t0:.1f}s")
+        if cgt:
+            np.save(cgt, gt)
+    idx = hnsw.HNSWIndex(X, model, w["metric"])
+    for leg in legs:
+        if LEG_OPTS[leg]["store"] != "native":
+            idx.build_store(LEG_OPTS[leg]["store"])
+    nq = len(Q)
+    rng = np.random.default_rng(77)
+    ks = rng.choice([10, 100], nq).astype(np.int32)
+    tq = torch.from_numpy(Q).to(dev)
+    tk = torch.from_numpy(ks).to(dev)
+    avg_deg = float(model.graphs[0][:, 0].mean())
+    results = {}
+    efs_list = [args.beam] if args.beam else EFS
+    for leg in legs:
+        opts = LEG_OPTS[leg]
+        kw = dict(store=opts["store"], rerank=opts["rerank"], summation=opts["summation"])
+        chosen = None
+        for ef in efs_list:
+            efs = np.maximum(ks, ef).astype(np.int32)
+            mx = int(efs.max())
+            oi = torch.zeros((nq, mx), dtype=torch.int32, device=dev)
+            od = torch.zeros((nq, mx), dtype=torch.float32, device=dev)
+            oc = torch.zeros(nq, dtype=torch.int32, device=dev)
+            te = torch.from_numpy(efs).to(dev)
+            idx.search_device(tq, 0, mx, oi, od, oc, k_per_query=tk, ef_per_query=te, **kw)
+            ids = oi.cpu().numpy().view(np.uint32)[:, :10]
+            rec = recall_plain(gt, ids, 10)
+            log(f"[bench] {leg} ef {ef}: recall@10 {rec:.4f}")
+            chosen = (ef, rec, efs, mx, oi, od, oc, te)
+            if rec >= args.target_recall:
+                break
+        ef, rec, efs, mx, oi, od, oc, te = chosen
+        ov = torch.zeros(nq, dtype=torch.int32, device=dev)
+        om = torch.zeros(nq, dtype=torch.int32, device=dev)
+        run = lambda a=True, ex=False: idx.search_device(
+            tq, 0, mx, oi, od, oc, k_per_query=tk, ef_per_query=te, out_visited=ov,
+            out_cmps=om, async_=a, exact_counts=ex, **kw)
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run()
+            kms.append(idx.base.last_kernel_ms())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        idx.base.search_status()
+        qps = nq * args.steps / dt
+        evals = om.cpu().numpy().astype(np.float64)
+        run(False, True)  # the reference's dist_cmps (count_cmps over every level)
+        vis = ov.cpu().numpy().astype(np.float64)
+        cmps = om.cpu().numpy().astype(np.float64)
+        rb = row_bytes(leg, w["d"], "f32")
+        nrr = efs.astype(np.float64) if opts["rerank"] else 0
+        per_q = vis * 4 * (1 + avg_deg) + cmps * rb + nrr * w["d"] * 4 + w["d"] * 4 + ks * 8
+        bytes_launch = float(per_q.sum())
+        kms_avg = float(np.mean(kms))
+        gbs = bytes_launch / (kms_avg * 1e-3) / 1e9
+        # open loop: Poisson arrivals through the continuous-batching server
+        srv = BatchServer(idx, mx, 2 << 30, **kw)
+        serving = {}
+        for frac in (0.5, 0.8):
+            rate = frac * qps
+            nreq = int(min(200_000, max(20_000, rate * 2.0)))
+            sel = np.arange(nreq) % nq
+            res = srv.serve(Q[sel], ks[sel], efs[sel], poisson_arrivals(nreq, rate, seed=5))
+            serving[f"{int(frac * 100)}pct"] = dict(offered_qps=rate, requests=nreq,
+                                                    **res.summary())
+            log(f"[bench] {leg} serve at {rate:,.0f} q/s: p50 {res.summary()['p50_ms']:.3f} ms "
+                f"p99 {res.summary()['p99_ms']:.3f} ms, mean batch {res.mean_batch:.0f}")
+        results[leg] = dict(ef=ef, ef_rule="max(ef, k)", recall_at_10=round(rec, 5),
+                            target_reached=rec >= args.target_recall, qps=qps,
+                            ms_per_step=dt * 1e3 / args.steps, kernel_ms=kms_avg,
+                            avg_visited=float(vis.mean()), avg_dist_cmps=float(cmps.mean()),
+                            avg_evals=float(evals.mean()), bytes_per_launch=bytes_launch,
+                            achieved_gbs=gbs, serving=serving)
+        log(f"[bench] {leg}: ef {ef} recall {rec:.4f} QPS {qps:,.0f} kernel {kms_avg:.3f} ms "
+            f"{gbs:,.0f} GB/s visited {vis.mean():.1f} cmps {cmps.mean():.1f}")
+    cpu = None
+    if not args.no_cpu and "fp32-tree" in results:
+        from oracle import pyoracle
+        ef = results["fp32-tree"]["ef"]
+        threads = min(64, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                      else os.cpu_count())
+        adj = [np.ascontiguousarray(g) for g in model.graphs]
+        ent = np.asarray(model.entrance, np.uint32)
+
+        def cpu_run(m):
+            for kk in (10, 100):
+                sel = np.nonzero(ks[:m] == kk)[0]
+                pyoracle.hnsw_search(adj, X, w["metric"], Q[sel], kk, max(ef, kk), ent,
+                                     nthreads=threads)
+        m = min(nq, 500)
+        t0 = time.perf_counter()
+        cpu_run(m)
+        rate = m / (time.perf_counter() - t0)
+        m = int(min(nq, max(500, rate * args.cpu_seconds)))
+        times = []
+        for rep in range(5):
+            _ = np.random.default_rng(rep).permutation(5_000_000)  # cache flush
+            t0 = time.perf_counter()
+            cpu_run(m)
+            times.append(time.perf_counter() - t0)
+        cpu = {"value": m / float(np.median(times)), "unit": "queries/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {m} of the {nq} queries (k in {{10,100}}, ef max(ef, k) at the "
+                         f"FP32 operating ef {ef}), 5 repetitions, median; oracle restatement "
+                         f"of HNSW.hpp search"}
+        log(f"[bench] host CPU ({threads} threads): {cpu['value']:,.0f} QPS")
+    head = w["headline"] if w["headline"] in results else next(iter(results))
+    h = results[head]
+    out = {
+        "metric": "HNSW serving QPS at recall@10=0.95 (k in {10,100}), 1 MI355X",
+        "value": h["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp8" if head == "fp8-rerank" else "f32",
+        "data": "synthetic (seeded generator; HNSW built on the GPU by this library)",
+        "config": {"workload": args.workload, "n": w["n"], "dim": w["d"], "nq": nq,
+                   "k": "U{10,100}", "graph": f"hnsw m={w['m']} efc={w['efc']}",
+                   "headline_leg": head, "legs": results, "graph_build_s": t_build},
+        "roofline": {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": h["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, w, head, h["ef"])},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

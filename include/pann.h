@@ -217,9 +217,12 @@ pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void 
  * entry tied with the k-th distance (:1590-1596).  Ids are mapped through
  * node_ids (device, NULL = node ids).  out_stride >= ef; out_count[q] =
  * entries written; out_visited / out_cmps sum every level (count_cmps).
- * opts.rerank re-scores layer 0 in FP32 before the cut. */
+ * opts.rerank re-scores layer 0 in FP32 before the cut.  k_pq / ef_pq
+ * (device int32[nq], may be NULL) give each query its own k and ef <= ef,
+ * so a batch mixes requests. */
 pann_status pann_hnsw_search(pann_index_t idx, const void *queries, uint32_t nq, uint32_t k,
-                             uint32_t ef, int64_t limit, const uint32_t *entrance,
+                             uint32_t ef, const int32_t *k_pq, const int32_t *ef_pq,
+                             int64_t limit, const uint32_t *entrance,
                              uint32_t nentrance, uint32_t top_level, const uint32_t *node_ids,
                              const pann_search_opts *opts, uint32_t *out_ids, float *out_dists,
                              uint32_t out_stride, uint32_t *out_count, uint32_t *out_visited,

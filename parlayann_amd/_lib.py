@@ -84,7 +84,7 @@ def lib():
         "pann_hnsw_search_layer": [vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32,
                                    C.c_int64, vp, vp, C.c_uint32, C.POINTER(SearchOpts), vp, vp,
                                    C.c_uint32, vp, vp, vp, vp],
-        "pann_hnsw_search": [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int64, vp,
+        "pann_hnsw_search": [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, C.c_int64, vp,
                              C.c_uint32, C.c_uint32, vp, C.POINTER(SearchOpts), vp, vp,
                              C.c_uint32, vp, vp, vp, vp],
         "pann_hnsw_select_neighbors": [vp, C.c_uint32, vp, C.c_uint32, vp, vp, vp, C.c_uint32,

@@ -10,9 +10,11 @@
 namespace pann {
 
 __global__ void k_hnsw_cut(uint32_t* ids, float* dists, uint32_t stride, uint32_t* count,
-                           uint32_t nq, uint32_t k, const uint32_t* node_ids) {
+                           uint32_t nq, uint32_t k_all, const int32_t* k_pq,
+                           const uint32_t* node_ids) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
+  const uint32_t k = k_pq ? (uint32_t)k_pq[q] : k_all;
   uint32_t* I = ids + (uint64_t)q * stride;
   const float* D = dists + (uint64_t)q * stride;
   uint32_t m = count[q] < stride ? count[q] : stride;
@@ -30,9 +32,10 @@ __global__ void k_hnsw_cut(uint32_t* ids, float* dists, uint32_t stride, uint32_
 }
 
 hipError_t launch_hnsw_cut(uint32_t* ids, float* dists, uint32_t stride, uint32_t* count,
-                           uint32_t nq, uint32_t k, const uint32_t* node_ids, hipStream_t s) {
+                           uint32_t nq, uint32_t k, const int32_t* k_pq,
+                           const uint32_t* node_ids, hipStream_t s) {
   hipLaunchKernelGGL(k_hnsw_cut, dim3((nq + 255) / 256), dim3(256), 0, s, ids, dists, stride,
-                     count, nq, k, node_ids);
+                     count, nq, k, k_pq, node_ids);
   return hipGetLastError();
 }
 

@@ -124,7 +124,8 @@ hipError_t launch_sort_visited(const uint64_t* vis, uint32_t nq, uint32_t cap,
 // HNSW::search's final sort-and-cut with ties (HNSW.hpp:1590-1596) and the
 // node -> point id map
 hipError_t launch_hnsw_cut(uint32_t* ids, float* dists, uint32_t stride, uint32_t* count,
-                           uint32_t nq, uint32_t k, const uint32_t* node_ids, hipStream_t s);
+                           uint32_t nq, uint32_t k, const int32_t* k_pq,
+                           const uint32_t* node_ids, hipStream_t s);
 hipError_t launch_update_neighbors(uint32_t* nbrs, uint32_t nbr_stride, uint32_t max_degree,
                                    const uint32_t* points, uint32_t np, const uint32_t* lists,
                                    const uint32_t* counts, uint32_t R, hipStream_t s);

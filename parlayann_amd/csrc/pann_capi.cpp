@@ -877,7 +877,8 @@ pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void*
 }
 
 pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq, uint32_t k,
-                             uint32_t ef, int64_t limit, const uint32_t* entrance,
+                             uint32_t ef, const int32_t* k_pq, const int32_t* ef_pq,
+                             int64_t limit, const uint32_t* entrance,
                              uint32_t nentrance, uint32_t top_level, const uint32_t* node_ids,
                              const pann_search_opts* opts, uint32_t* out_ids, float* out_dists,
                              uint32_t out_stride, uint32_t* out_count, uint32_t* out_visited,
@@ -942,12 +943,16 @@ pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq,
     a.out_visited = out_visited;
     a.out_cmps = out_cmps;  // count_cmps sums every level (:1553, :1097-1098)
     a.accumulate = first ? 0u : 1u;
+    if (last && ef_pq) {  // per-query ef: the whole frontier comes back
+      a.k_pq = ef_pq;
+      a.beam_pq = ef_pq;
+    }
     grid = std::min<uint32_t>(grid, nq);
     HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
             "HNSW layer launch");
   }
-  HIP_TRY(pann::launch_hnsw_cut(out_ids, out_dists, out_stride, out_count, nq, k, node_ids,
-                                stream),
+  HIP_TRY(pann::launch_hnsw_cut(out_ids, out_dists, out_stride, out_count, nq, k, k_pq,
+                                node_ids, stream),
           "HNSW cut launch");
   HIP_TRY(hipEventRecord(idx->ev1, stream), "hipEventRecord");
   idx->timed = true;

@@ -168,19 +168,23 @@ class HNSWIndex:
     def search_device(self, queries, k: int, ef: int, out_ids, out_dists, out_count, *,
                       limit: int = 0, store: str = "native", summation: str = "inorder",
                       rerank: bool = False, out_visited=None, out_cmps=None, stream=None,
-                      async_: bool = False):
+                      async_: bool = False, k_per_query=None, ef_per_query=None,
+                      exact_counts: bool = False):
         """HNSW::search for a batch of device queries (torch tensors).  Rows of
-        out_ids / out_dists (stride >= ef) hold out_count[q] results, point ids."""
+        out_ids / out_dists (stride >= ef) hold out_count[q] results, point ids.
+        k_per_query / ef_per_query (int32 tensors, ef_per_query <= ef) mix
+        requests in one batch."""
         import torch
         if self._node_ids is not None and self._dev_node_ids is None:
             self._dev_node_ids = torch.from_numpy(self._node_ids.view(np.int32)).to(queries.device)
         nq = int(queries.shape[0])
         ent = np.ascontiguousarray(self.model.entrance, dtype=np.uint32)
-        op = DeviceIndex._opts(store, summation, rerank, 0, async_, 0, False)
+        op = DeviceIndex._opts(store, summation, rerank, 0, async_, 0, exact_counts)
         if stream is not None and hasattr(stream, "cuda_stream"):
             stream = stream.cuda_stream
         L.check(L.lib().pann_hnsw_search(
-            self.base._h, _p(queries), nq, k, ef, limit, ent.ctypes.data, len(ent),
+            self.base._h, _p(queries), nq, k, ef, _p(k_per_query), _p(ef_per_query), limit,
+            ent.ctypes.data, len(ent),
             self.model.top_level, _p(self._dev_node_ids), C.byref(op), _p(out_ids),
             _p(out_dists), int(out_ids.shape[1]), _p(out_count), _p(out_visited), _p(out_cmps),
             stream))

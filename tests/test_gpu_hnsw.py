@@ -81,7 +81,7 @@ def _search(idx, Q, k, ef, exact=False, **kw):
     om = torch.zeros(nq, dtype=torch.int32, device=dev)
     ent = np.ascontiguousarray(idx.model.entrance, dtype=np.uint32)
     op = DeviceIndex._opts("native", "inorder", False, 0, False, 0, True)
-    L.check(L.lib().pann_hnsw_search(idx.base._h, tq.data_ptr(), nq, k, ef, 0,
+    L.check(L.lib().pann_hnsw_search(idx.base._h, tq.data_ptr(), nq, k, ef, None, None, 0,
                                      ent.ctypes.data, len(ent), idx.model.top_level, None,
                                      C.byref(op), oi.data_ptr(), od.data_ptr(), ef,
                                      oc.data_ptr(), ov.data_ptr(), om.data_ptr(), None))
@@ -141,3 +141,31 @@ def test_gpu_fp8_rerank_recall(oracle):
     # the oracle's search on an oracle build of this data reaches 0.9987
     assert rec["native"] > 0.99, rec
     assert rec["fp8"] > rec["native"] - 0.01, rec
+
+
+def test_gpu_mixed_batch_equals_alone(oracle, small_l2):
+    """Requests with their own k and ef in one launch get exactly what the
+    oracle returns for each request searched with its own k and ef."""
+    from parlayann_amd import hnsw
+    from parlayann_amd.serve import BatchServer, poisson_arrivals
+    X, lev, adj, ent = small_l2
+    idx = hnsw.HNSWIndex(X, model_of(X, lev, adj, ent, M, EFC), "l2")
+    Q = _queries(X, 240, seed=21)
+    rng = np.random.default_rng(4)
+    ks = rng.choice([1, 10, 50], len(Q)).astype(np.int32)
+    efs = np.maximum(ks, rng.choice([8, 24, 64], len(Q))).astype(np.int32)
+    srv = BatchServer(idx, 64, 1 << 20, summation="inorder")
+    ids, cnt = srv.search_all(Q, ks, efs)
+    for k in np.unique(ks):
+        for ef in np.unique(efs):
+            sel = np.nonzero((ks == k) & (efs == ef))[0]
+            if sel.size == 0:
+                continue
+            oi, od, oc, _, _ = oracle.hnsw_search(adj, X, "l2", Q[sel], int(k), int(ef), [ent])
+            assert np.array_equal(cnt[sel], oc), (k, ef)
+            for j, q in enumerate(sel):
+                assert np.array_equal(ids[q, :oc[j]], oi[j, :oc[j]]), (k, ef, q)
+    # open loop: every request is answered, latencies are positive
+    arr = poisson_arrivals(len(Q), 20000.0, seed=1)
+    res = srv.serve(Q, ks, efs, arr)
+    assert np.all(res.latency_s > 0) and res.batches >= 1
