@@ -528,14 +528,17 @@ def run_hnsw(args, w):
                          f"FP32 operating ef {ef}), 5 repetitions, median; oracle restatement "
                          f"of HNSW.hpp search"}
         log(f"[bench] host CPU ({threads} threads): {cpu['value']:,.0f} QPS")
-    head = w["headline"] if w["headline"] in results else next(iter(results))
+    # the headline is the fastest leg that reaches the recall target (FP8's three
+    # mantissa bits cannot separate this mixture's nearest neighbours)
+    ok = [l for l in results if results[l]["target_reached"]]
+    head = max(ok, key=lambda l: results[l]["qps"]) if ok else next(iter(results))
     h = results[head]
     out = {
         "metric": "HNSW serving QPS at recall@10=0.95 (k in {10,100}), 1 MI355X",
         "value": h["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp8" if head == "fp8-rerank" else "f32",
+        "dtype": {"fp8-rerank": "fp8", "bf16-rerank": "bf16"}.get(head, "f32"),
         "data": "synthetic (seeded generator; HNSW built on the GPU by this library)",
         "config": {"workload": args.workload, "n": w["n"], "dim": w["d"], "nq": nq,
                    "k": "U{10,100}", "graph": f"hnsw m={w['m']} efc={w['efc']}",

@@ -131,6 +131,18 @@ pann_status pann_search_host(pann_index_t idx, const void *queries, uint32_t nq,
 /* Status of the last asynchronous pann_search on `stream` (synchronises). */
 pann_status pann_search_status(pann_index_t idx, void *stream);
 
+/* filtered_beam_search without filtering (beamSearch.h:33-228) for a batch
+ * of host queries from one start set, returning each query's whole final
+ * frontier: out_ids/out_dists nq x beam_size (host), out_count[q] entries,
+ * sorted by (dist, id) -- the first member of beam_search's result
+ * (beamSearch.h:419-445).  A frontier shorter than beam_size is a result.
+ * opts.rerank is ignored.  out_visited / out_cmps may be NULL. */
+pann_status pann_beam_search_host(pann_index_t idx, const void *queries, uint32_t nq,
+                                  const pann_qparams *qp, const uint32_t *starts,
+                                  uint32_t nstarts, const pann_search_opts *opts,
+                                  uint32_t *out_ids, float *out_dists, uint32_t *out_count,
+                                  uint32_t *out_visited, uint32_t *out_cmps);
+
 /* Duration in milliseconds of the last pann_search kernel, measured with
  * HIP events on the launch stream (synchronises). */
 pann_status pann_last_kernel_ms(pann_index_t idx, float *ms);

@@ -480,11 +480,16 @@ pann_status check_status(pann_index_t idx, hipStream_t stream, bool allow_short 
 
 }  // namespace
 
-pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
+namespace {
+
+// pann_search with two extras: the frontier size per query (out_nfront) and
+// whether a frontier shorter than k is a result rather than an error.
+pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
                         const pann_qparams* qp, const int32_t* k_pq, const int32_t* beam_pq,
                         const uint32_t* starts, uint32_t nstarts, const pann_search_opts* opts,
                         uint32_t* out_ids, float* out_dists, uint32_t out_stride,
-                        uint32_t* out_visited, uint32_t* out_cmps, void* stream_) {
+                        uint32_t* out_visited, uint32_t* out_cmps, uint32_t* out_nfront,
+                        bool allow_short, void* stream_) {
   if (!idx || !qp || !starts) return fail(PANN_ERR_ARG, "null argument");
   if (nq == 0) return PANN_OK;
   if (!queries || !out_ids || !out_dists) return fail(PANN_ERR_ARG, "null buffer");
@@ -513,6 +518,7 @@ pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
   a.out_stride = out_stride;
   a.out_visited = out_visited;
   a.out_cmps = out_cmps;
+  a.out_nfront = out_nfront;
   grid = std::min<uint32_t>(grid, nq);
   HIP_TRY(hipEventRecord(idx->ev0, stream), "hipEventRecord");
   HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
@@ -520,7 +526,18 @@ pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
   HIP_TRY(hipEventRecord(idx->ev1, stream), "hipEventRecord");
   idx->timed = true;
   if (opts && opts->async) return PANN_OK;
-  return check_status(idx, stream);
+  return check_status(idx, stream, allow_short);
+}
+
+}  // namespace
+
+pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
+                        const pann_qparams* qp, const int32_t* k_pq, const int32_t* beam_pq,
+                        const uint32_t* starts, uint32_t nstarts, const pann_search_opts* opts,
+                        uint32_t* out_ids, float* out_dists, uint32_t out_stride,
+                        uint32_t* out_visited, uint32_t* out_cmps, void* stream) {
+  return search_impl(idx, queries, nq, qp, k_pq, beam_pq, starts, nstarts, opts, out_ids,
+                     out_dists, out_stride, out_visited, out_cmps, nullptr, false, stream);
 }
 
 pann_status pann_search_status(pann_index_t idx, void* stream) {
@@ -578,6 +595,57 @@ pann_status pann_search_host(pann_index_t idx, const void* queries, uint32_t nq,
     if (e == hipSuccess && out_visited) e = hipMemcpy(out_visited, dv, nq * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && out_cmps) e = hipMemcpy(out_cmps, dc, nq * 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) st = hip_fail(e, "search_host copy-out");
+  }
+  release();
+  return st;
+}
+
+pann_status pann_beam_search_host(pann_index_t idx, const void* queries, uint32_t nq,
+                                  const pann_qparams* qp, const uint32_t* starts,
+                                  uint32_t nstarts, const pann_search_opts* opts,
+                                  uint32_t* out_ids, float* out_dists, uint32_t* out_count,
+                                  uint32_t* out_visited, uint32_t* out_cmps) {
+  if (!idx || !qp || !queries || !out_ids || !out_dists || !out_count)
+    return fail(PANN_ERR_ARG, "null argument");
+  if (nq == 0) return PANN_OK;
+  if (qp->beam_size < 1 || qp->beam_size > 4096)
+    return fail(PANN_ERR_ARG, "beam_size must be in [1, 4096]");
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  const size_t qbytes = (size_t)nq * idx->dim * (idx->dtype == PANN_DT_F32 ? 4 : 1);
+  const size_t B = (size_t)qp->beam_size;
+  pann_qparams q2 = *qp;
+  q2.k = qp->beam_size;  // the whole frontier comes back
+  void* dq = nullptr;
+  uint32_t *di = nullptr, *dn = nullptr, *dv = nullptr, *dc = nullptr;
+  float* dd = nullptr;
+  auto release = [&]() {
+    for (void* p : {(void*)dq, (void*)di, (void*)dn, (void*)dv, (void*)dc, (void*)dd})
+      if (p) (void)hipFree(p);
+  };
+  hipError_t e = hipMalloc(&dq, qbytes);
+  if (e == hipSuccess) e = hipMalloc(&di, nq * B * 4);
+  if (e == hipSuccess) e = hipMalloc(&dd, nq * B * 4);
+  if (e == hipSuccess) e = hipMalloc(&dn, nq * 4);
+  if (e == hipSuccess) e = hipMalloc(&dv, nq * 4);
+  if (e == hipSuccess) e = hipMalloc(&dc, nq * 4);
+  if (e == hipSuccess) e = hipMemcpy(dq, queries, qbytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    release();
+    return hip_fail(e, "beam_search_host staging");
+  }
+  pann_search_opts o{};
+  if (opts) o = *opts;
+  o.async = 0;
+  o.rerank = 0;
+  pann_status st = search_impl(idx, dq, nq, &q2, nullptr, nullptr, starts, nstarts, &o, di, dd,
+                                (uint32_t)B, dv, dc, dn, true, nullptr);
+  if (st == PANN_OK) {
+    e = hipMemcpy(out_ids, di, nq * B * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out_dists, dd, nq * B * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out_count, dn, nq * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && out_visited) e = hipMemcpy(out_visited, dv, nq * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && out_cmps) e = hipMemcpy(out_cmps, dc, nq * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) st = hip_fail(e, "beam_search_host copy-out");
   }
   release();
   return st;
