@@ -39,7 +39,12 @@ typedef enum { PANN_METRIC_L2 = 0, PANN_METRIC_MIPS = 1 } pann_metric;
 typedef enum {
   PANN_STORE_NATIVE = 0, /* the points as given (f32 / u8 / i8) */
   PANN_STORE_BF16 = 1,   /* f32 points rounded to bfloat16 (RNE) */
-  PANN_STORE_FP8 = 2     /* f32 points as OCP FP8 e4m3 with per-dimension absmax scales */
+  PANN_STORE_FP8 = 2,    /* f32 points as OCP FP8 e4m3 with per-dimension absmax scales */
+  PANN_STORE_Q8 = 3      /* the reference's 8-bit scalar quantization of f32 points: uint8
+                            affine codes for L2 (Euclidian_Point<uint8_t>, euclidian_point.h:
+                            98-107, 188-243), int8 codes with a trimmed global scale for MIPS
+                            (Quantized_Mips_Point<8, true>, mips_point.h:292-470); queries are
+                            encoded the same way, distances are exact int32 */
 } pann_store;
 
 /* FP32 summation order of the native f32 store */
@@ -192,6 +197,11 @@ pann_status pann_robust_prune_groups(pann_index_t idx, const uint32_t *pts, uint
 pann_status pann_update_neighbors(pann_index_t idx, const uint32_t *pts, uint32_t np,
                                   const uint32_t *lists, const uint32_t *counts, uint32_t R,
                                   void *stream);
+
+/* Parameters of the PANN_STORE_Q8 copy (after pann_build_store): L2 slope
+ * and offset, MIPS bound (max_val) and scale; unused fields are 0. */
+pann_status pann_get_q8_params(pann_index_t idx, float *slope, int32_t *offset, float *max_val,
+                               float *scale);
 
 /* ------------------------------------------------------------------------
  * HNSW (algorithms/HNSW/HNSW.hpp).  A layered index is a pann_index whose

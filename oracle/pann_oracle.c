@@ -1302,3 +1302,96 @@ int oracle_build_hnsw(const oracle_hnsw *H, const oracle_points *P, const uint32
   free(rc.gstart);
   return atomic_load(&c.failed) ? -1 : 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* The reference's 8-bit scalar quantization of float points                 */
+/* (graph_index.cpp:86-99 builds these for its quant path)                   */
+/* ------------------------------------------------------------------------ */
+
+static int cmp_float(const void *a, const void *b) {
+  float x = *(const float *)a, y = *(const float *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* parlay::kth_smallest_copy(s, k): the element of rank k (0-based) */
+static float kth_smallest(const float *s, size_t n, size_t k) {
+  float *c = (float *)malloc(n * sizeof(float));
+  memcpy(c, s, n * sizeof(float));
+  qsort(c, n, sizeof(float), cmp_float);
+  float v = c[k < n ? k : n - 1];
+  free(c);
+  return v;
+}
+
+int oracle_q8_params_compute(const oracle_points *P, oracle_q8_params *out) {
+  if (P->dtype != OR_F32 || P->n == 0) return -1;
+  const size_t n = P->n;
+  const unsigned d = P->dim;
+  float *mins = (float *)malloc(n * sizeof(float));
+  float *maxs = (float *)malloc(n * sizeof(float));
+  int all_ints = 1;
+  for (size_t i = 0; i < n; i++) {
+    const float *x = (const float *)row_of(P, i);
+    float mn = 0.0f, mx = 0.0f; /* sequences start at 0.0 (euclidian_point.h:224-226, mips_point.h:420-429) */
+    for (unsigned j = 0; j < d; j++) {
+      if (P->metric == OR_L2) /* ni[i] && x >= 0 && x - (long) x == 0 (:229) */
+        all_ints = all_ints && x[j] >= 0 && (x[j] - (float)(long)x[j]) == 0;
+      mn = x[j] < mn ? x[j] : mn;
+      mx = x[j] > mx ? x[j] : mx;
+    }
+    mins[i] = mn;
+    maxs[i] = mx;
+  }
+  memset(out, 0, sizeof(*out));
+  if (P->metric == OR_L2) { /* Euclidian_Point<uint8_t>::generate_parameters (:219-243) */
+    float min_val = mins[0], max_val = maxs[0];
+    for (size_t i = 1; i < n; i++) {
+      min_val = mins[i] < min_val ? mins[i] : min_val;
+      max_val = maxs[i] > max_val ? maxs[i] : max_val;
+    }
+    if (all_ints) {
+      if (max_val < 256) max_val = 255;
+      min_val = 0;
+    }
+    out->slope = 255.0f / (max_val - min_val);            /* parameters(min, max, dims) (:104-107) */
+    out->offset = (int32_t)roundf(min_val * out->slope);
+  } else { /* Quantized_Mips_Point<8, true>::generate_parameters (:415-470), trimmed */
+    const double cutoff = .0001;
+    size_t min_rank = (size_t)(cutoff * (double)n);
+    size_t max_rank = (size_t)((1.0 - cutoff) * (double)(n - 1));
+    float min_val = kth_smallest(mins, n, min_rank);
+    float max_val = kth_smallest(maxs, n, max_rank);
+    float bound = max_val > -min_val ? max_val : -min_val;
+    out->max_val = bound;
+    out->scale = (float)(255 / 2) / bound;                  /* (range/2) / max_val (:304-305) */
+  }
+  free(mins);
+  free(maxs);
+  return 0;
+}
+
+void oracle_q8_translate(const oracle_q8_params *prm, int metric, const float *x, unsigned d,
+                         void *codes) {
+  if (metric == OR_L2) { /* Euclidian_Point<uint8_t>::translate_point (:188-214) */
+    uint8_t *c = (uint8_t *)codes;
+    if (prm->slope == 1.0f && prm->offset == 0) {
+      for (unsigned j = 0; j < d; j++) c[j] = (uint8_t)(int)x[j];
+    } else {
+      for (unsigned j = 0; j < d; j++) {
+        int64_t r = (int64_t)roundf(x[j] * prm->slope) - prm->offset;
+        if (r < 0) r = 0;
+        if (r > 255) r = 255;
+        c[j] = (uint8_t)r;
+      }
+    }
+  } else { /* Quantized_Mips_Point<8, true>::translate_point (:393-406) */
+    int8_t *c = (int8_t *)codes;
+    const float mv = prm->max_val, scale = prm->scale;
+    for (unsigned j = 0; j < d; j++) {
+      float pj = x[j];
+      if (pj < -mv) c[j] = (int8_t)(-127);
+      else if (pj > mv) c[j] = (int8_t)127;
+      else c[j] = (int8_t)(int32_t)roundf(pj * scale);
+    }
+  }
+}

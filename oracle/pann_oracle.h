@@ -141,6 +141,26 @@ int oracle_build_hnsw(const oracle_hnsw *H, const oracle_points *P, const uint32
                       unsigned efc, float alpha, float batch_base, uint32_t *entrance_out,
                       int nthreads);
 
+/* The reference's 8-bit scalar quantization of float points, as its Python
+ * quant path builds it (graph_index.cpp:86-99):
+ *  - L2: Euclidian_Point<uint8_t> (euclidian_point.h:98-107, 188-243): slope =
+ *    255 / (max - min), offset = round(min * slope) over all values (0
+ *    included; all-integer data in [0, 255] keeps slope 1, offset 0), codes
+ *    clamp(round(x * slope) - offset, 0, 255) (x truncated when slope 1 and
+ *    offset 0);
+ *  - MIPS: Quantized_Mips_Point<8, true> (mips_point.h:292-470): bound =
+ *    max(q_hi, -q_lo) of the 0.01%-trimmed per-point max / min, scale =
+ *    127 / bound, codes round(x * scale) clamped to +-127. */
+typedef struct {
+  float slope;
+  int32_t offset;
+  float max_val;
+  float scale;
+} oracle_q8_params;
+int oracle_q8_params_compute(const oracle_points *P, oracle_q8_params *out);
+void oracle_q8_translate(const oracle_q8_params *prm, int metric, const float *x, unsigned d,
+                         void *codes);
+
 /* The (dist, id) sort used everywhere above, exposed for its property test. */
 void oracle_sort_pairs(uint32_t *ids, float *dists, long n);
 

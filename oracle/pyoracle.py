@@ -38,6 +38,11 @@ class _HNSW(C.Structure):
                 ("max_degree", C.c_void_p)]
 
 
+class Q8Params(C.Structure):
+    _fields_ = [("slope", C.c_float), ("offset", C.c_int32), ("max_val", C.c_float),
+                ("scale", C.c_float)]
+
+
 class _QP(C.Structure):
     _fields_ = [("k", C.c_long), ("beam_size", C.c_long), ("limit", C.c_long),
                 ("degree_limit", C.c_long), ("rerank_factor", C.c_double),
@@ -96,6 +101,10 @@ def lib():
         L.oracle_groundtruth_f64.argtypes = [C.POINTER(_Points), vp, sz, C.c_int, C.c_int, u32p,
                                              vp]
         L.oracle_sort_pairs.argtypes = [u32p, f32p, C.c_long]
+        L.oracle_q8_params_compute.restype = C.c_int
+        L.oracle_q8_params_compute.argtypes = [C.POINTER(_Points), C.POINTER(Q8Params)]
+        L.oracle_q8_translate.restype = None
+        L.oracle_q8_translate.argtypes = [C.POINTER(Q8Params), C.c_int, vp, C.c_uint, vp]
         L.oracle_hnsw_search.restype = C.c_int
         L.oracle_hnsw_search.argtypes = [C.POINTER(_HNSW), C.POINTER(_Points), u32p, C.c_int, vp,
                                          sz, C.c_int, C.c_int, C.c_long, C.c_int, u32p, f32p, vp,
@@ -319,3 +328,22 @@ def build_hnsw(X, metric, m, efc, alpha, levels, batch_base=2.0, nthreads=None):
         tail = a[:, 1:]
         tail[np.arange(a.shape[1] - 1)[None, :] >= a[:, :1].astype(np.int64)] = 0
     return adj, int(ent[0])
+
+
+def q8_params(X, metric):
+    """The reference's scalar quantization parameters for float points."""
+    prm = Q8Params()
+    P = _points(np.ascontiguousarray(X, dtype=np.float32), metric)
+    if lib().oracle_q8_params_compute(C.byref(P), C.byref(prm)) != 0:
+        raise ValueError("q8 parameters need float points")
+    return prm
+
+
+def q8_translate(prm, metric, X):
+    """Codes of the rows of X (uint8 for L2, int8 for MIPS)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    out = np.zeros(X.shape, np.uint8 if MT[metric] == 0 else np.int8)
+    for i in range(X.shape[0]):
+        lib().oracle_q8_translate(C.byref(prm), MT[metric], X[i].ctypes.data, X.shape[1],
+                                  out[i].ctypes.data)
+    return out

@@ -87,6 +87,12 @@ hipError_t launch_bf16_store(const float* X, uint64_t stride_f, uint32_t n, uint
 hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                             uint32_t* absmax_bits, float* scales, uint8_t* Y, uint64_t stride_b,
                             hipStream_t s);
+// the reference's 8-bit scalar quantization (per-row extrema; codes of rows)
+hipError_t launch_row_minmax(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                             float* mins, float* maxs, uint32_t* notint, hipStream_t s);
+hipError_t launch_encode_q8(const float* X, uint64_t stride_f, uint32_t nrows, uint32_t dim,
+                            int metric, float slope, int32_t offset, float max_val, float scale,
+                            uint8_t* out, uint64_t out_stride, hipStream_t s);
 hipError_t launch_fp8_codec_probe(float* decode_out, const float* enc_in, uint8_t* enc_out,
                                   uint32_t n, hipStream_t s);
 }  // namespace pann

@@ -94,6 +94,12 @@ struct pann_index_s {
   uint64_t fp8_stride = 0;
   float* d_scales = nullptr;
   uint32_t* d_absmax = nullptr;
+  char* d_q8 = nullptr;            // the reference's 8-bit codes (PANN_STORE_Q8)
+  uint64_t q8_stride = 0;
+  float q8_slope = 0, q8_max = 0, q8_scale = 0;
+  int32_t q8_offset = 0;
+  uint8_t* d_qcodes = nullptr;     // per-search query codes for PANN_STORE_Q8
+  size_t qcodes_cap = 0;
   uint32_t* d_nbrs = nullptr;
   uint32_t nbr_stride = 0;
   // graph per level of a layered (HNSW) index; level 0 is d_nbrs above
@@ -263,16 +269,121 @@ pann_status pann_open(const pann_index_desc* desc, pann_index_t* out) {
   return PANN_OK;
 }
 
+namespace {
+
+// PANN_STORE_Q8: per-row extrema on the device, the reference's parameter
+// rules on the host, codes on the device.
+pann_status build_q8(pann_index_t idx, uint64_t fstride) {
+  const uint32_t n = idx->n;
+  float *d_mins = nullptr, *d_maxs = nullptr;
+  uint32_t* d_flag = nullptr;
+  auto release = [&]() {
+    for (void* p : {(void*)d_mins, (void*)d_maxs, (void*)d_flag})
+      if (p) (void)hipFree(p);
+  };
+  hipError_t e = hipMalloc(&d_mins, (size_t)n * 4);
+  if (e == hipSuccess) e = hipMalloc(&d_maxs, (size_t)n * 4);
+  if (e == hipSuccess) e = hipMalloc(&d_flag, 256);
+  if (e == hipSuccess)
+    e = pann::launch_row_minmax((const float*)idx->d_pts, fstride, n, idx->dim, d_mins, d_maxs,
+                                d_flag, nullptr);
+  std::vector<float> mins(n), maxs(n);
+  uint32_t notint = 0;
+  if (e == hipSuccess) e = hipMemcpy(mins.data(), d_mins, (size_t)n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(maxs.data(), d_maxs, (size_t)n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&notint, d_flag, 4, hipMemcpyDeviceToHost);
+  release();
+  if (e != hipSuccess) return hip_fail(e, "Q8 extrema");
+  float slope = 0, max_v = 0, scale = 0;
+  int32_t offset = 0;
+  if (idx->metric == PANN_METRIC_L2) {  // euclidian_point.h:219-243
+    float min_val = *std::min_element(mins.begin(), mins.end());
+    float max_val = *std::max_element(maxs.begin(), maxs.end());
+    if (!notint) {
+      if (max_val < 256) max_val = 255;
+      min_val = 0;
+    }
+    slope = 255.0f / (max_val - min_val);
+    offset = (int32_t)roundf(min_val * slope);
+  } else {  // mips_point.h:415-470, trimmed
+    const double cutoff = .0001;
+    const size_t min_rank = (size_t)(cutoff * (double)n);
+    const size_t max_rank = (size_t)((1.0 - cutoff) * (double)(n - 1));
+    std::nth_element(mins.begin(), mins.begin() + min_rank, mins.end());
+    std::nth_element(maxs.begin(), maxs.begin() + max_rank, maxs.end());
+    const float min_val = mins[min_rank], max_val = maxs[max_rank];
+    max_v = std::max(max_val, -min_val);
+    scale = (float)(255 / 2) / max_v;
+  }
+  if (!idx->d_q8) {
+    idx->q8_stride = round_up(idx->dim, 128);
+    HIP_TRY(hipMalloc(&idx->d_q8, (size_t)n * idx->q8_stride), "hipMalloc(q8)");
+    HIP_TRY(hipMemset(idx->d_q8, 0, (size_t)n * idx->q8_stride), "hipMemset(q8)");
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  }
+  idx->q8_slope = slope;
+  idx->q8_offset = offset;
+  idx->q8_max = max_v;
+  idx->q8_scale = scale;
+  HIP_TRY(pann::launch_encode_q8((const float*)idx->d_pts, fstride, n, idx->dim,
+                                 idx->metric == PANN_METRIC_L2 ? 0 : 1, slope, offset, max_v,
+                                 scale, (uint8_t*)idx->d_q8, idx->q8_stride, nullptr),
+          "Q8 encode");
+  return PANN_OK;
+}
+
+// For PANN_STORE_Q8 the kernel reads query codes: encode the float queries
+// into the index's scratch on `stream`.  Other stores read the queries as given.
+pann_status stage_queries(pann_index_t idx, int32_t store, const void* queries, uint32_t nq,
+                          pann::SearchArgs& a, hipStream_t stream) {
+  if (store != PANN_STORE_Q8) return PANN_OK;
+  const size_t need = (size_t)nq * idx->dim;
+  if (idx->qcodes_cap < need) {
+    if (idx->d_qcodes) {
+      HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      (void)hipFree(idx->d_qcodes);
+    }
+    idx->d_qcodes = nullptr;
+    idx->qcodes_cap = 0;
+    HIP_TRY(hipMalloc(&idx->d_qcodes, need), "hipMalloc(query codes)");
+    idx->qcodes_cap = need;
+  }
+  HIP_TRY(pann::launch_encode_q8((const float*)queries, idx->dim, nq, idx->dim,
+                                 idx->metric == PANN_METRIC_L2 ? 0 : 1, idx->q8_slope,
+                                 idx->q8_offset, idx->q8_max, idx->q8_scale, idx->d_qcodes,
+                                 idx->dim, stream),
+          "query encode");
+  a.queries = (const char*)idx->d_qcodes;
+  a.q_stride = idx->dim;
+  return PANN_OK;
+}
+
+}  // namespace
+
+pann_status pann_get_q8_params(pann_index_t idx, float* slope, int32_t* offset, float* max_val,
+                               float* scale) {
+  if (!idx) return fail(PANN_ERR_ARG, "null index");
+  if (!idx->d_q8) return fail(PANN_ERR_ARG, "Q8 store not built");
+  if (slope) *slope = idx->q8_slope;
+  if (offset) *offset = idx->q8_offset;
+  if (max_val) *max_val = idx->q8_max;
+  if (scale) *scale = idx->q8_scale;
+  return PANN_OK;
+}
+
 pann_status pann_build_store(pann_index_t idx, int32_t store) {
   if (!idx) return fail(PANN_ERR_ARG, "null index");
-  if (store != PANN_STORE_BF16 && store != PANN_STORE_FP8)
-    return fail(PANN_ERR_ARG, "store must be PANN_STORE_BF16 or PANN_STORE_FP8");
+  if (store != PANN_STORE_BF16 && store != PANN_STORE_FP8 && store != PANN_STORE_Q8)
+    return fail(PANN_ERR_ARG, "store must be PANN_STORE_BF16, PANN_STORE_FP8 or PANN_STORE_Q8");
   if (idx->dtype != PANN_DT_F32)
     return fail(PANN_ERR_ARG, "reduced-precision stores need float points");
   std::lock_guard<std::mutex> lk(idx->mu);
   HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
   const uint64_t fstride = idx->pts_stride / 4;
-  if (store == PANN_STORE_BF16) {
+  if (store == PANN_STORE_Q8) {
+    pann_status qst = build_q8(idx, fstride);
+    if (qst != PANN_OK) return qst;
+  } else if (store == PANN_STORE_BF16) {
     if (!idx->d_bf16) {
       idx->bf16_stride = round_up((uint64_t)idx->dim * 2, 128);
       HIP_TRY(hipMalloc(&idx->d_bf16, (size_t)idx->n * idx->bf16_stride), "hipMalloc(bf16)");
@@ -366,6 +477,12 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     pts = idx->d_bf16;
     stride = idx->bf16_stride;
     row_bytes = idx->dim * 2;
+  } else if (o.store == PANN_STORE_Q8) {
+    if (!idx->d_q8) return fail(PANN_ERR_ARG, "Q8 store not built (pann_build_store)");
+    store_kind = idx->metric == PANN_METRIC_L2 ? pann::ST_U8 : pann::ST_I8;
+    pts = idx->d_q8;
+    stride = idx->q8_stride;
+    row_bytes = idx->dim;
   } else if (o.store == PANN_STORE_FP8) {
     if (!idx->d_fp8) return fail(PANN_ERR_ARG, "FP8 store not built (pann_build_store)");
     store_kind = pann::ST_FP8;
@@ -519,6 +636,8 @@ pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
   a.out_visited = out_visited;
   a.out_cmps = out_cmps;
   a.out_nfront = out_nfront;
+  st = stage_queries(idx, opts ? opts->store : PANN_STORE_NATIVE, queries, nq, a, stream);
+  if (st != PANN_OK) return st;
   grid = std::min<uint32_t>(grid, nq);
   HIP_TRY(hipEventRecord(idx->ev0, stream), "hipEventRecord");
   HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
@@ -917,9 +1036,10 @@ pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void*
   pann_status st = prepare_search(idx, level, &qp, nullptr, 0, &ss, opts, nq, a, store_kind,
                                   sequential, grid, stream);
   if (st != PANN_OK) return st;
+  const int32_t store = opts ? opts->store : PANN_STORE_NATIVE;
   if (self_ids) {  // query q is point self_ids[q]; same_as skips it
-    a.queries = idx->d_pts;
-    a.q_stride = idx->pts_stride;
+    a.queries = store == PANN_STORE_Q8 ? idx->d_q8 : idx->d_pts;
+    a.q_stride = store == PANN_STORE_Q8 ? idx->q8_stride : idx->pts_stride;
     a.fqueries = idx->d_pts;
     a.fq_stride = idx->pts_stride;
     a.self_ids = self_ids;
@@ -929,6 +1049,8 @@ pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void*
     a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
     a.fqueries = (const char*)queries;
     a.fq_stride = (uint64_t)idx->dim * 4;
+    st = stage_queries(idx, store, queries, nq, a, stream);
+    if (st != PANN_OK) return st;
   }
   a.nq = nq;
   a.out_ids = out_ids;
@@ -1003,6 +1125,13 @@ pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq,
     a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
     a.fqueries = (const char*)queries;
     a.fq_stride = (uint64_t)idx->dim * 4;
+    if (first) {
+      st = stage_queries(idx, o.store, queries, nq, a, stream);
+      if (st != PANN_OK) return st;
+    } else if (o.store == PANN_STORE_Q8) {
+      a.queries = (const char*)idx->d_qcodes;
+      a.q_stride = idx->dim;
+    }
     a.nq = nq;
     a.out_ids = last ? out_ids : idx->d_eps;  // in place: each wave reads its start first
     a.out_dists = last ? out_dists : idx->d_epsd;
@@ -1083,7 +1212,7 @@ pann_status pann_close(pann_index_t idx) {
     if (p) (void)hipFree(p);
   for (size_t l = 1; l < idx->levels.size(); l++)
     if (idx->levels[l].d_nbrs) (void)hipFree(idx->levels[l].d_nbrs);
-  void* eps[] = {idx->d_eps, idx->d_epsd, idx->d_epsn};
+  void* eps[] = {idx->d_eps, idx->d_epsd, idx->d_epsn, idx->d_q8, idx->d_qcodes};
   for (void* p : eps)
     if (p) (void)hipFree(p);
   if (idx->ev0) (void)hipEventDestroy(idx->ev0);

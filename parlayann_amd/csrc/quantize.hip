@@ -7,6 +7,8 @@
 // The reference's analog is its scalar quantization of points
 // (euclidian_point.h:188-241, mips_point.h:425-501), which uses one global
 // scale; this store scales per dimension instead.
+#include <algorithm>
+
 #include "kernels.h"
 #include "device_common.h"
 #include "dist.h"
@@ -126,4 +128,90 @@ hipError_t launch_fp8_codec_probe(float* decode_out, const float* enc_in, uint8_
   if (n) hipLaunchKernelGGL(k_fp8_encode_n, dim3((n + 255) / 256), dim3(256), 0, s, enc_in, enc_out, n);
   return hipGetLastError();
 }
+}  // namespace pann
+
+// ---------------------------------------------------------------------------
+// The reference's 8-bit scalar quantization of float points (the quant path of
+// python/graph_index.cpp:86-99): uint8 affine codes for L2
+// (Euclidian_Point<uint8_t>, euclidian_point.h:98-107, 188-243) and int8
+// symmetric codes for MIPS (Quantized_Mips_Point<8, true>, mips_point.h:292-470).
+// The parameters are computed on the host from per-row extrema.
+// ---------------------------------------------------------------------------
+namespace pann {
+
+// per row: min(0, min_j x) and max(0, max_j x); notint |= some x is negative
+// or not an integer.  One wave per row, rows grid-strided.
+__global__ __launch_bounds__(256) void k_row_minmax(const float* __restrict__ X,
+                                                    uint64_t stride_f, uint32_t n, uint32_t dim,
+                                                    float* mins, float* maxs, uint32_t* notint) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  bool bad = false;
+  for (uint32_t r = wid; r < n; r += nw) {
+    float mn = 0.0f, mx = 0.0f;
+    const float* x = X + (uint64_t)r * stride_f;
+    for (uint32_t j = lane; j < dim; j += 64) {
+      const float v = x[j];
+      mn = fminf(mn, v);
+      mx = fmaxf(mx, v);
+      bad = bad || !(v >= 0.0f) || (v - truncf(v)) != 0.0f;
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+      mn = fminf(mn, __shfl_xor(mn, o));
+      mx = fmaxf(mx, __shfl_xor(mx, o));
+    }
+    if (lane == 0) {
+      mins[r] = mn;
+      maxs[r] = mx;
+    }
+  }
+  if (bad) atomicOr(notint, 1u);
+}
+
+// codes of nrows float rows; metric 0: uint8 affine, 1: int8 symmetric
+__global__ void k_encode_q8(const float* __restrict__ X, uint64_t stride_f, uint32_t nrows,
+                            uint32_t dim, int metric, float slope, int32_t offset, float max_val,
+                            float scale, uint8_t* __restrict__ out, uint64_t out_stride) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t total = (uint64_t)nrows * dim;
+  if (i >= total) return;
+  const uint64_t r = i / dim, j = i % dim;
+  const float x = X[r * stride_f + j];
+  uint8_t c;
+  if (metric == 0) {
+    if (slope == 1.0f && offset == 0) {
+      c = (uint8_t)(int)x;  // translate_point's direct copy (euclidian_point.h:193-195)
+    } else {
+      int64_t v = (int64_t)roundf(x * slope) - offset;
+      v = v < 0 ? 0 : (v > 255 ? 255 : v);
+      c = (uint8_t)v;
+    }
+  } else {
+    int32_t v = x < -max_val ? -127 : (x > max_val ? 127 : (int32_t)roundf(x * scale));
+    c = (uint8_t)(int8_t)v;
+  }
+  out[r * out_stride + j] = c;
+}
+
+hipError_t launch_row_minmax(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                             float* mins, float* maxs, uint32_t* notint, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u32_quant, dim3(1), dim3(64), 0, s, notint, 1u);
+  uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)n + 3) / 4, 65536);
+  hipLaunchKernelGGL(k_row_minmax, dim3(blocks), dim3(256), 0, s, X, stride_f, n, dim, mins,
+                     maxs, notint);
+  return hipGetLastError();
+}
+
+hipError_t launch_encode_q8(const float* X, uint64_t stride_f, uint32_t nrows, uint32_t dim,
+                            int metric, float slope, int32_t offset, float max_val, float scale,
+                            uint8_t* out, uint64_t out_stride, hipStream_t s) {
+  const uint64_t total = (uint64_t)nrows * dim;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_encode_q8, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s, X,
+                     stride_f, nrows, dim, metric, slope, offset, max_val, scale, out,
+                     out_stride);
+  return hipGetLastError();
+}
+
 }  // namespace pann

@@ -10,8 +10,9 @@ python/graph_index.cpp ``GraphIndex`` (batch_search :192-216, single_search
 intended QueryParams mapping (k = knn, beamSize = beam_width, limit =
 visit_limit if > 0 else n, degree_limit = max degree) instead of the stale
 ``QueryParams(knn, beam_width, 1.35, ...)`` call (graph_index.cpp:198).
-``quant=True`` searches the FP8 e4m3 store and reranks in FP32 (the
-reference's quant path uses its own uint8/int8 quantizers, :141-186).
+``quant=True`` searches the reference's own 8-bit scalar codes (uint8 affine
+for Euclidian, trimmed int8 for MIPS, :141-186) and reranks in FP32; the FP8
+and BF16 stores are available through DeviceIndex.
 """
 from __future__ import annotations
 
@@ -26,7 +27,7 @@ from . import formats
 _DT = {np.dtype(np.float32): L.DT_F32, np.dtype(np.uint8): L.DT_U8, np.dtype(np.int8): L.DT_I8}
 _METRIC = {"Euclidian": L.METRIC_L2, "l2": L.METRIC_L2, "L2": L.METRIC_L2,
            "mips": L.METRIC_MIPS, "ip": L.METRIC_MIPS, "MIPS": L.METRIC_MIPS}
-_STORE = {"native": L.STORE_NATIVE, "bf16": L.STORE_BF16, "fp8": L.STORE_FP8}
+_STORE = {"native": L.STORE_NATIVE, "bf16": L.STORE_BF16, "fp8": L.STORE_FP8, "q8": L.STORE_Q8}
 _SUM = {"inorder": L.SUM_INORDER, "tree": L.SUM_TREE}
 
 
@@ -74,11 +75,20 @@ class DeviceIndex:
 
     # -- stores --------------------------------------------------------------
     def build_store(self, kind: str):
-        """Reduced-precision copy of float points: 'bf16' or 'fp8' (e4m3,
-        per-dimension absmax scales)."""
+        """Reduced-precision copy of float points: 'bf16', 'fp8' (e4m3,
+        per-dimension absmax scales) or 'q8' (the reference's 8-bit scalar
+        quantization: uint8 affine for L2, trimmed int8 for MIPS)."""
         if kind not in self._stores:
             L.check(L.lib().pann_build_store(self._h, _STORE[kind]))
             self._stores.add(kind)
+
+    def q8_params(self) -> dict:
+        """Parameters of the 'q8' store (the reference's 8-bit scalar quantization)."""
+        sl, mx, sc = C.c_float(), C.c_float(), C.c_float()
+        off = C.c_int32()
+        L.check(L.lib().pann_get_q8_params(self._h, C.byref(sl), C.byref(off), C.byref(mx),
+                                           C.byref(sc)))
+        return {"slope": sl.value, "offset": off.value, "max_val": mx.value, "scale": sc.value}
 
     def fp8_scales(self) -> np.ndarray:
         s = np.zeros(self.dim, np.float32)
@@ -226,9 +236,9 @@ class GraphIndex:
             # marks missing ones with id 0xFFFFFFFF, distance +inf
             use_q = quant and self.dtype == np.float32
             if use_q:
-                self._hnsw.build_store("fp8")
+                self._hnsw.build_store("q8")
             r = self._hnsw.search(Q, knn, max(beam_width, knn), limit=limit or 0,
-                                  store="fp8" if use_q else "native", rerank=use_q)
+                                  store="q8" if use_q else "native", rerank=use_q)
             ids = np.full((len(Q), knn), 0xFFFFFFFF, np.uint32)
             dists = np.full((len(Q), knn), np.inf, np.float32)
             w = min(knn, r["ids"].shape[1])
@@ -237,10 +247,10 @@ class GraphIndex:
             dists[:, :w][have] = r["dists"][:, :w][have]
             return ids, dists
         use_q = quant and self.dtype == np.float32
-        if use_q:
-            self._dev.build_store("fp8")
+        if use_q:  # the reference's quant path: 8-bit scalar codes, float rerank
+            self._dev.build_store("q8")
         r = self._dev.search(Q, knn, beam_width, limit=limit,
-                             store="fp8" if use_q else "native", rerank=use_q)
+                             store="q8" if use_q else "native", rerank=use_q)
         return r["ids"], r["dists"]
 
     search = batch_search
