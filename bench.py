@@ -292,32 +292,48 @@ def main():
         tk, tb = torch.from_numpy(ks).to(dev), torch.from_numpy(bs).to(dev)
         ui = torch.zeros((nq, 100), dtype=torch.int32, device=dev)
         ud = torch.zeros((nq, 100), dtype=torch.float32, device=dev)
+        # the whole query set in one persistent launch, captured once into a
+        # hipGraph and replayed each step
+        gs = torch.cuda.Stream()
         for leg in [l for l in ("fp32-tree", "fp8-rerank") if l in legs]:
             opts = LEG_OPTS[leg]
-            run = lambda a=False: idx.search_device(tq, 100, 256, ui, ud, out_visited=ov,
-                                                    out_cmps=oc, k_per_query=tk,
-                                                    beam_per_query=tb, async_=a,
-                                                    rerank_factor=args.rerank_factor, **opts)
+            run = lambda a=False, st=None: idx.search_device(
+                tq, 100, 256, ui, ud, out_visited=ov, out_cmps=oc, k_per_query=tk,
+                beam_per_query=tb, async_=a, stream=st, rerank_factor=args.rerank_factor,
+                **opts)
             run()
             torch.cuda.synchronize()
             ids = ui.cpu().numpy().view(np.uint32)
             sel = ks >= 10
             rec = recall_plain(gt[sel], ids[sel][:, :10], 10)
+            with torch.cuda.stream(gs):
+                run(True, gs)  # first use of the stream, outside the capture
+            gs.synchronize()
+            idx.search_status(gs)
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, stream=gs):
+                run(True, gs)
             for _ in range(args.warmup):
-                run(True)
+                graph.replay()
             torch.cuda.synchronize()
-            kms = []
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in range(args.steps)]
             t0 = time.perf_counter()
-            for _ in range(args.steps):
-                run(True)
-                kms.append(idx.last_kernel_ms())
+            for e0, e1 in ev:
+                e0.record(gs)
+                graph.replay()
+                e1.record(gs)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
-            idx.search_status()
+            idx.search_status(gs)
+            gms = [e0.elapsed_time(e1) for e0, e1 in ev]
+            same = np.array_equal(ui.cpu().numpy().view(np.uint32)[sel][:, :10], ids[sel][:, :10])
             results["uneven-" + leg] = dict(
                 qps=nq * args.steps / dt, ms_per_step=dt * 1e3 / args.steps,
-                kernel_ms=float(np.mean(kms)), recall_at_10_k_ge_10=round(rec, 5),
-                k_choices=[1, 10, 100], beam_range=[16, 256])
+                graph_ms=float(np.mean(gms)), recall_at_10_k_ge_10=round(rec, 5),
+                replay_equals_eager=bool(same), k_choices=[1, 10, 100], beam_range=[16, 256],
+                launch="hipGraph replay of one persistent launch")
+            del graph
             log(f"[bench] uneven {leg}: QPS {nq * args.steps / dt:,.0f} recall@10(k>=10) {rec:.4f}")
 
     cpu = None

@@ -28,6 +28,8 @@
 
 #include <float.h>
 
+#include <atomic>
+
 namespace pann {
 
 struct WaveLds {
@@ -487,7 +489,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   const uint64_t* F = cur_buf ? L.F1 : L.F0;
   uint32_t* oi = a.out_ids + (uint64_t)q * a.out_stride;
   float* od = a.out_dists + (uint64_t)q * a.out_stride;
-  if (f < k) status |= PANN_ST_SHORT_BEAM;
+  if (f < k && !a.short_ok) status |= PANN_ST_SHORT_BEAM;
   if (a.full) {
     double nc = (double)k * a.rerank_factor;
     int nchk = nc < (double)f ? (int)nc : f;
@@ -556,9 +558,15 @@ template <int ST, int MT, int NCH, bool SEQ, bool EX>
 static hipError_t launch_k(const SearchArgs& a, uint32_t grid, hipStream_t s) {
   uint32_t lds = search_lds_bytes(a);
   auto kern = k_beam_search<ST, MT, NCH, SEQ, EX>;
-  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds);
-  if (e != hipSuccess) return e;
+  // raise the kernel's dynamic-LDS limit only when a launch needs more, so a
+  // repeated launch makes no attribute call (it may be under graph capture)
+  static std::atomic<uint32_t> lds_set{0};
+  if (lds > lds_set.load()) {
+    hipError_t e = hipFuncSetAttribute((const void*)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    lds_set.store(lds);
+  }
   hipLaunchKernelGGL(kern, dim3(grid), dim3(PANN_WAVE), lds, s, a);
   return hipGetLastError();
 }

@@ -43,6 +43,8 @@ struct SearchArgs {
   uint32_t starts_stride;
   // 1: add this launch's visits and evaluations to out_visited / out_cmps
   uint32_t accumulate;
+  // 1: a frontier shorter than k is a result, not PANN_ST_SHORT_BEAM
+  uint32_t short_ok;
   // QueryParams (types.h:233-268); per-query k / beam override the uniform ones
   int32_t k, beam;
   const int32_t* k_pq;

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -98,8 +99,6 @@ struct pann_index_s {
   uint64_t q8_stride = 0;
   float q8_slope = 0, q8_max = 0, q8_scale = 0;
   int32_t q8_offset = 0;
-  uint8_t* d_qcodes = nullptr;     // per-search query codes for PANN_STORE_Q8
-  size_t qcodes_cap = 0;
   uint32_t* d_nbrs = nullptr;
   uint32_t nbr_stride = 0;
   // graph per level of a layered (HNSW) index; level 0 is d_nbrs above
@@ -108,18 +107,29 @@ struct pann_index_s {
     uint32_t max_degree = 0, nbr_stride = 0;
   };
   std::vector<Level> levels;
-  uint32_t* d_eps = nullptr;  // per-query entry points of the HNSW descent
-  float* d_epsd = nullptr;
-  uint32_t* d_epsn = nullptr;
-  size_t eps_cap = 0;
-  uint32_t* d_counter = nullptr;
-  uint32_t* d_status = nullptr;
-  uint32_t* d_starts = nullptr;
-  unsigned long long* d_ghash = nullptr;
-  size_t ghash_words = 0;
-  uint32_t ghash_tag = 0;  // tags handed out so far (table zeroed when they wrap)
-  uint64_t* d_vis = nullptr;
-  size_t vis_words = 0;
+  // Launch state per stream: launches on different streams may run at the
+  // same time, so each stream has its own work counter, status word, start
+  // points, scratch and events.  Created at a stream's first use (outside
+  // any graph capture).
+  struct StreamCtx {
+    uint32_t* d_counter = nullptr;
+    uint32_t* d_status = nullptr;
+    uint32_t* d_starts = nullptr;
+    std::vector<uint32_t> starts_host;  // what d_starts holds (copied only on change)
+    unsigned long long* d_ghash = nullptr;
+    size_t ghash_words = 0;
+    uint32_t ghash_tag = 0;  // tags handed out so far (table zeroed when they wrap)
+    uint64_t* d_vis = nullptr;
+    size_t vis_words = 0;
+    uint32_t* d_eps = nullptr;  // per-query entry points of the HNSW descent
+    float* d_epsd = nullptr;
+    size_t eps_cap = 0;
+    uint8_t* d_qcodes = nullptr;  // per-search query codes for PANN_STORE_Q8
+    size_t qcodes_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  };
+  std::unordered_map<hipStream_t, StreamCtx> streams;
+  StreamCtx* last_timed = nullptr;
   uint32_t* d_dummy = nullptr;
   uint64_t* d_pkeys = nullptr;   // prune sort scratch for groups too large for LDS
   uint8_t* d_pflags = nullptr;
@@ -127,8 +137,6 @@ struct pann_index_s {
   int num_cus = 0;
   int64_t lds_per_cu = 0;
   int64_t max_lds_block = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool timed = false;
   std::mutex mu;
 };
 
@@ -136,32 +144,67 @@ namespace {
 
 // The overflow table holds (tag << 32 | id) slots; tag 0 never occurs, so a
 // zeroed table is empty for every query.
-pann_status ensure_ghash(pann_index_t idx, size_t words, uint32_t nq) {
-  if (idx->ghash_words < words) {
-    if (idx->d_ghash) (void)hipFree(idx->d_ghash);
-    idx->d_ghash = nullptr;
-    idx->ghash_words = 0;
-    HIP_TRY(hipMalloc(&idx->d_ghash, words * 8), "hipMalloc(visited overflow table)");
-    HIP_TRY(hipMemset(idx->d_ghash, 0, words * 8), "hipMemset(visited overflow table)");
-    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    idx->ghash_words = words;
-    idx->ghash_tag = 0;
+using StreamCtx = pann_index_s::StreamCtx;
+
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
+}
+
+// The caller holds idx->mu.
+pann_status stream_ctx(pann_index_t idx, hipStream_t s, StreamCtx** out) {
+  auto it = idx->streams.find(s);
+  if (it == idx->streams.end()) {
+    if (capturing(s))
+      return fail(PANN_ERR_ARG, "first use of a stream under graph capture: run the same "
+                                "search once on that stream before capturing it");
+    StreamCtx c;
+    hipError_t e = hipMalloc(&c.d_counter, 256);
+    if (e == hipSuccess) e = hipMalloc(&c.d_status, 256);
+    if (e == hipSuccess) e = hipMalloc(&c.d_starts, 64 * 4);
+    if (e == hipSuccess) e = hipEventCreate(&c.ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c.ev1);
+    if (e != hipSuccess) {
+      for (void* p : {(void*)c.d_counter, (void*)c.d_status, (void*)c.d_starts})
+        if (p) (void)hipFree(p);
+      if (c.ev0) (void)hipEventDestroy(c.ev0);
+      if (c.ev1) (void)hipEventDestroy(c.ev1);
+      return hip_fail(e, "stream context allocation");
+    }
+    it = idx->streams.emplace(s, std::move(c)).first;
   }
-  if ((uint64_t)idx->ghash_tag + nq + 1 >= 0xFFFFFFFFull) {  // tags about to wrap
-    HIP_TRY(hipMemset(idx->d_ghash, 0, idx->ghash_words * 8), "hipMemset(visited overflow table)");
+  *out = &it->second;
+  return PANN_OK;
+}
+
+// The overflow table holds (tag << 32 | id) slots; tag 0 never occurs, so a
+// zeroed table is empty for every query.
+pann_status ensure_ghash(StreamCtx& C, size_t words, uint32_t nq) {
+  if (C.ghash_words < words) {
+    if (C.d_ghash) (void)hipFree(C.d_ghash);
+    C.d_ghash = nullptr;
+    C.ghash_words = 0;
+    HIP_TRY(hipMalloc(&C.d_ghash, words * 8), "hipMalloc(visited overflow table)");
+    HIP_TRY(hipMemset(C.d_ghash, 0, words * 8), "hipMemset(visited overflow table)");
     HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    idx->ghash_tag = 0;
+    C.ghash_words = words;
+    C.ghash_tag = 0;
+  }
+  if ((uint64_t)C.ghash_tag + nq + 1 >= 0xFFFFFFFFull) {  // tags about to wrap
+    HIP_TRY(hipMemset(C.d_ghash, 0, C.ghash_words * 8), "hipMemset(visited overflow table)");
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    C.ghash_tag = 0;
   }
   return PANN_OK;
 }
 
-pann_status ensure_vis(pann_index_t idx, size_t words) {
-  if (idx->vis_words >= words) return PANN_OK;
-  if (idx->d_vis) (void)hipFree(idx->d_vis);
-  idx->d_vis = nullptr;
-  idx->vis_words = 0;
-  HIP_TRY(hipMalloc(&idx->d_vis, words * 8), "hipMalloc(visited lists)");
-  idx->vis_words = words;
+pann_status ensure_vis(StreamCtx& C, size_t words) {
+  if (C.vis_words >= words) return PANN_OK;
+  if (C.d_vis) (void)hipFree(C.d_vis);
+  C.d_vis = nullptr;
+  C.vis_words = 0;
+  HIP_TRY(hipMalloc(&C.d_vis, words * 8), "hipMalloc(visited lists)");
+  C.vis_words = words;
   return PANN_OK;
 }
 
@@ -255,12 +298,7 @@ pann_status pann_open(const pann_index_desc* desc, pann_index_t* out) {
   idx->levels[0].d_nbrs = idx->d_nbrs;
   idx->levels[0].max_degree = md;
   idx->levels[0].nbr_stride = idx->nbr_stride;
-  e = hipMalloc(&idx->d_counter, 256);
-  if (e == hipSuccess) e = hipMalloc(&idx->d_status, 256);
-  if (e == hipSuccess) e = hipMalloc(&idx->d_starts, 64 * 4);
-  if (e == hipSuccess) e = hipMalloc(&idx->d_dummy, 256);
-  if (e == hipSuccess) e = hipEventCreate(&idx->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&idx->ev1);
+  e = hipMalloc(&idx->d_dummy, 256);
   if (e != hipSuccess) return cleanup(hip_fail(e, "scratch allocation"));
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return cleanup(hip_fail(e, "hipDeviceSynchronize"));
@@ -334,26 +372,26 @@ pann_status build_q8(pann_index_t idx, uint64_t fstride) {
 
 // For PANN_STORE_Q8 the kernel reads query codes: encode the float queries
 // into the index's scratch on `stream`.  Other stores read the queries as given.
-pann_status stage_queries(pann_index_t idx, int32_t store, const void* queries, uint32_t nq,
-                          pann::SearchArgs& a, hipStream_t stream) {
+pann_status stage_queries(pann_index_t idx, StreamCtx& C, int32_t store, const void* queries,
+                          uint32_t nq, pann::SearchArgs& a, hipStream_t stream) {
   if (store != PANN_STORE_Q8) return PANN_OK;
   const size_t need = (size_t)nq * idx->dim;
-  if (idx->qcodes_cap < need) {
-    if (idx->d_qcodes) {
+  if (C.qcodes_cap < need) {
+    if (C.d_qcodes) {
       HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
-      (void)hipFree(idx->d_qcodes);
+      (void)hipFree(C.d_qcodes);
     }
-    idx->d_qcodes = nullptr;
-    idx->qcodes_cap = 0;
-    HIP_TRY(hipMalloc(&idx->d_qcodes, need), "hipMalloc(query codes)");
-    idx->qcodes_cap = need;
+    C.d_qcodes = nullptr;
+    C.qcodes_cap = 0;
+    HIP_TRY(hipMalloc(&C.d_qcodes, need), "hipMalloc(query codes)");
+    C.qcodes_cap = need;
   }
   HIP_TRY(pann::launch_encode_q8((const float*)queries, idx->dim, nq, idx->dim,
                                  idx->metric == PANN_METRIC_L2 ? 0 : 1, idx->q8_slope,
-                                 idx->q8_offset, idx->q8_max, idx->q8_scale, idx->d_qcodes,
+                                 idx->q8_offset, idx->q8_max, idx->q8_scale, C.d_qcodes,
                                  idx->dim, stream),
           "query encode");
-  a.queries = (const char*)idx->d_qcodes;
+  a.queries = (const char*)C.d_qcodes;
   a.q_stride = idx->dim;
   return PANN_OK;
 }
@@ -436,7 +474,11 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
                            const uint32_t* starts, uint32_t nstarts, const StartSets* ss,
                            const pann_search_opts* opts, uint32_t nq, pann::SearchArgs& a,
                            int& store_kind, bool& sequential, uint32_t& grid,
-                           hipStream_t stream) {
+                           hipStream_t stream, StreamCtx*& C) {
+  {
+    pann_status cst = stream_ctx(idx, stream, &C);
+    if (cst != PANN_OK) return cst;
+  }
   if (level >= idx->levels.size() || !idx->levels[level].d_nbrs)
     return fail(PANN_ERR_ARG, "graph level " + std::to_string(level) + " is not set");
   const auto& G = idx->levels[level];
@@ -515,10 +557,10 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   a.degree_limit = (int32_t)std::min<int64_t>(qp->degree_limit, G.max_degree);
   a.batch_factor = qp->batch_factor;
   a.rerank_factor = qp->rerank_factor;
-  a.status = idx->d_status;
-  a.work_counter = idx->d_counter;
+  a.status = C->d_status;
+  a.work_counter = C->d_counter;
   a.nstarts = ss ? 0u : nstarts;
-  a.starts = idx->d_starts;
+  a.starts = C->d_starts;
   if (ss) {
     a.starts_pq = ss->starts;
     a.nstarts_pq = ss->counts;
@@ -563,26 +605,38 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     uint32_t glog2 = std::max<uint32_t>(
         ceil_log2((uint64_t)2 * (10 + B) * a.degree_limit) + 1, hlog2 + 2);
     a.ghash_log2 = glog2;
-    pann_status st = ensure_ghash(idx, (size_t)grid << glog2, nq);
+    pann_status st = ensure_ghash(*C, (size_t)grid << glog2, nq);
     if (st != PANN_OK) return st;
-    a.ghash = idx->d_ghash;
-    a.ghash_tag_base = idx->ghash_tag;
-    idx->ghash_tag += nq;
+    a.ghash = C->d_ghash;
+    a.ghash_tag_base = C->ghash_tag;
+    C->ghash_tag += nq;
   }
-  if (!ss)
-    HIP_TRY(hipMemcpyAsync(idx->d_starts, starts, nstarts * 4, hipMemcpyHostToDevice, stream),
+  // start points go up only when they change, so a repeated search issues no
+  // host-to-device copy (and can be captured into a graph)
+  const bool cap = capturing(stream);
+  if (cap && o.exact_counts)
+    return fail(PANN_ERR_ARG, "exact_counts searches cannot be captured into a graph");
+  if (!ss && (C->starts_host.size() != nstarts ||
+              !std::equal(starts, starts + nstarts, C->starts_host.begin()))) {
+    if (cap)
+      return fail(PANN_ERR_ARG, "start points changed under graph capture: run the search "
+                                "once with these start points before capturing it");
+    HIP_TRY(hipMemcpyAsync(C->d_starts, starts, nstarts * 4, hipMemcpyHostToDevice, stream),
             "hipMemcpyAsync(starts)");
-  hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(1), 0, stream, idx->d_status);
+    HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    C->starts_host.assign(starts, starts + nstarts);
+  }
+  hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(1), 0, stream, C->d_status);
   HIP_TRY(hipGetLastError(), "status reset");
   return PANN_OK;
 }
 
 // allow_short: a frontier shorter than k is a result, not an error (HNSW
 // layers return whatever frontier they reach)
-pann_status check_status(pann_index_t idx, hipStream_t stream, bool allow_short = false) {
+pann_status check_status(StreamCtx& C, hipStream_t stream, bool allow_short = false) {
   HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
   uint32_t st = 0;
-  HIP_TRY(hipMemcpy(&st, idx->d_status, 4, hipMemcpyDeviceToHost), "hipMemcpy(status)");
+  HIP_TRY(hipMemcpy(&st, C.d_status, 4, hipMemcpyDeviceToHost), "hipMemcpy(status)");
   if (st & pann::PANN_ST_BAD_PARAMS)  // beamSearch.h:612-616 (k > beam) per query
     return fail(PANN_ERR_ARG, "a per-query k or beam is invalid (k > beam, beam > qp->beam_size, "
                               "or k > out_stride)");
@@ -619,8 +673,9 @@ pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
   int store_kind;
   bool sequential;
   uint32_t grid;
+  StreamCtx* C = nullptr;
   pann_status st = prepare_search(idx, 0, qp, starts, nstarts, nullptr, opts, nq, a, store_kind,
-                                  sequential, grid, stream);
+                                  sequential, grid, stream, C);
   if (st != PANN_OK) return st;
   const bool f32_queries = idx->dtype == PANN_DT_F32;
   a.queries = (const char*)queries;
@@ -636,16 +691,22 @@ pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
   a.out_visited = out_visited;
   a.out_cmps = out_cmps;
   a.out_nfront = out_nfront;
-  st = stage_queries(idx, opts ? opts->store : PANN_STORE_NATIVE, queries, nq, a, stream);
+  a.short_ok = allow_short ? 1u : 0u;
+  st = stage_queries(idx, *C, opts ? opts->store : PANN_STORE_NATIVE, queries, nq, a, stream);
   if (st != PANN_OK) return st;
   grid = std::min<uint32_t>(grid, nq);
-  HIP_TRY(hipEventRecord(idx->ev0, stream), "hipEventRecord");
+  const bool cap = capturing(stream);  // captured: no events, status read after replay
+  if (cap && !(opts && opts->async))
+    return fail(PANN_ERR_ARG, "a search under graph capture must be async");
+  if (!cap) HIP_TRY(hipEventRecord(C->ev0, stream), "hipEventRecord");
   HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
           "beam search launch");
-  HIP_TRY(hipEventRecord(idx->ev1, stream), "hipEventRecord");
-  idx->timed = true;
+  if (!cap) {
+    HIP_TRY(hipEventRecord(C->ev1, stream), "hipEventRecord");
+    idx->last_timed = C;
+  }
   if (opts && opts->async) return PANN_OK;
-  return check_status(idx, stream, allow_short);
+  return check_status(*C, stream, allow_short);
 }
 
 }  // namespace
@@ -662,14 +723,21 @@ pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
 pann_status pann_search_status(pann_index_t idx, void* stream) {
   if (!idx) return fail(PANN_ERR_ARG, "null index");
   HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
-  return check_status(idx, (hipStream_t)stream);
+  StreamCtx* C = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(idx->mu);
+    pann_status st = stream_ctx(idx, (hipStream_t)stream, &C);
+    if (st != PANN_OK) return st;
+  }
+  return check_status(*C, (hipStream_t)stream);
 }
 
 pann_status pann_last_kernel_ms(pann_index_t idx, float* ms) {
   if (!idx || !ms) return fail(PANN_ERR_ARG, "null argument");
-  if (!idx->timed) return fail(PANN_ERR_ARG, "no search has run");
-  HIP_TRY(hipEventSynchronize(idx->ev1), "hipEventSynchronize");
-  HIP_TRY(hipEventElapsedTime(ms, idx->ev0, idx->ev1), "hipEventElapsedTime");
+  StreamCtx* C = idx->last_timed;
+  if (!C) return fail(PANN_ERR_ARG, "no search has run");
+  HIP_TRY(hipEventSynchronize(C->ev1), "hipEventSynchronize");
+  HIP_TRY(hipEventElapsedTime(ms, C->ev0, C->ev1), "hipEventElapsedTime");
   return PANN_OK;
 }
 
@@ -791,10 +859,11 @@ pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint
   int store_kind;
   bool sequential;
   uint32_t grid;
+  StreamCtx* C = nullptr;
   pann_status st = prepare_search(idx, 0, &qp, &start, 1, nullptr, &o, nq, a, store_kind,
-                                  sequential, grid, stream);
+                                  sequential, grid, stream, C);
   if (st != PANN_OK) return st;
-  st = ensure_vis(idx, (size_t)nq * vis_cap);
+  st = ensure_vis(*C, (size_t)nq * vis_cap);
   if (st != PANN_OK) return st;
   a.queries = idx->d_pts;  // each query is the point's own row
   a.q_stride = idx->pts_stride;
@@ -802,7 +871,7 @@ pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint
   a.fq_stride = idx->pts_stride;
   a.nq = nq;
   a.self_ids = self_ids;
-  a.vis_out = idx->d_vis;
+  a.vis_out = C->d_vis;
   a.vis_cap = vis_cap;
   a.out_ids = idx->d_dummy;
   a.out_dists = (float*)idx->d_dummy;
@@ -812,9 +881,9 @@ pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint
   grid = std::min<uint32_t>(grid, nq);
   HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
           "build-mode search launch");
-  HIP_TRY(pann::launch_sort_visited(idx->d_vis, nq, vis_cap, vis_count, vis_ids, vis_dists, stream),
+  HIP_TRY(pann::launch_sort_visited(C->d_vis, nq, vis_cap, vis_count, vis_ids, vis_dists, stream),
           "visited sort launch");
-  return check_status(idx, stream);
+  return check_status(*C, stream);
 }
 
 namespace {
@@ -1033,8 +1102,9 @@ pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void*
   int store_kind;
   bool sequential;
   uint32_t grid;
+  StreamCtx* C = nullptr;
   pann_status st = prepare_search(idx, level, &qp, nullptr, 0, &ss, opts, nq, a, store_kind,
-                                  sequential, grid, stream);
+                                  sequential, grid, stream, C);
   if (st != PANN_OK) return st;
   const int32_t store = opts ? opts->store : PANN_STORE_NATIVE;
   if (self_ids) {  // query q is point self_ids[q]; same_as skips it
@@ -1049,7 +1119,7 @@ pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void*
     a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
     a.fqueries = (const char*)queries;
     a.fq_stride = (uint64_t)idx->dim * 4;
-    st = stage_queries(idx, store, queries, nq, a, stream);
+    st = stage_queries(idx, *C, store, queries, nq, a, stream);
     if (st != PANN_OK) return st;
   }
   a.nq = nq;
@@ -1059,11 +1129,12 @@ pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void*
   a.out_visited = out_visited;
   a.out_cmps = out_cmps;
   a.out_nfront = out_count;
+  a.short_ok = 1u;  // a layer returns whatever frontier it reaches
   grid = std::min<uint32_t>(grid, nq);
   HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
           "search_layer launch");
   if (opts && opts->async) return PANN_OK;
-  return check_status(idx, stream, true);
+  return check_status(*C, stream, true);
 }
 
 pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq, uint32_t k,
@@ -1086,21 +1157,27 @@ pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq,
   std::lock_guard<std::mutex> lk(idx->mu);
   HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
   hipStream_t stream = (hipStream_t)stream_;
-  if (idx->eps_cap < nq) {
-    if (idx->d_eps) (void)hipFree(idx->d_eps);
-    if (idx->d_epsd) (void)hipFree(idx->d_epsd);
-    idx->d_eps = nullptr;
-    idx->d_epsd = nullptr;
-    idx->eps_cap = 0;
-    HIP_TRY(hipMalloc(&idx->d_eps, (size_t)nq * 4), "hipMalloc(entry points)");
-    HIP_TRY(hipMalloc(&idx->d_epsd, (size_t)nq * 4), "hipMalloc(entry points)");
-    idx->eps_cap = nq;
+  StreamCtx* C = nullptr;
+  {
+    pann_status cst = stream_ctx(idx, stream, &C);
+    if (cst != PANN_OK) return cst;
+  }
+  if (C->eps_cap < nq) {
+    if (C->d_eps || C->d_epsd) HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (C->d_eps) (void)hipFree(C->d_eps);
+    if (C->d_epsd) (void)hipFree(C->d_epsd);
+    C->d_eps = nullptr;
+    C->d_epsd = nullptr;
+    C->eps_cap = 0;
+    HIP_TRY(hipMalloc(&C->d_eps, (size_t)nq * 4), "hipMalloc(entry points)");
+    HIP_TRY(hipMalloc(&C->d_epsd, (size_t)nq * 4), "hipMalloc(entry points)");
+    C->eps_cap = nq;
   }
   pann_search_opts o{};
   if (opts) o = *opts;
   const bool f32_queries = idx->dtype == PANN_DT_F32;
-  StartSets ss{idx->d_eps, nullptr, 1u};
-  HIP_TRY(hipEventRecord(idx->ev0, stream), "hipEventRecord");
+  StartSets ss{C->d_eps, nullptr, 1u};
+  HIP_TRY(hipEventRecord(C->ev0, stream), "hipEventRecord");
   // search_layer_to (HNSW.hpp:1545-1566): ef = 1 from the entrance through
   // levels top .. 1, keeping the closest; then layer 0 with ef (:1582-1586)
   for (int64_t l = top_level; l >= 0; l--) {
@@ -1117,29 +1194,31 @@ pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq,
     pann_search_opts lo = o;
     lo.rerank = last ? o.rerank : 0;
     const bool first = l == (int64_t)top_level;
+    StreamCtx* Cl = nullptr;
     pann_status st = prepare_search(idx, (uint32_t)l, &qp, entrance, nentrance,
                                     first ? nullptr : &ss, &lo, nq, a, store_kind, sequential,
-                                    grid, stream);
+                                    grid, stream, Cl);
     if (st != PANN_OK) return st;
     a.queries = (const char*)queries;
     a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
     a.fqueries = (const char*)queries;
     a.fq_stride = (uint64_t)idx->dim * 4;
     if (first) {
-      st = stage_queries(idx, o.store, queries, nq, a, stream);
+      st = stage_queries(idx, *C, o.store, queries, nq, a, stream);
       if (st != PANN_OK) return st;
     } else if (o.store == PANN_STORE_Q8) {
-      a.queries = (const char*)idx->d_qcodes;
+      a.queries = (const char*)C->d_qcodes;
       a.q_stride = idx->dim;
     }
     a.nq = nq;
-    a.out_ids = last ? out_ids : idx->d_eps;  // in place: each wave reads its start first
-    a.out_dists = last ? out_dists : idx->d_epsd;
+    a.out_ids = last ? out_ids : C->d_eps;  // in place: each wave reads its start first
+    a.out_dists = last ? out_dists : C->d_epsd;
     a.out_stride = last ? out_stride : 1u;
     a.out_nfront = last ? out_count : nullptr;
     a.out_visited = out_visited;
     a.out_cmps = out_cmps;  // count_cmps sums every level (:1553, :1097-1098)
     a.accumulate = first ? 0u : 1u;
+    a.short_ok = 1u;
     if (last && ef_pq) {  // per-query ef: the whole frontier comes back
       a.k_pq = ef_pq;
       a.beam_pq = ef_pq;
@@ -1151,10 +1230,10 @@ pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq,
   HIP_TRY(pann::launch_hnsw_cut(out_ids, out_dists, out_stride, out_count, nq, k, k_pq,
                                 node_ids, stream),
           "HNSW cut launch");
-  HIP_TRY(hipEventRecord(idx->ev1, stream), "hipEventRecord");
-  idx->timed = true;
+  HIP_TRY(hipEventRecord(C->ev1, stream), "hipEventRecord");
+  idx->last_timed = C;
   if (o.async) return PANN_OK;
-  return check_status(idx, stream, true);
+  return check_status(*C, stream, true);
 }
 
 pann_status pann_hnsw_select_neighbors(pann_index_t idx, uint32_t level, const uint32_t* pts,
@@ -1206,17 +1285,20 @@ pann_status pann_close(pann_index_t idx) {
   (void)hipSetDevice(idx->device);
   (void)hipDeviceSynchronize();
   void* ptrs[] = {idx->d_pts, idx->d_bf16, idx->d_fp8, idx->d_scales, idx->d_absmax, idx->d_nbrs,
-                  idx->d_counter, idx->d_status, idx->d_starts, idx->d_ghash, idx->d_vis,
-                  idx->d_dummy, idx->d_pkeys, idx->d_pflags};
+                  idx->d_q8, idx->d_dummy, idx->d_pkeys, idx->d_pflags};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (size_t l = 1; l < idx->levels.size(); l++)
     if (idx->levels[l].d_nbrs) (void)hipFree(idx->levels[l].d_nbrs);
-  void* eps[] = {idx->d_eps, idx->d_epsd, idx->d_epsn, idx->d_q8, idx->d_qcodes};
-  for (void* p : eps)
-    if (p) (void)hipFree(p);
-  if (idx->ev0) (void)hipEventDestroy(idx->ev0);
-  if (idx->ev1) (void)hipEventDestroy(idx->ev1);
+  for (auto& kv : idx->streams) {
+    StreamCtx& C = kv.second;
+    void* cp[] = {C.d_counter, C.d_status, C.d_starts, C.d_ghash, C.d_vis, C.d_eps, C.d_epsd,
+                  C.d_qcodes};
+    for (void* p : cp)
+      if (p) (void)hipFree(p);
+    if (C.ev0) (void)hipEventDestroy(C.ev0);
+    if (C.ev1) (void)hipEventDestroy(C.ev1);
+  }
   delete idx;
   return PANN_OK;
 }

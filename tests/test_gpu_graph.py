@@ -1,0 +1,56 @@
+"""The batched uneven search (per-query k and beam, one persistent launch)
+captured into a hipGraph and replayed: every replay, with eager work and
+synchronisation between replays, returns the eager result bit for bit."""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a HIP device")]
+
+
+@pytest.mark.parametrize("store", ["native", "q8"])
+def test_gpu_graph_replay_equals_eager(oracle, store):
+    import torch
+    from parlayann_amd import DeviceIndex, datagen
+    print(f"[graph] store={store}", flush=True)
+    X, Q = datagen.workload_f32_l2(n=5000, nq=700, d=32)
+    adj = oracle.build_vamana(X, "l2", 24, 48, 1.2, datagen.insertion_orders(len(X), 1, 9))
+    idx = DeviceIndex(X, adj, "l2")
+    if store != "native":
+        idx.build_store(store)
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(3)
+    ks = rng.choice([1, 10, 50], len(Q)).astype(np.int32)
+    bs = np.array([rng.integers(max(16, k), 97) for k in ks], np.int32)
+    tq = torch.from_numpy(Q).to(dev)
+    tk, tb = torch.from_numpy(ks).to(dev), torch.from_numpy(bs).to(dev)
+    oi = torch.zeros((len(Q), 50), dtype=torch.int32, device=dev)
+    od = torch.zeros((len(Q), 50), dtype=torch.float32, device=dev)
+    kw = dict(k_per_query=tk, beam_per_query=tb, store=store, rerank=store != "native",
+              summation="inorder")
+    # columns past a query's k are not written: fill them the way every
+    # replay below does
+    oi.fill_(-1)
+    od.fill_(0.0)
+    idx.search_device(tq, 50, 96, oi, od, **kw)
+    ref_i, ref_d = oi.clone(), od.clone()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):  # first use of the stream happens outside the capture
+        idx.search_device(tq, 50, 96, oi, od, stream=s, async_=True, **kw)
+    s.synchronize()
+    idx.search_status(s)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        idx.search_device(tq, 50, 96, oi, od, stream=s, async_=True, **kw)
+    for rep in range(4):
+        oi.fill_(-1)  # eager work between replays
+        od.fill_(0.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        idx.search_status(s)
+        assert torch.equal(oi, ref_i), rep
+        assert torch.equal(od, ref_d), rep
+    idx.close()

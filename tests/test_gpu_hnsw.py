@@ -154,7 +154,7 @@ def test_gpu_mixed_batch_equals_alone(oracle, small_l2):
     rng = np.random.default_rng(4)
     ks = rng.choice([1, 10, 50], len(Q)).astype(np.int32)
     efs = np.maximum(ks, rng.choice([8, 24, 64], len(Q))).astype(np.int32)
-    srv = BatchServer(idx, 64, 1 << 20, summation="inorder")
+    srv = BatchServer(idx, 64, 1 << 20, slots=2, summation="inorder")
     ids, cnt = srv.search_all(Q, ks, efs)
     for k in np.unique(ks):
         for ef in np.unique(efs):
@@ -165,7 +165,40 @@ def test_gpu_mixed_batch_equals_alone(oracle, small_l2):
             assert np.array_equal(cnt[sel], oc), (k, ef)
             for j, q in enumerate(sel):
                 assert np.array_equal(ids[q, :oc[j]], oi[j, :oc[j]]), (k, ef, q)
-    # open loop: every request is answered, latencies are positive
+    # open loop with two slots: batches overlap on the device and every
+    # request gets the same result as in the closed-loop run above
     arr = poisson_arrivals(len(Q), 20000.0, seed=1)
-    res = srv.serve(Q, ks, efs, arr)
+    res, sids, scnt = srv.serve(Q, ks, efs, arr, keep_results=True)
     assert np.all(res.latency_s > 0) and res.batches >= 1
+    assert np.array_equal(scnt, cnt)
+    for q in range(len(Q)):
+        assert np.array_equal(sids[q, :cnt[q]], ids[q, :cnt[q]]), q
+
+
+def test_gpu_concurrent_streams_equal_serial(oracle, small_l2):
+    """Async searches on two streams at once (each stream has its own launch
+    state in the library) give the same results as one at a time."""
+    import torch
+    from parlayann_amd import hnsw
+    X, lev, adj, ent = small_l2
+    idx = hnsw.HNSWIndex(X, model_of(X, lev, adj, ent, M, EFC), "l2")
+    dev = torch.device("cuda", 0)
+    Qa, Qb = _queries(X, 400, seed=31), _queries(X, 400, seed=32)
+    ref = [idx.search(Q, 10, 48) for Q in (Qa, Qb)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = []
+    for Q, st in zip((Qa, Qb), streams):
+        tq = torch.from_numpy(Q).to(dev)
+        oi = torch.zeros((len(Q), 48), dtype=torch.int32, device=dev)
+        od = torch.zeros((len(Q), 48), dtype=torch.float32, device=dev)
+        oc = torch.zeros(len(Q), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        idx.search_device(tq, 10, 48, oi, od, oc, stream=st, async_=True)
+        outs.append((oi, od, oc, st))
+    for (oi, od, oc, st), r in zip(outs, ref):
+        idx.base.search_status(st)
+        c = oc.cpu().numpy()
+        assert np.array_equal(c, r["count"])
+        ids = oi.cpu().numpy().view(np.uint32)
+        for q in range(len(c)):
+            assert np.array_equal(ids[q, :c[q]], r["ids"][q, :c[q]])
