@@ -46,12 +46,13 @@ WORKLOADS = {
     # precision: 10M x 96 unit anisotropic Gaussian, inner product, Vamana R=64 L=128 a=1.0
     "aniso-ip-10M": dict(kind="aniso", metric="ip", n=10_000_000, nq=100_000, d=96, R=64,
                          L=128, alpha=1.0, k=10, legs=["fp32-tree", "fp32-inorder",
-                                                       "bf16-rerank", "fp8-rerank"],
+                                                       "bf16-rerank", "fp8-rerank",
+                                                       "q8-rerank"],
                          headline="fp8-rerank", uneven=True),
     # serving: HNSW M=32 (layer-0 degree 64) efc=200 over 1M x 200 unit vectors
     # (500-centre mixture, queries shifted 0.5 sigma), IP, k in {10, 100}
     "hnsw-ip-1M": dict(kind="hnsw", metric="ip", n=1_000_000, nq=10_000, d=200, m=32, efc=200,
-                       k=10, legs=["fp32-tree", "bf16-rerank", "fp8-rerank"],
+                       k=10, legs=["fp32-tree", "bf16-rerank", "fp8-rerank", "q8-rerank"],
                        headline="fp8-rerank"),
 }
 EFS = [16, 20, 24, 32, 40, 48, 64, 80, 100, 128, 160, 200, 256, 320, 400]
@@ -81,6 +82,8 @@ LEG_OPTS = {
     "fp32-inorder": dict(store="native", summation="inorder", rerank=False),
     "bf16-rerank": dict(store="bf16", summation="tree", rerank=True),
     "fp8-rerank": dict(store="fp8", summation="tree", rerank=True),
+    # the reference's own 8-bit scalar quantization (its Python quant path)
+    "q8-rerank": dict(store="q8", summation="tree", rerank=True),
     "u8": dict(store="native", summation="tree", rerank=False),
 }
 
@@ -88,9 +91,22 @@ LEG_OPTS = {
 def row_bytes(leg, d, kind):
     if leg.startswith("bf16"):
         return 2 * d
-    if leg.startswith("fp8") or kind == "u8":
+    if leg.startswith("fp8") or leg.startswith("q8") or kind == "u8":
         return d
     return 4 * d
+
+
+def max_rel_dist_error(X, Q, ids, dists, metric, chunk=8192):
+    """max |d - d64| / |d64| over the returned items, d64 the FP64 distance."""
+    worst = 0.0
+    for s in range(0, len(Q), chunk):
+        I = ids[s:s + chunk].astype(np.int64)
+        P = X[I.reshape(-1)].astype(np.float64).reshape(I.shape + (X.shape[1],))
+        q = Q[s:s + chunk].astype(np.float64)[:, None, :]
+        d64 = ((P - q) ** 2).sum(-1) if metric == "l2" else -(P * q).sum(-1)
+        err = np.abs(dists[s:s + chunk].astype(np.float64) - d64) / np.maximum(np.abs(d64), 1e-30)
+        worst = max(worst, float(err.max()))
+    return worst
 
 
 def measured_traffic(workload, w, leg, beam):
@@ -132,6 +148,8 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against a copy of known size)")
     ap.add_argument("--cpu-seconds", type=float, default=6.0,
                     help="target wall time of one CPU-baseline repetition")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the launch-shape autotuner (library defaults)")
     ap.add_argument("--cache", default="",
                     help="directory holding the built graph and groundtruth of this workload "
                          "(written on the first run, loaded by later ones, e.g. profiler passes)")
@@ -142,7 +160,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from parlayann_amd import DeviceIndex, datagen, device_info
+    from parlayann_amd import DeviceIndex, autotune, datagen, device_info
     from parlayann_amd.groundtruth import groundtruth
     from parlayann_amd.recall import recall_plain, recall_with_ties
     from parlayann_amd.vamana import build_vamana
@@ -230,9 +248,9 @@ def main():
             rec = recall_plain(gt, ids, k)
             log(f"[bench] {leg} beam {B}: recall@10 {rec:.4f}")
             if best is None or rec > best[1] + 1e-4:
-                best = (B, rec, ids)
+                best = (B, rec, ids, od.cpu().numpy())
             if rec >= args.target_recall:
-                chosen = (B, rec, ids)
+                chosen = (B, rec, ids, od.cpu().numpy())
                 break
         reached = chosen is not None
         if not reached:
@@ -241,17 +259,25 @@ def main():
             log(f"[bench] {leg}: recall {args.target_recall} not reached; "
                 f"using the best beam {best[0]} (recall {best[1]:.4f})")
             chosen = best
-        B, rec, ids = chosen
+        B, rec, ids, dists = chosen
         rec_ties = recall_with_ties(X, Q, gt, ids, k, w["metric"])
+        mre = max_rel_dist_error(X, Q, ids, dists, w["metric"] if w["kind"] != "u8" else "l2")
+        tuned = {"waves_per_cu": 0, "visits_per_round": 0}
+        if not args.no_tune:
+            tuned = autotune.tune(idx, tq, k, B, rerank_factor=args.rerank_factor, **opts)
+            log(f"[bench] {leg}: tuned waves/CU {tuned['waves_per_cu']}, visits/round "
+                f"{tuned['visits_per_round']}")
+        shape = {"waves_per_cu": tuned["waves_per_cu"],
+                 "visits_per_round": tuned["visits_per_round"]}
         for _ in range(args.warmup):
             idx.search_device(tq, k, B, oi, od, out_visited=ov, out_cmps=oc, async_=True,
-                              rerank_factor=args.rerank_factor, **opts)
+                              rerank_factor=args.rerank_factor, **opts, **shape)
         torch.cuda.synchronize()
         kms = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             idx.search_device(tq, k, B, oi, od, out_visited=ov, out_cmps=oc, async_=True,
-                              rerank_factor=args.rerank_factor, **opts)
+                              rerank_factor=args.rerank_factor, **opts, **shape)
             kms.append(idx.last_kernel_ms())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
@@ -274,11 +300,12 @@ def main():
         kms_avg = float(np.mean(kms))
         gbs = bytes_launch / (kms_avg * 1e-3) / 1e9
         results[leg] = dict(beam=B, recall_at_10=round(rec, 5), recall_ties=round(rec_ties, 5),
-                            target_reached=reached,
+                            target_reached=reached, max_rel_dist_error=mre,
                             qps=qps, ms_per_step=dt * 1e3 / args.steps, kernel_ms=kms_avg,
                             avg_visited=float(vis.mean()), avg_dist_cmps=float(cmps.mean()),
                             avg_evals=float(evals.mean()),
-                            bytes_per_launch=bytes_launch, achieved_gbs=gbs)
+                            bytes_per_launch=bytes_launch, achieved_gbs=gbs,
+                            launch_shape=dict(shape, tuned=not args.no_tune))
         log(f"[bench] {leg}: beam {B} recall {rec:.4f} QPS {qps:,.0f} kernel {kms_avg:.3f} ms "
             f"{gbs:,.0f} GB/s visited {vis.mean():.1f} cmps {cmps.mean():.1f} "
             f"evals {evals.mean():.1f}")
@@ -297,10 +324,16 @@ def main():
         gs = torch.cuda.Stream()
         for leg in [l for l in ("fp32-tree", "fp8-rerank") if l in legs]:
             opts = LEG_OPTS[leg]
+            shape = {"waves_per_cu": 0, "visits_per_round": 0}
+            if not args.no_tune:
+                t = autotune.tune(idx, tq, 100, 256, k_per_query=tk, beam_per_query=tb,
+                                  out_stride=100, rerank_factor=args.rerank_factor, **opts)
+                shape = {"waves_per_cu": t["waves_per_cu"],
+                         "visits_per_round": t["visits_per_round"]}
             run = lambda a=False, st=None: idx.search_device(
                 tq, 100, 256, ui, ud, out_visited=ov, out_cmps=oc, k_per_query=tk,
                 beam_per_query=tb, async_=a, stream=st, rerank_factor=args.rerank_factor,
-                **opts)
+                **opts, **shape)
             run()
             torch.cuda.synchronize()
             ids = ui.cpu().numpy().view(np.uint32)
@@ -313,16 +346,18 @@ def main():
             graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(graph, stream=gs):
                 run(True, gs)
-            for _ in range(args.warmup):
-                graph.replay()
+            with torch.cuda.stream(gs):  # replays run on the capture stream
+                for _ in range(args.warmup):
+                    graph.replay()
             torch.cuda.synchronize()
             ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                   for _ in range(args.steps)]
             t0 = time.perf_counter()
-            for e0, e1 in ev:
-                e0.record(gs)
-                graph.replay()
-                e1.record(gs)
+            with torch.cuda.stream(gs):
+                for e0, e1 in ev:
+                    e0.record(gs)
+                    graph.replay()
+                    e1.record(gs)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             idx.search_status(gs)
@@ -332,7 +367,8 @@ def main():
                 qps=nq * args.steps / dt, ms_per_step=dt * 1e3 / args.steps,
                 graph_ms=float(np.mean(gms)), recall_at_10_k_ge_10=round(rec, 5),
                 replay_equals_eager=bool(same), k_choices=[1, 10, 100], beam_range=[16, 256],
-                launch="hipGraph replay of one persistent launch")
+                launch="hipGraph replay of one persistent launch",
+                launch_shape=dict(shape, tuned=not args.no_tune))
             del graph
             log(f"[bench] uneven {leg}: QPS {nq * args.steps / dt:,.0f} recall@10(k>=10) {rec:.4f}")
 

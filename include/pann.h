@@ -93,6 +93,9 @@ typedef struct {
                           the frontier and pending candidates afterwards -- same ids,
                           distances and visited counts, out_cmps counts evaluations, no
                           capacity limit */
+  int32_t visits_per_round; /* 0 or 2: while a merge is deferred, the next visit's rows are
+                               gathered with the current one's; 1: one visit per round.
+                               Results do not depend on it (a tuning knob) */
 } pann_search_opts;
 
 /* Creates the device index: uploads the points (rows padded to 128 bytes)

@@ -48,7 +48,8 @@ class QParams(C.Structure):
 class SearchOpts(C.Structure):
     _fields_ = [("store", C.c_int32), ("summation", C.c_int32), ("rerank", C.c_int32),
                 ("hash_slots_log2", C.c_int32), ("async_", C.c_int32),
-                ("waves_per_cu", C.c_int32), ("exact_counts", C.c_int32)]
+                ("waves_per_cu", C.c_int32), ("exact_counts", C.c_int32),
+                ("visits_per_round", C.c_int32)]
 
 
 _lib = None

@@ -1,0 +1,90 @@
+"""Launch-shape autotuner for the batched beam search.
+
+The search's results do not depend on its launch shape, only its speed does.
+For an index, a store and a beam (or a per-query k / beam mix) the tuner
+times every (waves per CU, visits gathered per round) pair on a sample of the
+caller's queries -- the persistent grid's occupancy cap, and whether the next
+visit's rows are gathered together with the current one's while a merge is
+deferred -- and keeps the fastest.  Winners are cached in a JSON file keyed
+by device, index shape, store and beam, so later runs reuse them.
+"""
+from __future__ import annotations
+
+import json
+import os
+from typing import Optional
+
+WAVES_PER_CU = (4, 6, 8, 12, 16)
+VISITS_PER_ROUND = (1, 2)
+
+
+def cache_path() -> str:
+    return os.environ.get("PANN_TUNE_CACHE") or os.path.join(
+        os.path.expanduser("~"), ".cache", "parlayann_amd", "tune.json")
+
+
+def _load() -> dict:
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table: dict):
+    p = cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        tmp = p + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, p)
+    except OSError:
+        pass  # a read-only home: tuning still applies to this process
+
+
+def tune(index, queries, k: int, beam: int, *, store: str = "native",
+         summation: str = "tree", rerank: bool = False, k_per_query=None,
+         beam_per_query=None, out_stride: Optional[int] = None, sample: int = 16384,
+         reps: int = 3, use_cache: bool = True, rerank_factor: float = 100.0) -> dict:
+    """Fastest {waves_per_cu, visits_per_round} for DeviceIndex `index` on
+    device queries (a torch tensor).  Returns the choice and the timings."""
+    import torch
+
+    dev = queries.device
+    uneven = k_per_query is not None or beam_per_query is not None
+    key = "|".join(str(x) for x in (
+        torch.cuda.get_device_name(dev), index.n, index.dim, index.max_degree,
+        str(index.dtype), index.metric, store, summation, int(rerank), k, beam,
+        "uneven" if uneven else "uniform"))
+    table = _load() if use_cache else {}
+    if use_cache and key in table:
+        return dict(table[key], cached=True)
+    m = min(int(queries.shape[0]), sample)
+    q = queries[:m]
+    stride = out_stride or k
+    oi = torch.empty((m, max(stride, 1)), dtype=torch.int32, device=dev)
+    od = torch.empty((m, max(stride, 1)), dtype=torch.float32, device=dev)
+    kpq = k_per_query[:m] if k_per_query is not None else None
+    bpq = beam_per_query[:m] if beam_per_query is not None else None
+    timings = []
+    for wpc in WAVES_PER_CU:
+        for vpr in VISITS_PER_ROUND:
+            kw = dict(store=store, summation=summation, rerank=rerank, waves_per_cu=wpc,
+                      visits_per_round=vpr, k_per_query=kpq, beam_per_query=bpq,
+                      rerank_factor=rerank_factor)
+            index.search_device(q, k, beam, oi, od, **kw)  # warm
+            ms = []
+            for _ in range(reps):
+                index.search_device(q, k, beam, oi, od, async_=True, **kw)
+                ms.append(index.last_kernel_ms())
+            index.search_status()
+            timings.append((min(ms), wpc, vpr))
+    best = min(timings)
+    res = {"waves_per_cu": best[1], "visits_per_round": best[2], "sample_ms": best[0],
+           "sample": m, "timings": [{"waves_per_cu": w, "visits_per_round": v, "ms": t}
+                                    for t, w, v in timings]}
+    if use_cache:
+        table[key] = res
+        _store(table)
+    return dict(res, cached=False)

@@ -406,7 +406,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
       // gathered together with cur's.  Its results are committed only if the
       // first visit defers; its accepted ids are inserted after cur's, so an
       // id both lists share is taken once, for cur, as in the reference.
-      const bool have2 = nu >= 0 && (int64_t)nvis < a.limit;
+      const bool have2 = a.lookahead && nu >= 0 && (int64_t)nvis < a.limit;
       const uint64_t key2 = have2 ? F[nu] : 0;
       const uint32_t id2 = key_id(key2);
       const uint32_t nb2 = !have2 ? PANN_SENTINEL : (id2 == pref_id ? nb_pref : adjacency(id2));

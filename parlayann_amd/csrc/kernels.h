@@ -45,6 +45,9 @@ struct SearchArgs {
   uint32_t accumulate;
   // 1: a frontier shorter than k is a result, not PANN_ST_SHORT_BEAM
   uint32_t short_ok;
+  // 1: while a merge is deferred, gather the next visit's rows in the same
+  // round (two visits per round); 0: one visit per round
+  uint32_t lookahead;
   // QueryParams (types.h:233-268); per-query k / beam override the uniform ones
   int32_t k, beam;
   const int32_t* k_pq;

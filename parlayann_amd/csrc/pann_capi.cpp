@@ -574,6 +574,9 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   a.ccap = (uint32_t)round_up((uint64_t)(defer_cap + a.degree_limit + 1), 16);
   a.ccap = std::max<uint32_t>(a.ccap, 64);  // also holds the start points
   a.exact_seen = o.exact_counts ? 1u : 0u;
+  if (o.visits_per_round < 0 || o.visits_per_round > 2)
+    return fail(PANN_ERR_ARG, "visits_per_round must be 0, 1 or 2");
+  a.lookahead = o.visits_per_round == 1 ? 0u : 1u;
   uint32_t hlog2;
   if (o.exact_counts) {
     // The reference's seen set (every id met): a small LDS table keeps more

@@ -107,10 +107,11 @@ class DeviceIndex:
                          float(rerank_factor), float(batch_factor))
 
     @staticmethod
-    def _opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu, exact_counts):
+    def _opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu, exact_counts,
+              visits_per_round=0):
         return L.SearchOpts(_STORE[store], _SUM[summation], int(bool(rerank)),
                             int(hash_slots_log2), int(bool(async_)), int(waves_per_cu),
-                            int(bool(exact_counts)))
+                            int(bool(exact_counts)), int(visits_per_round))
 
     def search(self, queries, k: int, beam: int, *, limit=None, degree_limit=None,
                batch_factor: float = 0.125, rerank_factor: float = 100.0,
@@ -146,7 +147,7 @@ class DeviceIndex:
                       summation: str = "inorder", rerank: bool = False,
                       starts: Sequence[int] = (0,), hash_slots_log2: int = 0,
                       waves_per_cu: int = 0, stream=None, async_: bool = False,
-                      exact_counts: bool = False):
+                      exact_counts: bool = False, visits_per_round: int = 0):
         """Device-buffer search: queries/outputs are torch tensors on the
         index's device.  `beam` is the maximum beam when beam_per_query is
         given.  Launches on `stream` (torch stream or raw handle)."""
@@ -154,7 +155,7 @@ class DeviceIndex:
         st = np.ascontiguousarray(starts, dtype=np.uint32)
         qp = self._qparams(k, beam, limit, degree_limit, rerank_factor, batch_factor)
         op = self._opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu,
-                        exact_counts)
+                        exact_counts, visits_per_round)
         if stream is not None and hasattr(stream, "cuda_stream"):
             stream = stream.cuda_stream
         L.check(L.lib().pann_search(self._h, _ptr(queries), nq, C.byref(qp), _ptr(k_per_query),
