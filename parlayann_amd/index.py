@@ -117,7 +117,7 @@ class DeviceIndex:
                batch_factor: float = 0.125, rerank_factor: float = 100.0,
                store: str = "native", summation: str = "inorder", rerank: bool = False,
                starts: Sequence[int] = (0,), hash_slots_log2: int = 0, waves_per_cu: int = 0,
-               exact_counts: bool = False):
+               exact_counts: bool = False, visits_per_round: int = 0):
         """Host-buffer batched search (qsearchAll, beamSearch.h:599-646).
         Returns dict(ids uint32[nq,k], dists float32[nq,k], visited, dist_cmps);
         dist_cmps is the reference's count with exact_counts, else the number
@@ -134,7 +134,7 @@ class DeviceIndex:
         st = np.ascontiguousarray(starts, dtype=np.uint32)
         qp = self._qparams(k, beam, limit, degree_limit, rerank_factor, batch_factor)
         op = self._opts(store, summation, rerank, hash_slots_log2, False, waves_per_cu,
-                        exact_counts)
+                        exact_counts, visits_per_round)
         L.check(L.lib().pann_search_host(self._h, Q.ctypes.data, nq, C.byref(qp), st.ctypes.data,
                                          len(st), C.byref(op), ids.ctypes.data,
                                          dists.ctypes.data, vis.ctypes.data, cmps.ctypes.data))

@@ -54,3 +54,20 @@ def test_gpu_graph_replay_equals_eager(oracle, store):
         assert torch.equal(oi, ref_i), rep
         assert torch.equal(od, ref_d), rep
     idx.close()
+
+
+def test_gpu_results_independent_of_launch_shape(oracle):
+    """The autotuner may pick any (waves per CU, visits per round): every
+    choice returns the oracle's ids, distances and visit counts."""
+    from parlayann_amd import DeviceIndex, autotune, datagen
+    X, Q = datagen.workload_u8_l2(n=8000, nq=300, d=32)
+    adj = oracle.build_vamana(X, "l2", 24, 48, 1.2, datagen.insertion_orders(len(X), 1, 4))
+    o = oracle.search_batch(adj, X, "l2", Q, 10, 48)
+    idx = DeviceIndex(X, adj, "l2")
+    for wpc in autotune.WAVES_PER_CU:
+        for vpr in autotune.VISITS_PER_ROUND:
+            r = idx.search(Q, 10, 48, waves_per_cu=wpc, visits_per_round=vpr)
+            assert np.array_equal(r["ids"], o["ids"]), (wpc, vpr)
+            assert np.array_equal(r["dists"], o["dists"]), (wpc, vpr)
+            assert np.array_equal(r["visited"], o["visited"]), (wpc, vpr)
+    idx.close()
