@@ -154,6 +154,8 @@ def main():
                     help="directory holding the built graph and groundtruth of this workload "
                          "(written on the first run, loaded by later ones, e.g. profiler passes)")
     args = ap.parse_args()
+    if args.cache:  # the autotuner's table lives beside the graph / groundtruth cache
+        os.environ.setdefault("PANN_TUNE_CACHE", os.path.join(args.cache, "tune.json"))
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
                                    "of it (--gpus must be 1)"}))
@@ -266,7 +268,8 @@ def main():
         if not args.no_tune:
             tuned = autotune.tune(idx, tq, k, B, rerank_factor=args.rerank_factor, **opts)
             log(f"[bench] {leg}: tuned waves/CU {tuned['waves_per_cu']}, visits/round "
-                f"{tuned['visits_per_round']}")
+                f"{tuned['visits_per_round']}" + (" (cached)" if tuned["cached"] else "") +
+                f" [{autotune.cache_path()} {tuned['key']}]")
         shape = {"waves_per_cu": tuned["waves_per_cu"],
                  "visits_per_round": tuned["visits_per_round"]}
         for _ in range(args.warmup):

@@ -53,13 +53,15 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
 
     dev = queries.device
     uneven = k_per_query is not None or beam_per_query is not None
+    from .index import device_info  # HIP's own figures (device names can be blank)
+    di = device_info(dev.index or 0)
     key = "|".join(str(x) for x in (
-        torch.cuda.get_device_name(dev), index.n, index.dim, index.max_degree,
+        f"cus{di['num_cus']}-hbm{di['hbm_bytes']}", index.n, index.dim, index.max_degree,
         str(index.dtype), index.metric, store, summation, int(rerank), k, beam,
         "uneven" if uneven else "uniform"))
     table = _load() if use_cache else {}
     if use_cache and key in table:
-        return dict(table[key], cached=True)
+        return dict(table[key], cached=True, key=key)
     m = min(int(queries.shape[0]), sample)
     q = queries[:m]
     stride = out_stride or k
@@ -87,4 +89,4 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
     if use_cache:
         table[key] = res
         _store(table)
-    return dict(res, cached=False)
+    return dict(res, cached=False, key=key)

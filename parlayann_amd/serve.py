@@ -79,6 +79,16 @@ class BatchServer:
         self.dev = torch.device("cuda", device)
         self.slots = [_Slot(self.max_batch, dim, self.max_ef, qdt, self.dev)
                       for _ in range(slots)]
+        # one full-size batch per slot before serving: the library allocates a
+        # stream's launch state and scratch at first use, which must not stall
+        # a timed run
+        B = self.max_batch
+        z = np.zeros((B, dim), np.float32 if qdt == torch.float32 else np.uint8)
+        one = np.ones(B, np.int32)
+        for s in self.slots:
+            self._launch(s, z, one, one, 0, B)
+        for s in self.slots:
+            self._retire(s)
 
     def _launch(self, s: _Slot, Q, ks, efs, lo, hi):
         import torch
