@@ -96,6 +96,9 @@ typedef struct {
   int32_t visits_per_round; /* 0 or 2: while a merge is deferred, the next visit's rows are
                                gathered with the current one's; 1: one visit per round.
                                Results do not depend on it (a tuning knob) */
+  int32_t eval_cache_log2;  /* exact_counts = 0: log2 of the slots of an LDS cache of evaluated
+                               ids (0 = none, 6..13); a cached id is not evaluated again.
+                               Results do not depend on it (a tuning knob) */
 } pann_search_opts;
 
 /* Creates the device index: uploads the points (rows padded to 128 bytes)

@@ -2,10 +2,12 @@
 
 The search's results do not depend on its launch shape, only its speed does.
 For an index, a store and a beam (or a per-query k / beam mix) the tuner
-times every (waves per CU, visits gathered per round) pair on a sample of the
-caller's queries -- the persistent grid's occupancy cap, and whether the next
-visit's rows are gathered together with the current one's while a merge is
-deferred -- and keeps the fastest.  Winners are cached in a JSON file keyed
+times every (waves per CU, visits gathered per round, evaluated-id cache
+size) choice on a sample of the caller's queries -- the persistent grid's
+occupancy cap, whether the next visit's rows are gathered together with the
+current one's while a merge is deferred, and how many LDS slots remember
+evaluated ids (fewer rows gathered again, against fewer waves per CU) -- and
+keeps the fastest.  Winners are cached in a JSON file keyed
 by device, index shape, store and beam, so later runs reuse them.
 """
 from __future__ import annotations
@@ -16,6 +18,7 @@ from typing import Optional
 
 WAVES_PER_CU = (4, 6, 8, 12, 16)
 VISITS_PER_ROUND = (1, 2)
+EVAL_CACHE_LOG2 = (0, 8, 10)
 
 
 def cache_path() -> str:
@@ -46,7 +49,7 @@ def _store(table: dict):
 def tune(index, queries, k: int, beam: int, *, store: str = "native",
          summation: str = "tree", rerank: bool = False, k_per_query=None,
          beam_per_query=None, out_stride: Optional[int] = None, sample: int = 16384,
-         reps: int = 3, use_cache: bool = True, rerank_factor: float = 100.0) -> dict:
+         reps: int = 2, use_cache: bool = True, rerank_factor: float = 100.0) -> dict:
     """Fastest {waves_per_cu, visits_per_round} for DeviceIndex `index` on
     device queries (a torch tensor).  Returns the choice and the timings."""
     import torch
@@ -72,20 +75,22 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
     timings = []
     for wpc in WAVES_PER_CU:
         for vpr in VISITS_PER_ROUND:
-            kw = dict(store=store, summation=summation, rerank=rerank, waves_per_cu=wpc,
-                      visits_per_round=vpr, k_per_query=kpq, beam_per_query=bpq,
-                      rerank_factor=rerank_factor)
-            index.search_device(q, k, beam, oi, od, **kw)  # warm
-            ms = []
-            for _ in range(reps):
-                index.search_device(q, k, beam, oi, od, async_=True, **kw)
-                ms.append(index.last_kernel_ms())
-            index.search_status()
-            timings.append((min(ms), wpc, vpr))
+            for ec in EVAL_CACHE_LOG2:
+                kw = dict(store=store, summation=summation, rerank=rerank, waves_per_cu=wpc,
+                          visits_per_round=vpr, eval_cache_log2=ec, k_per_query=kpq,
+                          beam_per_query=bpq, rerank_factor=rerank_factor)
+                index.search_device(q, k, beam, oi, od, **kw)  # warm
+                ms = []
+                for _ in range(reps):
+                    index.search_device(q, k, beam, oi, od, async_=True, **kw)
+                    ms.append(index.last_kernel_ms())
+                index.search_status()
+                timings.append((min(ms), wpc, vpr, ec))
     best = min(timings)
-    res = {"waves_per_cu": best[1], "visits_per_round": best[2], "sample_ms": best[0],
-           "sample": m, "timings": [{"waves_per_cu": w, "visits_per_round": v, "ms": t}
-                                    for t, w, v in timings]}
+    res = {"waves_per_cu": best[1], "visits_per_round": best[2], "eval_cache_log2": best[3],
+           "sample_ms": best[0], "sample": m,
+           "timings": [{"waves_per_cu": w, "visits_per_round": v, "eval_cache_log2": e, "ms": t}
+                       for t, w, v, e in timings]}
     if use_cache:
         table[key] = res
         _store(table)

@@ -41,12 +41,13 @@ struct WaveLds {
   float* DL;
   float* QL;
   uint32_t* H;
+  uint32_t* RC;
 };
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap, uint32_t dim,
-                                                   uint32_t hslots_log2) {
+                                                   uint32_t hslots_log2, uint32_t rc_log2) {
   uint32_t b = 0;
   b += 2 * align16(fcap * 8);
   b += 2 * align16(fcap);
@@ -54,6 +55,7 @@ __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap,
   b += align16(128 * 4) * 2;
   b += align16(dim * 4);
   b += (1u << hslots_log2) * 4;
+  b += rc_log2 ? (1u << rc_log2) * 4 : 0u;
   return b;
 }
 
@@ -70,6 +72,8 @@ __device__ inline WaveLds carve(char* p, const SearchArgs& a) {
   L.DL = (float*)(p + o); o += align16(128 * 4);
   L.QL = (float*)(p + o); o += align16(a.dim * 4);
   L.H = (uint32_t*)(p + o);
+  o += (1u << a.hslots_log2) * 4;
+  L.RC = (uint32_t*)(p + o);
   return L;
 }
 
@@ -188,6 +192,10 @@ __device__ __forceinline__ int lower_bound_u64(const uint64_t* A, int n, uint64_
   return lo;
 }
 
+__device__ __forceinline__ uint32_t rc_slot(uint32_t id, uint32_t log2n) {
+  return (id * 0x9E3779B1u) >> (32u - log2n);
+}
+
 // whether sorted A[0..n) holds x
 __device__ __forceinline__ bool sorted_contains(const uint64_t* A, int n, uint64_t x) {
   int lo = 0, len = n;
@@ -244,9 +252,13 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     const float* fq = SEQ ? (const float*)qrow : (const float*)(a.fqueries + qr * a.fq_stride);
     for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
   }
-  // ---- clear the visited hash ----
+  // ---- clear the visited hash and the cache of evaluated ids ----
   for (uint32_t i = lane * 4; i < hslots; i += PANN_WAVE * 4)
     *(uint4*)(L.H + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
+  const uint32_t rcl = EX ? 0u : a.rc_log2;
+  if (rcl)
+    for (uint32_t i = lane * 4; i < (1u << rcl); i += PANN_WAVE * 4)
+      *(uint4*)(L.RC + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
   wave_lds_sync();
 
   bool ovf = false;
@@ -308,6 +320,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
       const float d = L.DL[r];
       key = make_key(d, L.NL[r]);
       if (!(d >= cutoff) && !sorted_contains(F, f, key)) ins = lds_insert(L.H, hlog2, key_id(key));
+      if (rcl) L.RC[rc_slot(key_id(key), rcl)] = key_id(key);  // evaluated and committed
     }
     const uint64_t mi = ballot(ins);
     if (ins) L.C[c + (int)mbcnt(mi)] = key;
@@ -414,8 +427,14 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
       const int nxt = have2 ? nnu : nu;
       pref_id = nxt >= 0 ? key_id(F[nxt]) : PANN_SENTINEL;
       nb_pref = nxt >= 0 ? adjacency(pref_id) : PANN_SENTINEL;
-      const bool e1 = nb != PANN_SENTINEL && nb != self;
-      const bool e2 = have2 && nb2 != PANN_SENTINEL && nb2 != self;
+      // an id found in the cache of committed evaluations was seen before:
+      // the reference's seen set skips it, and its outcome here would be
+      // the same (rejected again, or already in F or C), so it is not
+      // evaluated again
+      const bool e1 = nb != PANN_SENTINEL && nb != self &&
+                      !(rcl && L.RC[rc_slot(nb, rcl)] == nb);
+      const bool e2 = have2 && nb2 != PANN_SENTINEL && nb2 != self &&
+                      !(rcl && L.RC[rc_slot(nb2, rcl)] == nb2);
       const uint64_t m1 = ballot(e1), m2 = ballot(e2);
       const int n1 = __popcll(m1), n2 = __popcll(m2);
       if (e1) L.NL[mbcnt(m1)] = nb;
@@ -551,7 +570,7 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 // host-side launch
 // ---------------------------------------------------------------------------
 uint32_t search_lds_bytes(const SearchArgs& a) {
-  return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2);
+  return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2, a.rc_log2);
 }
 
 template <int ST, int MT, int NCH, bool SEQ, bool EX>

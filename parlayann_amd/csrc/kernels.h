@@ -78,6 +78,9 @@ struct SearchArgs {
   uint32_t exact_seen;
   // LDS layout
   uint32_t fcap, ccap, hslots_log2;
+  // log2 of the slots of the LDS cache of evaluated ids (0 = none): a
+  // direct-mapped table, a slot holding the last committed id that hashed there
+  uint32_t rc_log2;
 };
 
 uint32_t search_lds_bytes(const SearchArgs& a);

@@ -577,6 +577,9 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   if (o.visits_per_round < 0 || o.visits_per_round > 2)
     return fail(PANN_ERR_ARG, "visits_per_round must be 0, 1 or 2");
   a.lookahead = o.visits_per_round == 1 ? 0u : 1u;
+  if (o.eval_cache_log2 != 0 && (o.eval_cache_log2 < 6 || o.eval_cache_log2 > 13))
+    return fail(PANN_ERR_ARG, "eval_cache_log2 must be 0 or in [6, 13]");
+  a.rc_log2 = o.exact_counts ? 0u : (uint32_t)o.eval_cache_log2;
   uint32_t hlog2;
   if (o.exact_counts) {
     // The reference's seen set (every id met): a small LDS table keeps more

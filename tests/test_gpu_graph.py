@@ -57,8 +57,9 @@ def test_gpu_graph_replay_equals_eager(oracle, store):
 
 
 def test_gpu_results_independent_of_launch_shape(oracle):
-    """The autotuner may pick any (waves per CU, visits per round): every
-    choice returns the oracle's ids, distances and visit counts."""
+    """The autotuner may pick any (waves per CU, visits per round, evaluated-id
+    cache size): every choice returns the oracle's ids, distances and visit
+    counts."""
     from parlayann_amd import DeviceIndex, autotune, datagen
     X, Q = datagen.workload_u8_l2(n=8000, nq=300, d=32)
     adj = oracle.build_vamana(X, "l2", 24, 48, 1.2, datagen.insertion_orders(len(X), 1, 4))
@@ -66,8 +67,12 @@ def test_gpu_results_independent_of_launch_shape(oracle):
     idx = DeviceIndex(X, adj, "l2")
     for wpc in autotune.WAVES_PER_CU:
         for vpr in autotune.VISITS_PER_ROUND:
-            r = idx.search(Q, 10, 48, waves_per_cu=wpc, visits_per_round=vpr)
-            assert np.array_equal(r["ids"], o["ids"]), (wpc, vpr)
-            assert np.array_equal(r["dists"], o["dists"]), (wpc, vpr)
-            assert np.array_equal(r["visited"], o["visited"]), (wpc, vpr)
+            for ec in autotune.EVAL_CACHE_LOG2 + (6, 13):
+                r = idx.search(Q, 10, 48, waves_per_cu=wpc, visits_per_round=vpr,
+                               eval_cache_log2=ec)
+                assert np.array_equal(r["ids"], o["ids"]), (wpc, vpr, ec)
+                assert np.array_equal(r["dists"], o["dists"]), (wpc, vpr, ec)
+                assert np.array_equal(r["visited"], o["visited"]), (wpc, vpr, ec)
+                # the cache only removes evaluations, never below the reference's count
+                assert np.all(r["dist_cmps"].astype(np.uint64) >= o["dist_cmps"]), ec
     idx.close()
