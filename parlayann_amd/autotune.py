@@ -18,7 +18,7 @@ from typing import Optional
 
 WAVES_PER_CU = (4, 6, 8, 12, 16)
 VISITS_PER_ROUND = (1, 2)
-EVAL_CACHE_LOG2 = (0, 8, 10)
+EVAL_CACHE_LOG2 = (0, 8, 10, 12)
 
 
 def cache_path() -> str:
