@@ -163,7 +163,7 @@ def main():
 
     import torch
     from parlayann_amd import DeviceIndex, autotune, datagen, device_info
-    from parlayann_amd.groundtruth import groundtruth
+    from parlayann_amd.groundtruth import groundtruth_device
     from parlayann_amd.recall import recall_plain, recall_with_ties
     from parlayann_amd.vamana import build_vamana
 
@@ -205,12 +205,14 @@ def main():
         if cg:
             os.makedirs(args.cache, exist_ok=True)
             np.save(cg, adj)
+    idx = DeviceIndex(X, adj, w["metric"])
     t0 = time.time()
     if cgt and os.path.exists(cgt):
         gt = np.load(cgt)
     else:
-        gt, _ = groundtruth(X, Q, 100, w["metric"])
-        log(f"[bench] FP64 groundtruth (k=100) in {time.time() - t0:.1f}s")
+        # brute-force candidates on the matrix cores, exact FP64 re-score
+        gt, _ = groundtruth_device(idx, X, Q, 100, w["metric"])
+        log(f"[bench] FP64 groundtruth (k=100, MFMA candidates) in {time.time() - t0:.1f}s")
         if cgt:
             np.save(cgt, gt)
 
@@ -223,7 +225,6 @@ def main():
             torch.neg(a_, out=b_)
         torch.cuda.synchronize()
         del a_, b_
-    idx = DeviceIndex(X, adj, w["metric"])
     for leg in legs:
         st = LEG_OPTS[leg]["store"]
         if st != "native":
@@ -433,7 +434,7 @@ def run_hnsw(args, w):
     continuous-batching server at 50% and 80% of the closed-loop rate."""
     import torch
     from parlayann_amd import hnsw
-    from parlayann_amd.groundtruth import groundtruth
+    from parlayann_amd.groundtruth import groundtruth_device
     from parlayann_amd.recall import recall_plain
     from parlayann_amd.serve import BatchServer, poisson_arrivals
 
@@ -466,15 +467,15 @@ def run_hnsw(args, w):
         if cm:
             os.makedirs(args.cache, exist_ok=True)
             hnsw.write_model(cm, model)
+    idx = hnsw.HNSWIndex(X, model, w["metric"])
     if cgt and os.path.exists(cgt):
         gt = np.load(cgt)
     else:
         t0 = time.time()
-        gt, _ = groundtruth(X, Q, 100, w["metric"])
-        log(f"[bench] FP64 groundtruth (k=100) in {time.time() - t0:.1f}s")
+        gt, _ = groundtruth_device(idx.base, X, Q, 100, w["metric"])
+        log(f"[bench] FP64 groundtruth (k=100, MFMA candidates) in {time.time() - t0:.1f}s")
         if cgt:
             np.save(cgt, gt)
-    idx = hnsw.HNSWIndex(X, model, w["metric"])
     for leg in legs:
         if LEG_OPTS[leg]["store"] != "native":
             idx.build_store(LEG_OPTS[leg]["store"])

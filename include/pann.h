@@ -209,6 +209,17 @@ pann_status pann_update_neighbors(pann_index_t idx, const uint32_t *pts, uint32_
 pann_status pann_get_q8_params(pann_index_t idx, float *slope, int32_t *offset, float *max_val,
                                float *scale);
 
+/* Brute-force candidates for groundtruth (compute_groundtruth.cpp:22-60) on
+ * the matrix cores (v_mfma_f32_32x32x2f32): for each of nq device queries
+ * (float, dim values each) the kk points of smallest FP32 distance, sorted,
+ * as device nq x kk ids (0xFFFFFFFF past n) and distances (L2: |x|^2 - 2 q.x,
+ * i.e. without the query's |q|^2; MIPS: -q.x).  kk <= 224.  The exact FP64
+ * re-scoring of the candidates, with kk - k of margin for FP32 rounding, is
+ * the caller's.  Synchronises `stream`. */
+pann_status pann_bruteforce_candidates(pann_index_t idx, const float *queries, uint32_t nq,
+                                      uint32_t kk, uint32_t *out_ids, float *out_dists,
+                                      void *stream);
+
 /* ------------------------------------------------------------------------
  * HNSW (algorithms/HNSW/HNSW.hpp).  A layered index is a pann_index whose
  * level-0 graph (pann_open, degree 2m) is joined by one graph per upper level

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpann.so")
-SOURCES = ["beam_search.hip", "quantize.hip", "prune.hip", "hnsw.hip", "pann_capi.cpp"]
+SOURCES = ["beam_search.hip", "quantize.hip", "prune.hip", "hnsw.hip", "bruteforce.hip", "pann_capi.cpp"]
 ARCH = os.environ.get("PANN_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

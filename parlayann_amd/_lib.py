@@ -24,7 +24,7 @@ EXPORTS = ["pann_open", "pann_build_store", "pann_get_fp8_scales", "pann_get_q8_
            "pann_download_graph", "pann_close", "pann_last_error", "pann_device_info",
            "pann_hnsw_set_level", "pann_hnsw_search_layer", "pann_hnsw_search",
            "pann_hnsw_select_neighbors", "pann_hnsw_add_reverse", "pann_hnsw_set_neighbors",
-           "pann_hnsw_download_level"]
+           "pann_hnsw_download_level", "pann_bruteforce_candidates"]
 
 
 class PannError(RuntimeError):
@@ -98,6 +98,7 @@ def lib():
                                   C.c_uint32, vp, vp, vp],
         "pann_hnsw_set_neighbors": [vp, C.c_uint32, vp, C.c_uint32, vp, vp, C.c_uint32, vp],
         "pann_hnsw_download_level": [vp, C.c_uint32, vp],
+        "pann_bruteforce_candidates": [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp],
         "pann_close": [vp],
         "pann_device_info": [C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                              C.POINTER(C.c_int64)],

@@ -95,6 +95,28 @@ hipError_t launch_bf16_store(const float* X, uint64_t stride_f, uint32_t n, uint
 hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                             uint32_t* absmax_bits, float* scales, uint8_t* Y, uint64_t stride_b,
                             hipStream_t s);
+// brute-force candidates on the matrix cores (bruteforce.hip)
+struct BFArgs {
+  const void* X;          // points (f32 / u8 / i8), row stride xs elements
+  uint64_t xs;
+  const float* xnorm;     // |x|^2 per point (L2), NULL for MIPS
+  uint32_t n, dim, dim_pad;
+  const float* Q;         // queries, row stride qs floats
+  uint64_t qs;
+  uint32_t nq;
+  int metric;             // 0 L2, 1 MIPS
+  uint32_t nranges;       // point ranges per query tile
+  uint64_t range_len;
+  uint32_t kk, cap;       // kept per query; per-(query, range) buffer, a power of two >= kk + 32
+  uint64_t* buf;          // nq x nranges x cap keys
+  uint32_t* counts;       // nq x nranges
+};
+uint32_t bf_lds_bytes(uint32_t dim_pad, uint32_t cap);
+hipError_t launch_bf(const BFArgs& a, int point_dtype, uint32_t* out_ids, float* out_dists,
+                     hipStream_t s);
+hipError_t launch_row_sqnorm(const void* X, int point_dtype, uint64_t xs, uint32_t n,
+                             uint32_t dim, float* out, hipStream_t s);
+
 // the reference's 8-bit scalar quantization (per-row extrema; codes of rows)
 hipError_t launch_row_minmax(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                              float* mins, float* maxs, uint32_t* notint, hipStream_t s);

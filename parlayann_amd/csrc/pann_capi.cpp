@@ -95,6 +95,7 @@ struct pann_index_s {
   uint64_t fp8_stride = 0;
   float* d_scales = nullptr;
   uint32_t* d_absmax = nullptr;
+  float* d_xnorm = nullptr;        // |x|^2 per point (brute force, L2)
   char* d_q8 = nullptr;            // the reference's 8-bit codes (PANN_STORE_Q8)
   uint64_t q8_stride = 0;
   float q8_slope = 0, q8_max = 0, q8_scale = 0;
@@ -1286,12 +1287,74 @@ pann_status pann_hnsw_download_level(pann_index_t idx, uint32_t level, uint32_t*
   return download_level(idx, level, graph_host);
 }
 
+pann_status pann_bruteforce_candidates(pann_index_t idx, const float* queries, uint32_t nq,
+                                      uint32_t kk, uint32_t* out_ids, float* out_dists,
+                                      void* stream_) {
+  if (!idx || !queries || !out_ids || !out_dists) return fail(PANN_ERR_ARG, "null argument");
+  if (nq == 0) return PANN_OK;
+  if (kk == 0 || kk > 224) return fail(PANN_ERR_ARG, "kk must be in [1, 224]");
+  if (idx->dim > 1024) return fail(PANN_ERR_UNSUPPORTED, "dim > 1024");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  hipStream_t stream = (hipStream_t)stream_;
+  const int pdt = idx->dtype == PANN_DT_F32 ? 0 : (idx->dtype == PANN_DT_U8 ? 1 : 2);
+  const uint64_t xs = idx->pts_stride / elem_bytes(idx->dtype);
+  if (idx->metric == PANN_METRIC_L2 && !idx->d_xnorm) {
+    HIP_TRY(hipMalloc(&idx->d_xnorm, (size_t)idx->n * 4), "hipMalloc(norms)");
+    HIP_TRY(pann::launch_row_sqnorm(idx->d_pts, pdt, xs, idx->n, idx->dim, idx->d_xnorm, stream),
+            "norms launch");
+  }
+  pann::BFArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.X = idx->d_pts;
+  a.xs = xs;
+  a.xnorm = idx->metric == PANN_METRIC_L2 ? idx->d_xnorm : nullptr;
+  a.n = idx->n;
+  a.dim = idx->dim;
+  a.dim_pad = (idx->dim + 1) & ~1u;
+  a.qs = idx->dim;
+  a.metric = idx->metric == PANN_METRIC_L2 ? 0 : 1;
+  a.kk = kk;
+  a.cap = 1;
+  while (a.cap < kk + 32) a.cap <<= 1;
+  const uint32_t lds = pann::bf_lds_bytes(a.dim_pad, a.cap);
+  if ((int64_t)lds > idx->max_lds_block) return fail(PANN_ERR_UNSUPPORTED, "dim too large");
+  // point ranges: enough (query tile, range) waves to fill the device
+  const uint32_t chunk = 16384;
+  const uint32_t tiles = (std::min(nq, chunk) + 31) / 32;
+  a.nranges = std::max<uint32_t>(1, std::min<uint32_t>(64, (uint32_t)(idx->num_cus * 16) / tiles));
+  a.nranges = std::min<uint32_t>(a.nranges, std::max<uint32_t>(1, idx->n / 4096));
+  a.range_len = (idx->n + a.nranges - 1) / a.nranges;
+  uint64_t* buf = nullptr;
+  uint32_t* counts = nullptr;
+  const uint32_t cq = std::min(nq, chunk);
+  hipError_t e = hipMalloc(&buf, (size_t)cq * a.nranges * a.cap * 8);
+  if (e == hipSuccess) e = hipMalloc(&counts, (size_t)cq * a.nranges * 4);
+  if (e != hipSuccess) {
+    if (buf) (void)hipFree(buf);
+    return hip_fail(e, "hipMalloc(brute-force scratch)");
+  }
+  a.buf = buf;
+  a.counts = counts;
+  for (uint32_t q0 = 0; q0 < nq && e == hipSuccess; q0 += chunk) {
+    a.nq = std::min(chunk, nq - q0);
+    a.Q = queries + (uint64_t)q0 * idx->dim;
+    e = pann::launch_bf(a, pdt, out_ids + (uint64_t)q0 * kk, out_dists + (uint64_t)q0 * kk, stream);
+  }
+  hipError_t e2 = hipStreamSynchronize(stream);
+  (void)hipFree(buf);
+  (void)hipFree(counts);
+  if (e != hipSuccess) return hip_fail(e, "brute-force launch");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamSynchronize");
+  return PANN_OK;
+}
+
 pann_status pann_close(pann_index_t idx) {
   if (!idx) return PANN_OK;
   (void)hipSetDevice(idx->device);
   (void)hipDeviceSynchronize();
   void* ptrs[] = {idx->d_pts, idx->d_bf16, idx->d_fp8, idx->d_scales, idx->d_absmax, idx->d_nbrs,
-                  idx->d_q8, idx->d_dummy, idx->d_pkeys, idx->d_pflags};
+                  idx->d_q8, idx->d_xnorm, idx->d_dummy, idx->d_pkeys, idx->d_pflags};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (size_t l = 1; l < idx->levels.size(); l++)

@@ -173,7 +173,8 @@ class HNSWIndex:
         """HNSW::search for a batch of device queries (torch tensors).  Rows of
         out_ids / out_dists (stride >= ef) hold out_count[q] results, point ids.
         k_per_query / ef_per_query (int32 tensors, ef_per_query <= ef) mix
-        requests in one batch."""
+        requests in one batch.  With async_ the tensors must stay alive, and
+        unchanged, until search_status on the same stream."""
         import torch
         if self._node_ids is not None and self._dev_node_ids is None:
             self._dev_node_ids = torch.from_numpy(self._node_ids.view(np.int32)).to(queries.device)

@@ -153,7 +153,9 @@ class DeviceIndex:
                eval_cache_log2: int = 0):
         """Device-buffer search: queries/outputs are torch tensors on the
         index's device.  `beam` is the maximum beam when beam_per_query is
-        given.  Launches on `stream` (torch stream or raw handle)."""
+        given.  Launches on `stream` (torch stream or raw handle).  With
+        async_ the tensors must stay alive, and unchanged, until
+        search_status on the same stream."""
         nq = int(queries.shape[0])
         st = np.ascontiguousarray(starts, dtype=np.uint32)
         qp = self._qparams(k, beam, limit, degree_limit, rerank_factor, batch_factor)

@@ -76,3 +76,29 @@ def test_gpu_results_independent_of_launch_shape(oracle):
                 # the cache only removes evaluations, never below the reference's count
                 assert np.all(r["dist_cmps"].astype(np.uint64) >= o["dist_cmps"]), ec
     idx.close()
+
+
+@pytest.mark.parametrize("metric,dtype,k", [("l2", "f32", 10), ("ip", "f32", 100),
+                                            ("l2", "u8", 10), ("l2", "f32", 100)])
+def test_gpu_bruteforce_groundtruth_equals_fp64(oracle, metric, dtype, k):
+    """K7 (brute-force candidates on MFMA) + FP64 re-score gives exactly the
+    oracle's FP64 brute-force top k in (distance, id) order."""
+    from parlayann_amd import DeviceIndex, datagen
+    from parlayann_amd.groundtruth import groundtruth_device
+    from parlayann_amd.index import _normalize
+    print(f"[bruteforce] metric={metric} dtype={dtype} k={k}", flush=True)
+    if dtype == "u8":
+        X, Q = datagen.workload_u8_l2(n=30000, nq=700, d=64)
+    elif metric == "ip":
+        X, Q = datagen.workload_aniso_ip(n=30000, nq=700, d=96)
+        X, Q = _normalize(X), _normalize(Q)
+    else:
+        X, Q = datagen.workload_f32_l2(n=30007, nq=701, d=50)
+    adj = np.zeros((len(X), 2), np.uint32)
+    idx = DeviceIndex(X, adj, metric)
+    ids, d = groundtruth_device(idx, X, Q, k, metric)
+    oi, od = oracle.groundtruth_f64(X, metric, Q, k)
+    assert np.array_equal(ids, oi)
+    # both FP64; the host re-score sums pairwise, the oracle in order
+    assert np.allclose(d, od, rtol=1e-12, atol=1e-12)
+    idx.close()

@@ -194,8 +194,10 @@ def test_gpu_concurrent_streams_equal_serial(oracle, small_l2):
         oc = torch.zeros(len(Q), dtype=torch.int32, device=dev)
         torch.cuda.synchronize()
         idx.search_device(tq, 10, 48, oi, od, oc, stream=st, async_=True)
-        outs.append((oi, od, oc, st))
-    for (oi, od, oc, st), r in zip(outs, ref):
+        # the queries must outlive the async search: torch's allocator would
+        # hand freed memory to the next tensor while the kernel still reads it
+        outs.append((oi, od, oc, st, tq))
+    for (oi, od, oc, st, _), r in zip(outs, ref):
         idx.base.search_status(st)
         c = oc.cpu().numpy()
         assert np.array_equal(c, r["count"])
