@@ -215,7 +215,8 @@ pann_status pann_get_q8_params(pann_index_t idx, float *slope, int32_t *offset, 
  * as device nq x kk ids (0xFFFFFFFF past n) and distances (L2: |x|^2 - 2 q.x,
  * i.e. without the query's |q|^2; MIPS: -q.x).  kk <= 224.  The exact FP64
  * re-scoring of the candidates, with kk - k of margin for FP32 rounding, is
- * the caller's.  Synchronises `stream`. */
+ * the caller's.  dim <= 256 (PANN_ERR_UNSUPPORTED beyond).  Synchronises
+ * `stream`. */
 pann_status pann_bruteforce_candidates(pann_index_t idx, const float *queries, uint32_t nq,
                                       uint32_t kk, uint32_t *out_ids, float *out_dists,
                                       void *stream);

@@ -10,18 +10,22 @@
 // (distance, id) order of the first k are the caller's, so kk carries a
 // margin over k for FP32 rounding.
 //
-// k_bf_topk: one wave per (32-query tile, point range).  The query tile and
-// each 32-point tile are staged in LDS (integer points widened to float) and
-// multiplied with v_mfma_f32_32x32x2f32: lane l supplies A[l%32][l/32] and
-// B[l/32][l%32], register r of lane l receives C[(r/4)*8+(l/32)*4+r%4][l%32].
-// A distance below its query's threshold is appended to the query's buffer
-// (capacity >= kk + 32: room for one more tile); when a buffer could
-// overflow it is sorted and cut to kk, and the kk-th distance becomes the
-// threshold.  k_bf_merge sorts each query's per-range lists into its kk best.
+// k_bf_topk: one wave per (32 or 64 queries, point range), multiplying with
+// v_mfma_f32_32x32x2f32 (lane l supplies A[l%32][l/32] and B[l/32][l%32],
+// register r of lane l receives C[(r/4)*8+(l/32)*4+r%4][l%32]).  The K index
+// is permuted so that each lane streams a contiguous half row of its point
+// into the B operand and keeps the matching half of its queries in
+// registers; integer points are widened to float on the way.  A distance
+// below its query's threshold is appended to the query's buffer (capacity
+// >= kk + 32: room for one more tile); when a buffer could overflow it is
+// sorted and cut to kk, and the kk-th distance becomes the threshold.
+// k_bf_merge sorts each query's per-range lists into its kk best.
 #include "kernels.h"
 #include "device_common.h"
 
 #include <float.h>
+
+#include <type_traits>
 
 namespace pann {
 
@@ -45,33 +49,76 @@ __device__ void lds_bitonic(uint64_t* K, int np, int lane) {
     }
 }
 
+// A lane's half row: lane l holds, of point row0 + l % 32, the dp / 2
+// contiguous elements [h * dp / 2, (h + 1) * dp / 2), h = l / 32, as nw
+// 16-byte words (rows are zero padded to 128 bytes, and dp is dim rounded up
+// to 32 bytes, so every word lies inside its row).
+template <typename PT, int NV>
+__device__ __forceinline__ void load_half_rows(uint4 (&w)[NV], const PT* X, uint64_t xs,
+                                               uint64_t row0, uint64_t rows_end, uint32_t half,
+                                               uint32_t nw, int lane) {
+  const uint64_t row = row0 + (uint64_t)(lane & 31);
+  const bool live = row < rows_end;
+  const uint4* p = (const uint4*)(X + row * xs + (uint64_t)(lane >> 5) * half);
+#pragma unroll
+  for (int t = 0; t < NV; t++)
+    w[t] = (live && (uint32_t)t < nw) ? p[t] : make_uint4(0u, 0u, 0u, 0u);
+}
+
 template <typename PT>
-__global__ __launch_bounds__(PANN_WAVE) void k_bf_topk(BFArgs a) {
+__device__ __forceinline__ float word_elem(const uint4& w, int e) {
+  if constexpr (sizeof(PT) == 4) {
+    return __uint_as_float(e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w);
+  } else {
+    const uint32_t word = (e >> 2) == 0 ? w.x : (e >> 2) == 1 ? w.y : (e >> 2) == 2 ? w.z : w.w;
+    const uint32_t byte = (word >> (8 * (e & 3))) & 0xFFu;
+    return std::is_same<PT, int8_t>::value ? (float)(int8_t)byte : (float)byte;
+  }
+}
+
+// One wave per (32 * QS queries, point range).  The dot products are an
+// MFMA GEMM whose K index is permuted: at step s, lane l (h = l / 32) feeds
+// dimension h * dp / 2 + s of query / point l % 32 as its A / B element, so
+// each lane streams its own contiguous half row from global memory straight
+// into the matrix core (no LDS staging) and holds its queries' matching
+// halves in registers for the whole range.  The next tile's words are in
+// flight while the current tile is multiplied.
+template <typename PT, int NV, int QS>
+__global__ __launch_bounds__(PANN_WAVE) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_bf_topk(BFArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
+  constexpr int EPW = 16 / (int)sizeof(PT);  // elements per 16-byte word
+  constexpr int SMAX = NV * EPW;              // steps per lane, upper bound
+  constexpr int NQ = 32 * QS;                 // queries per wave
   const int lane = threadIdx.x;
-  const uint32_t dp = a.dim_pad;  // dim rounded up to 2
-  const uint32_t ld = dp + 1;     // odd LDS row stride: the 32 rows fall in distinct banks
-  float* QL = (float*)smem;                 // 32 x ld
-  float* PL = QL + 32 * ld;                 // 32 x ld
-  float* NR = PL + 32 * ld;                 // 32 norms
-  float* thr = NR + 32;                     // 32 thresholds
-  uint32_t* cnt = (uint32_t*)(thr + 32);    // 32 counters
-  uint64_t* S = (uint64_t*)(cnt + 32);      // sort scratch, cap keys
+  const int li = lane & 31, h = lane >> 5;
+  float* thr = (float*)smem;                   // NQ thresholds
+  uint32_t* cnt = (uint32_t*)(thr + NQ);       // NQ counters
+  uint64_t* S = (uint64_t*)(cnt + NQ);         // sort scratch, cap keys
+  const uint32_t half = a.dim_pad / 2;
+  const uint32_t nw = half / EPW;
 
   const uint32_t qt = blockIdx.x / a.nranges, rg = blockIdx.x % a.nranges;
-  const uint32_t q0 = qt * 32;
+  const uint32_t q0 = qt * NQ;
   const uint64_t pb = (uint64_t)rg * a.range_len;
   const uint64_t pe = pb + a.range_len < a.n ? pb + a.range_len : a.n;
-  for (uint32_t e = lane; e < 32 * dp; e += PANN_WAVE) {
-    const uint32_t i = e / dp, j = e % dp;
-    QL[i * ld + j] = (q0 + i < a.nq && j < a.dim) ? a.Q[(uint64_t)(q0 + i) * a.qs + j] : 0.0f;
+
+  // this lane's halves of its queries (zero past dim)
+  float qa[QS][SMAX];
+#pragma unroll
+  for (int g = 0; g < QS; g++) {
+    const uint32_t row = q0 + (uint32_t)(g * 32 + li);
+    const float* qr = a.Q + (uint64_t)(row < a.nq ? row : 0) * a.qs;
+#pragma unroll
+    for (int s = 0; s < SMAX; s++) {
+      const uint32_t j = (uint32_t)h * half + (uint32_t)s;
+      qa[g][s] = (row < a.nq && (uint32_t)s < half && j < a.dim) ? qr[j] : 0.0f;
+    }
   }
-  if (lane < 32) {
-    thr[lane] = FLT_MAX;
-    cnt[lane] = 0;
+  for (int i = lane; i < NQ; i += PANN_WAVE) {
+    thr[i] = FLT_MAX;
+    cnt[i] = 0;
   }
   wave_lds_sync();
-  const int li = lane & 31, lk = lane >> 5;
   uint64_t* bufq = a.buf + ((uint64_t)q0 * a.nranges + rg) * a.cap;  // query q0 + i: + i*nranges*cap
 
   auto compact = [&](uint32_t i) {  // sort query i's buffer, keep kk
@@ -90,41 +137,65 @@ __global__ __launch_bounds__(PANN_WAVE) void k_bf_topk(BFArgs a) {
     wave_lds_sync();
   };
 
+  const PT* X = (const PT*)a.X;
+  uint4 wc[NV], wn[NV];
+  load_half_rows<PT, NV>(wc, X, a.xs, pb, pe, half, nw, lane);
+  float nc = (a.xnorm && pb + li < pe) ? a.xnorm[pb + li] : 0.0f, nn = 0.0f;
   for (uint64_t t0 = pb; t0 < pe; t0 += 32) {
-    for (uint32_t e = lane; e < 32 * dp; e += PANN_WAVE) {
-      const uint32_t i = e / dp, j = e % dp;
-      PL[i * ld + j] =
-          (t0 + i < pe && j < a.dim) ? (float)((const PT*)a.X)[(t0 + i) * a.xs + j] : 0.0f;
+    if (t0 + 32 < pe) {
+      load_half_rows<PT, NV>(wn, X, a.xs, t0 + 32, pe, half, nw, lane);
+      nn = (a.xnorm && t0 + 32 + li < pe) ? a.xnorm[t0 + 32 + li] : 0.0f;
     }
-    if (lane < 32) NR[lane] = (a.xnorm && t0 + lane < pe) ? a.xnorm[t0 + lane] : 0.0f;
-    wave_lds_sync();
-    f32x16 acc = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f,
-                  0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    for (uint32_t s = 0; s < dp / 2; s++) {
-      const float av = QL[li * ld + 2 * s + lk];
-      const float bv = PL[li * ld + 2 * s + lk];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+    f32x16 acc[QS];
+#pragma unroll
+    for (int g = 0; g < QS; g++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[g][r] = 0.0f;
+#pragma unroll
+    for (int t = 0; t < NV; t++) {  // words past nw are zero, as are their query halves
+#pragma unroll
+      for (int e = 0; e < EPW; e++) {
+        const float b = word_elem<PT>(wc[t], e);
+#pragma unroll
+        for (int g = 0; g < QS; g++)
+          acc[g] = __builtin_amdgcn_mfma_f32_32x32x2f32(qa[g][t * EPW + e], b, acc[g], 0, 0, 0);
+      }
     }
     const uint64_t pid = t0 + (uint64_t)li;
+    if (pid < pe) {
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const uint32_t i = (uint32_t)((r >> 2) * 8 + lk * 4 + (r & 3));
-      if (pid < pe && q0 + i < a.nq) {
-        const float d = a.metric == 0 ? NR[li] - 2.0f * acc[r] : -acc[r];
-        if (d < thr[i]) {
-          const uint32_t pos = atomicAdd(&cnt[i], 1u);
-          bufq[(uint64_t)i * a.nranges * a.cap + pos] = make_key(d, (uint32_t)pid);
+      for (int g = 0; g < QS; g++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          const uint32_t i = (uint32_t)(g * 32 + (r >> 2) * 8 + h * 4 + (r & 3));
+          const float d = a.metric == 0 ? nc - 2.0f * acc[g][r] : -acc[g][r];
+          if (q0 + i < a.nq && d < thr[i]) {
+            const uint32_t pos = atomicAdd(&cnt[i], 1u);
+            bufq[(uint64_t)i * a.nranges * a.cap + pos] = make_key(d, (uint32_t)pid);
+          }
         }
-      }
     }
     wave_lds_sync();
     // room for one more tile: a buffer past cap - 32 is cut back to kk
-    for (uint32_t i = 0; i < 32; i++)
-      if (cnt[i] > a.cap - 32) compact(i);
+    uint64_t full = __ballot(lane < NQ && cnt[lane < NQ ? lane : 0] > a.cap - 32);
+    while (full) {
+      const uint32_t i = (uint32_t)__ffsll((long long)full) - 1;
+      full &= full - 1;
+      compact(i);
+    }
+#pragma unroll
+    for (int t = 0; t < NV; t++) wc[t] = wn[t];
+    nc = nn;
   }
-  for (uint32_t i = 0; i < 32; i++)
-    if (q0 + i < a.nq && cnt[i] > 0) compact(i);
-  if (lane < 32 && q0 + lane < a.nq) a.counts[(uint64_t)(q0 + lane) * a.nranges + rg] = cnt[lane];
+  // the merge reads each buffer's first kk keys: cut longer ones
+  uint64_t over = __ballot(lane < NQ && q0 + lane < a.nq && cnt[lane < NQ ? lane : 0] > a.kk);
+  while (over) {
+    const uint32_t i = (uint32_t)__ffsll((long long)over) - 1;
+    over &= over - 1;
+    compact(i);
+  }
+  for (int i = lane; i < NQ; i += PANN_WAVE)
+    if (q0 + i < a.nq) a.counts[(uint64_t)(q0 + i) * a.nranges + rg] = cnt[i];
 }
 
 // one wave per query: the sorted per-range lists into the query's kk best
@@ -173,32 +244,45 @@ __global__ void k_row_sqnorm(const PT* X, uint64_t xs, uint32_t n, uint32_t dim,
   }
 }
 
-uint32_t bf_lds_bytes(uint32_t dim_pad, uint32_t cap) {
-  return 2 * 32 * (dim_pad + 1) * 4 + 32 * 4 * 3 + ((cap * 8 + 15) & ~15u) + 16;
+uint32_t bf_lds_bytes(uint32_t cap) { return 64 * 4 * 2 + cap * 8; }
+
+uint32_t bf_dim_pad(uint32_t dim, int point_dtype) {
+  return point_dtype == 0 ? (dim + 7) & ~7u : (dim + 31) & ~31u;  // 32 bytes
+}
+
+template <typename PT, int NV, int QS>
+static hipError_t launch_topk_t(const BFArgs& a, hipStream_t s) {
+  const uint32_t lds = bf_lds_bytes(a.cap);
+  const uint32_t blocks = ((a.nq + 32 * QS - 1) / (32 * QS)) * a.nranges;
+  hipLaunchKernelGGL((k_bf_topk<PT, NV, QS>), dim3(blocks), dim3(PANN_WAVE), lds, s, a);
+  return hipGetLastError();
+}
+
+// words per lane nw = dim_pad / 2 / (16 / sizeof(PT)); two query tiles per
+// wave while their register halves stay within 64 steps
+template <typename PT>
+static hipError_t launch_topk_nv(const BFArgs& a, hipStream_t s) {
+  const uint32_t nw = a.dim_pad / 2 / (16 / (uint32_t)sizeof(PT));
+  if constexpr (sizeof(PT) == 4) {
+    if (nw <= 4) return launch_topk_t<PT, 4, 2>(a, s);
+    if (nw <= 8) return launch_topk_t<PT, 8, 2>(a, s);
+    if (nw <= 12) return launch_topk_t<PT, 12, 2>(a, s);
+    if (nw <= 16) return launch_topk_t<PT, 16, 2>(a, s);
+    if (nw <= 24) return launch_topk_t<PT, 24, 1>(a, s);
+    if (nw <= 32) return launch_topk_t<PT, 32, 1>(a, s);
+  } else {
+    if (nw <= 2) return launch_topk_t<PT, 2, 2>(a, s);
+    if (nw <= 4) return launch_topk_t<PT, 4, 2>(a, s);
+    if (nw <= 8) return launch_topk_t<PT, 8, 1>(a, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_bf(const BFArgs& a, int point_dtype, uint32_t* out_ids, float* out_dists,
                      hipStream_t s) {
-  const uint32_t lds = bf_lds_bytes(a.dim_pad, a.cap);
-  const uint32_t blocks = ((a.nq + 31) / 32) * a.nranges;
-  hipError_t e;
-  if (point_dtype == 0) {
-    e = hipFuncSetAttribute((const void*)k_bf_topk<float>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bf_topk<float>, dim3(blocks), dim3(PANN_WAVE), lds, s, a);
-  } else if (point_dtype == 1) {
-    e = hipFuncSetAttribute((const void*)k_bf_topk<uint8_t>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bf_topk<uint8_t>, dim3(blocks), dim3(PANN_WAVE), lds, s, a);
-  } else {
-    e = hipFuncSetAttribute((const void*)k_bf_topk<int8_t>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bf_topk<int8_t>, dim3(blocks), dim3(PANN_WAVE), lds, s, a);
-  }
-  e = hipGetLastError();
+  hipError_t e = point_dtype == 0 ? launch_topk_nv<float>(a, s)
+                 : point_dtype == 1 ? launch_topk_nv<uint8_t>(a, s)
+                                    : launch_topk_nv<int8_t>(a, s);
   if (e != hipSuccess) return e;
   uint32_t np = 2;
   while (np < a.nranges * a.kk) np <<= 1;

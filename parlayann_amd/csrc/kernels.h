@@ -100,7 +100,7 @@ struct BFArgs {
   const void* X;          // points (f32 / u8 / i8), row stride xs elements
   uint64_t xs;
   const float* xnorm;     // |x|^2 per point (L2), NULL for MIPS
-  uint32_t n, dim, dim_pad;
+  uint32_t n, dim, dim_pad;  // dim_pad: bf_dim_pad(dim, dtype)
   const float* Q;         // queries, row stride qs floats
   uint64_t qs;
   uint32_t nq;
@@ -111,7 +111,8 @@ struct BFArgs {
   uint64_t* buf;          // nq x nranges x cap keys
   uint32_t* counts;       // nq x nranges
 };
-uint32_t bf_lds_bytes(uint32_t dim_pad, uint32_t cap);
+uint32_t bf_lds_bytes(uint32_t cap);
+uint32_t bf_dim_pad(uint32_t dim, int point_dtype);  // dim rounded up to 32 bytes
 hipError_t launch_bf(const BFArgs& a, int point_dtype, uint32_t* out_ids, float* out_dists,
                      hipStream_t s);
 hipError_t launch_row_sqnorm(const void* X, int point_dtype, uint64_t xs, uint32_t n,

@@ -1293,7 +1293,8 @@ pann_status pann_bruteforce_candidates(pann_index_t idx, const float* queries, u
   if (!idx || !queries || !out_ids || !out_dists) return fail(PANN_ERR_ARG, "null argument");
   if (nq == 0) return PANN_OK;
   if (kk == 0 || kk > 224) return fail(PANN_ERR_ARG, "kk must be in [1, 224]");
-  if (idx->dim > 1024) return fail(PANN_ERR_UNSUPPORTED, "dim > 1024");
+  if (idx->dim > 256)
+    return fail(PANN_ERR_UNSUPPORTED, "brute force: dim > 256");
   std::lock_guard<std::mutex> lk(idx->mu);
   HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
   hipStream_t stream = (hipStream_t)stream_;
@@ -1311,20 +1312,27 @@ pann_status pann_bruteforce_candidates(pann_index_t idx, const float* queries, u
   a.xnorm = idx->metric == PANN_METRIC_L2 ? idx->d_xnorm : nullptr;
   a.n = idx->n;
   a.dim = idx->dim;
-  a.dim_pad = (idx->dim + 1) & ~1u;
+  a.dim_pad = pann::bf_dim_pad(idx->dim, pdt);  // inside the 128-byte padded rows
   a.qs = idx->dim;
   a.metric = idx->metric == PANN_METRIC_L2 ? 0 : 1;
   a.kk = kk;
   a.cap = 1;
   while (a.cap < kk + 32) a.cap <<= 1;
-  const uint32_t lds = pann::bf_lds_bytes(a.dim_pad, a.cap);
-  if ((int64_t)lds > idx->max_lds_block) return fail(PANN_ERR_UNSUPPORTED, "dim too large");
-  // point ranges: enough (query tile, range) waves to fill the device
-  const uint32_t chunk = 16384;
-  const uint32_t tiles = (std::min(nq, chunk) + 31) / 32;
-  a.nranges = std::max<uint32_t>(1, std::min<uint32_t>(64, (uint32_t)(idx->num_cus * 16) / tiles));
-  a.nranges = std::min<uint32_t>(a.nranges, std::max<uint32_t>(1, idx->n / 4096));
+  // point ranges: a multiple of 8 (the XCD count), so that blockIdx
+  // (query tile * nranges + range) puts every wave of a range on one XCD and
+  // its L2 serves the range's rows to all of them; more ranges for few
+  // queries, each range >= 4096 points, and nranges * kk within the merge
+  const uint32_t tiles = (nq + 63) / 64;
+  uint32_t nr = std::max<uint32_t>(8, (uint32_t)(idx->num_cus * 8 + tiles - 1) / tiles);
+  nr = std::min<uint32_t>((nr + 7) & ~7u, 64);
+  nr = std::min<uint32_t>(nr, std::max<uint32_t>(1, idx->n / 4096));
+  nr = std::min<uint32_t>(nr, 4096 / kk);
+  a.nranges = std::max<uint32_t>(1, nr);
   a.range_len = (idx->n + a.nranges - 1) / a.nranges;
+  // queries per launch: scratch (cap keys per query and range) within 1 GiB
+  const uint64_t per_q = (uint64_t)a.nranges * (a.cap * 8 + 4);
+  const uint32_t chunk = (uint32_t)std::max<uint64_t>(
+      64, std::min<uint64_t>(nq, ((1ull << 30) / per_q) & ~63ull));
   uint64_t* buf = nullptr;
   uint32_t* counts = nullptr;
   const uint32_t cq = std::min(nq, chunk);
