@@ -109,6 +109,43 @@ def max_rel_dist_error(X, Q, ids, dists, metric, chunk=8192):
     return worst
 
 
+def gather_ceiling(dev):
+    """The device's random row-gather ceiling (parlayann_amd/libpann_calib.so):
+    2^26 8-lane reads of the first read_bytes of pseudo-random 128-byte rows of
+    a 4 GiB buffer.  Every read fetches its whole 128-byte line, so the line
+    rate, not HBM's streaming bandwidth, bounds the search's gathers."""
+    import ctypes
+    import torch
+    lib = ctypes.CDLL(os.path.join(ROOT, "parlayann_amd", "libpann_calib.so"))
+    fn = lib.pann_calib_gather
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                   ctypes.c_uint64, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                   ctypes.c_void_p]
+    nb, nreads = 4 << 30, 1 << 26
+    buf = torch.zeros(nb, dtype=torch.uint8, device=dev)
+    sink = torch.zeros(16, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    torch.cuda.synchronize()
+    out = {}
+    # (row stride, bytes read): whole lines, part lines, and 96-byte rows packed
+    # at a 96-byte stride (half of them straddle two lines)
+    for stride, rb in ((128, 128), (128, 96), (128, 32), (96, 96)):
+        ms = []
+        for _ in range(3):
+            t = ctypes.c_float()
+            rc = fn(buf.data_ptr(), nb, stride, rb, nreads, sink.data_ptr(), ctypes.byref(t),
+                    stream)
+            if rc != 0:
+                raise RuntimeError(f"pann_calib_gather failed: hipError {rc}")
+            ms.append(t.value)
+        best = min(ms)
+        out[f"read_{rb}B_of_{stride}B_rows"] = {"ms": best, "rows_per_s": nreads / best * 1e3,
+                                                "line_gbs": nreads * 128 / best / 1e6}
+    del buf, sink
+    return out
+
+
 def measured_traffic(workload, w, leg, beam):
     """HBM bytes per launch of the search kernel for this exact workload, leg
     and beam, from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary
@@ -146,6 +183,8 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB read + 1 GiB write elementwise kernel first (calibrates "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against a copy of known size)")
+    ap.add_argument("--no-calib-gather", action="store_true",
+                    help="skip the random row-gather ceiling measurement (about 0.1 s)")
     ap.add_argument("--cpu-seconds", type=float, default=6.0,
                     help="target wall time of one CPU-baseline repetition")
     ap.add_argument("--no-tune", action="store_true",
@@ -216,6 +255,11 @@ def main():
         if cgt:
             np.save(cgt, gt)
 
+    gceil = None if args.no_calib_gather else gather_ceiling(dev)
+    if gceil:
+        g = gceil["read_128B_of_128B_rows"]
+        log(f"[bench] random 128-byte row gather ceiling: {g['rows_per_s'] / 1e9:.1f} G rows/s "
+            f"= {g['line_gbs']:,.0f} GB/s of lines")
     if args.calib_copy:
         # an elementwise kernel that reads 1 GiB and writes 1 GiB (its name
         # contains "neg"): the scale of rocprofv3's FETCH_SIZE / WRITE_SIZE
@@ -415,11 +459,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp8-rerank": "fp8", "bf16-rerank": "bf16", "u8": "u8"}.get(head, "f32"),
+        "dtype": {"fp8-rerank": "fp8", "bf16-rerank": "bf16", "u8": "u8",
+                  "q8-rerank": "i8" if w["metric"] == "ip" else "u8"}.get(head, "f32"),
         "data": "synthetic (seeded generator; graph built on the GPU by this library)",
         "config": {"workload": args.workload, "n": w["n"], "dim": w["d"], "nq": nq, "k": k,
                    "graph": f"vamana R={w['R']} L={w['L']} alpha={w['alpha']}",
-                   "headline_leg": head, "legs": results, "graph_build_s": t_build},
+                   "headline_leg": head, "legs": results, "graph_build_s": t_build,
+                   "gather_ceiling": gceil},
         "roofline": {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": h["achieved_gbs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, w, head, h["beam"])},
@@ -476,6 +522,7 @@ def run_hnsw(args, w):
         log(f"[bench] FP64 groundtruth (k=100, MFMA candidates) in {time.time() - t0:.1f}s")
         if cgt:
             np.save(cgt, gt)
+    gceil = None if args.no_calib_gather else gather_ceiling(dev)
     for leg in legs:
         if LEG_OPTS[leg]["store"] != "native":
             idx.build_store(LEG_OPTS[leg]["store"])
@@ -594,11 +641,13 @@ def run_hnsw(args, w):
         "value": h["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp8-rerank": "fp8", "bf16-rerank": "bf16"}.get(head, "f32"),
+        "dtype": {"fp8-rerank": "fp8", "bf16-rerank": "bf16",
+                  "q8-rerank": "i8" if w["metric"] == "ip" else "u8"}.get(head, "f32"),
         "data": "synthetic (seeded generator; HNSW built on the GPU by this library)",
         "config": {"workload": args.workload, "n": w["n"], "dim": w["d"], "nq": nq,
                    "k": "U{10,100}", "graph": f"hnsw m={w['m']} efc={w['efc']}",
-                   "headline_leg": head, "legs": results, "graph_build_s": t_build},
+                   "headline_leg": head, "legs": results, "graph_build_s": t_build,
+                   "gather_ceiling": gceil},
         "roofline": {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": h["achieved_gbs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, w, head, h["ef"])},

@@ -16,6 +16,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpann.so")
 SOURCES = ["beam_search.hip", "quantize.hip", "prune.hip", "hnsw.hip", "bruteforce.hip", "pann_capi.cpp"]
+# the device's random row-gather ceiling, for bench.py's roofline (not part of libpann)
+CALIB_SRC = "calib.hip"
+CALIB_LIB = os.path.join(HERE, "libpann_calib.so")
 ARCH = os.environ.get("PANN_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -34,12 +37,14 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
-    subprocess.run(cmd, check=True, capture_output=not verbose)
-    os.replace(tmp, LIB)
+    srcs = SOURCES + [CALIB_SRC]
+    with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(_compile, srcs))
+    for lib, o in ((LIB, objs[:-1]), (CALIB_LIB, objs[-1:])):
+        tmp = lib + ".tmp"
+        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *o]
+        subprocess.run(cmd, check=True, capture_output=not verbose)
+        os.replace(tmp, lib)
     return LIB
 
 
