@@ -67,6 +67,21 @@ pann_status upload_graph(uint32_t n, uint32_t md, uint32_t stride, const uint32_
   return PANN_OK;
 }
 
+// QueryParams checks that hold for any batch, an empty one included: the
+// reference's qsearchAll aborts on k > beam before looking at the queries.
+pann_status check_qparams(const pann_qparams* qp) {
+  const int64_t B = qp->beam_size;
+  if (qp->k < 0 || B < 1) return fail(PANN_ERR_ARG, "beam_size must be >= 1 and k >= 0");
+  if (qp->k > B)  // beamSearch.h:612-616
+    return fail(PANN_ERR_ARG, "beam search parameter Q = " + std::to_string(B) +
+                                  " same size or smaller than k = " + std::to_string(qp->k));
+  if (B > 4096) return fail(PANN_ERR_UNSUPPORTED, "beam_size > 4096 is not supported");
+  if (qp->degree_limit < 1) return fail(PANN_ERR_ARG, "degree_limit must be >= 1");
+  if (!(qp->batch_factor >= 0.0) || !(qp->rerank_factor >= 0.0))
+    return fail(PANN_ERR_ARG, "batch_factor and rerank_factor must be >= 0");
+  return PANN_OK;
+}
+
 pann_status check_graph(uint32_t n, uint32_t md, const uint32_t* graph) {
   // graph.h:60-63 / :240-243 abort on bad degrees and ids; here they are errors
   for (uint64_t i = 0; i < n; i++) {
@@ -484,14 +499,10 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     return fail(PANN_ERR_ARG, "graph level " + std::to_string(level) + " is not set");
   const auto& G = idx->levels[level];
   const int64_t B = qp->beam_size;
-  if (qp->k < 0 || B < 1) return fail(PANN_ERR_ARG, "beam_size must be >= 1 and k >= 0");
-  if (qp->k > B)  // beamSearch.h:612-616
-    return fail(PANN_ERR_ARG, "beam search parameter Q = " + std::to_string(B) +
-                                  " same size or smaller than k = " + std::to_string(qp->k));
-  if (B > 4096) return fail(PANN_ERR_UNSUPPORTED, "beam_size > 4096 is not supported");
-  if (qp->degree_limit < 1) return fail(PANN_ERR_ARG, "degree_limit must be >= 1");
-  if (!(qp->batch_factor >= 0.0) || !(qp->rerank_factor >= 0.0))
-    return fail(PANN_ERR_ARG, "batch_factor and rerank_factor must be >= 0");
+  {
+    pann_status qst = check_qparams(qp);
+    if (qst != PANN_OK) return qst;
+  }
   if (!ss) {
     if (nstarts == 0)  // beamSearch.h:51-53
       return fail(PANN_ERR_ARG, "beam search expects at least one start point");
@@ -669,6 +680,10 @@ pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
                         uint32_t* out_visited, uint32_t* out_cmps, uint32_t* out_nfront,
                         bool allow_short, void* stream_) {
   if (!idx || !qp || !starts) return fail(PANN_ERR_ARG, "null argument");
+  {
+    pann_status qst = check_qparams(qp);
+    if (qst != PANN_OK) return qst;
+  }
   if (nq == 0) return PANN_OK;
   if (!queries || !out_ids || !out_dists) return fail(PANN_ERR_ARG, "null buffer");
   if (out_stride < (uint32_t)std::max<int64_t>(qp->k, 0))
@@ -753,8 +768,12 @@ pann_status pann_search_host(pann_index_t idx, const void* queries, uint32_t nq,
                              const pann_search_opts* opts, uint32_t* out_ids, float* out_dists,
                              uint32_t* out_visited, uint32_t* out_cmps) {
   if (!idx || !qp || !queries || !out_ids || !out_dists) return fail(PANN_ERR_ARG, "null argument");
-  if (nq == 0) return PANN_OK;
   if (qp->k < 1) return fail(PANN_ERR_ARG, "k must be >= 1");
+  {
+    pann_status qst = check_qparams(qp);
+    if (qst != PANN_OK) return qst;
+  }
+  if (nq == 0) return PANN_OK;
   HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
   const size_t qbytes = (size_t)nq * idx->dim * (idx->dtype == PANN_DT_F32 ? 4 : 1);
   const size_t k = (size_t)qp->k;
