@@ -309,14 +309,17 @@ def main():
         B, rec, ids, dists = chosen
         rec_ties = recall_with_ties(X, Q, gt, ids, k, w["metric"])
         mre = max_rel_dist_error(X, Q, ids, dists, w["metric"] if w["kind"] != "u8" else "l2")
-        tuned = {"waves_per_cu": 0, "visits_per_round": 0, "eval_cache_log2": 0}
+        tuned = {"waves_per_cu": 0, "visits_per_round": 0, "eval_cache_log2": 0,
+                 "frontier_filter": 0}
         if not args.no_tune:
             tuned = autotune.tune(idx, tq, k, B, rerank_factor=args.rerank_factor, **opts)
             log(f"[bench] {leg}: tuned waves/CU {tuned['waves_per_cu']}, visits/round "
-                f"{tuned['visits_per_round']}, eval cache 2^{tuned['eval_cache_log2']}" +
+                f"{tuned['visits_per_round']}, eval cache 2^{tuned['eval_cache_log2']}, "
+                f"frontier filter {('auto', 'on', 'off')[tuned['frontier_filter']]}" +
                 (" (cached)" if tuned["cached"] else "") +
                 f" [{autotune.cache_path()} {tuned['key']}]")
-        shape = {k_: tuned[k_] for k_ in ("waves_per_cu", "visits_per_round", "eval_cache_log2")}
+        shape = {k_: tuned[k_] for k_ in ("waves_per_cu", "visits_per_round", "eval_cache_log2",
+                                          "frontier_filter")}
         for _ in range(args.warmup):
             idx.search_device(tq, k, B, oi, od, out_visited=ov, out_cmps=oc, async_=True,
                               rerank_factor=args.rerank_factor, **opts, **shape)
@@ -372,12 +375,13 @@ def main():
         gs = torch.cuda.Stream()
         for leg in [l for l in ("fp32-tree", "fp8-rerank") if l in legs]:
             opts = LEG_OPTS[leg]
-            shape = {"waves_per_cu": 0, "visits_per_round": 0, "eval_cache_log2": 0}
+            shape = {"waves_per_cu": 0, "visits_per_round": 0, "eval_cache_log2": 0,
+                     "frontier_filter": 0}
             if not args.no_tune:
                 t = autotune.tune(idx, tq, 100, 256, k_per_query=tk, beam_per_query=tb,
                                   out_stride=100, rerank_factor=args.rerank_factor, **opts)
                 shape = {k_: t[k_] for k_ in ("waves_per_cu", "visits_per_round",
-                                              "eval_cache_log2")}
+                                              "eval_cache_log2", "frontier_filter")}
             run = lambda a=False, st=None: idx.search_device(
                 tq, 100, 256, ui, ud, out_visited=ov, out_cmps=oc, k_per_query=tk,
                 beam_per_query=tb, async_=a, stream=st, rerank_factor=args.rerank_factor,

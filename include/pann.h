@@ -99,6 +99,11 @@ typedef struct {
   int32_t eval_cache_log2;  /* exact_counts = 0: log2 of the slots of an LDS cache of evaluated
                                ids (0 = none, 6..13); a cached id is not evaluated again.
                                Results do not depend on it (a tuning knob) */
+  int32_t frontier_filter;  /* exact_counts = 0: a Bloom filter of the frontier's ids, each hit
+                               checked exactly, keeps ids already in the frontier from being
+                               gathered again.  0 = automatic (on when a gathered row spans
+                               more than one 128-byte line), 1 = on, 2 = off; beams above 512
+                               run without it.  Results do not depend on it (a tuning knob) */
 } pann_search_opts;
 
 /* Creates the device index: uploads the points (rows padded to 128 bytes)

@@ -7,8 +7,10 @@ size) choice on a sample of the caller's queries -- the persistent grid's
 occupancy cap, whether the next visit's rows are gathered together with the
 current one's while a merge is deferred, and how many LDS slots remember
 evaluated ids (fewer rows gathered again, against fewer waves per CU) -- and
-keeps the fastest.  Winners are cached in a JSON file keyed
-by device, index shape, store and beam, so later runs reuse them.
+keeps the fastest; the winner is then timed with the frontier-id filter
+forced on and off (fewer rows gathered again, against a filter test before
+every gather).  Winners are cached in a JSON file keyed by device, index
+shape, store and beam, so later runs reuse them.
 """
 from __future__ import annotations
 
@@ -19,6 +21,7 @@ from typing import Optional
 WAVES_PER_CU = (4, 6, 8, 12, 16)
 VISITS_PER_ROUND = (1, 2)
 EVAL_CACHE_LOG2 = (0, 8, 10, 12)
+FRONTIER_FILTER = (0, 1, 2)  # automatic, on, off
 
 
 def cache_path() -> str:
@@ -50,8 +53,9 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
          summation: str = "tree", rerank: bool = False, k_per_query=None,
          beam_per_query=None, out_stride: Optional[int] = None, sample: int = 16384,
          reps: int = 2, use_cache: bool = True, rerank_factor: float = 100.0) -> dict:
-    """Fastest {waves_per_cu, visits_per_round} for DeviceIndex `index` on
-    device queries (a torch tensor).  Returns the choice and the timings."""
+    """Fastest {waves_per_cu, visits_per_round, eval_cache_log2,
+    frontier_filter} for DeviceIndex `index` on device queries (a torch
+    tensor).  Returns the choice and the timings."""
     import torch
 
     dev = queries.device
@@ -63,7 +67,7 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
         str(index.dtype), index.metric, store, summation, int(rerank), k, beam,
         "uneven" if uneven else "uniform"))
     table = _load() if use_cache else {}
-    if use_cache and key in table:
+    if use_cache and key in table and "frontier_filter" in table[key]:
         return dict(table[key], cached=True, key=key)
     m = min(int(queries.shape[0]), sample)
     q = queries[:m]
@@ -72,25 +76,31 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
     od = torch.empty((m, max(stride, 1)), dtype=torch.float32, device=dev)
     kpq = k_per_query[:m] if k_per_query is not None else None
     bpq = beam_per_query[:m] if beam_per_query is not None else None
+    def time_shape(wpc, vpr, ec, ff):
+        kw = dict(store=store, summation=summation, rerank=rerank, waves_per_cu=wpc,
+                  visits_per_round=vpr, eval_cache_log2=ec, frontier_filter=ff,
+                  k_per_query=kpq, beam_per_query=bpq, rerank_factor=rerank_factor)
+        index.search_device(q, k, beam, oi, od, **kw)  # warm
+        ms = []
+        for _ in range(reps):
+            index.search_device(q, k, beam, oi, od, async_=True, **kw)
+            ms.append(index.last_kernel_ms())
+        index.search_status()
+        return min(ms)
+
     timings = []
     for wpc in WAVES_PER_CU:
         for vpr in VISITS_PER_ROUND:
             for ec in EVAL_CACHE_LOG2:
-                kw = dict(store=store, summation=summation, rerank=rerank, waves_per_cu=wpc,
-                          visits_per_round=vpr, eval_cache_log2=ec, k_per_query=kpq,
-                          beam_per_query=bpq, rerank_factor=rerank_factor)
-                index.search_device(q, k, beam, oi, od, **kw)  # warm
-                ms = []
-                for _ in range(reps):
-                    index.search_device(q, k, beam, oi, od, async_=True, **kw)
-                    ms.append(index.last_kernel_ms())
-                index.search_status()
-                timings.append((min(ms), wpc, vpr, ec))
+                timings.append((time_shape(wpc, vpr, ec, 0), wpc, vpr, ec, 0))
+    _, bw, bv, be, _ = min(timings)
+    for ff in FRONTIER_FILTER[1:]:
+        timings.append((time_shape(bw, bv, be, ff), bw, bv, be, ff))
     best = min(timings)
     res = {"waves_per_cu": best[1], "visits_per_round": best[2], "eval_cache_log2": best[3],
-           "sample_ms": best[0], "sample": m,
-           "timings": [{"waves_per_cu": w, "visits_per_round": v, "eval_cache_log2": e, "ms": t}
-                       for t, w, v, e in timings]}
+           "frontier_filter": best[4], "sample_ms": best[0], "sample": m,
+           "timings": [{"waves_per_cu": w, "visits_per_round": v, "eval_cache_log2": e,
+                        "frontier_filter": f_, "ms": t} for t, w, v, e, f_ in timings]}
     if use_cache:
         table[key] = res
         _store(table)

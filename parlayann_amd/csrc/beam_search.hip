@@ -42,12 +42,14 @@ struct WaveLds {
   float* QL;
   uint32_t* H;
   uint32_t* RC;
+  uint32_t* FB;
 };
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap, uint32_t dim,
-                                                   uint32_t hslots_log2, uint32_t rc_log2) {
+                                                   uint32_t hslots_log2, uint32_t rc_log2,
+                                                   uint32_t fb_log2) {
   uint32_t b = 0;
   b += 2 * align16(fcap * 8);
   b += 2 * align16(fcap);
@@ -56,6 +58,7 @@ __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap,
   b += align16(dim * 4);
   b += (1u << hslots_log2) * 4;
   b += rc_log2 ? (1u << rc_log2) * 4 : 0u;
+  b += fb_log2 ? (1u << fb_log2) / 8 : 0u;
   return b;
 }
 
@@ -74,6 +77,8 @@ __device__ inline WaveLds carve(char* p, const SearchArgs& a) {
   L.H = (uint32_t*)(p + o);
   o += (1u << a.hslots_log2) * 4;
   L.RC = (uint32_t*)(p + o);
+  o += a.rc_log2 ? (1u << a.rc_log2) * 4 : 0u;
+  L.FB = (uint32_t*)(p + o);
   return L;
 }
 
@@ -196,6 +201,19 @@ __device__ __forceinline__ uint32_t rc_slot(uint32_t id, uint32_t log2n) {
   return (id * 0x9E3779B1u) >> (32u - log2n);
 }
 
+// Frontier-id Bloom filter: two bits per id, from two multiplicative hashes
+// other than the tables' own.
+__device__ __forceinline__ uint32_t fb_h1(uint32_t id, uint32_t log2n) {
+  return (id * 0x85EBCA77u) >> (32u - log2n);
+}
+__device__ __forceinline__ uint32_t fb_h2(uint32_t id, uint32_t log2n) {
+  return ((id ^ 0x5BD1E995u) * 0xC2B2AE3Du) >> (32u - log2n);
+}
+__device__ __forceinline__ bool fb_test(const uint32_t* FB, uint32_t log2n, uint32_t id) {
+  const uint32_t a = fb_h1(id, log2n), b = fb_h2(id, log2n);
+  return ((FB[a >> 5] >> (a & 31u)) & (FB[b >> 5] >> (b & 31u)) & 1u) != 0u;
+}
+
 // whether sorted A[0..n) holds x
 __device__ __forceinline__ bool sorted_contains(const uint64_t* A, int n, uint64_t x) {
   int lo = 0, len = n;
@@ -256,6 +274,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   for (uint32_t i = lane * 4; i < hslots; i += PANN_WAVE * 4)
     *(uint4*)(L.H + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
   const uint32_t rcl = EX ? 0u : a.rc_log2;
+  const uint32_t fbl = EX ? 0u : a.fb_log2;
   if (rcl)
     for (uint32_t i = lane * 4; i < (1u << rcl); i += PANN_WAVE * 4)
       *(uint4*)(L.RC + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
@@ -298,6 +317,61 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   int f = ns;
   uint32_t nvis = 0;
   uint32_t cmps = (uint32_t)ns;
+
+  // The frontier-id filter: a Bloom filter of the ids in the frontier,
+  // rebuilt whenever the frontier changes (here and after every merge).
+  // A neighbour it reports is looked up exactly in F before its row is
+  // gathered; one in F was seen before, so the reference's seen set skips it
+  // and so does this search (its evaluation would only be discarded).
+  auto fb_rebuild = [&](const uint64_t* Fx, int fx) {
+    for (uint32_t i = lane; i < ((1u << fbl) >> 5); i += PANN_WAVE) L.FB[i] = 0u;
+    wave_lds_sync();
+    for (int i = lane; i < fx; i += PANN_WAVE) {
+      const uint32_t id = key_id(Fx[i]);
+      const uint32_t h1 = fb_h1(id, fbl), h2 = fb_h2(id, fbl);
+      atomicOr(&L.FB[h1 >> 5], 1u << (h1 & 31u));
+      atomicOr(&L.FB[h2 >> 5], 1u << (h2 & 31u));
+    }
+    wave_lds_sync();
+  };
+  // Which of two candidate lists (ids a where ca, ids b where cb) are in
+  // F[0..f), f <= 512.  A wave-wide operation, called by all 64 lanes: each
+  // lane holds the ids of F[lane + 64 i] in registers (loaded once, only when
+  // the filter reports something), and each filter hit costs one compare per
+  // register and a ballot.
+  auto in_frontier2 = [&](const uint64_t* F, uint32_t ida, bool ca, uint32_t idb, bool cb,
+                          bool& fa, bool& fbb) {
+    uint64_t ma = ballot(ca && fb_test(L.FB, fbl, ida));
+    uint64_t mb = ballot(cb && fb_test(L.FB, fbl, idb));
+    fa = fbb = false;
+    if ((ma | mb) == 0) return;
+    const uint32_t* Fw = (const uint32_t*)F;  // key_id = the low word of a key
+    uint32_t fid[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int t = lane + i * PANN_WAVE;
+      fid[i] = t < f ? Fw[2 * t] : PANN_SENTINEL;
+    }
+    while (ma) {
+      const int j = __ffsll((long long)ma) - 1;
+      ma &= ma - 1;
+      const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)ida, j);
+      bool any = false;
+#pragma unroll
+      for (int i = 0; i < 8; i++) any |= fid[i] == x;
+      if (ballot(any) != 0 && lane == j) fa = true;
+    }
+    while (mb) {
+      const int j = __ffsll((long long)mb) - 1;
+      mb &= mb - 1;
+      const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)idb, j);
+      bool any = false;
+#pragma unroll
+      for (int i = 0; i < 8; i++) any |= fid[i] == x;
+      if (ballot(any) != 0 && lane == j) fbb = true;
+    }
+  };
+  if (fbl) fb_rebuild(L.F0, f);
   int c = 0;
 
   // ---- main loop (beamSearch.h:116-223) ----
@@ -319,6 +393,8 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     if (r < hi) {
       const float d = L.DL[r];
       key = make_key(d, L.NL[r]);
+      // (with the frontier-id filter on no evaluated id is in F; the exact
+      // check stays, at the cost of one binary search per id under the cutoff)
       if (!(d >= cutoff) && !sorted_contains(F, f, key)) ins = lds_insert(L.H, hlog2, key_id(key));
       if (rcl) L.RC[rc_slot(key_id(key), rcl)] = key_id(key);  // evaluated and committed
     }
@@ -410,7 +486,9 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
       //   * ids once accepted and since evicted from the frontier: eviction
       //     leaves the frontier full of B smaller keys, so they now fail the
       //     cutoff too;
-      //   * ids in the frontier or pending in C: found by accept_range.
+      //   * ids in the frontier: skipped before their gather when the
+      //     frontier-id filter is on, else found by accept_range;
+      //   * ids pending in C: found by accept_range.
       // Candidates, merges and visits are therefore the reference's; only the
       // count of distance evaluations exceeds its dist_cmps.
       //
@@ -431,10 +509,16 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
       // the reference's seen set skips it, and its outcome here would be
       // the same (rejected again, or already in F or C), so it is not
       // evaluated again
-      const bool e1 = nb != PANN_SENTINEL && nb != self &&
-                      !(rcl && L.RC[rc_slot(nb, rcl)] == nb);
-      const bool e2 = have2 && nb2 != PANN_SENTINEL && nb2 != self &&
-                      !(rcl && L.RC[rc_slot(nb2, rcl)] == nb2);
+      bool e1 = nb != PANN_SENTINEL && nb != self && !(rcl && L.RC[rc_slot(nb, rcl)] == nb);
+      bool e2 = have2 && nb2 != PANN_SENTINEL && nb2 != self &&
+                !(rcl && L.RC[rc_slot(nb2, rcl)] == nb2);
+      if (fbl) {  // ids in the frontier were seen: not gathered again
+        // a wave-wide call: every lane takes part, whatever its own flags
+        bool f1, f2;
+        in_frontier2(F, nb, e1, nb2, e2, f1, f2);
+        e1 = e1 && !f1;
+        e2 = e2 && !f2;
+      }
       const uint64_t m1 = ballot(e1), m2 = ballot(e2);
       const int n1 = __popcll(m1), n2 = __popcll(m2);
       if (e1) L.NL[mbcnt(m1)] = nb;
@@ -502,6 +586,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     u = wave_min_i32(first);
     if (u == 0x7FFFFFFF) u = -1;
     wave_lds_sync();
+    if (fbl) fb_rebuild(cur_buf ? L.F1 : L.F0, f);
   }
 
   // ---- results (beam_search_rerank :557-596) ----
@@ -570,7 +655,7 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 // host-side launch
 // ---------------------------------------------------------------------------
 uint32_t search_lds_bytes(const SearchArgs& a) {
-  return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2, a.rc_log2);
+  return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2, a.rc_log2, a.fb_log2);
 }
 
 template <int ST, int MT, int NCH, bool SEQ, bool EX>

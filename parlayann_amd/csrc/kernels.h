@@ -81,6 +81,7 @@ struct SearchArgs {
   // log2 of the slots of the LDS cache of evaluated ids (0 = none): a
   // direct-mapped table, a slot holding the last committed id that hashed there
   uint32_t rc_log2;
+  uint32_t fb_log2;         // bits of the frontier-id Bloom filter (log2), 0 = none
 };
 
 uint32_t search_lds_bytes(const SearchArgs& a);

@@ -58,8 +58,8 @@ def test_gpu_graph_replay_equals_eager(oracle, store):
 
 def test_gpu_results_independent_of_launch_shape(oracle):
     """The autotuner may pick any (waves per CU, visits per round, evaluated-id
-    cache size): every choice returns the oracle's ids, distances and visit
-    counts."""
+    cache size, frontier-id filter): every choice returns the oracle's ids,
+    distances and visit counts."""
     from parlayann_amd import DeviceIndex, autotune, datagen
     X, Q = datagen.workload_u8_l2(n=8000, nq=300, d=32)
     adj = oracle.build_vamana(X, "l2", 24, 48, 1.2, datagen.insertion_orders(len(X), 1, 4))
@@ -68,13 +68,36 @@ def test_gpu_results_independent_of_launch_shape(oracle):
     for wpc in autotune.WAVES_PER_CU:
         for vpr in autotune.VISITS_PER_ROUND:
             for ec in autotune.EVAL_CACHE_LOG2 + (6, 13):
-                r = idx.search(Q, 10, 48, waves_per_cu=wpc, visits_per_round=vpr,
-                               eval_cache_log2=ec)
-                assert np.array_equal(r["ids"], o["ids"]), (wpc, vpr, ec)
-                assert np.array_equal(r["dists"], o["dists"]), (wpc, vpr, ec)
-                assert np.array_equal(r["visited"], o["visited"]), (wpc, vpr, ec)
-                # the cache only removes evaluations, never below the reference's count
-                assert np.all(r["dist_cmps"].astype(np.uint64) >= o["dist_cmps"]), ec
+                for ff in autotune.FRONTIER_FILTER:
+                    r = idx.search(Q, 10, 48, waves_per_cu=wpc, visits_per_round=vpr,
+                                   eval_cache_log2=ec, frontier_filter=ff)
+                    shape = (wpc, vpr, ec, ff)
+                    assert np.array_equal(r["ids"], o["ids"]), shape
+                    assert np.array_equal(r["dists"], o["dists"]), shape
+                    assert np.array_equal(r["visited"], o["visited"]), shape
+                    # cache and filter only remove evaluations, never below the
+                    # reference's count
+                    assert np.all(r["dist_cmps"].astype(np.uint64) >= o["dist_cmps"]), shape
+    idx.close()
+
+
+@pytest.mark.parametrize("beam", [48, 512, 600])
+def test_gpu_frontier_filter_exact_f32(oracle, beam):
+    """FP32 rows of 200 bytes (the filter's automatic choice is on) with the
+    in-order summation: bit-exact against the oracle with the filter on, off
+    and automatic, at beams up to its 512-id limit and past it (600: off)."""
+    from parlayann_amd import DeviceIndex, datagen
+    X, Q = datagen.workload_f32_l2(n=6000, nq=120, d=50)
+    adj = oracle.build_vamana(X, "l2", 24, 48, 1.2, datagen.insertion_orders(len(X), 1, 9))
+    o = oracle.search_batch(adj, X, "l2", Q, 10, beam)
+    idx = DeviceIndex(X, adj, "l2")
+    for ff in (0, 1, 2):
+        for vpr in (1, 2):
+            r = idx.search(Q, 10, beam, summation="inorder", frontier_filter=ff,
+                           visits_per_round=vpr)
+            for key in ("ids", "dists", "visited"):
+                assert np.array_equal(r[key], o[key]), (key, ff, vpr)
+            assert np.all(r["dist_cmps"].astype(np.uint64) >= o["dist_cmps"]), (ff, vpr)
     idx.close()
 
 
