@@ -239,7 +239,7 @@ __device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const
   }
 }
 
-template <int ST, int MT, int NCH, bool SEQ, bool EX>
+template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
 __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, int lane,
                            unsigned long long* ghash) {
   const uint32_t gtag = a.ghash_tag_base + q + 1u;
@@ -274,7 +274,9 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   for (uint32_t i = lane * 4; i < hslots; i += PANN_WAVE * 4)
     *(uint4*)(L.H + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
   const uint32_t rcl = EX ? 0u : a.rc_log2;
-  const uint32_t fbl = EX ? 0u : a.fb_log2;
+  // the filter is a separate instantiation (FB), so that kernels without it
+  // keep their register count and occupancy
+  const uint32_t fbl = FB ? a.fb_log2 : 0u;
   if (rcl)
     for (uint32_t i = lane * 4; i < (1u << rcl); i += PANN_WAVE * 4)
       *(uint4*)(L.RC + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
@@ -631,7 +633,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   wave_lds_sync();
 }
 
-template <int ST, int MT, int NCH, bool SEQ, bool EX>
+template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
 __global__ __launch_bounds__(PANN_WAVE) void k_beam_search(SearchArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int lane = threadIdx.x;
@@ -642,7 +644,7 @@ __global__ __launch_bounds__(PANN_WAVE) void k_beam_search(SearchArgs a) {
     if (lane == 0) q = atomicAdd(a.work_counter, 1u);
     q = readfirstlane(q);
     if (q >= a.nq) break;
-    search_one<ST, MT, NCH, SEQ, EX>(a, L, q, lane, ghash);
+    search_one<ST, MT, NCH, SEQ, EX, FB>(a, L, q, lane, ghash);
   }
 }
 
@@ -658,10 +660,10 @@ uint32_t search_lds_bytes(const SearchArgs& a) {
   return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2, a.rc_log2, a.fb_log2);
 }
 
-template <int ST, int MT, int NCH, bool SEQ, bool EX>
+template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
 static hipError_t launch_k(const SearchArgs& a, uint32_t grid, hipStream_t s) {
   uint32_t lds = search_lds_bytes(a);
-  auto kern = k_beam_search<ST, MT, NCH, SEQ, EX>;
+  auto kern = k_beam_search<ST, MT, NCH, SEQ, EX, FB>;
   // raise the kernel's dynamic-LDS limit only when a launch needs more, so a
   // repeated launch makes no attribute call (it may be under graph capture)
   static std::atomic<uint32_t> lds_set{0};
@@ -679,8 +681,9 @@ static hipError_t launch_k(const SearchArgs& a, uint32_t grid, hipStream_t s) {
 // kernel carries none of its state
 template <int ST, int MT, int NCH, bool SEQ>
 static hipError_t launch_t(const SearchArgs& a, uint32_t grid, hipStream_t s) {
-  return a.exact_seen ? launch_k<ST, MT, NCH, SEQ, true>(a, grid, s)
-                      : launch_k<ST, MT, NCH, SEQ, false>(a, grid, s);
+  if (a.exact_seen) return launch_k<ST, MT, NCH, SEQ, true, false>(a, grid, s);
+  return a.fb_log2 ? launch_k<ST, MT, NCH, SEQ, false, true>(a, grid, s)
+                   : launch_k<ST, MT, NCH, SEQ, false, false>(a, grid, s);
 }
 
 static int nch_for(int row_bytes) {
