@@ -7,10 +7,11 @@ size) choice on a sample of the caller's queries -- the persistent grid's
 occupancy cap, whether the next visit's rows are gathered together with the
 current one's while a merge is deferred, and how many LDS slots remember
 evaluated ids (fewer rows gathered again, against fewer waves per CU) -- and
-keeps the fastest; the winner is then timed with the frontier-id filter
-forced on and off (fewer rows gathered again, against a filter test before
-every gather).  Winners are cached in a JSON file keyed by device, index
-shape, store and beam, so later runs reuse them.
+keeps the four fastest; those and the frontier-id filter forced on and off
+for the best (fewer rows gathered again, against a filter test before every
+gather) are then timed on the whole query batch, whose throughput a small
+sample's tail can misstate.  Winners are cached in a JSON file keyed by
+device, index shape, store and beam, so later runs reuse them.
 """
 from __future__ import annotations
 
@@ -52,7 +53,8 @@ def _store(table: dict):
 def tune(index, queries, k: int, beam: int, *, store: str = "native",
          summation: str = "tree", rerank: bool = False, k_per_query=None,
          beam_per_query=None, out_stride: Optional[int] = None, sample: int = 16384,
-         reps: int = 2, use_cache: bool = True, rerank_factor: float = 100.0) -> dict:
+         reps: int = 2, use_cache: bool = True, rerank_factor: float = 100.0,
+         confirm: int = 4, confirm_max: int = 131072) -> dict:
     """Fastest {waves_per_cu, visits_per_round, eval_cache_log2,
     frontier_filter} for DeviceIndex `index` on device queries (a torch
     tensor).  Returns the choice and the timings."""
@@ -67,40 +69,44 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
         str(index.dtype), index.metric, store, summation, int(rerank), k, beam,
         "uneven" if uneven else "uniform"))
     table = _load() if use_cache else {}
-    if use_cache and key in table and "frontier_filter" in table[key]:
+    if use_cache and key in table and "queries" in table[key]:
         return dict(table[key], cached=True, key=key)
-    m = min(int(queries.shape[0]), sample)
-    q = queries[:m]
     stride = out_stride or k
-    oi = torch.empty((m, max(stride, 1)), dtype=torch.int32, device=dev)
-    od = torch.empty((m, max(stride, 1)), dtype=torch.float32, device=dev)
-    kpq = k_per_query[:m] if k_per_query is not None else None
-    bpq = beam_per_query[:m] if beam_per_query is not None else None
-    def time_shape(wpc, vpr, ec, ff):
+    mfull = min(int(queries.shape[0]), confirm_max)
+    oi = torch.empty((mfull, max(stride, 1)), dtype=torch.int32, device=dev)
+    od = torch.empty((mfull, max(stride, 1)), dtype=torch.float32, device=dev)
+
+    def time_shape(m, wpc, vpr, ec, ff):
         kw = dict(store=store, summation=summation, rerank=rerank, waves_per_cu=wpc,
                   visits_per_round=vpr, eval_cache_log2=ec, frontier_filter=ff,
-                  k_per_query=kpq, beam_per_query=bpq, rerank_factor=rerank_factor)
-        index.search_device(q, k, beam, oi, od, **kw)  # warm
+                  k_per_query=k_per_query[:m] if k_per_query is not None else None,
+                  beam_per_query=beam_per_query[:m] if beam_per_query is not None else None,
+                  rerank_factor=rerank_factor)
+        index.search_device(queries[:m], k, beam, oi[:m], od[:m], **kw)  # warm
         ms = []
         for _ in range(reps):
-            index.search_device(q, k, beam, oi, od, async_=True, **kw)
+            index.search_device(queries[:m], k, beam, oi[:m], od[:m], async_=True, **kw)
             ms.append(index.last_kernel_ms())
         index.search_status()
         return min(ms)
 
+    m = min(mfull, sample)
     timings = []
     for wpc in WAVES_PER_CU:
         for vpr in VISITS_PER_ROUND:
             for ec in EVAL_CACHE_LOG2:
-                timings.append((time_shape(wpc, vpr, ec, 0), wpc, vpr, ec, 0))
-    _, bw, bv, be, _ = min(timings)
-    for ff in FRONTIER_FILTER[1:]:
-        timings.append((time_shape(bw, bv, be, ff), bw, bv, be, ff))
-    best = min(timings)
+                timings.append((time_shape(m, wpc, vpr, ec, 0), wpc, vpr, ec, 0))
+    timings.sort()
+    _, bw, bv, be, _ = timings[0]
+    shapes = [t[1:] for t in timings[:confirm]] + [(bw, bv, be, ff) for ff in FRONTIER_FILTER[1:]]
+    full = [(time_shape(mfull, *sh), *sh) for sh in shapes]
+    best = min(full)
     res = {"waves_per_cu": best[1], "visits_per_round": best[2], "eval_cache_log2": best[3],
-           "frontier_filter": best[4], "sample_ms": best[0], "sample": m,
+           "frontier_filter": best[4], "ms": best[0], "queries": mfull, "sample": m,
            "timings": [{"waves_per_cu": w, "visits_per_round": v, "eval_cache_log2": e,
-                        "frontier_filter": f_, "ms": t} for t, w, v, e, f_ in timings]}
+                        "frontier_filter": f_, "ms": t, "queries": n_}
+                       for n_, rows in ((m, timings), (mfull, full))
+                       for t, w, v, e, f_ in rows]}
     if use_cache:
         table[key] = res
         _store(table)

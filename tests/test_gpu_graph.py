@@ -138,3 +138,29 @@ def test_gpu_bruteforce_groundtruth_equals_fp64(oracle, metric, dtype, k):
     # both FP64; the host re-score sums pairwise, the oracle in order
     assert np.allclose(d, od, rtol=1e-12, atol=1e-12)
     idx.close()
+
+
+def test_gpu_autotune_returns_a_valid_shape(oracle, tmp_path, monkeypatch):
+    """tune() grids the launch shapes on a sample, confirms the best few on
+    the whole batch, caches the winner; the chosen shape reproduces the
+    oracle's results and a second call is served from the cache."""
+    import torch
+    from parlayann_amd import DeviceIndex, autotune, datagen
+    monkeypatch.setenv("PANN_TUNE_CACHE", str(tmp_path / "tune.json"))
+    X, Q = datagen.workload_u8_l2(n=6000, nq=600, d=32)
+    adj = oracle.build_vamana(X, "l2", 24, 48, 1.2, datagen.insertion_orders(len(X), 1, 5))
+    o = oracle.search_batch(adj, X, "l2", Q, 10, 48)
+    idx = DeviceIndex(X, adj, "l2")
+    tq = torch.from_numpy(Q).to(torch.device("cuda"))
+    res = autotune.tune(idx, tq, 10, 48, sample=128)
+    assert not res["cached"] and res["queries"] == len(Q) and res["sample"] == 128
+    shape = {k: res[k] for k in ("waves_per_cu", "visits_per_round", "eval_cache_log2",
+                                 "frontier_filter")}
+    assert shape["waves_per_cu"] in autotune.WAVES_PER_CU
+    assert shape["frontier_filter"] in autotune.FRONTIER_FILTER
+    r = idx.search(Q, 10, 48, **shape)
+    for key in ("ids", "dists", "visited"):
+        assert np.array_equal(r[key], o[key]), key
+    again = autotune.tune(idx, tq, 10, 48, sample=128)
+    assert again["cached"] and {k: again[k] for k in shape} == shape
+    idx.close()
