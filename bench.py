@@ -109,6 +109,18 @@ def max_rel_dist_error(X, Q, ids, dists, metric, chunk=8192):
     return worst
 
 
+def calib_copy(dev):
+    """An elementwise kernel that reads 1 GiB and writes 1 GiB (its name
+    contains "neg"): the scale of rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    a_ = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    b_ = torch.empty_like(a_)
+    for _ in range(3):
+        torch.neg(a_, out=b_)
+    torch.cuda.synchronize()
+    del a_, b_
+
+
 def gather_ceiling(dev):
     """The device's random row-gather ceiling (parlayann_amd/libpann_calib.so):
     2^26 8-lane reads of the first read_bytes of pseudo-random 128-byte rows of
@@ -180,6 +192,8 @@ def main():
     ap.add_argument("--beam", type=int, default=0,
                     help="fixed operating beam (skips the recall sweep; used for profiling)")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="HNSW workload: skip the open-loop serving runs (profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB read + 1 GiB write elementwise kernel first (calibrates "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE against a copy of known size)")
@@ -261,14 +275,7 @@ def main():
         log(f"[bench] random 128-byte row gather ceiling: {g['rows_per_s'] / 1e9:.1f} G rows/s "
             f"= {g['line_gbs']:,.0f} GB/s of lines")
     if args.calib_copy:
-        # an elementwise kernel that reads 1 GiB and writes 1 GiB (its name
-        # contains "neg"): the scale of rocprofv3's FETCH_SIZE / WRITE_SIZE
-        a_ = torch.ones(1 << 28, dtype=torch.float32, device=dev)
-        b_ = torch.empty_like(a_)
-        for _ in range(3):
-            torch.neg(a_, out=b_)
-        torch.cuda.synchronize()
-        del a_, b_
+        calib_copy(dev)
     for leg in legs:
         st = LEG_OPTS[leg]["store"]
         if st != "native":
@@ -527,6 +534,8 @@ def run_hnsw(args, w):
         if cgt:
             np.save(cgt, gt)
     gceil = None if args.no_calib_gather else gather_ceiling(dev)
+    if args.calib_copy:
+        calib_copy(dev)
     for leg in legs:
         if LEG_OPTS[leg]["store"] != "native":
             idx.build_store(LEG_OPTS[leg]["store"])
@@ -559,9 +568,18 @@ def run_hnsw(args, w):
         ef, rec, efs, mx, oi, od, oc, te = chosen
         ov = torch.zeros(nq, dtype=torch.int32, device=dev)
         om = torch.zeros(nq, dtype=torch.int32, device=dev)
-        run = lambda a=True, ex=False: idx.search_device(
+        launch = {"exact_counts": False, "hash_slots_log2": 0, "waves_per_cu": 0}
+        if not args.no_tune:
+            t = autotune.tune_hnsw(idx, tq, tk, te, mx, **kw)
+            launch = {k_: t[k_] for k_ in launch}
+            log(f"[bench] {leg}: tuned seen set {'exact' if t['exact_counts'] else 'default'}"
+                f" (LDS slots 2^{t['hash_slots_log2'] or 'auto'}), waves/CU "
+                f"{t['waves_per_cu'] or 'auto'}" + (" (cached)" if t["cached"] else ""))
+        ex_t = launch["exact_counts"]
+        shape = {k_: v for k_, v in launch.items() if k_ != "exact_counts"}
+        run = lambda a=True, ex=ex_t: idx.search_device(
             tq, 0, mx, oi, od, oc, k_per_query=tk, ef_per_query=te, out_visited=ov,
-            out_cmps=om, async_=a, exact_counts=ex, **kw)
+            out_cmps=om, async_=a, exact_counts=ex, **kw, **shape)
         for _ in range(args.warmup):
             run()
         torch.cuda.synchronize()
@@ -585,9 +603,9 @@ def run_hnsw(args, w):
         kms_avg = float(np.mean(kms))
         gbs = bytes_launch / (kms_avg * 1e-3) / 1e9
         # open loop: Poisson arrivals through the continuous-batching server
-        srv = BatchServer(idx, mx, 2 << 30, **kw)
         serving = {}
-        for frac in (0.5, 0.8):
+        srv = None if args.no_serve else BatchServer(idx, mx, 2 << 30, launch=launch, **kw)
+        for frac in (() if args.no_serve else (0.5, 0.8)):
             rate = frac * qps
             nreq = int(min(200_000, max(20_000, rate * 2.0)))
             sel = np.arange(nreq) % nq
@@ -601,7 +619,8 @@ def run_hnsw(args, w):
                             ms_per_step=dt * 1e3 / args.steps, kernel_ms=kms_avg,
                             avg_visited=float(vis.mean()), avg_dist_cmps=float(cmps.mean()),
                             avg_evals=float(evals.mean()), bytes_per_launch=bytes_launch,
-                            achieved_gbs=gbs, serving=serving)
+                            achieved_gbs=gbs, serving=serving,
+                            launch_shape=dict(launch, tuned=not args.no_tune))
         log(f"[bench] {leg}: ef {ef} recall {rec:.4f} QPS {qps:,.0f} kernel {kms_avg:.3f} ms "
             f"{gbs:,.0f} GB/s visited {vis.mean():.1f} cmps {cmps.mean():.1f}")
     cpu = None

@@ -111,3 +111,56 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
         table[key] = res
         _store(table)
     return dict(res, cached=False, key=key)
+
+
+HNSW_SHAPES = ((False, 0), (True, 0), (True, 12))  # (exact seen set, its LDS slots log2)
+HNSW_WAVES = (0, 12, 8)
+
+
+def tune_hnsw(index, queries, k_per_query, ef_per_query, max_ef: int, *, store: str = "native",
+              summation: str = "tree", rerank: bool = False, reps: int = 3,
+              use_cache: bool = True) -> dict:
+    """Fastest seen set and occupancy for HNSWIndex `index` on a batch of
+    device queries with per-query k and ef: the default set (every
+    neighbour evaluated, checked against the frontier and pending
+    candidates) or the reference's exact set, which at small ef and high
+    degree gathers far fewer rows (an id already seen is not evaluated
+    again); results are the same.  Returns {exact_counts, hash_slots_log2,
+    waves_per_cu, ms}."""
+    import torch
+    from .index import device_info
+    dev = queries.device
+    di = device_info(dev.index or 0)
+    base = index.base
+    key = "|".join(str(x) for x in (
+        "hnsw", f"cus{di['num_cus']}-hbm{di['hbm_bytes']}", base.n, base.dim,
+        index.model.m, index.model.top_level, str(base.dtype), base.metric, store, summation,
+        int(rerank), int(queries.shape[0]), max_ef))
+    table = _load() if use_cache else {}
+    if use_cache and key in table:
+        return dict(table[key], cached=True, key=key)
+    nq = int(queries.shape[0])
+    oi = torch.empty((nq, max_ef), dtype=torch.int32, device=dev)
+    od = torch.empty((nq, max_ef), dtype=torch.float32, device=dev)
+    oc = torch.empty(nq, dtype=torch.int32, device=dev)
+    timings = []
+    for ex, hs in HNSW_SHAPES:
+        for wpc in HNSW_WAVES:
+            kw = dict(store=store, summation=summation, rerank=rerank, exact_counts=ex,
+                      hash_slots_log2=hs, waves_per_cu=wpc, k_per_query=k_per_query,
+                      ef_per_query=ef_per_query)
+            index.search_device(queries, 0, max_ef, oi, od, oc, **kw)  # warm
+            ms = []
+            for _ in range(reps):
+                index.search_device(queries, 0, max_ef, oi, od, oc, async_=True, **kw)
+                ms.append(base.last_kernel_ms())
+            base.search_status()
+            timings.append((min(ms), ex, hs, wpc))
+    best = min(timings)
+    res = {"exact_counts": best[1], "hash_slots_log2": best[2], "waves_per_cu": best[3],
+           "ms": best[0], "timings": [{"exact_counts": e, "hash_slots_log2": h,
+                                       "waves_per_cu": w, "ms": t} for t, e, h, w in timings]}
+    if use_cache:
+        table[key] = res
+        _store(table)
+    return dict(res, cached=False, key=key)

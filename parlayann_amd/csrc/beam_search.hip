@@ -239,26 +239,49 @@ __device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const
   }
 }
 
+// One beam search of a query: the graph it walks, its beam, k, limit and
+// start points.  A launch runs the HNSW upper levels as passes at beam 1
+// (not final: the nearest id is handed on, nothing is written) and then the
+// main search (final: results and counters written).
+struct Pass {
+  const uint32_t* nbrs;
+  uint32_t nbr_stride;
+  int dl;              // neighbours read per row: min(degree_limit, max degree)
+  int B, k;
+  int64_t limit;
+  const uint32_t* sp;  // start ids, or nullptr: the single id start1
+  int ns;
+  uint32_t start1;
+  uint32_t pass;       // index of this pass within the query (seen-table tag)
+  bool final;
+  uint32_t vis0, cmps0;  // visits and evaluations of the earlier passes
+};
+struct PassOut {
+  uint32_t best;  // id of the nearest frontier entry
+  uint32_t vis, cmps;
+  bool ok;
+};
+
 template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
-__device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, int lane,
-                           unsigned long long* ghash) {
-  const uint32_t gtag = a.ghash_tag_base + q + 1u;
+__device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, int lane,
+                              unsigned long long* ghash, const Pass& P) {
+  const uint32_t gtag = a.ghash_tag_base + q * (a.nup + 1u) + P.pass + 1u;
   using QT = typename QueryElem<ST>::T;
-  const int B = a.beam_pq ? a.beam_pq[q] : a.beam;
-  const int k = a.k_pq ? a.k_pq[q] : a.k;
+  const int B = P.B;
+  const int k = P.k;
   const uint32_t self = a.self_ids ? a.self_ids[q] : PANN_SENTINEL;
   // |C| < batch_factor * B  <=>  |C| < ceil(batch_factor * B) for integer |C|
   const int thr = (int)ceil(a.batch_factor * (double)B);
-  const bool defer_ok = a.limit >= 2 * (int64_t)B;
-  const int dl = a.degree_limit < (int)a.max_degree ? a.degree_limit : (int)a.max_degree;
+  const bool defer_ok = P.limit >= 2 * (int64_t)B;
+  const int dl = P.dl;
   const uint32_t hlog2 = a.hslots_log2;
   const uint32_t hslots = 1u << hlog2;
-  const uint32_t* sp = a.starts_pq ? a.starts_pq + (uint64_t)q * a.starts_stride : a.starts;
-  const int ns = a.starts_pq ? (a.nstarts_pq ? (int)a.nstarts_pq[q] : 1) : (int)a.nstarts;
-  if (B < 1 || B > (int)a.fcap || k < 0 || k > B || k > (int)a.out_stride || ns < 1 ||
-      ns > B) {  // per-query values outside what this launch was sized for
+  const uint32_t* sp = P.sp;
+  const int ns = P.ns;
+  if (B < 1 || B > (int)a.fcap || k < 0 || k > B || (P.final && k > (int)a.out_stride) ||
+      ns < 1 || ns > B) {  // per-query values outside what this launch was sized for
     if (lane == 0) atomicOr(a.status, PANN_ST_BAD_PARAMS);
-    return;
+    return PassOut{PANN_SENTINEL, P.vis0, P.cmps0, false};
   }
 
   // ---- query into registers (group mode) and LDS (sequential / rerank) ----
@@ -293,7 +316,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
   for (int t = 0; t < ns; t += PANN_WAVE) {
     const int m = ns - t < PANN_WAVE ? ns - t : PANN_WAVE;
     if (lane < m) {
-      const uint32_t s = sp[t + lane];
+      const uint32_t s = sp ? sp[t + lane] : P.start1;
       L.NL[lane] = s;
       if constexpr (EX) lds_insert(L.H, hlog2, s);
     }
@@ -405,20 +428,20 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     c += __popcll(mi);
   };
   auto record_visit = [&](uint64_t k) {
-    if (a.vis_out && lane == 0) {
+    if (P.final && a.vis_out && lane == 0) {
       if (nvis < a.vis_cap) a.vis_out[(uint64_t)q * a.vis_cap + nvis] = k;
       else status |= PANN_ST_VISITED_OVERFLOW;
     }
     nvis++;
   };
   auto adjacency = [&](uint32_t id) -> uint32_t {
-    return lane < dl ? a.nbrs[(uint64_t)id * a.nbr_stride + lane] : PANN_SENTINEL;
+    return lane < dl ? P.nbrs[(uint64_t)id * P.nbr_stride + lane] : PANN_SENTINEL;
   };
 
   for (;;) {
     uint64_t* F = cur_buf ? L.F1 : L.F0;
     uint8_t* FV = cur_buf ? L.FV1 : L.FV0;
-    if (u < 0 || (int64_t)nvis >= a.limit) break;
+    if (u < 0 || (int64_t)nvis >= P.limit) break;
     const uint64_t cur_key = F[u];
     const uint32_t cur = key_id(cur_key);
     if (lane == 0) FV[u] = 1;
@@ -499,7 +522,7 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
       // gathered together with cur's.  Its results are committed only if the
       // first visit defers; its accepted ids are inserted after cur's, so an
       // id both lists share is taken once, for cur, as in the reference.
-      const bool have2 = a.lookahead && nu >= 0 && (int64_t)nvis < a.limit;
+      const bool have2 = a.lookahead && nu >= 0 && (int64_t)nvis < P.limit;
       const uint64_t key2 = have2 ? F[nu] : 0;
       const uint32_t id2 = key_id(key2);
       const uint32_t nb2 = !have2 ? PANN_SENTINEL : (id2 == pref_id ? nb_pref : adjacency(id2));
@@ -591,8 +614,14 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     if (fbl) fb_rebuild(cur_buf ? L.F1 : L.F0, f);
   }
 
-  // ---- results (beam_search_rerank :557-596) ----
   const uint64_t* F = cur_buf ? L.F1 : L.F0;
+  if (!P.final) {  // an upper HNSW level: W[0], the nearest (HNSW.hpp:1557-1558)
+    const uint32_t best = key_id(F[0]);
+    if (lane == 0 && status) atomicOr(a.status, status);
+    wave_lds_sync();
+    return PassOut{best, P.vis0 + nvis, P.cmps0 + cmps, true};
+  }
+  // ---- results (beam_search_rerank :557-596) ----
   uint32_t* oi = a.out_ids + (uint64_t)q * a.out_stride;
   float* od = a.out_dists + (uint64_t)q * a.out_stride;
   if (f < k && !a.short_ok) status |= PANN_ST_SHORT_BEAM;
@@ -625,12 +654,14 @@ __device__ void search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, in
     }
   }
   if (lane == 0) {
-    if (a.out_visited) a.out_visited[q] = (a.accumulate ? a.out_visited[q] : 0u) + nvis;
-    if (a.out_cmps) a.out_cmps[q] = (a.accumulate ? a.out_cmps[q] : 0u) + cmps;
+    if (a.out_visited)
+      a.out_visited[q] = (a.accumulate ? a.out_visited[q] : 0u) + P.vis0 + nvis;
+    if (a.out_cmps) a.out_cmps[q] = (a.accumulate ? a.out_cmps[q] : 0u) + P.cmps0 + cmps;
     if (a.out_nfront) a.out_nfront[q] = (uint32_t)f;
     if (status) atomicOr(a.status, status);
   }
   wave_lds_sync();
+  return PassOut{PANN_SENTINEL, P.vis0 + nvis, P.cmps0 + cmps, true};
 }
 
 template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
@@ -644,7 +675,40 @@ __global__ __launch_bounds__(PANN_WAVE) void k_beam_search(SearchArgs a) {
     if (lane == 0) q = atomicAdd(a.work_counter, 1u);
     q = readfirstlane(q);
     if (q >= a.nq) break;
-    search_one<ST, MT, NCH, SEQ, EX, FB>(a, L, q, lane, ghash);
+    // passes 0 .. nup-1: the HNSW upper levels at beam 1 (search_layer_to,
+    // HNSW.hpp:1545-1566), the first from the start points, each next one
+    // from the last one's nearest id; pass nup: the search itself.  One call
+    // site, so the search is inlined once.
+    PassOut po{PANN_SENTINEL, 0u, 0u, true};
+    for (uint32_t l = 0; l <= a.nup; l++) {
+      const bool up = l < a.nup;
+      Pass P;
+      P.nbrs = up ? a.up[l].nbrs : a.nbrs;
+      P.nbr_stride = up ? a.up[l].nbr_stride : a.nbr_stride;
+      // an upper level: degree_limit = its max degree, limit = n
+      P.dl = up ? (int)a.up[l].max_degree
+                : (a.degree_limit < (int)a.max_degree ? a.degree_limit : (int)a.max_degree);
+      P.B = readfirstlane(up ? 1 : (a.beam_pq ? a.beam_pq[q] : a.beam));
+      P.k = readfirstlane(up ? 1 : (a.k_pq ? a.k_pq[q] : a.k));
+      P.limit = up ? (int64_t)a.n : a.limit;
+      if (l == 0) {
+        P.sp = a.starts_pq ? a.starts_pq + (uint64_t)q * a.starts_stride : a.starts;
+        P.ns = a.starts_pq ? (a.nstarts_pq ? (int)a.nstarts_pq[q] : 1) : (int)a.nstarts;
+      } else {
+        P.sp = nullptr;
+        P.ns = 1;
+      }
+      P.start1 = po.best;
+      P.pass = l;
+      P.final = !up;
+      P.vis0 = po.vis;
+      P.cmps0 = po.cmps;
+      po = search_one<ST, MT, NCH, SEQ, EX, FB>(a, L, q, lane, ghash, P);
+      po.best = readfirstlane(po.best);  // the same in every lane: keep it scalar
+      po.vis = readfirstlane(po.vis);
+      po.cmps = readfirstlane(po.cmps);
+      if (!po.ok) break;
+    }
   }
 }
 

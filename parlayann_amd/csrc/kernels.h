@@ -13,6 +13,14 @@ enum : uint32_t {
   PANN_ST_BAD_PARAMS = 8u,        // a per-query k / beam outside the launch's sizing
 };
 
+// An HNSW level searched at beam 1 before the main search (HNSW.hpp:1545-1566).
+constexpr uint32_t PANN_MAX_UP = 16;
+struct UpperLevel {
+  const uint32_t* nbrs;  // n rows of nbr_stride uint32, sentinel past the degree
+  uint32_t nbr_stride;
+  uint32_t max_degree;
+};
+
 struct SearchArgs {
   // search store: n rows of row_bytes valid bytes, pts_stride apart (multiple of 128)
   const char* pts;
@@ -72,7 +80,7 @@ struct SearchArgs {
   uint32_t* work_counter;
   unsigned long long* ghash;  // grid x 2^ghash_log2 slots of (tag << 32 | id)
   uint32_t ghash_log2;
-  uint32_t ghash_tag_base;    // query q of this launch uses tag base + q + 1
+  uint32_t ghash_tag_base;    // pass p of query q uses tag base + q * (nup + 1) + p + 1
   // 1: the seen set holds every id at first encounter (dist_cmps equal to the
   // reference's); 0: it holds accepted ids only (same results, fewer inserts)
   uint32_t exact_seen;
@@ -82,6 +90,12 @@ struct SearchArgs {
   // direct-mapped table, a slot holding the last committed id that hashed there
   uint32_t rc_log2;
   uint32_t fb_log2;         // bits of the frontier-id Bloom filter (log2), 0 = none
+  // HNSW descent in the same launch: before the search above, each query is
+  // searched at beam 1 on up[0], up[1], .. up[nup - 1] (top level first), the
+  // first pass from the start points, each later pass and the main search
+  // from the previous pass's nearest id; visits and evaluations add up
+  uint32_t nup;
+  UpperLevel up[PANN_MAX_UP];
 };
 
 uint32_t search_lds_bytes(const SearchArgs& a);

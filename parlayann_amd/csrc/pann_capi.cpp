@@ -490,7 +490,7 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
                            const uint32_t* starts, uint32_t nstarts, const StartSets* ss,
                            const pann_search_opts* opts, uint32_t nq, pann::SearchArgs& a,
                            int& store_kind, bool& sequential, uint32_t& grid,
-                           hipStream_t stream, StreamCtx*& C) {
+                           hipStream_t stream, StreamCtx*& C, uint32_t passes = 1) {
   {
     pann_status cst = stream_ctx(idx, stream, &C);
     if (cst != PANN_OK) return cst;
@@ -637,11 +637,13 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     uint32_t glog2 = std::max<uint32_t>(
         ceil_log2((uint64_t)2 * (10 + B) * a.degree_limit) + 1, hlog2 + 2);
     a.ghash_log2 = glog2;
-    pann_status st = ensure_ghash(*C, (size_t)grid << glog2, nq);
+    // every pass of every query gets its own tag (the HNSW descent runs
+    // passes * nq searches in one launch)
+    pann_status st = ensure_ghash(*C, (size_t)grid << glog2, nq * passes);
     if (st != PANN_OK) return st;
     a.ghash = C->d_ghash;
     a.ghash_tag_base = C->ghash_tag;
-    C->ghash_tag += nq;
+    C->ghash_tag += nq * passes;
   }
   // start points go up only when they change, so a repeated search issues no
   // host-to-device copy (and can be captured into a graph)
@@ -1218,6 +1220,57 @@ pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq,
   const bool f32_queries = idx->dtype == PANN_DT_F32;
   StartSets ss{C->d_eps, nullptr, 1u};
   HIP_TRY(hipEventRecord(C->ev0, stream), "hipEventRecord");
+  if (top_level <= pann::PANN_MAX_UP) {
+    // one launch: each wave descends its query through levels top .. 1 at
+    // beam 1 and then searches layer 0 with ef, so the descents of some
+    // queries overlap the layer-0 searches of others
+    const auto& G = idx->levels[0];
+    pann_qparams qp{(int64_t)ef, (int64_t)ef, limit > 0 ? limit : (int64_t)idx->n,
+                    (int64_t)G.max_degree, 100.0, 0.125};
+    pann::SearchArgs a;
+    int store_kind;
+    bool sequential;
+    uint32_t grid;
+    StreamCtx* Cl = nullptr;
+    pann_status st = prepare_search(idx, 0, &qp, entrance, nentrance, nullptr, &o, nq, a,
+                                    store_kind, sequential, grid, stream, Cl, top_level + 1);
+    if (st != PANN_OK) return st;
+    a.queries = (const char*)queries;
+    a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
+    a.fqueries = (const char*)queries;
+    a.fq_stride = (uint64_t)idx->dim * 4;
+    st = stage_queries(idx, *C, o.store, queries, nq, a, stream);
+    if (st != PANN_OK) return st;
+    a.nq = nq;
+    a.out_ids = out_ids;
+    a.out_dists = out_dists;
+    a.out_stride = out_stride;
+    a.out_nfront = out_count;
+    a.out_visited = out_visited;
+    a.out_cmps = out_cmps;  // count_cmps sums every level (:1553, :1097-1098)
+    a.accumulate = 0u;
+    a.short_ok = 1u;
+    if (ef_pq) {  // per-query ef: the whole frontier comes back
+      a.k_pq = ef_pq;
+      a.beam_pq = ef_pq;
+    }
+    a.nup = top_level;
+    for (uint32_t i = 0; i < top_level; i++) {
+      const auto& U = idx->levels[top_level - i];
+      a.up[i] = pann::UpperLevel{U.d_nbrs, U.nbr_stride, U.max_degree};
+    }
+    grid = std::min<uint32_t>(grid, nq);
+    HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
+            "HNSW search launch");
+    HIP_TRY(pann::launch_hnsw_cut(out_ids, out_dists, out_stride, out_count, nq, k, k_pq,
+                                  node_ids, stream),
+            "HNSW cut launch");
+    HIP_TRY(hipEventRecord(C->ev1, stream), "hipEventRecord");
+    idx->last_timed = C;
+    if (o.async) return PANN_OK;
+    return check_status(*C, stream, true);
+  }
+  // more levels than one launch takes: a launch per level
   // search_layer_to (HNSW.hpp:1545-1566): ef = 1 from the entrance through
   // levels top .. 1, keeping the closest; then layer 0 with ef (:1582-1586)
   for (int64_t l = top_level; l >= 0; l--) {

@@ -169,18 +169,25 @@ class HNSWIndex:
                       limit: int = 0, store: str = "native", summation: str = "inorder",
                       rerank: bool = False, out_visited=None, out_cmps=None, stream=None,
                       async_: bool = False, k_per_query=None, ef_per_query=None,
-                      exact_counts: bool = False):
+                      exact_counts: bool = False, hash_slots_log2: int = 0,
+                      waves_per_cu: int = 0, visits_per_round: int = 0,
+                      eval_cache_log2: int = 0, frontier_filter: int = 0):
         """HNSW::search for a batch of device queries (torch tensors).  Rows of
         out_ids / out_dists (stride >= ef) hold out_count[q] results, point ids.
         k_per_query / ef_per_query (int32 tensors, ef_per_query <= ef) mix
-        requests in one batch.  With async_ the tensors must stay alive, and
-        unchanged, until search_status on the same stream."""
+        requests in one batch.  The launch options are DeviceIndex.search's
+        (exact_counts also selects the reference's exact seen set, which at
+        small ef and high degree gathers fewer rows).  With async_ the tensors
+        must stay alive, and unchanged, until search_status on the same
+        stream."""
         import torch
         if self._node_ids is not None and self._dev_node_ids is None:
             self._dev_node_ids = torch.from_numpy(self._node_ids.view(np.int32)).to(queries.device)
         nq = int(queries.shape[0])
         ent = np.ascontiguousarray(self.model.entrance, dtype=np.uint32)
-        op = DeviceIndex._opts(store, summation, rerank, 0, async_, 0, exact_counts)
+        op = DeviceIndex._opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu,
+                               exact_counts, visits_per_round, eval_cache_log2,
+                               frontier_filter)
         if stream is not None and hasattr(stream, "cuda_stream"):
             stream = stream.cuda_stream
         L.check(L.lib().pann_hnsw_search(

@@ -64,11 +64,14 @@ class _Slot:
 class BatchServer:
     def __init__(self, index, max_ef: int, budget_bytes: int, *, slots: int = 2,
                  store: str = "native", rerank: bool = False, summation: str = "tree",
-                 device: int = 0):
+                 device: int = 0, launch: dict | None = None):
+        """launch: HNSWIndex.search_device launch options (e.g. the seen set
+        and occupancy autotune.tune_hnsw chose); results do not depend on them."""
         import torch
         self.index = index
         self.max_ef = int(max_ef)
         self.store, self.rerank, self.summation = store, rerank, summation
+        self.launch = dict(launch or {})
         dim = index.model.dim
         qdt = torch.float32 if index.base.dtype == np.float32 else torch.uint8
         qbytes = dim * (4 if qdt == torch.float32 else 1)
@@ -103,7 +106,8 @@ class BatchServer:
             self.index.search_device(s.q_dev[:m], 0, self.max_ef, s.oi[:m], s.od[:m], s.oc[:m],
                                      store=self.store, rerank=self.rerank,
                                      summation=self.summation, k_per_query=s.k_dev[:m],
-                                     ef_per_query=s.ef_dev[:m], stream=s.stream, async_=True)
+                                     ef_per_query=s.ef_dev[:m], stream=s.stream, async_=True,
+                                     **self.launch)
             s.oi_host[:m].copy_(s.oi[:m], non_blocking=True)
             s.oc_host[:m].copy_(s.oc[:m], non_blocking=True)
             s.done.record(s.stream)

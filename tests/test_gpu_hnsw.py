@@ -63,6 +63,67 @@ def test_gpu_search_equals_oracle(oracle, small_l2, k, ef, exact):
     idx.close()
 
 
+@pytest.mark.parametrize("extra", [0, 13, 20])
+@pytest.mark.parametrize("exact", [False, True])
+def test_gpu_search_level_counts(oracle, small_l2, extra, exact):
+    """Up to 16 upper levels run in the search's own launch, more in a launch
+    each: the top level repeated `extra` times (a valid model, the descent
+    repeats a level) gives the oracle's results either way."""
+    from parlayann_amd import hnsw
+    X, lev, adj, ent = small_l2
+    adj2 = list(adj) + [adj[-1]] * extra
+    lev2 = lev.copy()
+    lev2[ent] = len(adj2) - 1  # the entrance sits on the top level
+    print(f"[hnsw levels] upper levels={len(adj2) - 1} exact_counts={exact}", flush=True)
+    Q = _queries(X, nq=200, seed=3)
+    idx = hnsw.HNSWIndex(X, model_of(X, lev2, adj2, ent, M, EFC), "l2")
+    r = _search(idx, Q, 10, 32, exact)
+    oi, od, oc, om, ov = oracle.hnsw_search(adj2, X, "l2", Q, 10, 32, [ent])
+    assert np.array_equal(r["count"], oc)
+    for i in range(len(Q)):
+        c = oc[i]
+        assert np.array_equal(r["ids"][i, :c], oi[i, :c]), i
+        assert np.array_equal(r["dists"][i, :c], od[i, :c]), i
+    assert np.array_equal(r["visited"], ov)
+    if exact:
+        assert np.array_equal(r["dist_cmps"].astype(np.uint64), om)
+    idx.close()
+
+
+def test_gpu_tune_hnsw_choice_is_exact(oracle, small_l2, tmp_path, monkeypatch):
+    """autotune.tune_hnsw picks a seen set and occupancy; the chosen options
+    return the oracle's results through the serving path's call."""
+    import torch
+    from parlayann_amd import autotune, hnsw
+    monkeypatch.setenv("PANN_TUNE_CACHE", str(tmp_path / "tune.json"))
+    X, lev, adj, ent = small_l2
+    Q = _queries(X, nq=256, seed=5)
+    idx = hnsw.HNSWIndex(X, model_of(X, lev, adj, ent, M, EFC), "l2")
+    dev = torch.device("cuda", 0)
+    tq = torch.from_numpy(Q).to(dev)
+    ks = np.where(np.arange(len(Q)) % 2 == 0, 10, 40).astype(np.int32)
+    efs = np.maximum(ks, 32).astype(np.int32)
+    tk, te = torch.from_numpy(ks).to(dev), torch.from_numpy(efs).to(dev)
+    t = autotune.tune_hnsw(idx, tq, tk, te, 40, summation="inorder")
+    assert (t["exact_counts"], t["hash_slots_log2"]) in autotune.HNSW_SHAPES
+    oi = torch.zeros((len(Q), 40), dtype=torch.int32, device=dev)
+    od = torch.zeros((len(Q), 40), dtype=torch.float32, device=dev)
+    oc = torch.zeros(len(Q), dtype=torch.int32, device=dev)
+    idx.search_device(tq, 0, 40, oi, od, oc, k_per_query=tk, ef_per_query=te,
+                      summation="inorder", exact_counts=t["exact_counts"],
+                      hash_slots_log2=t["hash_slots_log2"], waves_per_cu=t["waves_per_cu"])
+    gi, gd, gc = oi.cpu().numpy().view(np.uint32), od.cpu().numpy(), oc.cpu().numpy()
+    for k in (10, 40):
+        sel = np.nonzero(ks == k)[0]
+        oi_, od_, oc_, _, _ = oracle.hnsw_search(adj, X, "l2", Q[sel], k, max(k, 32), [ent])
+        assert np.array_equal(gc[sel], oc_)
+        for j, i in enumerate(sel):
+            assert np.array_equal(gi[i, :oc_[j]], oi_[j, :oc_[j]]), i
+            assert np.array_equal(gd[i, :oc_[j]], od_[j, :oc_[j]]), i
+    assert autotune.tune_hnsw(idx, tq, tk, te, 40, summation="inorder")["cached"]
+    idx.close()
+
+
 def _search(idx, Q, k, ef, exact=False, **kw):
     import torch
     from parlayann_amd import _lib as L
