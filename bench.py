@@ -490,7 +490,7 @@ def run_hnsw(args, w):
     queries (inputs resident), then open-loop Poisson arrivals through the
     continuous-batching server at 50% and 80% of the closed-loop rate."""
     import torch
-    from parlayann_amd import hnsw
+    from parlayann_amd import autotune, hnsw
     from parlayann_amd.groundtruth import groundtruth_device
     from parlayann_amd.recall import recall_plain
     from parlayann_amd.serve import BatchServer, poisson_arrivals
