@@ -317,16 +317,19 @@ def main():
         rec_ties = recall_with_ties(X, Q, gt, ids, k, w["metric"])
         mre = max_rel_dist_error(X, Q, ids, dists, w["metric"] if w["kind"] != "u8" else "l2")
         tuned = {"waves_per_cu": 0, "visits_per_round": 0, "eval_cache_log2": 0,
-                 "frontier_filter": 0}
+                 "frontier_filter": 0, "exact_counts": False, "hash_slots_log2": 0}
         if not args.no_tune:
             tuned = autotune.tune(idx, tq, k, B, rerank_factor=args.rerank_factor, **opts)
-            log(f"[bench] {leg}: tuned waves/CU {tuned['waves_per_cu']}, visits/round "
-                f"{tuned['visits_per_round']}, eval cache 2^{tuned['eval_cache_log2']}, "
-                f"frontier filter {('auto', 'on', 'off')[tuned['frontier_filter']]}" +
+            log(f"[bench] {leg}: tuned waves/CU {tuned['waves_per_cu']}, " +
+                (f"exact seen set (LDS slots 2^{tuned['hash_slots_log2'] or 'auto'})"
+                 if tuned["exact_counts"] else
+                 f"visits/round {tuned['visits_per_round']}, eval cache "
+                 f"2^{tuned['eval_cache_log2']}, frontier filter "
+                 f"{('auto', 'on', 'off')[tuned['frontier_filter']]}") +
                 (" (cached)" if tuned["cached"] else "") +
                 f" [{autotune.cache_path()} {tuned['key']}]")
         shape = {k_: tuned[k_] for k_ in ("waves_per_cu", "visits_per_round", "eval_cache_log2",
-                                          "frontier_filter")}
+                                          "frontier_filter", "exact_counts", "hash_slots_log2")}
         for _ in range(args.warmup):
             idx.search_device(tq, k, B, oi, od, out_visited=ov, out_cmps=oc, async_=True,
                               rerank_factor=args.rerank_factor, **opts, **shape)
@@ -346,6 +349,7 @@ def main():
         # (beamSearch.h:178) per query: the algorithmic gather count; the timed
         # kernel re-evaluates ids it rejected earlier instead of remembering them
         idx.search_device(tq, k, B, oi, od, out_visited=ov, out_cmps=oc, exact_counts=True,
+                          hash_slots_log2=shape["hash_slots_log2"],
                           rerank_factor=args.rerank_factor, **opts)
         torch.cuda.synchronize()
         vis = ov.cpu().numpy().astype(np.float64)
@@ -385,8 +389,10 @@ def main():
             shape = {"waves_per_cu": 0, "visits_per_round": 0, "eval_cache_log2": 0,
                      "frontier_filter": 0}
             if not args.no_tune:
+                # replayed from a captured hipGraph: the default seen set
                 t = autotune.tune(idx, tq, 100, 256, k_per_query=tk, beam_per_query=tb,
-                                  out_stride=100, rerank_factor=args.rerank_factor, **opts)
+                                  out_stride=100, rerank_factor=args.rerank_factor,
+                                  allow_exact=False, **opts)
                 shape = {k_: t[k_] for k_ in ("waves_per_cu", "visits_per_round",
                                               "eval_cache_log2", "frontier_filter")}
             run = lambda a=False, st=None: idx.search_device(

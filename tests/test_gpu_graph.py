@@ -155,9 +155,11 @@ def test_gpu_autotune_returns_a_valid_shape(oracle, tmp_path, monkeypatch):
     res = autotune.tune(idx, tq, 10, 48, sample=128)
     assert not res["cached"] and res["queries"] == len(Q) and res["sample"] == 128
     shape = {k: res[k] for k in ("waves_per_cu", "visits_per_round", "eval_cache_log2",
-                                 "frontier_filter")}
-    assert shape["waves_per_cu"] in autotune.WAVES_PER_CU
+                                 "frontier_filter", "exact_counts", "hash_slots_log2")}
+    assert shape["waves_per_cu"] in autotune.WAVES_PER_CU + (0,)
     assert shape["frontier_filter"] in autotune.FRONTIER_FILTER
+    assert not shape["exact_counts"] or (shape["hash_slots_log2"], shape["waves_per_cu"]) \
+        in autotune.EXACT_SHAPES
     r = idx.search(Q, 10, 48, **shape)
     for key in ("ids", "dists", "visited"):
         assert np.array_equal(r[key], o[key]), key
