@@ -233,7 +233,12 @@ class GraphIndex:
 
     def batch_search(self, queries, knn: int, beam_width: int, quant: bool = False,
                      visit_limit: int = -1):
-        """(ids uint32[nq, knn], dists float32[nq, knn]) (graph_index.cpp:192-216)."""
+        """(ids uint32[nq, knn], dists float32[nq, knn]) (graph_index.cpp:192-216).
+        queries: numpy (or anything numpy converts, a CPU torch tensor too); a
+        torch tensor on the index's GPU is searched in place and the results
+        come back as device tensors (ids int32 holding the uint32 bits)."""
+        if _on_device(queries):
+            return self._batch_search_device(queries, knn, beam_width, quant, visit_limit)
         Q = self._prep_queries(queries, quant)
         limit = visit_limit if visit_limit > 0 else None
         if self._hnsw is not None:
@@ -261,8 +266,53 @@ class GraphIndex:
 
     search = batch_search
 
+    def _batch_search_device(self, queries, knn, beam_width, quant, visit_limit):
+        import torch
+        want = {np.dtype(np.float32): torch.float32, np.dtype(np.uint8): torch.uint8,
+                np.dtype(np.int8): torch.int8}[np.dtype(self.dtype)]
+        if queries.dtype != want:
+            raise TypeError(f"queries must be {want} for this index")
+        Q = queries.reshape(1, -1) if queries.dim() == 1 else queries
+        Q = Q.contiguous()
+        if quant and self.metric == "mips" and self.dtype == np.float32:
+            # graph_index.cpp:171, as _normalize: double norm, x * float(1/norm)
+            norm = torch.sqrt((Q.double() * Q.double()).sum(1))
+            norm[norm == 0] = 1.0
+            Q = (Q * (1.0 / norm).float()[:, None]).contiguous()
+        use_q = quant and self.dtype == np.float32
+        nq = Q.shape[0]
+        dev = Q.device
+        limit = visit_limit if visit_limit > 0 else None
+        if self._hnsw is not None:
+            if use_q:
+                self._hnsw.build_store("q8")
+            ef = max(beam_width, knn)
+            oi = torch.empty((nq, ef), dtype=torch.int32, device=dev)
+            od = torch.empty((nq, ef), dtype=torch.float32, device=dev)
+            oc = torch.empty(nq, dtype=torch.int32, device=dev)
+            self._hnsw.search_device(Q, knn, ef, oi, od, oc, limit=limit or 0,
+                                     store="q8" if use_q else "native", rerank=use_q)
+            w = min(knn, ef)
+            ids = torch.full((nq, knn), -1, dtype=torch.int32, device=dev)
+            dists = torch.full((nq, knn), float("inf"), dtype=torch.float32, device=dev)
+            have = torch.arange(w, device=dev)[None, :] < oc[:, None]
+            ids[:, :w] = torch.where(have, oi[:, :w], ids[:, :w])
+            dists[:, :w] = torch.where(have, od[:, :w], dists[:, :w])
+            return ids, dists
+        if use_q:
+            self._dev.build_store("q8")
+        oi = torch.empty((nq, knn), dtype=torch.int32, device=dev)
+        od = torch.empty((nq, knn), dtype=torch.float32, device=dev)
+        self._dev.search_device(Q, knn, beam_width, oi, od, limit=limit,
+                                store="q8" if use_q else "native", rerank=use_q)
+        return oi, od
+
     def single_search(self, q, knn: int, beam_width: int, quant: bool = False,
                       visit_limit: int = -1):
+        if _on_device(q):
+            ids, _ = self._batch_search_device(q.reshape(1, -1), knn, beam_width, quant,
+                                               visit_limit)
+            return ids[0]
         ids, _ = self.batch_search(np.asarray(q)[None, :], knn, beam_width, quant, visit_limit)
         return ids[0]
 
@@ -284,6 +334,11 @@ class GraphIndex:
                              "l2" if self.metric == "Euclidian" else "mips")
         print(f"Recall: {r:.6f}")
         return r
+
+
+def _on_device(x) -> bool:
+    """A torch tensor in GPU memory (numpy and CPU tensors go the host way)."""
+    return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
 
 
 def _normalize(X: np.ndarray) -> np.ndarray:
