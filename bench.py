@@ -428,14 +428,31 @@ def main():
             idx.search_status(gs)
             gms = [e0.elapsed_time(e1) for e0, e1 in ev]
             same = np.array_equal(ui.cpu().numpy().view(np.uint32)[sel][:, :10], ids[sel][:, :10])
+            del graph
+            # algorithmic bytes as for the uniform legs, per query with its own
+            # k and beam: the reference's counts from one untimed eager launch
+            # with its exact seen set
+            idx.search_device(tq, 100, 256, ui, ud, out_visited=ov, out_cmps=oc,
+                              k_per_query=tk, beam_per_query=tb, exact_counts=True,
+                              rerank_factor=args.rerank_factor, **opts)
+            torch.cuda.synchronize()
+            vis = ov.cpu().numpy().astype(np.float64)
+            cmps = oc.cpu().numpy().astype(np.float64)
+            rb = row_bytes(leg, w["d"], w["kind"])
+            nrr = (np.minimum(ks * args.rerank_factor, bs) if opts["rerank"] else 0)
+            per_q = vis * 4 * (1 + avg_deg) + cmps * rb + nrr * w["d"] * 4 + w["d"] * 4 + ks * 8
+            gms_avg = float(np.mean(gms))
+            gbs = float(per_q.sum()) / (gms_avg * 1e-3) / 1e9
             results["uneven-" + leg] = dict(
                 qps=nq * args.steps / dt, ms_per_step=dt * 1e3 / args.steps,
-                graph_ms=float(np.mean(gms)), recall_at_10_k_ge_10=round(rec, 5),
+                graph_ms=gms_avg, recall_at_10_k_ge_10=round(rec, 5),
                 replay_equals_eager=bool(same), k_choices=[1, 10, 100], beam_range=[16, 256],
                 launch="hipGraph replay of one persistent launch",
+                avg_visited=float(vis.mean()), avg_dist_cmps=float(cmps.mean()),
+                bytes_per_launch=float(per_q.sum()), achieved_gbs=gbs,
                 launch_shape=dict(shape, tuned=not args.no_tune))
-            del graph
-            log(f"[bench] uneven {leg}: QPS {nq * args.steps / dt:,.0f} recall@10(k>=10) {rec:.4f}")
+            log(f"[bench] uneven {leg}: QPS {nq * args.steps / dt:,.0f} recall@10(k>=10) {rec:.4f}"
+                f" replay {gms_avg:.2f} ms {gbs:,.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
