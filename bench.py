@@ -628,6 +628,10 @@ def run_hnsw(args, w):
         # open loop: Poisson arrivals through the continuous-batching server
         serving = {}
         srv = None if args.no_serve else BatchServer(idx, mx, 2 << 30, launch=launch, **kw)
+        if srv is not None:  # untimed: host buffers and clocks settle before the runs
+            nw = int(min(20_000, max(2_000, 0.25 * qps)))
+            srv.serve(Q[np.arange(nw) % nq], ks[np.arange(nw) % nq], efs[np.arange(nw) % nq],
+                      poisson_arrivals(nw, 0.5 * qps, seed=4))
         for frac in (() if args.no_serve else (0.5, 0.8)):
             rate = frac * qps
             nreq = int(min(200_000, max(20_000, rate * 2.0)))
