@@ -137,6 +137,11 @@ __global__ __launch_bounds__(PANN_WAVE) __attribute__((amdgpu_waves_per_eu(1, 2)
     wave_lds_sync();
   };
 
+  // lane j < NQ holds query j's threshold; the selection fetches it with
+  // v_readlane at a compile-time lane, off the LDS pipe (thresholds change
+  // only when a buffer is cut, which reloads them)
+  static_assert(NQ <= PANN_WAVE, "one threshold per lane");
+  float thr_reg = FLT_MAX;
   const PT* X = (const PT*)a.X;
   uint4 wc[NV], wn[NV];
   load_half_rows<PT, NV>(wc, X, a.xs, pb, pe, half, nw, lane);
@@ -167,9 +172,13 @@ __global__ __launch_bounds__(PANN_WAVE) __attribute__((amdgpu_waves_per_eu(1, 2)
       for (int g = 0; g < QS; g++)
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-          const uint32_t i = (uint32_t)(g * 32 + (r >> 2) * 8 + h * 4 + (r & 3));
+          const int i0 = g * 32 + (r >> 2) * 8 + (r & 3);  // query of this element, h = 0
+          const uint32_t i = (uint32_t)(i0 + h * 4);
+          const float t0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr_reg), i0));
+          const float t1 =
+              __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr_reg), i0 + 4));
           const float d = a.metric == 0 ? nc - 2.0f * acc[g][r] : -acc[g][r];
-          if (q0 + i < a.nq && d < thr[i]) {
+          if (q0 + i < a.nq && d < (h ? t1 : t0)) {
             const uint32_t pos = atomicAdd(&cnt[i], 1u);
             bufq[(uint64_t)i * a.nranges * a.cap + pos] = make_key(d, (uint32_t)pid);
           }
@@ -178,10 +187,13 @@ __global__ __launch_bounds__(PANN_WAVE) __attribute__((amdgpu_waves_per_eu(1, 2)
     wave_lds_sync();
     // room for one more tile: a buffer past cap - 32 is cut back to kk
     uint64_t full = __ballot(lane < NQ && cnt[lane < NQ ? lane : 0] > a.cap - 32);
-    while (full) {
-      const uint32_t i = (uint32_t)__ffsll((long long)full) - 1;
-      full &= full - 1;
-      compact(i);
+    if (full) {
+      while (full) {
+        const uint32_t i = (uint32_t)__ffsll((long long)full) - 1;
+        full &= full - 1;
+        compact(i);
+      }
+      thr_reg = lane < NQ ? thr[lane] : FLT_MAX;
     }
 #pragma unroll
     for (int t = 0; t < NV; t++) wc[t] = wn[t];
