@@ -102,8 +102,8 @@ typedef struct {
   int32_t frontier_filter;  /* exact_counts = 0: a Bloom filter of the frontier's ids, each hit
                                checked exactly, keeps ids already in the frontier from being
                                gathered again.  0 = automatic (on for float32 rows wider than
-                               one 128-byte line), 1 = on, 2 = off; beams above 512 run
-                               without it.  Results do not depend on it (a tuning knob) */
+                               one 128-byte line), 1 = on, 2 = off.  Results do not depend
+                               on it (a tuning knob) */
 } pann_search_opts;
 
 /* Creates the device index: uploads the points (rows padded to 128 bytes)

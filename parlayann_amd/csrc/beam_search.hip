@@ -360,40 +360,41 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
     wave_lds_sync();
   };
   // Which of two candidate lists (ids a where ca, ids b where cb) are in
-  // F[0..f), f <= 512.  A wave-wide operation, called by all 64 lanes: each
-  // lane holds the ids of F[lane + 64 i] in registers (loaded once, only when
-  // the filter reports something), and each filter hit costs one compare per
-  // register and a ballot.
+  // F[0..f).  A wave-wide operation, called by all 64 lanes: only when the
+  // filter reports something, each lane loads the ids of F[c + lane + 64 i]
+  // (i < 8) for each 512-entry chunk c, and each filter hit still unresolved
+  // costs one compare per register and a ballot.
   auto in_frontier2 = [&](const uint64_t* F, uint32_t ida, bool ca, uint32_t idb, bool cb,
                           bool& fa, bool& fbb) {
     uint64_t ma = ballot(ca && fb_test(L.FB, fbl, ida));
     uint64_t mb = ballot(cb && fb_test(L.FB, fbl, idb));
     fa = fbb = false;
-    if ((ma | mb) == 0) return;
     const uint32_t* Fw = (const uint32_t*)F;  // key_id = the low word of a key
-    uint32_t fid[8];
+    for (int c0 = 0; c0 < f && (ma | mb) != 0; c0 += 8 * PANN_WAVE) {
+      uint32_t fid[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const int t = lane + i * PANN_WAVE;
-      fid[i] = t < f ? Fw[2 * t] : PANN_SENTINEL;
-    }
-    while (ma) {
-      const int j = __ffsll((long long)ma) - 1;
-      ma &= ma - 1;
-      const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)ida, j);
-      bool any = false;
+      for (int i = 0; i < 8; i++) {
+        const int t = c0 + lane + i * PANN_WAVE;
+        fid[i] = t < f ? Fw[2 * t] : PANN_SENTINEL;
+      }
+      for (uint64_t m = ma; m; m &= m - 1) {
+        const int j = __ffsll((long long)m) - 1;
+        const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)ida, j);
+        bool any = false;
 #pragma unroll
-      for (int i = 0; i < 8; i++) any |= fid[i] == x;
-      if (ballot(any) != 0 && lane == j) fa = true;
-    }
-    while (mb) {
-      const int j = __ffsll((long long)mb) - 1;
-      mb &= mb - 1;
-      const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)idb, j);
-      bool any = false;
+        for (int i = 0; i < 8; i++) any |= fid[i] == x;
+        if (ballot(any) != 0 && lane == j) fa = true;
+      }
+      for (uint64_t m = mb; m; m &= m - 1) {
+        const int j = __ffsll((long long)m) - 1;
+        const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)idb, j);
+        bool any = false;
 #pragma unroll
-      for (int i = 0; i < 8; i++) any |= fid[i] == x;
-      if (ballot(any) != 0 && lane == j) fbb = true;
+        for (int i = 0; i < 8; i++) any |= fid[i] == x;
+        if (ballot(any) != 0 && lane == j) fbb = true;
+      }
+      ma &= ~ballot(fa);  // found: no later chunk needs to look
+      mb &= ~ballot(fbb);
     }
   };
   if (fbl) fb_rebuild(L.F0, f);

@@ -593,7 +593,8 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     return fail(PANN_ERR_ARG, "eval_cache_log2 must be 0 or in [6, 13]");
   a.rc_log2 = o.exact_counts ? 0u : (uint32_t)o.eval_cache_log2;
   // frontier-id Bloom filter: 8 bits per frontier slot (2 of them set per
-  // id); its exact check holds a frontier of up to 512 ids in registers.
+  // id), at most 8192 bits; its exact check scans the frontier 512 ids at a
+  // time from registers.
   // Automatic: on for float32 rows wider than one 128-byte line, where a
   // gather it saves outweighs the filter test before every gather (measured
   // +3..7 %); the 1- and 2-line rows of the reduced-precision stores lose
@@ -603,9 +604,9 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   const bool fb_on =
       o.frontier_filter == 1 ||
       (o.frontier_filter == 0 && store_kind == pann::ST_F32 && a.row_bytes > 128);
-  a.fb_log2 = (o.exact_counts || a.fcap > 512 || !fb_on)
+  a.fb_log2 = (o.exact_counts || !fb_on)
                   ? 0u
-                  : std::min<uint32_t>(std::max<uint32_t>(ceil_log2((uint64_t)8 * a.fcap), 9), 12);
+                  : std::min<uint32_t>(std::max<uint32_t>(ceil_log2((uint64_t)8 * a.fcap), 9), 13);
   uint32_t hlog2;
   if (o.exact_counts) {
     // The reference's seen set (every id met): a small LDS table keeps more

@@ -81,11 +81,12 @@ def test_gpu_results_independent_of_launch_shape(oracle):
     idx.close()
 
 
-@pytest.mark.parametrize("beam", [48, 512, 600])
+@pytest.mark.parametrize("beam", [48, 512, 600, 1500])
 def test_gpu_frontier_filter_exact_f32(oracle, beam):
     """FP32 rows of 200 bytes (the filter's automatic choice is on) with the
     in-order summation: bit-exact against the oracle with the filter on, off
-    and automatic, at beams up to its 512-id limit and past it (600: off)."""
+    and automatic, at beams inside one 512-id chunk of the exact check and
+    across several."""
     from parlayann_amd import DeviceIndex, datagen
     X, Q = datagen.workload_f32_l2(n=6000, nq=120, d=50)
     adj = oracle.build_vamana(X, "l2", 24, 48, 1.2, datagen.insertion_orders(len(X), 1, 9))
