@@ -52,8 +52,9 @@ WORKLOADS = {
     # serving: HNSW M=32 (layer-0 degree 64) efc=200 over 1M x 200 unit vectors
     # (500-centre mixture, queries shifted 0.5 sigma), IP, k in {10, 100}
     "hnsw-ip-1M": dict(kind="hnsw", metric="ip", n=1_000_000, nq=10_000, d=200, m=32, efc=200,
-                       k=10, legs=["fp32-tree", "bf16-rerank", "fp8-rerank", "q8-rerank"],
-                       headline="fp8-rerank"),
+                       k=10, legs=["fp32-tree", "bf16-rerank", "fp8-rerank", "fp8r-rerank",
+                                   "q8-rerank"],
+                       headline="fp8r-rerank"),
 }
 EFS = [16, 20, 24, 32, 40, 48, 64, 80, 100, 128, 160, 200, 256, 320, 400]
 
@@ -82,6 +83,8 @@ LEG_OPTS = {
     "fp32-inorder": dict(store="native", summation="inorder", rerank=False),
     "bf16-rerank": dict(store="bf16", summation="tree", rerank=True),
     "fp8-rerank": dict(store="fp8", summation="tree", rerank=True),
+    # FP8 of each point's residual to the nearest of 2048 sampled points (MIPS)
+    "fp8r-rerank": dict(store="fp8r", summation="tree", rerank=True),
     # the reference's own 8-bit scalar quantization (its Python quant path)
     "q8-rerank": dict(store="q8", summation="tree", rerank=True),
     "u8": dict(store="native", summation="tree", rerank=False),
@@ -91,6 +94,8 @@ LEG_OPTS = {
 def row_bytes(leg, d, kind):
     if leg.startswith("bf16"):
         return 2 * d
+    if leg.startswith("fp8r"):
+        return d + 4  # the row's centroid id
     if leg.startswith("fp8") or leg.startswith("q8") or kind == "u8":
         return d
     return 4 * d
@@ -493,7 +498,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp8-rerank": "fp8", "bf16-rerank": "bf16", "u8": "u8",
+        "dtype": {"fp8-rerank": "fp8", "fp8r-rerank": "fp8", "bf16-rerank": "bf16", "u8": "u8",
                   "q8-rerank": "i8" if w["metric"] == "ip" else "u8"}.get(head, "f32"),
         "data": "synthetic (seeded generator; graph built on the GPU by this library)",
         "config": {"workload": args.workload, "n": w["n"], "dim": w["d"], "nq": nq, "k": k,

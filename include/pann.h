@@ -40,11 +40,17 @@ typedef enum {
   PANN_STORE_NATIVE = 0, /* the points as given (f32 / u8 / i8) */
   PANN_STORE_BF16 = 1,   /* f32 points rounded to bfloat16 (RNE) */
   PANN_STORE_FP8 = 2,    /* f32 points as OCP FP8 e4m3 with per-dimension absmax scales */
-  PANN_STORE_Q8 = 3      /* the reference's 8-bit scalar quantization of f32 points: uint8
+  PANN_STORE_Q8 = 3,     /* the reference's 8-bit scalar quantization of f32 points: uint8
                             affine codes for L2 (Euclidian_Point<uint8_t>, euclidian_point.h:
                             98-107, 188-243), int8 codes with a trimmed global scale for MIPS
                             (Quantized_Mips_Point<8, true>, mips_point.h:292-470); queries are
                             encoded the same way, distances are exact int32 */
+  PANN_STORE_FP8_RESIDUAL = 4 /* MIPS only, dim <= 256: x - c as OCP FP8 e4m3 with
+                            per-dimension absmax scales, c the nearest (L2) of ncent points
+                            sampled from the index; the search adds q.c from a per-query
+                            table.  For clustered data, whose neighbour gaps are below
+                            e4m3's error on x itself.  No counterpart in the reference,
+                            which quantizes x directly (mips_point.h:292-470) */
 } pann_store;
 
 /* FP32 summation order of the native f32 store */
@@ -119,6 +125,21 @@ pann_status pann_build_store(pann_index_t idx, int32_t store);
 
 /* Copies the FP8 per-dimension scales (dim floats) to host memory. */
 pann_status pann_get_fp8_scales(pann_index_t idx, float *scales_host);
+
+/* Builds PANN_STORE_FP8_RESIDUAL with ncent centroids: the points
+ * floor(n / ncent) * i, i < ncent; each point's centroid is its nearest in L2
+ * (FP32 distances on the brute-force kernel); scales s_j = max_i |x_ij -
+ * c_ij| / 448.  ncent: a multiple of 4 in [4, min(4096, n)], 0 = min(2048,
+ * n / 64 rounded down to a multiple of 4).  pann_build_store(idx,
+ * PANN_STORE_FP8_RESIDUAL) is this with ncent = 0.  Same role as
+ * pann_build_store (point_range.h:55-66); no counterpart in the reference. */
+pann_status pann_build_fp8_residual(pann_index_t idx, uint32_t ncent);
+
+/* The FP8 residual store's parameters, to host memory (any output but ncent
+ * may be NULL): ncent, the centroids (ncent x dim floats), each point's
+ * centroid (n) and the per-dimension scales (dim). */
+pann_status pann_get_fp8_residual(pann_index_t idx, uint32_t *ncent, float *centroids,
+                                  uint32_t *assign, float *scales);
 
 /* Batched beam search, device buffers.  Replaces qsearchAll
  * (beamSearch.h:599-646) over beam_search_rerank (:528-597) over

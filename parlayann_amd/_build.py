@@ -40,9 +40,12 @@ def build(verbose: bool = False) -> str:
     srcs = SOURCES + [CALIB_SRC]
     with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, srcs))
-    for lib, o in ((LIB, objs[:-1]), (CALIB_LIB, objs[-1:])):
+    # libpann links rocBLAS for one plain FP32 GEMM (the FP8 residual store's
+    # per-query centroid table)
+    for lib, o, libs in ((LIB, objs[:-1], ["-L/opt/rocm/lib", "-lrocblas"]),
+                         (CALIB_LIB, objs[-1:], [])):
         tmp = lib + ".tmp"
-        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *o]
+        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *o, *libs]
         subprocess.run(cmd, check=True, capture_output=not verbose)
         os.replace(tmp, lib)
     return LIB

@@ -12,11 +12,12 @@ PANN_OK, PANN_ERR_ARG, PANN_ERR_DEVICE, PANN_ERR_OOM, PANN_ERR_CAPACITY = 0, 1, 
 PANN_ERR_SHORT_BEAM, PANN_ERR_UNSUPPORTED = 5, 6
 DT_F32, DT_U8, DT_I8 = 0, 1, 2
 METRIC_L2, METRIC_MIPS = 0, 1
-STORE_NATIVE, STORE_BF16, STORE_FP8, STORE_Q8 = 0, 1, 2, 3
+STORE_NATIVE, STORE_BF16, STORE_FP8, STORE_Q8, STORE_FP8_RESIDUAL = 0, 1, 2, 3, 4
 SUM_INORDER, SUM_TREE = 0, 1
 
 # every symbol include/pann.h declares
 EXPORTS = ["pann_open", "pann_build_store", "pann_get_fp8_scales", "pann_get_q8_params",
+           "pann_build_fp8_residual", "pann_get_fp8_residual",
            "pann_search",
            "pann_search_host", "pann_beam_search_host", "pann_search_status", "pann_last_kernel_ms",
            "pann_search_visited", "pann_robust_prune_batch", "pann_robust_prune_groups",
@@ -69,6 +70,8 @@ def lib():
         "pann_open": [C.POINTER(IndexDesc), C.POINTER(vp)],
         "pann_build_store": [vp, C.c_int32],
         "pann_get_fp8_scales": [vp, vp],
+        "pann_build_fp8_residual": [vp, C.c_uint32],
+        "pann_get_fp8_residual": [vp, vp, vp, vp, vp],
         "pann_get_q8_params": [vp, vp, vp, vp, vp],
         "pann_search": [vp, vp, C.c_uint32, C.POINTER(QParams), vp, vp, vp, C.c_uint32,
                         C.POINTER(SearchOpts), vp, vp, C.c_uint32, vp, vp, vp],

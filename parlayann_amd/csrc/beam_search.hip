@@ -43,13 +43,14 @@ struct WaveLds {
   uint32_t* H;
   uint32_t* RC;
   uint32_t* FB;
+  float* CT;  // FP8 residual store: q . c for every centroid c
 };
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap, uint32_t dim,
                                                    uint32_t hslots_log2, uint32_t rc_log2,
-                                                   uint32_t fb_log2) {
+                                                   uint32_t fb_log2, uint32_t ncent) {
   uint32_t b = 0;
   b += 2 * align16(fcap * 8);
   b += 2 * align16(fcap);
@@ -59,6 +60,7 @@ __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap,
   b += (1u << hslots_log2) * 4;
   b += rc_log2 ? (1u << rc_log2) * 4 : 0u;
   b += fb_log2 ? (1u << fb_log2) / 8 : 0u;
+  b += align16(ncent * 4);
   return b;
 }
 
@@ -79,6 +81,8 @@ __device__ inline WaveLds carve(char* p, const SearchArgs& a) {
   L.RC = (uint32_t*)(p + o);
   o += a.rc_log2 ? (1u << a.rc_log2) * 4 : 0u;
   L.FB = (uint32_t*)(p + o);
+  o += a.fb_log2 ? (1u << a.fb_log2) / 8 : 0u;
+  L.CT = (float*)(p + o);
   return L;
 }
 
@@ -304,6 +308,17 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
     for (uint32_t i = lane * 4; i < (1u << rcl); i += PANN_WAVE * 4)
       *(uint4*)(L.RC + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
   wave_lds_sync();
+  if constexpr (ST == ST_FP8R) {
+    // the query's row of q . c (one per centroid, ncent a multiple of 4),
+    // into LDS once per query (its first pass)
+    if (P.pass == 0) {
+      const float* row = a.qtable + qr * a.ncent;
+      for (uint32_t c0 = 4u * lane; c0 < a.ncent; c0 += 4u * PANN_WAVE)
+        *(float4*)(L.CT + c0) = *(const float4*)(row + c0);
+      wave_lds_sync();
+    }
+    D.T = L.CT;
+  }
 
   bool ovf = false;
   uint32_t lds_count = 0, g_count = 0;
@@ -722,7 +737,7 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 // host-side launch
 // ---------------------------------------------------------------------------
 uint32_t search_lds_bytes(const SearchArgs& a) {
-  return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2, a.rc_log2, a.fb_log2);
+  return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2, a.rc_log2, a.fb_log2, a.ncent);
 }
 
 template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
@@ -795,6 +810,14 @@ hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool s
   } else if (store == ST_FP8) {
     if (metric == M_L2) { PANN_NCH_CASES(ST_FP8, M_L2, false) }
     else { PANN_NCH_CASES(ST_FP8, M_IP, false) }
+  } else if (store == ST_FP8R && metric == M_IP) {  // rows of dim + 4 <= 384 bytes
+    if (!a.qtable || a.ncent == 0) return hipErrorInvalidValue;
+    switch (nch) {
+      case 1: return launch_t<ST_FP8R, M_IP, 1, false>(a, grid, s);
+      case 2: return launch_t<ST_FP8R, M_IP, 2, false>(a, grid, s);
+      case 3: return launch_t<ST_FP8R, M_IP, 3, false>(a, grid, s);
+      default: return hipErrorInvalidValue;
+    }
   }
   return hipErrorInvalidValue;
 }

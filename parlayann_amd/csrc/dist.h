@@ -15,7 +15,7 @@
 
 namespace pann {
 
-enum StoreKind : int { ST_F32 = 0, ST_U8 = 1, ST_I8 = 2, ST_BF16 = 3, ST_FP8 = 4 };
+enum StoreKind : int { ST_F32 = 0, ST_U8 = 1, ST_I8 = 2, ST_BF16 = 3, ST_FP8 = 4, ST_FP8R = 5 };
 enum MetricKind : int { M_L2 = 0, M_IP = 1 };
 
 __host__ __device__ constexpr int store_elem_bytes(int st) {
@@ -171,6 +171,31 @@ struct GroupDist<ST_FP8, MT, NCH> {
   __device__ float finish(float v) const { return MT == M_L2 ? v : -v; }
 };
 
+// FP8 residual rows (inner product only): x ~ c + s * fp8, c the row's
+// centroid, named by the row's last 4 bytes (four 6-bit digits, finite e4m3
+// codes that meet a zero query).  -q.x = -(q.c + sum_j q_j s_j fp8_j): the
+// lane holding the id word (sub 7 of the last chunk) adds q.c from the
+// query's centroid table T (LDS, filled before the search).
+__device__ __forceinline__ uint32_t centroid_of(uint32_t w) {
+  return (w & 63u) | ((w >> 8) & 63u) << 6 | ((w >> 16) & 63u) << 12 | ((w >> 24) & 63u) << 18;
+}
+
+template <int MT, int NCH>
+struct GroupDist<ST_FP8R, MT, NCH> : GroupDist<ST_FP8, M_IP, NCH> {
+  static_assert(MT == M_IP, "the FP8 residual store is inner product only");
+  const float* T;
+  bool idl;
+  __device__ void load_query(const float* qv, const float* scales, int dim, int sub) {
+    GroupDist<ST_FP8, M_IP, NCH>::load_query(qv, scales, dim, sub);
+    idl = sub == 7;
+  }
+  __device__ float partial(const uint4* raw) const {
+    float acc = GroupDist<ST_FP8, M_IP, NCH>::partial(raw);
+    if (idl) acc += T[centroid_of(raw[NCH - 1].w)];
+    return acc;
+  }
+};
+
 // uint8 rows: exact int32 via v_dot4_u32_u8.
 //   L2: sum (q-p)^2 = sum q^2 + sum p^2 - 2 sum q.p   (mod 2^32, exact result)
 template <int MT, int NCH>
@@ -271,7 +296,9 @@ __device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, cons
   // rows in flight per lane: every 8-lane group gathers UP rows per round,
   // sized so the raw chunks (UP * NCH * 4 VGPRs) keep the kernel at or
   // under ~128 VGPRs, i.e. no fewer waves per CU than LDS already allows
-  constexpr int UP = NCH <= 2 ? 8 : (NCH <= 4 ? 4 : 2);
+  // (FP8 at two chunks: a float per query element leaves room for only 4)
+  constexpr int UP = (ST == ST_FP8 || ST == ST_FP8R) && NCH == 2 ? 4
+                     : NCH <= 2 ? 8 : (NCH <= 4 ? 4 : 2);
   const int sub = lane & 7, grp = lane >> 3;
   // Branch-free gathers: slots past ncmp re-read row NL[0] (an L1 hit) and are
   // discarded; chunks past row_bytes read the row's zero padding (rows are

@@ -27,6 +27,10 @@ struct SearchArgs {
   uint64_t pts_stride;
   uint32_t row_bytes;
   const float* scales;  // FP8 per-dimension scales (nullptr otherwise)
+  // FP8 residual store: q . c for every query and centroid (nq x ncent,
+  // computed before the launch); each row's last 4 bytes name its centroid
+  const float* qtable;
+  uint32_t ncent;
   // FP32 store for the rerank branch (nullptr: plain branch)
   const char* full;
   uint64_t full_stride;
@@ -107,9 +111,12 @@ hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool s
 namespace pann {
 hipError_t launch_bf16_store(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                              uint16_t* Y, uint64_t stride_h, hipStream_t s);
+// cent != nullptr: the residual store, x - cent[assign[i]] is encoded and the
+// row's last 4 bytes (stride_b - 4) hold assign[i] (four 6-bit digits)
 hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                             uint32_t* absmax_bits, float* scales, uint8_t* Y, uint64_t stride_b,
-                            hipStream_t s);
+                            hipStream_t s, const float* cent = nullptr, uint64_t cstride_f = 0,
+                            const uint32_t* assign = nullptr);
 // brute-force candidates on the matrix cores (bruteforce.hip)
 struct BFArgs {
   const void* X;          // points (f32 / u8 / i8), row stride xs elements

@@ -7,6 +7,7 @@
 #include "../../include/pann.h"
 
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -111,6 +112,16 @@ struct pann_index_s {
   float* d_scales = nullptr;
   uint32_t* d_absmax = nullptr;
   float* d_xnorm = nullptr;        // |x|^2 per point (brute force, L2)
+  // FP8 residual store (PANN_STORE_FP8_RESIDUAL): e4m3 codes of x - c with
+  // per-dimension scales, c the nearest of ncent sampled points
+  char* d_fp8r = nullptr;
+  uint64_t fp8r_stride = 0;
+  float* d_rscales = nullptr;
+  uint32_t* d_rabsmax = nullptr;
+  float* d_cent = nullptr;     // ncent x pts_stride (the sampled rows)
+  float* d_cent_t = nullptr;   // dim x ncent (transposed, for the per-query table)
+  uint32_t* d_assign = nullptr;  // n: each point's centroid
+  uint32_t ncent = 0;
   char* d_q8 = nullptr;            // the reference's 8-bit codes (PANN_STORE_Q8)
   uint64_t q8_stride = 0;
   float q8_slope = 0, q8_max = 0, q8_scale = 0;
@@ -142,6 +153,9 @@ struct pann_index_s {
     size_t eps_cap = 0;
     uint8_t* d_qcodes = nullptr;  // per-search query codes for PANN_STORE_Q8
     size_t qcodes_cap = 0;
+    float* d_qtable = nullptr;    // PANN_STORE_FP8_RESIDUAL: nq x ncent q . c
+    size_t qtable_cap = 0;
+    rocblas_handle blas = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
   };
   std::unordered_map<hipStream_t, StreamCtx> streams;
@@ -387,9 +401,40 @@ pann_status build_q8(pann_index_t idx, uint64_t fstride) {
 }
 
 // For PANN_STORE_Q8 the kernel reads query codes: encode the float queries
-// into the index's scratch on `stream`.  Other stores read the queries as given.
+// into the index's scratch on `stream`.  For PANN_STORE_FP8_RESIDUAL it reads
+// q . c for every centroid: one FP32 GEMM (rocBLAS) into the scratch.  Other
+// stores read the queries as given.
 pann_status stage_queries(pann_index_t idx, StreamCtx& C, int32_t store, const void* queries,
                           uint32_t nq, pann::SearchArgs& a, hipStream_t stream) {
+  if (store == PANN_STORE_FP8_RESIDUAL) {
+    const size_t need = (size_t)nq * idx->ncent;
+    if (C.qtable_cap < need) {
+      if (C.d_qtable) {
+        HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        (void)hipFree(C.d_qtable);
+      }
+      C.d_qtable = nullptr;
+      C.qtable_cap = 0;
+      HIP_TRY(hipMalloc(&C.d_qtable, need * 4), "hipMalloc(centroid table)");
+      C.qtable_cap = need;
+    }
+    if (!C.blas && rocblas_create_handle(&C.blas) != rocblas_status_success) {
+      C.blas = nullptr;
+      return fail(PANN_ERR_DEVICE, "rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(C.blas, stream) != rocblas_status_success)
+      return fail(PANN_ERR_DEVICE, "rocblas_set_stream failed");
+    // row-major T (nq x ncent) = Q (nq x dim) . Ct (dim x ncent); in rocBLAS's
+    // column-major terms T^T (ncent x nq) = Ct^T (ncent x dim) . Q^T (dim x nq)
+    const float one = 1.0f, zero = 0.0f;
+    const rocblas_int m = (rocblas_int)idx->ncent, nn = (rocblas_int)nq, kd = (rocblas_int)idx->dim;
+    if (rocblas_sgemm(C.blas, rocblas_operation_none, rocblas_operation_none, m, nn, kd, &one,
+                      idx->d_cent_t, m, (const float*)queries, kd, &zero, C.d_qtable, m) !=
+        rocblas_status_success)
+      return fail(PANN_ERR_DEVICE, "rocblas_sgemm (centroid table) failed");
+    a.qtable = C.d_qtable;
+    return PANN_OK;
+  }
   if (store != PANN_STORE_Q8) return PANN_OK;
   const size_t need = (size_t)nq * idx->dim;
   if (C.qcodes_cap < need) {
@@ -427,8 +472,11 @@ pann_status pann_get_q8_params(pann_index_t idx, float* slope, int32_t* offset, 
 
 pann_status pann_build_store(pann_index_t idx, int32_t store) {
   if (!idx) return fail(PANN_ERR_ARG, "null index");
+  if (store == PANN_STORE_FP8_RESIDUAL) return pann_build_fp8_residual(idx, 0);
   if (store != PANN_STORE_BF16 && store != PANN_STORE_FP8 && store != PANN_STORE_Q8)
-    return fail(PANN_ERR_ARG, "store must be PANN_STORE_BF16, PANN_STORE_FP8 or PANN_STORE_Q8");
+    return fail(PANN_ERR_ARG,
+                "store must be PANN_STORE_BF16, PANN_STORE_FP8, PANN_STORE_Q8 or "
+                "PANN_STORE_FP8_RESIDUAL");
   if (idx->dtype != PANN_DT_F32)
     return fail(PANN_ERR_ARG, "reduced-precision stores need float points");
   std::lock_guard<std::mutex> lk(idx->mu);
@@ -543,6 +591,13 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     pts = idx->d_fp8;
     stride = idx->fp8_stride;
     row_bytes = idx->dim;
+  } else if (o.store == PANN_STORE_FP8_RESIDUAL) {
+    if (!idx->d_fp8r)
+      return fail(PANN_ERR_ARG, "FP8 residual store not built (pann_build_store)");
+    store_kind = pann::ST_FP8R;
+    pts = idx->d_fp8r;
+    stride = idx->fp8r_stride;
+    row_bytes = (uint32_t)idx->fp8r_stride;  // the centroid id sits in the last 4 bytes
   } else {
     return fail(PANN_ERR_ARG, "unknown store");
   }
@@ -555,7 +610,8 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   a.pts = pts;
   a.pts_stride = stride;
   a.row_bytes = row_bytes;
-  a.scales = idx->d_scales;
+  a.scales = store_kind == pann::ST_FP8R ? idx->d_rscales : idx->d_scales;
+  a.ncent = store_kind == pann::ST_FP8R ? idx->ncent : 0u;
   a.full = o.rerank ? idx->d_pts : nullptr;
   a.full_stride = idx->pts_stride;
   a.nbrs = G.d_nbrs;
@@ -1374,6 +1430,65 @@ pann_status pann_hnsw_download_level(pann_index_t idx, uint32_t level, uint32_t*
   return download_level(idx, level, graph_host);
 }
 
+// The brute-force candidates of nq queries (rows qs floats apart) among the
+// n rows of X (point dtype pdt, rows xs elements apart, xnorm = |x|^2 for
+// L2): the kk best per query, sorted.  Synchronises `stream`.
+static pann_status run_bruteforce(const pann_index_s* idx, const void* X, int pdt, uint64_t xs,
+                                  const float* xnorm, uint32_t n, int metric, const float* Q,
+                                  uint64_t qs, uint32_t nq, uint32_t kk, uint32_t* out_ids,
+                                  float* out_dists, hipStream_t stream) {
+  pann::BFArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.X = X;
+  a.xs = xs;
+  a.xnorm = metric == PANN_METRIC_L2 ? xnorm : nullptr;
+  a.n = n;
+  a.dim = idx->dim;
+  a.dim_pad = pann::bf_dim_pad(idx->dim, pdt);  // inside the 128-byte padded rows
+  a.qs = qs;
+  a.metric = metric == PANN_METRIC_L2 ? 0 : 1;
+  a.kk = kk;
+  a.cap = 1;
+  while (a.cap < kk + 32) a.cap <<= 1;
+  // point ranges: a multiple of 8 (the XCD count), so that blockIdx
+  // (query tile * nranges + range) puts every wave of a range on one XCD and
+  // its L2 serves the range's rows to all of them; more ranges for few
+  // queries, each range >= 4096 points, and nranges * kk within the merge
+  const uint32_t tiles = (nq + 63) / 64;
+  uint32_t nr = std::max<uint32_t>(8, (uint32_t)(idx->num_cus * 8 + tiles - 1) / tiles);
+  nr = std::min<uint32_t>((nr + 7) & ~7u, 64);
+  nr = std::min<uint32_t>(nr, std::max<uint32_t>(1, n / 4096));
+  nr = std::min<uint32_t>(nr, 4096 / kk);
+  a.nranges = std::max<uint32_t>(1, nr);
+  a.range_len = (n + a.nranges - 1) / a.nranges;
+  // queries per launch: scratch (cap keys per query and range) within 1 GiB
+  const uint64_t per_q = (uint64_t)a.nranges * (a.cap * 8 + 4);
+  const uint32_t chunk = (uint32_t)std::max<uint64_t>(
+      64, std::min<uint64_t>(nq, ((1ull << 30) / per_q) & ~63ull));
+  uint64_t* buf = nullptr;
+  uint32_t* counts = nullptr;
+  const uint32_t cq = std::min(nq, chunk);
+  hipError_t e = hipMalloc(&buf, (size_t)cq * a.nranges * a.cap * 8);
+  if (e == hipSuccess) e = hipMalloc(&counts, (size_t)cq * a.nranges * 4);
+  if (e != hipSuccess) {
+    if (buf) (void)hipFree(buf);
+    return hip_fail(e, "hipMalloc(brute-force scratch)");
+  }
+  a.buf = buf;
+  a.counts = counts;
+  for (uint32_t q0 = 0; q0 < nq && e == hipSuccess; q0 += chunk) {
+    a.nq = std::min(chunk, nq - q0);
+    a.Q = Q + (uint64_t)q0 * qs;
+    e = pann::launch_bf(a, pdt, out_ids + (uint64_t)q0 * kk, out_dists + (uint64_t)q0 * kk, stream);
+  }
+  hipError_t e2 = hipStreamSynchronize(stream);
+  (void)hipFree(buf);
+  (void)hipFree(counts);
+  if (e != hipSuccess) return hip_fail(e, "brute-force launch");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamSynchronize");
+  return PANN_OK;
+}
+
 pann_status pann_bruteforce_candidates(pann_index_t idx, const float* queries, uint32_t nq,
                                       uint32_t kk, uint32_t* out_ids, float* out_dists,
                                       void* stream_) {
@@ -1392,55 +1507,93 @@ pann_status pann_bruteforce_candidates(pann_index_t idx, const float* queries, u
     HIP_TRY(pann::launch_row_sqnorm(idx->d_pts, pdt, xs, idx->n, idx->dim, idx->d_xnorm, stream),
             "norms launch");
   }
-  pann::BFArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.X = idx->d_pts;
-  a.xs = xs;
-  a.xnorm = idx->metric == PANN_METRIC_L2 ? idx->d_xnorm : nullptr;
-  a.n = idx->n;
-  a.dim = idx->dim;
-  a.dim_pad = pann::bf_dim_pad(idx->dim, pdt);  // inside the 128-byte padded rows
-  a.qs = idx->dim;
-  a.metric = idx->metric == PANN_METRIC_L2 ? 0 : 1;
-  a.kk = kk;
-  a.cap = 1;
-  while (a.cap < kk + 32) a.cap <<= 1;
-  // point ranges: a multiple of 8 (the XCD count), so that blockIdx
-  // (query tile * nranges + range) puts every wave of a range on one XCD and
-  // its L2 serves the range's rows to all of them; more ranges for few
-  // queries, each range >= 4096 points, and nranges * kk within the merge
-  const uint32_t tiles = (nq + 63) / 64;
-  uint32_t nr = std::max<uint32_t>(8, (uint32_t)(idx->num_cus * 8 + tiles - 1) / tiles);
-  nr = std::min<uint32_t>((nr + 7) & ~7u, 64);
-  nr = std::min<uint32_t>(nr, std::max<uint32_t>(1, idx->n / 4096));
-  nr = std::min<uint32_t>(nr, 4096 / kk);
-  a.nranges = std::max<uint32_t>(1, nr);
-  a.range_len = (idx->n + a.nranges - 1) / a.nranges;
-  // queries per launch: scratch (cap keys per query and range) within 1 GiB
-  const uint64_t per_q = (uint64_t)a.nranges * (a.cap * 8 + 4);
-  const uint32_t chunk = (uint32_t)std::max<uint64_t>(
-      64, std::min<uint64_t>(nq, ((1ull << 30) / per_q) & ~63ull));
-  uint64_t* buf = nullptr;
-  uint32_t* counts = nullptr;
-  const uint32_t cq = std::min(nq, chunk);
-  hipError_t e = hipMalloc(&buf, (size_t)cq * a.nranges * a.cap * 8);
-  if (e == hipSuccess) e = hipMalloc(&counts, (size_t)cq * a.nranges * 4);
-  if (e != hipSuccess) {
-    if (buf) (void)hipFree(buf);
-    return hip_fail(e, "hipMalloc(brute-force scratch)");
-  }
-  a.buf = buf;
-  a.counts = counts;
-  for (uint32_t q0 = 0; q0 < nq && e == hipSuccess; q0 += chunk) {
-    a.nq = std::min(chunk, nq - q0);
-    a.Q = queries + (uint64_t)q0 * idx->dim;
-    e = pann::launch_bf(a, pdt, out_ids + (uint64_t)q0 * kk, out_dists + (uint64_t)q0 * kk, stream);
-  }
-  hipError_t e2 = hipStreamSynchronize(stream);
-  (void)hipFree(buf);
-  (void)hipFree(counts);
-  if (e != hipSuccess) return hip_fail(e, "brute-force launch");
-  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamSynchronize");
+  return run_bruteforce(idx, idx->d_pts, pdt, xs, idx->d_xnorm, idx->n, idx->metric, queries,
+                        idx->dim, nq, kk, out_ids, out_dists, stream);
+}
+
+pann_status pann_build_fp8_residual(pann_index_t idx, uint32_t ncent) {
+  if (!idx) return fail(PANN_ERR_ARG, "null index");
+  if (idx->dtype != PANN_DT_F32)
+    return fail(PANN_ERR_ARG, "reduced-precision stores need float points");
+  if (idx->metric != PANN_METRIC_MIPS)
+    return fail(PANN_ERR_UNSUPPORTED, "the FP8 residual store is for inner-product indexes");
+  if (idx->dim > 256) return fail(PANN_ERR_UNSUPPORTED, "FP8 residual store: dim > 256");
+  if (ncent == 0) ncent = std::min<uint32_t>(2048, (idx->n / 64) & ~3u);
+  if (ncent < 4 || ncent % 4 != 0 || ncent > 4096 || ncent > idx->n)
+    return fail(PANN_ERR_ARG, "ncent must be a multiple of 4 in [4, min(4096, n)]");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  void* old[] = {idx->d_fp8r, idx->d_rscales, idx->d_rabsmax, idx->d_cent, idx->d_cent_t,
+                 idx->d_assign};
+  for (void* p : old)
+    if (p) (void)hipFree(p);
+  idx->d_fp8r = nullptr;
+  idx->d_rscales = nullptr;
+  idx->d_rabsmax = nullptr;
+  idx->d_cent = idx->d_cent_t = nullptr;
+  idx->d_assign = nullptr;
+  idx->ncent = 0;
+  const uint64_t ps = idx->pts_stride, fs = ps / 4;
+  const uint32_t n = idx->n, dim = idx->dim;
+  idx->fp8r_stride = round_up((uint64_t)dim + 4, 128);
+  HIP_TRY(hipMalloc(&idx->d_fp8r, (size_t)n * idx->fp8r_stride), "hipMalloc(fp8 residual)");
+  HIP_TRY(hipMemset(idx->d_fp8r, 0, (size_t)n * idx->fp8r_stride), "hipMemset(fp8 residual)");
+  HIP_TRY(hipMalloc(&idx->d_rscales, dim * 4), "hipMalloc(scales)");
+  HIP_TRY(hipMalloc(&idx->d_rabsmax, dim * 4), "hipMalloc(absmax)");
+  HIP_TRY(hipMalloc(&idx->d_cent, (size_t)ncent * ps), "hipMalloc(centroids)");
+  HIP_TRY(hipMalloc(&idx->d_cent_t, (size_t)ncent * dim * 4), "hipMalloc(centroids)");
+  HIP_TRY(hipMalloc(&idx->d_assign, (size_t)n * 4), "hipMalloc(assignment)");
+  // centroids: every (n / ncent)-th point
+  const uint64_t step = n / ncent;
+  HIP_TRY(hipMemcpy2D(idx->d_cent, ps, idx->d_pts, ps * step, ps, ncent, hipMemcpyDeviceToDevice),
+          "hipMemcpy2D(centroids)");
+  std::vector<float> c((size_t)ncent * fs), ct((size_t)ncent * dim);
+  HIP_TRY(hipMemcpy(c.data(), idx->d_cent, c.size() * 4, hipMemcpyDeviceToHost),
+          "hipMemcpy(centroids)");
+  for (uint32_t i = 0; i < ncent; i++)
+    for (uint32_t j = 0; j < dim; j++) ct[(size_t)j * ncent + i] = c[(size_t)i * fs + j];
+  HIP_TRY(hipMemcpy(idx->d_cent_t, ct.data(), ct.size() * 4, hipMemcpyHostToDevice),
+          "hipMemcpy(centroids)");
+  // each point's nearest centroid in L2, on the brute-force kernel
+  float* cnorm = nullptr;
+  float* cd = nullptr;
+  hipError_t e = hipMalloc(&cnorm, (size_t)ncent * 4);
+  if (e == hipSuccess) e = hipMalloc(&cd, (size_t)n * 4);
+  if (e == hipSuccess) e = pann::launch_row_sqnorm(idx->d_cent, 0, fs, ncent, dim, cnorm, nullptr);
+  pann_status st = e == hipSuccess ? PANN_OK : hip_fail(e, "centroid norms");
+  if (st == PANN_OK)
+    st = run_bruteforce(idx, idx->d_cent, 0, fs, cnorm, ncent, PANN_METRIC_L2,
+                        (const float*)idx->d_pts, fs, n, 1, idx->d_assign, cd, nullptr);
+  if (cnorm) (void)hipFree(cnorm);
+  if (cd) (void)hipFree(cd);
+  if (st != PANN_OK) return st;
+  HIP_TRY(pann::launch_fp8_store((const float*)idx->d_pts, fs, n, dim, idx->d_rabsmax,
+                                 idx->d_rscales, (uint8_t*)idx->d_fp8r, idx->fp8r_stride, nullptr,
+                                 idx->d_cent, fs, idx->d_assign),
+          "fp8 residual store kernel");
+  HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  idx->ncent = ncent;
+  return PANN_OK;
+}
+
+pann_status pann_get_fp8_residual(pann_index_t idx, uint32_t* ncent, float* centroids,
+                                  uint32_t* assign, float* scales) {
+  if (!idx || !ncent) return fail(PANN_ERR_ARG, "null argument");
+  if (!idx->d_fp8r || idx->ncent == 0) return fail(PANN_ERR_ARG, "FP8 residual store not built");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  *ncent = idx->ncent;
+  if (centroids)
+    HIP_TRY(hipMemcpy2D(centroids, (size_t)idx->dim * 4, idx->d_cent, idx->pts_stride,
+                        (size_t)idx->dim * 4, idx->ncent, hipMemcpyDeviceToHost),
+            "hipMemcpy2D(centroids)");
+  if (assign)
+    HIP_TRY(hipMemcpy(assign, idx->d_assign, (size_t)idx->n * 4, hipMemcpyDeviceToHost),
+            "hipMemcpy(assignment)");
+  if (scales)
+    HIP_TRY(hipMemcpy(scales, idx->d_rscales, (size_t)idx->dim * 4, hipMemcpyDeviceToHost),
+            "hipMemcpy(scales)");
   return PANN_OK;
 }
 
@@ -1448,8 +1601,10 @@ pann_status pann_close(pann_index_t idx) {
   if (!idx) return PANN_OK;
   (void)hipSetDevice(idx->device);
   (void)hipDeviceSynchronize();
-  void* ptrs[] = {idx->d_pts, idx->d_bf16, idx->d_fp8, idx->d_scales, idx->d_absmax, idx->d_nbrs,
-                  idx->d_q8, idx->d_xnorm, idx->d_dummy, idx->d_pkeys, idx->d_pflags};
+  void* ptrs[] = {idx->d_pts,    idx->d_bf16,    idx->d_fp8,   idx->d_scales, idx->d_absmax,
+                  idx->d_nbrs,   idx->d_q8,      idx->d_xnorm, idx->d_dummy,  idx->d_pkeys,
+                  idx->d_pflags, idx->d_fp8r,    idx->d_rscales, idx->d_rabsmax, idx->d_cent,
+                  idx->d_cent_t, idx->d_assign};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (size_t l = 1; l < idx->levels.size(); l++)
@@ -1457,9 +1612,10 @@ pann_status pann_close(pann_index_t idx) {
   for (auto& kv : idx->streams) {
     StreamCtx& C = kv.second;
     void* cp[] = {C.d_counter, C.d_status, C.d_starts, C.d_ghash, C.d_vis, C.d_eps, C.d_epsd,
-                  C.d_qcodes};
+                  C.d_qcodes, C.d_qtable};
     for (void* p : cp)
       if (p) (void)hipFree(p);
+    if (C.blas) (void)rocblas_destroy_handle(C.blas);
     if (C.ev0) (void)hipEventDestroy(C.ev0);
     if (C.ev1) (void)hipEventDestroy(C.ev1);
   }

@@ -38,14 +38,27 @@ __device__ __forceinline__ uint8_t f32_to_e4m3_rne(float v) {
   return (uint8_t)(code | sgn);
 }
 
+// The value encoded for row r, column j: x_rj, or x_rj - c_{assign[r], j}
+// for the residual store.
+__device__ __forceinline__ float store_value(const float* X, uint64_t stride_f, const float* cent,
+                                             uint64_t cstride_f, const uint32_t* assign,
+                                             uint64_t r, uint64_t j) {
+  float v = X[r * stride_f + j];
+  if (cent) v -= cent[(uint64_t)assign[r] * cstride_f + j];
+  return v;
+}
+
 __global__ void k_absmax_cols(const float* __restrict__ X, uint64_t stride_f, uint32_t n,
-                              uint32_t dim, uint32_t* __restrict__ out_bits) {
+                              uint32_t dim, uint32_t* __restrict__ out_bits,
+                              const float* __restrict__ cent, uint64_t cstride_f,
+                              const uint32_t* __restrict__ assign) {
   // each block: a slab of rows, one thread per column (looping)
   uint32_t r0 = blockIdx.x * 256u;
   uint32_t r1 = r0 + 256u < n ? r0 + 256u : n;
   for (uint32_t j = threadIdx.x; j < dim; j += blockDim.x) {
     float m = 0.0f;
-    for (uint32_t r = r0; r < r1; r++) m = fmaxf(m, fabsf(X[(uint64_t)r * stride_f + j]));
+    for (uint32_t r = r0; r < r1; r++)
+      m = fmaxf(m, fabsf(store_value(X, stride_f, cent, cstride_f, assign, r, j)));
     atomicMax(&out_bits[j], __float_as_uint(m));  // non-negative floats order as uints
   }
 }
@@ -70,12 +83,28 @@ __global__ void k_encode_bf16(const float* __restrict__ X, uint64_t stride_f, ui
 
 __global__ void k_encode_fp8(const float* __restrict__ X, uint64_t stride_f, uint32_t n,
                              uint32_t dim, const float* __restrict__ scales,
-                             uint8_t* __restrict__ Y, uint64_t stride_b) {
+                             uint8_t* __restrict__ Y, uint64_t stride_b,
+                             const float* __restrict__ cent, uint64_t cstride_f,
+                             const uint32_t* __restrict__ assign) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t total = (uint64_t)n * dim;
   if (i >= total) return;
   uint64_t r = i / dim, j = i % dim;
-  Y[r * stride_b + j] = f32_to_e4m3_rne(X[r * stride_f + j] / scales[j]);
+  Y[r * stride_b + j] =
+      f32_to_e4m3_rne(store_value(X, stride_f, cent, cstride_f, assign, r, j) / scales[j]);
+}
+
+// The residual store's centroid id in the row's last 4 bytes, one 6-bit digit
+// per byte (least significant first): every byte is a finite e4m3 code
+// (<= 0x3F), so the search may decode it with the row; the query is zero
+// there.
+__global__ void k_put_centroid_ids(uint8_t* __restrict__ Y, uint64_t stride_b, uint32_t n,
+                                   const uint32_t* __restrict__ assign) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint32_t c = assign[r];
+  uint32_t w = (c & 63u) | ((c >> 6) & 63u) << 8 | ((c >> 12) & 63u) << 16 | ((c >> 18) & 63u) << 24;
+  *(uint32_t*)(Y + (uint64_t)r * stride_b + stride_b - 4) = w;
 }
 
 __global__ void k_zero_u32_quant(uint32_t* p, uint32_t n) {
@@ -106,16 +135,20 @@ hipError_t launch_bf16_store(const float* X, uint64_t stride_f, uint32_t n, uint
 
 hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                             uint32_t* absmax_bits, float* scales, uint8_t* Y, uint64_t stride_b,
-                            hipStream_t s) {
+                            hipStream_t s, const float* cent, uint64_t cstride_f,
+                            const uint32_t* assign) {
   hipLaunchKernelGGL(k_zero_u32_quant, dim3((dim + 255) / 256), dim3(256), 0, s, absmax_bits, dim);
   hipLaunchKernelGGL(k_absmax_cols, dim3((n + 255) / 256), dim3(128), 0, s, X, stride_f, n, dim,
-                     absmax_bits);
+                     absmax_bits, cent, cstride_f, assign);
   hipLaunchKernelGGL(k_scales_from_absmax, dim3((dim + 255) / 256), dim3(256), 0, s, absmax_bits,
                      dim, scales);
   uint64_t total = (uint64_t)n * dim;
   uint32_t blocks = (uint32_t)((total + 255) / 256);
   hipLaunchKernelGGL(k_encode_fp8, dim3(blocks), dim3(256), 0, s, X, stride_f, n, dim, scales, Y,
-                     stride_b);
+                     stride_b, cent, cstride_f, assign);
+  if (cent)
+    hipLaunchKernelGGL(k_put_centroid_ids, dim3((n + 255) / 256), dim3(256), 0, s, Y, stride_b, n,
+                       assign);
   return hipGetLastError();
 }
 

@@ -162,8 +162,8 @@ class HNSWIndex:
             X = _normalize(X)
         return cls(X, read_model(index_path), metric, device)
 
-    def build_store(self, kind: str):
-        self.base.build_store(kind)
+    def build_store(self, kind: str, ncent: int = 0):
+        self.base.build_store(kind, ncent)
 
     def search_device(self, queries, k: int, ef: int, out_ids, out_dists, out_count, *,
                       limit: int = 0, store: str = "native", summation: str = "inorder",

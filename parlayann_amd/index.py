@@ -27,7 +27,8 @@ from . import formats
 _DT = {np.dtype(np.float32): L.DT_F32, np.dtype(np.uint8): L.DT_U8, np.dtype(np.int8): L.DT_I8}
 _METRIC = {"Euclidian": L.METRIC_L2, "l2": L.METRIC_L2, "L2": L.METRIC_L2,
            "mips": L.METRIC_MIPS, "ip": L.METRIC_MIPS, "MIPS": L.METRIC_MIPS}
-_STORE = {"native": L.STORE_NATIVE, "bf16": L.STORE_BF16, "fp8": L.STORE_FP8, "q8": L.STORE_Q8}
+_STORE = {"native": L.STORE_NATIVE, "bf16": L.STORE_BF16, "fp8": L.STORE_FP8, "q8": L.STORE_Q8,
+          "fp8r": L.STORE_FP8_RESIDUAL}
 _SUM = {"inorder": L.SUM_INORDER, "tree": L.SUM_TREE}
 
 
@@ -74,13 +75,31 @@ class DeviceIndex:
             pass
 
     # -- stores --------------------------------------------------------------
-    def build_store(self, kind: str):
+    def build_store(self, kind: str, ncent: int = 0):
         """Reduced-precision copy of float points: 'bf16', 'fp8' (e4m3,
-        per-dimension absmax scales) or 'q8' (the reference's 8-bit scalar
-        quantization: uint8 affine for L2, trimmed int8 for MIPS)."""
-        if kind not in self._stores:
+        per-dimension absmax scales), 'q8' (the reference's 8-bit scalar
+        quantization: uint8 affine for L2, trimmed int8 for MIPS) or 'fp8r'
+        (MIPS: e4m3 of each point minus the nearest of `ncent` sampled points,
+        0 = automatic)."""
+        if kind == "fp8r":
+            if kind not in self._stores or ncent:
+                L.check(L.lib().pann_build_fp8_residual(self._h, int(ncent)))
+                self._stores.add(kind)
+        elif kind not in self._stores:
             L.check(L.lib().pann_build_store(self._h, _STORE[kind]))
             self._stores.add(kind)
+
+    def fp8_residual(self) -> dict:
+        """The 'fp8r' store's centroids (ncent x dim), each point's centroid
+        and the per-dimension scales."""
+        nc = C.c_uint32()
+        L.check(L.lib().pann_get_fp8_residual(self._h, C.byref(nc), None, None, None))
+        cent = np.zeros((nc.value, self.dim), np.float32)
+        assign = np.zeros(self.n, np.uint32)
+        scales = np.zeros(self.dim, np.float32)
+        L.check(L.lib().pann_get_fp8_residual(self._h, C.byref(nc), cent.ctypes.data,
+                                              assign.ctypes.data, scales.ctypes.data))
+        return {"centroids": cent, "assign": assign, "scales": scales}
 
     def q8_params(self) -> dict:
         """Parameters of the 'q8' store (the reference's 8-bit scalar quantization)."""

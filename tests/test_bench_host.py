@@ -44,6 +44,7 @@ def test_row_bytes_per_store():
     assert bench.row_bytes("fp32-inorder", 96, "f32") == 384
     assert bench.row_bytes("bf16-rerank", 96, "f32") == 192
     assert bench.row_bytes("fp8-rerank", 96, "f32") == 96
+    assert bench.row_bytes("fp8r-rerank", 200, "f32") == 204  # + the centroid id
     assert bench.row_bytes("q8-rerank", 200, "f32") == 200
     assert bench.row_bytes("u8", 128, "u8") == 128
 

@@ -191,17 +191,19 @@ def test_gpu_fp8_rerank_recall(oracle):
     mdl = hnsw.build_hnsw(X, "ip", 16, 100, 1.0, seed=2)
     idx = hnsw.HNSWIndex(X, mdl, "ip")
     idx.build_store("fp8")
+    idx.build_store("fp8r", ncent=64)
     Q = _queries(X, 300, seed=3)
     Q /= np.linalg.norm(Q, axis=1, keepdims=True)
     gt, _ = oracle.groundtruth_f64(X, "ip", Q, 10)
     rec = {}
-    for store, rr in (("native", False), ("fp8", True)):
+    for store, rr in (("native", False), ("fp8", True), ("fp8r", True)):
         r = idx.search(Q, 10, 64, store=store, rerank=rr, summation="tree")
         rec[store] = np.mean([len(set(r["ids"][i, :10].tolist()) & set(gt[i].tolist())) / 10
                               for i in range(len(Q))])
     # the oracle's search on an oracle build of this data reaches 0.9987
     assert rec["native"] > 0.99, rec
     assert rec["fp8"] > rec["native"] - 0.01, rec
+    assert rec["fp8r"] > rec["native"] - 0.01, rec
 
 
 def test_gpu_mixed_batch_equals_alone(oracle, small_l2):

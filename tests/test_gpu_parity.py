@@ -273,6 +273,50 @@ def test_fp8_store_codec():
         assert np.allclose(r["dists"][i], exp, rtol=1e-5, atol=1e-6 * np.abs(exp).max())
 
 
+def test_fp8_residual_store_codec():
+    """FP8 residual store: the centroids are points floor(n / ncent) * i; a
+    point's centroid is its nearest in L2; scales s_j = max|x_j - c_j| / 448;
+    codes = nearest e4m3fn value of (x - c) / s.  Checked through the search:
+    on a complete graph with beam = n every row's distance comes back and must
+    equal -(q.c + sum_j q_j s_j dec(code_ij)), from a brute-force encoder."""
+    _say("fp8 residual codec")
+    rng = np.random.default_rng(5)
+    n, d, nc = 64, 20, 8
+    centers = rng.standard_normal((6, d)) * 3.0
+    X = (centers[rng.integers(0, 6, n)] + 0.1 * rng.standard_normal((n, d))
+         * 10.0 ** rng.uniform(-3, 0, (n, d))).astype(np.float32)
+    Q = rng.standard_normal((8, d)).astype(np.float32)
+    idx = DeviceIndex(X, _complete(n), "ip")
+    idx.build_store("fp8r", ncent=nc)
+    p = idx.fp8_residual()
+    step = n // nc
+    cent = X[np.arange(nc) * step]
+    assert np.array_equal(p["centroids"], cent)
+    D = ((X[:, None, :].astype(np.float64) - cent[None].astype(np.float64)) ** 2).sum(-1)
+    srt = np.sort(D, axis=1)
+    clear = srt[:, 1] - srt[:, 0] > 1e-4 * srt[:, 1]  # no FP32 near-tie between two centroids
+    assert clear.mean() > 0.9
+    a = D.argmin(1)
+    assert np.array_equal(p["assign"][clear], a[clear])
+    a = p["assign"].astype(np.int64)
+    R = (X - cent[a]).astype(np.float32)
+    s = p["scales"]
+    assert np.array_equal(s, (np.abs(R).max(0) / np.float32(448.0)).astype(np.float32))
+    tab = _e4m3_table()
+    codes = _e4m3_encode_bruteforce((R / s[None, :]).astype(np.float32), tab)
+    deq = tab[codes] * s[None, :].astype(np.float64)
+    r = idx.search(Q, n, n, store="fp8r")
+    for i in range(len(Q)):
+        ids = r["ids"][i].astype(np.int64)
+        q = Q[i].astype(np.float64)
+        exp = -(cent[a[ids]].astype(np.float64) @ q + deq[ids] @ q)
+        assert np.allclose(r["dists"][i], exp, rtol=1e-5, atol=1e-5 * np.abs(exp).max())
+    # the store is inner product only
+    from parlayann_amd._lib import PannError
+    with pytest.raises(PannError):
+        DeviceIndex(X, _complete(n), "l2").build_store("fp8r")
+
+
 def test_bf16_store_codec():
     import torch
     _say("bf16 codec")
