@@ -4,6 +4,9 @@
 // FP8:  OCP e4m3fn with a per-dimension scale s_j = max_i |x_ij| / 448
 //       (448 = largest finite e4m3), code = RNE(x_ij / s_j) saturated to
 //       +-448; subnormals (|v| < 2^-6) are kept (step 2^-9).
+// FP8 residual: the same codes of x_i - c_{a(i)}, c the nearest sampled
+//       point, the scales taken over the residuals; the centroid id a(i) sits
+//       in the row's last 4 bytes.
 // The reference's analog is its scalar quantization of points
 // (euclidian_point.h:188-241, mips_point.h:425-501), which uses one global
 // scale; this store scales per dimension instead.
