@@ -273,15 +273,16 @@ def test_fp8_store_codec():
         assert np.allclose(r["dists"][i], exp, rtol=1e-5, atol=1e-6 * np.abs(exp).max())
 
 
-def test_fp8_residual_store_codec():
+@pytest.mark.parametrize("d", [20, 200, 253])  # rows of 1, 2 and 3 128-byte chunks
+def test_fp8_residual_store_codec(d):
     """FP8 residual store: the centroids are points floor(n / ncent) * i; a
     point's centroid is its nearest in L2; scales s_j = max|x_j - c_j| / 448;
     codes = nearest e4m3fn value of (x - c) / s.  Checked through the search:
     on a complete graph with beam = n every row's distance comes back and must
     equal -(q.c + sum_j q_j s_j dec(code_ij)), from a brute-force encoder."""
-    _say("fp8 residual codec")
+    _say(f"fp8 residual codec d={d}")
     rng = np.random.default_rng(5)
-    n, d, nc = 64, 20, 8
+    n, nc = 64, 8
     centers = rng.standard_normal((6, d)) * 3.0
     X = (centers[rng.integers(0, 6, n)] + 0.1 * rng.standard_normal((n, d))
          * 10.0 ** rng.uniform(-3, 0, (n, d))).astype(np.float32)
