@@ -130,7 +130,7 @@ pann_status pann_get_fp8_scales(pann_index_t idx, float *scales_host);
  * floor(n / ncent) * i, i < ncent; each point's centroid is its nearest in L2
  * (FP32 distances on the brute-force kernel); scales s_j = max_i |x_ij -
  * c_ij| / 448.  ncent: a multiple of 4 in [4, min(4096, n)], 0 = min(2048,
- * n / 64 rounded down to a multiple of 4).  pann_build_store(idx,
+ * n / 64 rounded down to a multiple of 4), at least 4.  n >= 4.  pann_build_store(idx,
  * PANN_STORE_FP8_RESIDUAL) is this with ncent = 0.  Same role as
  * pann_build_store (point_range.h:55-66); no counterpart in the reference. */
 pann_status pann_build_fp8_residual(pann_index_t idx, uint32_t ncent);

@@ -1518,7 +1518,8 @@ pann_status pann_build_fp8_residual(pann_index_t idx, uint32_t ncent) {
   if (idx->metric != PANN_METRIC_MIPS)
     return fail(PANN_ERR_UNSUPPORTED, "the FP8 residual store is for inner-product indexes");
   if (idx->dim > 256) return fail(PANN_ERR_UNSUPPORTED, "FP8 residual store: dim > 256");
-  if (ncent == 0) ncent = std::min<uint32_t>(2048, (idx->n / 64) & ~3u);
+  if (idx->n < 4) return fail(PANN_ERR_UNSUPPORTED, "FP8 residual store: fewer than 4 points");
+  if (ncent == 0) ncent = std::max<uint32_t>(4, std::min<uint32_t>(2048, (idx->n / 64) & ~3u));
   if (ncent < 4 || ncent % 4 != 0 || ncent > 4096 || ncent > idx->n)
     return fail(PANN_ERR_ARG, "ncent must be a multiple of 4 in [4, min(4096, n)]");
   std::lock_guard<std::mutex> lk(idx->mu);

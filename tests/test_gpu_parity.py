@@ -318,6 +318,22 @@ def test_fp8_residual_store_codec(d):
         DeviceIndex(X, _complete(n), "l2").build_store("fp8r")
 
 
+def test_fp8_residual_small_index():
+    """Automatic centroid count on a 10-point index (the minimum, 4), and the
+    rerank over it: with beam = n every point comes back in exact order."""
+    _say("fp8 residual small index")
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((10, 8)).astype(np.float32)
+    Q = rng.standard_normal((3, 8)).astype(np.float32)
+    idx = DeviceIndex(X, _complete(10), "ip")
+    idx.build_store("fp8r")
+    assert idx.fp8_residual()["centroids"].shape == (4, 8)
+    r = idx.search(Q, 10, 10, store="fp8r", rerank=True)
+    for i in range(len(Q)):
+        exact = -(X.astype(np.float64) @ Q[i].astype(np.float64))
+        assert np.array_equal(r["ids"][i], np.argsort(exact, kind="stable"))
+
+
 def test_bf16_store_codec():
     import torch
     _say("bf16 codec")
