@@ -592,7 +592,7 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     stride = idx->fp8_stride;
     row_bytes = idx->dim;
   } else if (o.store == PANN_STORE_FP8_RESIDUAL) {
-    if (!idx->d_fp8r)
+    if (!idx->d_fp8r || idx->ncent == 0)  // ncent is set only once a build completed
       return fail(PANN_ERR_ARG, "FP8 residual store not built (pann_build_store)");
     store_kind = pann::ST_FP8R;
     pts = idx->d_fp8r;
