@@ -83,8 +83,10 @@ __device__ __forceinline__ float word_elem(const uint4& w, int e) {
 // into the matrix core (no LDS staging) and holds its queries' matching
 // halves in registers for the whole range.  The next tile's words are in
 // flight while the current tile is multiplied.
-template <typename PT, int NV, int QS>
-__global__ __launch_bounds__(PANN_WAVE) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_bf_topk(BFArgs a) {
+// MINW: the waves per SIMD the register allocation must allow (2 where the
+// compiler would otherwise settle on 1 but can fit 256 registers unspilled)
+template <typename PT, int NV, int QS, int MINW>
+__global__ __launch_bounds__(PANN_WAVE) __attribute__((amdgpu_waves_per_eu(MINW, 2))) void k_bf_topk(BFArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   constexpr int EPW = 16 / (int)sizeof(PT);  // elements per 16-byte word
   constexpr int SMAX = NV * EPW;              // steps per lane, upper bound
@@ -262,29 +264,32 @@ uint32_t bf_dim_pad(uint32_t dim, int point_dtype) {
   return point_dtype == 0 ? (dim + 7) & ~7u : (dim + 31) & ~31u;  // 32 bytes
 }
 
-template <typename PT, int NV, int QS>
+template <typename PT, int NV, int QS, int MINW = 1>
 static hipError_t launch_topk_t(const BFArgs& a, hipStream_t s) {
   const uint32_t lds = bf_lds_bytes(a.cap);
   const uint32_t blocks = ((a.nq + 32 * QS - 1) / (32 * QS)) * a.nranges;
-  hipLaunchKernelGGL((k_bf_topk<PT, NV, QS>), dim3(blocks), dim3(PANN_WAVE), lds, s, a);
+  hipLaunchKernelGGL((k_bf_topk<PT, NV, QS, MINW>), dim3(blocks), dim3(PANN_WAVE), lds, s, a);
   return hipGetLastError();
 }
 
-// words per lane nw = dim_pad / 2 / (16 / sizeof(PT)); two query tiles per
-// wave while their register halves stay within 64 steps
+// words per lane nw = dim_pad / 2 / (16 / sizeof(PT)).  One query tile (32
+// queries) per wave where that leaves room for two waves per SIMD, so one
+// wave's selection overlaps the other's multiplies (measured: f32 d = 64 and
+// 96, u8 d = 128 gain 12-33 % over two tiles per wave at one wave per SIMD);
+// two tiles for the smallest rows, which reach two waves per SIMD with them
 template <typename PT>
 static hipError_t launch_topk_nv(const BFArgs& a, hipStream_t s) {
   const uint32_t nw = a.dim_pad / 2 / (16 / (uint32_t)sizeof(PT));
   if constexpr (sizeof(PT) == 4) {
     if (nw <= 4) return launch_topk_t<PT, 4, 2>(a, s);
-    if (nw <= 8) return launch_topk_t<PT, 8, 2>(a, s);
-    if (nw <= 12) return launch_topk_t<PT, 12, 2>(a, s);
-    if (nw <= 16) return launch_topk_t<PT, 16, 2>(a, s);
+    if (nw <= 8) return launch_topk_t<PT, 8, 1>(a, s);
+    if (nw <= 12) return launch_topk_t<PT, 12, 1>(a, s);
+    if (nw <= 16) return launch_topk_t<PT, 16, 1, 2>(a, s);
     if (nw <= 24) return launch_topk_t<PT, 24, 1>(a, s);
     if (nw <= 32) return launch_topk_t<PT, 32, 1>(a, s);
   } else {
     if (nw <= 2) return launch_topk_t<PT, 2, 2>(a, s);
-    if (nw <= 4) return launch_topk_t<PT, 4, 2>(a, s);
+    if (nw <= 4) return launch_topk_t<PT, 4, 1>(a, s);
     if (nw <= 8) return launch_topk_t<PT, 8, 1>(a, s);
   }
   return hipErrorInvalidValue;

@@ -105,19 +105,24 @@ def test_gpu_frontier_filter_exact_f32(oracle, beam):
 @pytest.mark.parametrize("metric,dtype,k", [("l2", "f32", 10), ("ip", "f32", 100),
                                             ("l2", "u8", 10), ("l2", "f32", 100),
                                             ("ip", "f32-200", 10), ("l2", "i8", 100),
-                                            ("ip", "u8-200", 10)])
+                                            ("ip", "u8-200", 10), ("l2", "f32-128", 10),
+                                            ("l2", "u8-128", 100)])
 def test_gpu_bruteforce_groundtruth_equals_fp64(oracle, metric, dtype, k):
     """K7 (brute-force candidates on MFMA) + FP64 re-score gives exactly the
     oracle's FP64 brute-force top k in (distance, id) order.  The dims cover
-    every register layout of the kernel: f32 dim 50 / 96 / 200 (two query
-    tiles per wave up to 64 steps per lane, one beyond), u8 / i8 dim 64 /
-    100 / 200."""
+    every register layout of the kernel: f32 dim 50 / 96 / 128 / 200, u8 / i8
+    dim 64 / 100 / 128 / 200 (one or two query tiles per wave, one or two
+    waves per SIMD)."""
     from parlayann_amd import DeviceIndex, datagen
     from parlayann_amd.groundtruth import groundtruth_device
     from parlayann_amd.index import _normalize
     print(f"[bruteforce] metric={metric} dtype={dtype} k={k}", flush=True)
     if dtype == "u8":
         X, Q = datagen.workload_u8_l2(n=30000, nq=700, d=64)
+    elif dtype == "u8-128":
+        X, Q = datagen.workload_u8_l2(n=24000, nq=400, d=128)
+    elif dtype == "f32-128":
+        X, Q = datagen.workload_f32_l2(n=30007, nq=401, d=128)
     elif dtype == "u8-200":
         X, Q = datagen.workload_u8_l2(n=20011, nq=333, d=200)
     elif dtype == "i8":
