@@ -300,10 +300,14 @@ __device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, cons
   constexpr int UP = (ST == ST_FP8 || ST == ST_FP8R) && NCH == 2 ? 4
                      : NCH <= 2 ? 8 : (NCH <= 4 ? 4 : 2);
   const int sub = lane & 7, grp = lane >> 3;
+  // 128-byte lines a row occupies: NCH rounds 5..7 up to 8, and the chunks
+  // past the row's last line are not loaded (they would be the next row's
+  // line, or past the last row's end)
+  const int nl = (row_bytes + 127) >> 7;
   // Branch-free gathers: slots past ncmp re-read row NL[0] (an L1 hit) and are
-  // discarded; chunks past row_bytes read the row's zero padding (rows are
-  // padded to a 128-byte multiple, the query is zero there), which adds
-  // nothing to either metric and no cache lines.
+  // discarded; bytes past row_bytes within its last line read the row's zero
+  // padding (rows are padded to a 128-byte multiple, the query is zero
+  // there), which adds nothing to either metric and no cache lines.
   for (int r0 = 0; r0 < ncmp; r0 += 8 * UP) {
     uint4 raw[UP][NCH];
 #pragma unroll
@@ -311,7 +315,12 @@ __device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, cons
       const int r = r0 + u * 8 + grp;
       const char* row = base + (uint64_t)NL[r < ncmp ? r : 0] * stride;
 #pragma unroll
-      for (int c = 0; c < NCH; c++) raw[u][c] = ld16(row + c * 128 + sub * 16);
+      for (int c = 0; c < NCH; c++) {
+        if (NCH <= 4 || c < nl)  // uniform: the same for every row of the launch
+          raw[u][c] = ld16(row + c * 128 + sub * 16);
+        else
+          raw[u][c] = make_uint4(0u, 0u, 0u, 0u);
+      }
     }
 #pragma unroll
     for (int u = 0; u < UP; u++) {

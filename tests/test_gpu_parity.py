@@ -130,6 +130,29 @@ def test_f32_tree_ids_up_to_ties(oracle, f32_case, ip_case, metric):
     assert rel[same].max() <= 1e-5
 
 
+@pytest.mark.parametrize("d", [140, 180, 200])  # f32 rows of 5, 6 and 7 128-byte lines
+def test_f32_tree_rows_of_5_to_7_lines(d):
+    """The gather's 8-chunk shape on rows of 5-7 lines: the chunks past a
+    row's last line are not loaded (they are the next row's, or past the last
+    row).  On a complete graph with beam = n every row, the last one included,
+    is evaluated; ids come back in exact order and distances match FP64."""
+    _say(f"f32 tree rows d={d}")
+    rng = np.random.default_rng(d)
+    n = 61
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    Q = rng.standard_normal((6, d)).astype(np.float32)
+    for metric in ("l2", "ip"):
+        r = DeviceIndex(X, _complete(n), metric).search(Q, n, n, summation="tree")
+        for i in range(len(Q)):
+            q = Q[i].astype(np.float64)
+            ex = (((X.astype(np.float64) - q) ** 2).sum(1) if metric == "l2"
+                  else -(X.astype(np.float64) @ q))
+            assert np.array_equal(r["ids"][i], np.argsort(ex, kind="stable")), (metric, i)
+            got = r["dists"][i].astype(np.float64)
+            want = ex[r["ids"][i]]
+            assert np.all(np.abs(got - want) <= 1e-5 * np.maximum(np.abs(want), 1.0)), (metric, i)
+
+
 def test_hash_overflow_path_is_exact(oracle, u8_case, f32_case):
     """A 256-slot LDS table overflows into the per-wave global table after
     128 ids; results must not change."""
