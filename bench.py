@@ -30,9 +30,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# the reference's beam sweep tops out at 1000 (check_nn_recall.h:214-216)
-BEAMS = [10, 12, 16, 20, 24, 32, 40, 48, 64, 80, 100, 128, 160, 200, 256, 320, 400, 512, 640,
-         800, 1000]
+# the reference's beam sweep tops out at 1000 (check_nn_recall.h:214-216); the operating
+# point is the smallest beam of this list whose recall reaches the target, so the list
+# steps by at most 1/8 of the beam above 100
+BEAMS = [10, 12, 14, 16, 18, 20, 22, 24, 28, 32, 36, 40, 45, 50, 56, 64, 72, 80, 90, 100, 112,
+         128, 144, 160, 180, 200, 224, 256, 288, 320, 352, 384, 416, 448, 512, 576, 640, 720,
+         800, 900, 1000]
 
 WORKLOADS = {
     # configs: plain FP32 L2 beam search, 100k x 128 Gaussian mixture, Vamana R=32 L=64 a=1.2
@@ -56,7 +59,7 @@ WORKLOADS = {
                                    "q8-rerank"],
                        headline="fp8r-rerank"),
 }
-EFS = [16, 20, 24, 32, 40, 48, 64, 80, 100, 128, 160, 200, 256, 320, 400]
+EFS = [16, 18, 20, 22, 24, 28, 32, 36, 40, 48, 64, 80, 100, 128, 160, 200, 256, 320, 400]
 
 
 def log(*a):
