@@ -237,7 +237,13 @@ template <int ST, int MT, int NCH, bool SEQ>
 __device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
                                           const WaveLds& L, int ncmp, int lane) {
   if constexpr (SEQ) {
-    eval_seq_f32<MT>(L.QL, a.pts, a.pts_stride, (int)a.dim, L.NL, ncmp, L.DL, lane);
+    // rows of up to 8 lines: the group gather with the in-order lane chain;
+    // longer rows (NCH 8 only): one lane per row
+    if (NCH < 8 || a.row_bytes <= 8u * 128u)
+      eval_group_inorder<MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, L.NL, ncmp, L.DL,
+                                  lane);
+    else
+      eval_seq_f32<MT>(L.QL, a.pts, a.pts_stride, (int)a.dim, L.NL, ncmp, L.DL, lane);
   } else {
     eval_group<ST, MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, L.NL, ncmp, L.DL, lane);
   }
@@ -292,7 +298,7 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
   GroupDist<ST, MT, NCH> D;
   const uint64_t qr = a.self_ids ? (uint64_t)self : (uint64_t)q;  // build mode: the point's own row
   const char* qrow = a.queries + qr * a.q_stride;
-  if constexpr (!SEQ) D.load_query((const QT*)qrow, a.scales, (int)a.dim, lane & 7);
+  D.load_query((const QT*)qrow, a.scales, (int)a.dim, lane & 7);
   if (SEQ || a.full) {
     const float* fq = SEQ ? (const float*)qrow : (const float*)(a.fqueries + qr * a.fq_stride);
     for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
@@ -789,12 +795,13 @@ static int nch_for(int row_bytes) {
 hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool sequential,
                               uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, a.work_counter, 1u);
+  int nch = nch_for((int)a.row_bytes);
   if (sequential) {
     if (store != ST_F32) return hipErrorInvalidValue;
-    if (metric == M_L2) return launch_t<ST_F32, M_L2, 1, true>(a, grid, s);
-    return launch_t<ST_F32, M_IP, 1, true>(a, grid, s);
+    if (nch < 0) nch = 8;  // rows past 8 lines: the one-lane-per-row sum
+    if (metric == M_L2) { PANN_NCH_CASES(ST_F32, M_L2, true) }
+    else { PANN_NCH_CASES(ST_F32, M_IP, true) }
   }
-  int nch = nch_for((int)a.row_bytes);
   if (store == ST_F32) {
     if (metric == M_L2) { PANN_NCH_CASES(ST_F32, M_L2, false) }
     else { PANN_NCH_CASES(ST_F32, M_IP, false) }

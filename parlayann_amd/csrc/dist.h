@@ -332,6 +332,87 @@ __device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, cons
 }
 
 // ---------------------------------------------------------------------------
+// In-order FP32 over the group layout.  Lane `sub` of a row's 8-lane group
+// holds elements c*32 + sub*4 + e (e < 4) of chunk c.  The running sum walks
+// the lanes in element order: at step (c, s) lane s takes the sum from lane
+// s - 1 (DPP row_shr:1) or, at s = 0, from lane 7 of its own group, which
+// ended chunk c - 1 (row_ror:9 maps lane 7 to lane 0 and lane 15 to lane 8 of
+// each 16-lane row), and adds its four terms one rounding at a time.  Lane 7
+// ends with the row's sum.  Every other lane computes a discarded value.  The
+// terms and their order are those of seq_f32_row below, and the zero padding
+// past dim adds exact zeros to a sum that is never -0, so the result is
+// bit-identical to it and to the reference's float loop.
+// ---------------------------------------------------------------------------
+template <int MT, int NCH>
+__device__ __forceinline__ float group_inorder_sum(const GroupDist<ST_F32, MT, NCH>& D,
+                                                   const uint4* raw) {
+#pragma clang fp contract(off)
+  float t[NCH][4];
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+    const float p[4] = {__uint_as_float(raw[c].x), __uint_as_float(raw[c].y),
+                        __uint_as_float(raw[c].z), __uint_as_float(raw[c].w)};
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      if (MT == M_L2) {
+        const float df = D.q[c][e] - p[e];
+        t[c][e] = df * df;
+      } else {
+        t[c][e] = D.q[c][e] * p[e];
+      }
+    }
+  }
+  float acc = 0.0f;
+#pragma unroll
+  for (int c = 0; c < NCH; c++)
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      float in;
+      if (c == 0 && s == 0) in = 0.0f;
+      else if (s == 0) in = __int_as_float(dpp_mov<0x129>(__float_as_int(acc)));
+      else in = __int_as_float(dpp_mov<0x111>(__float_as_int(acc)));
+      acc = in + t[c][0];
+      acc = acc + t[c][1];
+      acc = acc + t[c][2];
+      acc = acc + t[c][3];
+    }
+  return MT == M_L2 ? acc : -acc;
+}
+
+// eval_group's gather with the in-order sum: DL[r] comes from lane 7 of the
+// row's group.
+template <int MT, int NCH>
+__device__ __forceinline__ void eval_group_inorder(const GroupDist<ST_F32, MT, NCH>& D,
+                                                   const char* base, uint64_t stride,
+                                                   int row_bytes, const uint32_t* NL, int ncmp,
+                                                   float* DL, int lane) {
+  constexpr int UP = NCH <= 2 ? 8 : (NCH <= 4 ? 4 : 2);
+  const int sub = lane & 7, grp = lane >> 3;
+  const int nl = (row_bytes + 127) >> 7;
+  for (int r0 = 0; r0 < ncmp; r0 += 8 * UP) {
+    uint4 raw[UP][NCH];
+#pragma unroll
+    for (int u = 0; u < UP; u++) {
+      const int r = r0 + u * 8 + grp;
+      const char* row = base + (uint64_t)NL[r < ncmp ? r : 0] * stride;
+#pragma unroll
+      for (int c = 0; c < NCH; c++) {
+        if (NCH <= 4 || c < nl)
+          raw[u][c] = ld16(row + c * 128 + sub * 16);
+        else
+          raw[u][c] = make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UP; u++) {
+      const int r = r0 + u * 8 + grp;
+      const float s = group_inorder_sum<MT, NCH>(D, raw[u]);
+      if (sub == 7 && r < ncmp) DL[r] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Sequential FP32 (one lane per row, in-order sum of separately rounded terms;
 // the reference's float loop).  The query sits in LDS (QL, dim floats).
 // ---------------------------------------------------------------------------

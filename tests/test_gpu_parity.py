@@ -110,6 +110,28 @@ def test_f32_inorder_bit_exact(oracle, f32_case, ip_case, metric, B):
     assert np.array_equal(f["visited"], o["visited"])
 
 
+@pytest.mark.parametrize("dim", [5, 32, 40, 100, 200, 256, 300])
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+def test_f32_inorder_bit_exact_dims(oracle, metric, dim):
+    """In-order FP32 at every row shape: rows of 1, 1, 2, 4, 7 and 8 128-byte
+    lines (8-lane gather, sum carried lane to lane in element order) and rows
+    past 8 lines (one lane per row)."""
+    rng = np.random.default_rng(1000 + dim)
+    X = rng.standard_normal((3000, dim)).astype(np.float32)
+    Q = rng.standard_normal((100, dim)).astype(np.float32)
+    _say(f"f32 inorder {metric} dim={dim}")
+    perm = datagen.insertion_orders(len(X), 1, 21)
+    adj = oracle.build_vamana(X, metric, 16, 32, 1.2 if metric == "l2" else 1.0, perm)
+    o = oracle.search_batch(adj, X, metric, Q, 10, 32)
+    assert not o["hash_grew"].any()
+    idx = DeviceIndex(X, adj, metric)
+    g = idx.search(Q, 10, 32, summation="inorder", exact_counts=True)
+    _assert_exact(g, o, 10)
+    f = idx.search(Q, 10, 32, summation="inorder")
+    assert np.array_equal(f["ids"], o["ids"]) and np.array_equal(f["dists"], o["dists"])
+    idx.close()
+
+
 @pytest.mark.parametrize("metric", ["l2", "ip"])
 def test_f32_tree_ids_up_to_ties(oracle, f32_case, ip_case, metric):
     X, Q, adj = f32_case if metric == "l2" else ip_case
