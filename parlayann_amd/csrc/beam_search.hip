@@ -655,7 +655,18 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
       int m = nchk - t < PANN_WAVE ? nchk - t : PANN_WAVE;
       if (lane < m) L.NL[lane] = key_id(F[t + lane]);
       wave_lds_sync();
-      eval_seq_f32<MT>(L.QL, a.full, a.full_stride, (int)a.dim, L.NL, m, L.DL, lane);
+      // Rows of up to 8 lines: 8-lane gather and in-order lane chain; longer rows: one lane
+      // per row.  The FP8 kernels keep one lane per row throughout: the 8-lane rerank adds
+      // about 4 VGPRs to every kernel, which takes the one-line FP8 kernel from 128 VGPRs
+      // (4 waves per SIMD) to 132 (3 waves), a larger loss than the rerank gains.
+      if (ST != ST_FP8 && a.dim <= 128u)
+        eval_group_inorder_lds<MT, 4, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
+                                         lane);
+      else if (ST != ST_FP8 && a.dim <= 256u)
+        eval_group_inorder_lds<MT, 8, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
+                                         lane);
+      else
+        eval_seq_f32<MT>(L.QL, a.full, a.full_stride, (int)a.dim, L.NL, m, L.DL, lane);
       wave_lds_sync();
       if (lane < m) R[t + lane] = make_key(L.DL[lane], L.NL[lane]);
       wave_lds_sync();

@@ -343,40 +343,94 @@ __device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, cons
 // past dim adds exact zeros to a sum that is never -0, so the result is
 // bit-identical to it and to the reference's float loop.
 // ---------------------------------------------------------------------------
+// The four separately rounded terms of one 16-byte chunk (no fused multiply-add).
+template <int MT>
+__device__ __forceinline__ void inorder_terms(const float* q, const uint4& raw, float* t) {
+#pragma clang fp contract(off)
+  const float p[4] = {__uint_as_float(raw.x), __uint_as_float(raw.y), __uint_as_float(raw.z),
+                      __uint_as_float(raw.w)};
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    if (MT == M_L2) {
+      const float df = q[e] - p[e];
+      t[e] = df * df;
+    } else {
+      t[e] = q[e] * p[e];
+    }
+  }
+}
+
+// The eight steps of one 128-byte chunk; `first` starts the sum at zero.
+__device__ __forceinline__ float inorder_chunk(float acc, const float* t, bool first) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    float in;
+    if (s == 0) in = first ? 0.0f : __int_as_float(dpp_mov<0x129>(__float_as_int(acc)));
+    else in = __int_as_float(dpp_mov<0x111>(__float_as_int(acc)));
+    acc = in + t[0];
+    acc = acc + t[1];
+    acc = acc + t[2];
+    acc = acc + t[3];
+  }
+  return acc;
+}
+
 template <int MT, int NCH>
 __device__ __forceinline__ float group_inorder_sum(const GroupDist<ST_F32, MT, NCH>& D,
                                                    const uint4* raw) {
-#pragma clang fp contract(off)
   float t[NCH][4];
 #pragma unroll
-  for (int c = 0; c < NCH; c++) {
-    const float p[4] = {__uint_as_float(raw[c].x), __uint_as_float(raw[c].y),
-                        __uint_as_float(raw[c].z), __uint_as_float(raw[c].w)};
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      if (MT == M_L2) {
-        const float df = D.q[c][e] - p[e];
-        t[c][e] = df * df;
-      } else {
-        t[c][e] = D.q[c][e] * p[e];
-      }
-    }
-  }
+  for (int c = 0; c < NCH; c++) inorder_terms<MT>(D.q[c], raw[c], t[c]);
   float acc = 0.0f;
 #pragma unroll
-  for (int c = 0; c < NCH; c++)
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-      float in;
-      if (c == 0 && s == 0) in = 0.0f;
-      else if (s == 0) in = __int_as_float(dpp_mov<0x129>(__float_as_int(acc)));
-      else in = __int_as_float(dpp_mov<0x111>(__float_as_int(acc)));
-      acc = in + t[c][0];
-      acc = acc + t[c][1];
-      acc = acc + t[c][2];
-      acc = acc + t[c][3];
-    }
+  for (int c = 0; c < NCH; c++) acc = inorder_chunk(acc, t[c], c == 0);
   return MT == M_L2 ? acc : -acc;
+}
+
+// The same sum over FP32 rows of at most MAXC lines, the line count known
+// only at run time and the query read from LDS (QL, dim floats): the fused
+// rerank, whose rows are not the search store's.  UP rows per group in flight.
+template <int MT, int MAXC, int UP>
+__device__ __forceinline__ void eval_group_inorder_lds(const float* QL, int dim, const char* base,
+                                                       uint64_t stride, const uint32_t* NL,
+                                                       int ncmp, float* DL, int lane) {
+  const int sub = lane & 7, grp = lane >> 3;
+  const int nl = (dim * 4 + 127) >> 7;  // 1..MAXC, checked by the caller
+  for (int r0 = 0; r0 < ncmp; r0 += 8 * UP) {
+    uint4 raw[UP][MAXC];
+#pragma unroll
+    for (int u = 0; u < UP; u++) {
+      const int r = r0 + u * 8 + grp;
+      const char* row = base + (uint64_t)NL[r < ncmp ? r : 0] * stride;
+#pragma unroll
+      for (int c = 0; c < MAXC; c++)
+        raw[u][c] = c < nl ? ld16(row + c * 128 + sub * 16) : make_uint4(0u, 0u, 0u, 0u);
+    }
+    float acc[UP] = {};
+#pragma unroll
+    for (int c = 0; c < MAXC; c++) {
+      if (c < nl) {  // uniform
+        float q[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int j = c * 32 + sub * 4 + e;
+          q[e] = j < dim ? QL[j] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < UP; u++) {
+          float t[4];
+          inorder_terms<MT>(q, raw[u][c], t);
+          acc[u] = inorder_chunk(c == 0 ? 0.0f : acc[u], t, c == 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UP; u++) {
+      const int r = r0 + u * 8 + grp;
+      if (sub == 7 && r < ncmp) DL[r] = MT == M_L2 ? acc[u] : -acc[u];
+    }
+  }
 }
 
 // eval_group's gather with the in-order sum: DL[r] comes from lane 7 of the

@@ -258,6 +258,28 @@ def test_reduced_precision_rerank(oracle, ip_case, store):
     assert np.all(np.diff(red["dists"], axis=1) >= 0)
 
 
+@pytest.mark.parametrize("dim", [20, 96, 128, 200, 256, 300])
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+def test_rerank_distances_bit_exact_dims(oracle, metric, dim):
+    """The fused FP32 rerank at rows of 1 to 4 lines (two rows per 8-lane
+    group in flight), 5 to 8 lines (one) and past 8 lines (one lane per row):
+    every returned distance is the oracle's in-order FP32 distance."""
+    rng = np.random.default_rng(2000 + dim)
+    X = rng.standard_normal((3000, dim)).astype(np.float32)
+    Q = rng.standard_normal((100, dim)).astype(np.float32)
+    _say(f"bf16 + rerank {metric} dim={dim}")
+    perm = datagen.insertion_orders(len(X), 1, 22)
+    adj = oracle.build_vamana(X, metric, 16, 32, 1.2 if metric == "l2" else 1.0, perm)
+    idx = DeviceIndex(X, adj, metric)
+    idx.build_store("bf16")
+    r = idx.search(Q, 10, 40, store="bf16", rerank=True)
+    for i in range(len(Q)):
+        for a, d in zip(r["ids"][i], r["dists"][i]):
+            assert d == oracle.distance(X[a], Q[i], metric), (i, a)
+    assert np.all(np.diff(r["dists"], axis=1) >= 0)
+    idx.close()
+
+
 def _e4m3_table():
     v = np.zeros(256, np.float64)
     for c in range(256):
