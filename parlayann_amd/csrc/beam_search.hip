@@ -50,13 +50,14 @@ __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15
 
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap, uint32_t dim,
                                                    uint32_t hslots_log2, uint32_t rc_log2,
-                                                   uint32_t fb_log2, uint32_t ncent) {
+                                                   uint32_t fb_log2, uint32_t ncent,
+                                                   uint32_t q_own) {
   uint32_t b = 0;
   b += 2 * align16(fcap * 8);
   b += 2 * align16(fcap);
   b += 2 * align16(ccap * 8);
   b += align16(128 * 4) * 2;
-  b += align16(dim * 4);
+  b += q_own ? align16(dim * 4) : 0u;
   b += (1u << hslots_log2) * 4;
   b += rc_log2 ? (1u << rc_log2) * 4 : 0u;
   b += fb_log2 ? (1u << fb_log2) / 8 : 0u;
@@ -75,7 +76,9 @@ __device__ inline WaveLds carve(char* p, const SearchArgs& a) {
   L.Cs = (uint64_t*)(p + o); o += align16(a.ccap * 8);
   L.NL = (uint32_t*)(p + o); o += align16(128 * 4);
   L.DL = (float*)(p + o); o += align16(128 * 4);
-  L.QL = (float*)(p + o); o += align16(a.dim * 4);
+  // without a region of its own the query row shares the candidate buffers C, Cs
+  L.QL = a.q_own ? (float*)(p + o) : (float*)L.C;
+  o += a.q_own ? align16(a.dim * 4) : 0u;
   L.H = (uint32_t*)(p + o);
   o += (1u << a.hslots_log2) * 4;
   L.RC = (uint32_t*)(p + o);
@@ -299,10 +302,9 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
   const uint64_t qr = a.self_ids ? (uint64_t)self : (uint64_t)q;  // build mode: the point's own row
   const char* qrow = a.queries + qr * a.q_stride;
   D.load_query((const QT*)qrow, a.scales, (int)a.dim, lane & 7);
-  if (SEQ || a.full) {
-    const float* fq = SEQ ? (const float*)qrow : (const float*)(a.fqueries + qr * a.fq_stride);
+  const float* fq = SEQ ? (const float*)qrow : (const float*)(a.fqueries + qr * a.fq_stride);
+  if (a.q_own && (SEQ || a.full))
     for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
-  }
   // ---- clear the visited hash and the cache of evaluated ids ----
   for (uint32_t i = lane * 4; i < hslots; i += PANN_WAVE * 4)
     *(uint4*)(L.H + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
@@ -651,6 +653,10 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
     double nc = (double)k * a.rerank_factor;
     int nchk = nc < (double)f ? (int)nc : f;
     uint64_t* R = cur_buf ? L.F0 : L.F1;
+    if (!a.q_own) {  // into the candidate buffers: no candidate is pending after the search
+      for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
+      wave_lds_sync();
+    }
     for (int t = 0; t < nchk; t += PANN_WAVE) {
       int m = nchk - t < PANN_WAVE ? nchk - t : PANN_WAVE;
       if (lane < m) L.NL[lane] = key_id(F[t + lane]);
@@ -754,7 +760,8 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 // host-side launch
 // ---------------------------------------------------------------------------
 uint32_t search_lds_bytes(const SearchArgs& a) {
-  return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2, a.rc_log2, a.fb_log2, a.ncent);
+  return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2, a.rc_log2, a.fb_log2, a.ncent,
+                        a.q_own);
 }
 
 template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>

@@ -94,6 +94,11 @@ struct SearchArgs {
   // direct-mapped table, a slot holding the last committed id that hashed there
   uint32_t rc_log2;
   uint32_t fb_log2;         // bits of the frontier-id Bloom filter (log2), 0 = none
+  // 1: the FP32 query row has an LDS region of its own for the whole search (the
+  // one-lane-per-row in-order sum of rows past 8 lines, or a row too long for
+  // the candidate buffers); 0: the rerank stages it in the candidate buffers,
+  // which are dead by then, and no other step reads it from LDS
+  uint32_t q_own;
   // HNSW descent in the same launch: before the search above, each query is
   // searched at beam 1 on up[0], up[1], .. up[nup - 1] (top level first), the
   // first pass from the start points, each later pass and the main search
