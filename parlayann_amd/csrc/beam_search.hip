@@ -661,14 +661,13 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
       int m = nchk - t < PANN_WAVE ? nchk - t : PANN_WAVE;
       if (lane < m) L.NL[lane] = key_id(F[t + lane]);
       wave_lds_sync();
-      // Rows of up to 8 lines: 8-lane gather and in-order lane chain; longer rows: one lane
-      // per row.  The FP8 kernels keep one lane per row throughout: the 8-lane rerank adds
-      // about 4 VGPRs to every kernel, which takes the one-line FP8 kernel from 128 VGPRs
-      // (4 waves per SIMD) to 132 (3 waves), a larger loss than the rerank gains.
-      if (ST != ST_FP8 && a.dim <= 128u)
+      // Rows of up to 8 lines: 8-lane gather and in-order lane chain; longer rows, and the
+      // FP8 L2 kernels (their query scales leave no registers for it): one lane per row
+      constexpr bool group_rr = !(ST == ST_FP8 && MT == M_L2);
+      if (group_rr && a.dim <= 128u)
         eval_group_inorder_lds<MT, 4, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
                                          lane);
-      else if (ST != ST_FP8 && a.dim <= 256u)
+      else if (group_rr && a.dim <= 256u)
         eval_group_inorder_lds<MT, 8, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
                                          lane);
       else
@@ -703,8 +702,11 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
   return PassOut{PANN_SENTINEL, P.vis0 + nvis, P.cmps0 + cmps, true};
 }
 
+// The one-line FP8 inner-product kernel is held to 128 VGPRs (4 waves per SIMD, what
+// its LDS allows at the beams it serves); left alone, the 8-lane rerank takes it to 132.
 template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
-__global__ __launch_bounds__(PANN_WAVE) void k_beam_search(SearchArgs a) {
+__global__ __launch_bounds__(PANN_WAVE, (ST == ST_FP8 && MT == M_IP && NCH == 1 && !EX) ? 4 : 1)
+void k_beam_search(SearchArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int lane = threadIdx.x;
   WaveLds L = carve(smem, a);
