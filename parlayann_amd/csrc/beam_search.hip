@@ -21,8 +21,10 @@
 //
 // Memory: one wave per workgroup, persistent grid pulling query indices from
 // a device counter.  LDS per wave: two frontier buffers (u64 keys + visited
-// flags), candidate buffers, the gather list, the query row (FP32 paths) and
-// the visited hash.
+// flags), candidate buffers, the gather list and the visited hash.  The FP32
+// query row for the rerank is staged in the candidate buffers once the search
+// is over; it has a region of its own only for the one-lane-per-row in-order
+// sum of rows past 8 lines, or when it is longer than those buffers.
 #include "kernels.h"
 #include "dist.h"
 

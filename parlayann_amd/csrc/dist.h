@@ -6,10 +6,11 @@
 //    per instruction; partial sums are reduced with DPP inside the 8-lane
 //    group.  Integer (uint8/int8) sums are exact in any order, so group mode is
 //    bit-exact for them (euclidian_point.h:45-81, mips_point.h:44-58).
-//  * sequential: one lane per row, terms summed strictly in index order with
-//    separately rounded multiply and add -- the arithmetic of the reference's
-//    float loop (euclidian_point.h:83-90, mips_point.h:60-66), so FP32 results
-//    are bit-identical to it.
+//  * in-order: terms summed strictly in index order with separately rounded
+//    multiply and add -- the arithmetic of the reference's float loop
+//    (euclidian_point.h:83-90, mips_point.h:60-66), so FP32 results are
+//    bit-identical to it.  Rows of up to 8 lines keep the group gather and
+//    carry the running sum from lane to lane; longer rows take one lane each.
 #pragma once
 #include "device_common.h"
 
