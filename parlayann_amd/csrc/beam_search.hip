@@ -663,9 +663,11 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
       int m = nchk - t < PANN_WAVE ? nchk - t : PANN_WAVE;
       if (lane < m) L.NL[lane] = key_id(F[t + lane]);
       wave_lds_sync();
-      // Rows of up to 8 lines: 8-lane gather and in-order lane chain; longer rows, and the
-      // FP8 L2 kernels (their query scales leave no registers for it): one lane per row
-      constexpr bool group_rr = !(ST == ST_FP8 && MT == M_L2);
+      // Rows of up to 8 lines: 8-lane gather and in-order lane chain, in the kernels of the
+      // reduced-precision stores.  One lane per row for longer rows, for the FP8 L2 kernels
+      // (their query scales leave no registers for it) and for the FP32 store's own
+      // kernels, whose rerank is rare and whose search slowed by 1.5 % with it compiled in.
+      constexpr bool group_rr = ST != ST_F32 && !(ST == ST_FP8 && MT == M_L2);
       if (group_rr && a.dim <= 128u)
         eval_group_inorder_lds<MT, 4, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
                                          lane);
