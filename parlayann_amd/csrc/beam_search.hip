@@ -23,7 +23,7 @@
 // a device counter.  LDS per wave: two frontier buffers (u64 keys + visited
 // flags), candidate buffers, the gather list and the visited hash.  The FP32
 // query row for the rerank is staged in the candidate buffers once the search
-// is over; it has a region of its own only for the one-lane-per-row in-order
+// is over; it has a region of its own only for the in-order
 // sum of rows past 8 lines, or when it is longer than those buffers.
 #include "kernels.h"
 #include "dist.h"
@@ -242,13 +242,14 @@ template <int ST, int MT, int NCH, bool SEQ>
 __device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
                                           const WaveLds& L, int ncmp, int lane) {
   if constexpr (SEQ) {
-    // rows of up to 8 lines: the group gather with the in-order lane chain;
-    // longer rows (NCH 8 only): one lane per row
+    // rows of up to 8 lines: the group gather with the in-order lane chain and the
+    // query in registers; longer rows (NCH 8 only): 8 lines at a time, query in LDS
     if (NCH < 8 || a.row_bytes <= 8u * 128u)
       eval_group_inorder<MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, L.NL, ncmp, L.DL,
                                   lane);
     else
-      eval_seq_f32<MT>(L.QL, a.pts, a.pts_stride, (int)a.dim, L.NL, ncmp, L.DL, lane);
+      eval_group_inorder_lds<MT, 8, 1>(L.QL, (int)a.dim, a.pts, a.pts_stride, L.NL, ncmp, L.DL,
+                                       lane);
   } else {
     eval_group<ST, MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, L.NL, ncmp, L.DL, lane);
   }
@@ -663,15 +664,15 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
       int m = nchk - t < PANN_WAVE ? nchk - t : PANN_WAVE;
       if (lane < m) L.NL[lane] = key_id(F[t + lane]);
       wave_lds_sync();
-      // Rows of up to 8 lines: 8-lane gather and in-order lane chain, in the kernels of the
-      // reduced-precision stores.  One lane per row for longer rows, for the FP8 L2 kernels
-      // (their query scales leave no registers for it) and for the FP32 store's own
-      // kernels, whose rerank is rare and whose search slowed by 1.5 % with it compiled in.
+      // 8-lane gather and in-order lane chain in the kernels of the reduced-precision
+      // stores.  One lane per row for the FP8 L2 kernels (their query scales leave no
+      // registers for it) and for the FP32 store's own kernels, whose rerank is rare and
+      // whose search slowed by 1.5 % with it compiled in.
       constexpr bool group_rr = ST != ST_F32 && !(ST == ST_FP8 && MT == M_L2);
       if (group_rr && a.dim <= 128u)
         eval_group_inorder_lds<MT, 4, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
                                          lane);
-      else if (group_rr && a.dim <= 256u)
+      else if (group_rr)
         eval_group_inorder_lds<MT, 8, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
                                          lane);
       else
@@ -822,7 +823,7 @@ hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool s
   int nch = nch_for((int)a.row_bytes);
   if (sequential) {
     if (store != ST_F32) return hipErrorInvalidValue;
-    if (nch < 0) nch = 8;  // rows past 8 lines: the one-lane-per-row sum
+    if (nch < 0) nch = 8;  // rows past 8 lines: 8 lines at a time, query in LDS
     if (metric == M_L2) { PANN_NCH_CASES(ST_F32, M_L2, true) }
     else { PANN_NCH_CASES(ST_F32, M_IP, true) }
   }

@@ -10,7 +10,9 @@
 //    multiply and add -- the arithmetic of the reference's float loop
 //    (euclidian_point.h:83-90, mips_point.h:60-66), so FP32 results are
 //    bit-identical to it.  Rows of up to 8 lines keep the group gather and
-//    carry the running sum from lane to lane; longer rows take one lane each.
+//    carry the running sum from lane to lane; longer rows go 8 lines at a
+//    time with the query in LDS.  One lane per row remains for the rerank of
+//    the FP32 store's kernels and of the FP8 L2 kernels.
 #pragma once
 #include "device_common.h"
 
@@ -389,40 +391,49 @@ __device__ __forceinline__ float group_inorder_sum(const GroupDist<ST_F32, MT, N
   return MT == M_L2 ? acc : -acc;
 }
 
-// The same sum over FP32 rows of at most MAXC lines, the line count known
-// only at run time and the query read from LDS (QL, dim floats): the fused
-// rerank, whose rows are not the search store's.  UP rows per group in flight.
+// The same sum over FP32 rows of any length, the line count known only at run
+// time and the query read from LDS (QL, dim floats): the fused rerank, whose
+// rows are not the search store's, and rows past 8 lines.  Rows are gathered
+// MAXC lines at a time, UP rows per group in flight; the sum carries over from
+// one block of lines to the next.
 template <int MT, int MAXC, int UP>
 __device__ __forceinline__ void eval_group_inorder_lds(const float* QL, int dim, const char* base,
                                                        uint64_t stride, const uint32_t* NL,
                                                        int ncmp, float* DL, int lane) {
   const int sub = lane & 7, grp = lane >> 3;
-  const int nl = (dim * 4 + 127) >> 7;  // 1..MAXC, checked by the caller
+  const int nl = (dim * 4 + 127) >> 7;  // lines per row, taken MAXC at a time
   for (int r0 = 0; r0 < ncmp; r0 += 8 * UP) {
-    uint4 raw[UP][MAXC];
+    const char* row[UP];
 #pragma unroll
     for (int u = 0; u < UP; u++) {
       const int r = r0 + u * 8 + grp;
-      const char* row = base + (uint64_t)NL[r < ncmp ? r : 0] * stride;
-#pragma unroll
-      for (int c = 0; c < MAXC; c++)
-        raw[u][c] = c < nl ? ld16(row + c * 128 + sub * 16) : make_uint4(0u, 0u, 0u, 0u);
+      row[u] = base + (uint64_t)NL[r < ncmp ? r : 0] * stride;
     }
     float acc[UP] = {};
+    for (int c0 = 0; c0 < nl; c0 += MAXC) {  // uniform
+      uint4 raw[UP][MAXC];
 #pragma unroll
-    for (int c = 0; c < MAXC; c++) {
-      if (c < nl) {  // uniform
-        float q[4];
+      for (int u = 0; u < UP; u++)
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          const int j = c * 32 + sub * 4 + e;
-          q[e] = j < dim ? QL[j] : 0.0f;
-        }
+        for (int c = 0; c < MAXC; c++)
+          raw[u][c] = c0 + c < nl ? ld16(row[u] + (c0 + c) * 128 + sub * 16)
+                                  : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-        for (int u = 0; u < UP; u++) {
-          float t[4];
-          inorder_terms<MT>(q, raw[u][c], t);
-          acc[u] = inorder_chunk(c == 0 ? 0.0f : acc[u], t, c == 0);
+      for (int c = 0; c < MAXC; c++) {
+        if (c0 + c < nl) {  // uniform
+          float q[4];
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const int j = (c0 + c) * 32 + sub * 4 + e;
+            q[e] = j < dim ? QL[j] : 0.0f;
+          }
+          const bool first = c0 + c == 0;
+#pragma unroll
+          for (int u = 0; u < UP; u++) {
+            float t[4];
+            inorder_terms<MT>(q, raw[u][c], t);
+            acc[u] = inorder_chunk(first ? 0.0f : acc[u], t, first);
+          }
         }
       }
     }

@@ -95,7 +95,7 @@ struct SearchArgs {
   uint32_t rc_log2;
   uint32_t fb_log2;         // bits of the frontier-id Bloom filter (log2), 0 = none
   // 1: the FP32 query row has an LDS region of its own for the whole search (the
-  // one-lane-per-row in-order sum of rows past 8 lines, or a row too long for
+  // in-order sum of rows past 8 lines, or a row too long for
   // the candidate buffers); 0: the rerank stages it in the candidate buffers,
   // which are dead by then, and no other step reads it from LDS
   uint32_t q_own;

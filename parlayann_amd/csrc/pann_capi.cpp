@@ -678,7 +678,7 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     hlog2 = std::max<uint32_t>(ceil_log2((uint64_t)2 * a.ccap), 6);
   }
   a.hslots_log2 = hlog2;
-  // the query row's own LDS region: for the one-lane in-order sum of rows past 8
+  // the query row's own LDS region: for the in-order sum of rows past 8
   // lines, or when the row does not fit the two candidate buffers (the rerank's
   // staging place otherwise)
   a.q_own = (sequential && a.row_bytes > 8u * 128u) ||

@@ -110,12 +110,12 @@ def test_f32_inorder_bit_exact(oracle, f32_case, ip_case, metric, B):
     assert np.array_equal(f["visited"], o["visited"])
 
 
-@pytest.mark.parametrize("dim", [5, 32, 40, 100, 200, 256, 300])
+@pytest.mark.parametrize("dim", [5, 32, 40, 100, 200, 256, 300, 600])
 @pytest.mark.parametrize("metric", ["l2", "ip"])
 def test_f32_inorder_bit_exact_dims(oracle, metric, dim):
     """In-order FP32 at every row shape: rows of 1, 1, 2, 4, 7 and 8 128-byte
     lines (8-lane gather, sum carried lane to lane in element order) and rows
-    past 8 lines (one lane per row)."""
+    of 10 and 19 lines (8 lines at a time, query in LDS)."""
     rng = np.random.default_rng(1000 + dim)
     X = rng.standard_normal((3000, dim)).astype(np.float32)
     Q = rng.standard_normal((100, dim)).astype(np.float32)
@@ -262,7 +262,7 @@ def test_reduced_precision_rerank(oracle, ip_case, store):
 @pytest.mark.parametrize("metric", ["l2", "ip"])
 def test_rerank_distances_bit_exact_dims(oracle, metric, dim):
     """The fused FP32 rerank at rows of 1 to 4 lines (two rows per 8-lane
-    group in flight), 5 to 8 lines (one) and past 8 lines (one lane per row):
+    group in flight), 5 to 8 lines and past 8 lines (one, 8 lines at a time):
     every returned distance is the oracle's in-order FP32 distance."""
     rng = np.random.default_rng(2000 + dim)
     X = rng.standard_normal((3000, dim)).astype(np.float32)
