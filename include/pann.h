@@ -55,7 +55,12 @@ typedef enum {
 
 /* FP32 summation order of the native f32 store */
 typedef enum {
-  PANN_SUM_INORDER = 0, /* one lane per row, in-order sum: bit-identical to the reference */
+  PANN_SUM_INORDER = 0, /* in-order sum, the reference's float loop (euclidian_point.h:83-90,
+                           mips_point.h:60-66) as g++ -O3 -march=<FMA target> compiles it:
+                           separately rounded terms up to the last multiple of 4, the last
+                           d % 4 terms fused; bit-identical to that build at every d.  Rows
+                           are gathered 8 lanes per row and the running sum is carried from
+                           lane to lane in element order (DPP) */
   PANN_SUM_TREE = 1     /* 8 lanes per row, DPP tree reduction (faster, not bit-identical) */
 } pann_summation;
 
@@ -95,9 +100,12 @@ typedef struct {
   int32_t async;       /* 1: do not synchronise; call pann_search_status() later */
   int32_t waves_per_cu;/* persistent waves per CU (0 = automatic) */
   int32_t exact_counts;/* 1: out_cmps equals the reference's dist_cmps (every id seen is
-                          remembered); 0: every neighbour is evaluated and checked against
-                          the frontier and pending candidates afterwards -- same ids,
-                          distances and visited counts, out_cmps counts evaluations, no
+                          remembered in an LDS table with a per-wave global overflow table
+                          of fixed size: a query that fills it fails with
+                          PANN_ERR_CAPACITY, where the reference's hashset grows,
+                          hashset.h:25-38); 0: every neighbour is evaluated and checked
+                          against the frontier and pending candidates afterwards -- same
+                          ids, distances and visited counts, out_cmps counts evaluations, no
                           capacity limit */
   int32_t visits_per_round; /* 0 or 2: while a merge is deferred, the next visit's rows are
                                gathered with the current one's; 1: one visit per round.
@@ -159,7 +167,9 @@ pann_status pann_search(pann_index_t idx, const void *queries, uint32_t nq,
                         uint32_t *out_visited, uint32_t *out_cmps, void *stream);
 
 /* Same search with host buffers (copies in and out, synchronises).  This is
- * what a C++ qsearchAll caller binds directly. */
+ * what a C++ qsearchAll caller binds directly.  The device staging for the
+ * copies is kept per index and grown geometrically, so repeated calls of the
+ * same or smaller size allocate nothing.  Calls on one index are serialised. */
 pann_status pann_search_host(pann_index_t idx, const void *queries, uint32_t nq,
                              const pann_qparams *qp, const uint32_t *starts, uint32_t nstarts,
                              const pann_search_opts *opts, uint32_t *out_ids, float *out_dists,
@@ -179,6 +189,24 @@ pann_status pann_beam_search_host(pann_index_t idx, const void *queries, uint32_
                                   uint32_t nstarts, const pann_search_opts *opts,
                                   uint32_t *out_ids, float *out_dists, uint32_t *out_count,
                                   uint32_t *out_visited, uint32_t *out_cmps);
+
+/* pann_beam_search_host plus each query's visited list -- the second member
+ * of filtered_beam_search's result (beamSearch.h:225-227): out_vis_ids /
+ * out_vis_dists nq x vis_cap (host), out_vis_count[q] entries, sorted by
+ * (dist, id).  vis_cap in [1, 4096]; a longer list is PANN_ERR_CAPACITY. */
+pann_status pann_beam_search_visited_host(pann_index_t idx, const void *queries, uint32_t nq,
+                                          const pann_qparams *qp, const uint32_t *starts,
+                                          uint32_t nstarts, const pann_search_opts *opts,
+                                          uint32_t *out_ids, float *out_dists,
+                                          uint32_t *out_count, uint32_t vis_cap,
+                                          uint32_t *out_vis_ids, float *out_vis_dists,
+                                          uint32_t *out_vis_count, uint32_t *out_cmps);
+
+/* Number of device allocations this library has made since it was loaded
+ * (every hipMalloc of the index, its stores and its per-stream scratch).  No
+ * reference counterpart: it lets a caller check that steady-state searches
+ * allocate nothing. */
+pann_status pann_device_alloc_count(uint64_t *count);
 
 /* Duration in milliseconds of the last pann_search kernel, measured with
  * HIP events on the launch stream (synchronises). */

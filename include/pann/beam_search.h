@@ -1,18 +1,54 @@
 /*
  * pann/beam_search.h -- the reference's C++ search surface over libpann.
  *
- * Keeps the names, parameter order and QueryParams fields/constructors of
- * ParlayANN's search API (namespace parlayANN):
- *   QueryParams                 types.h:233-268
+ * Keeps the names, parameter order and result types of ParlayANN's search
+ * API (namespace parlayANN), so that a caller's search code compiles against
+ * it unchanged apart from the container type:
+ *   QueryParams                 types.h:233-268 (fields and both constructors)
  *   stats<indexType>            stats.h:121-155
- *   beam_search                 beamSearch.h:419-445 (one query, frontier)
- *   beam_search_rerank          beamSearch.h:528-597 (one query, k results)
- *   qsearchAll                  beamSearch.h:599-646 (a batch)
- * std::vector replaces parlay::sequence (parlaylib is not a dependency).  The
- * point set and graph live on the GPU in a DeviceIndex (the analogue of
- * PointRange + Graph after loading, point_range.h:74-117, graph.h:151-208);
- * every search runs there, and there is no CPU path.  Errors the reference
- * reports with abort() throw std::runtime_error carrying pann_last_error().
+ *   EarlyStopping, EStop        beamSearch.h:23-29, :411-417
+ *   filtered_beam_search        beamSearch.h:33-45  (frontier, sorted visited list,
+ *                                                     distance comparisons)
+ *   beam_search                 beamSearch.h:419-427, :437-445
+ *   beam_search_rerank          beamSearch.h:528-597
+ *   qsearchAll                  beamSearch.h:599-646
+ *   Graph<indexType>            graph.h:130-255 (file constructor, size, max_degree,
+ *                                                 operator[], save)
+ *   PointRange<Point>           point_range.h:43-141 (file constructor, size,
+ *                                                 dimension, operator[], the
+ *                                                 quantizing PointRange(pr))
+ *   Euclidian_Point<T>, Mips_Point<T>             euclidian_point.h:92-248,
+ *                                                 mips_point.h:68-144 (row views)
+ * std::vector replaces parlay::sequence (parlaylib is not a dependency).
+ *
+ * Every search runs on the GPU: the first search over a (Graph, PointRange)
+ * pair uploads both to the device (pann_open) and keeps them there; a graph
+ * changed through edgeRange::update_neighbors is uploaded again at its next
+ * search.  There is no CPU path.  Reduced-precision copies are point types
+ * of their own, built on the device from the float points the way the
+ * reference's quantizing PointRange constructor builds its copy
+ * (point_range.h:55-66):
+ *   BF16_Point<P>  bfloat16 rows (no reference counterpart)
+ *   FP8_Point<P>   OCP FP8 e4m3 rows, per-dimension absmax scales (none)
+ *   Q8_Point<P>    the reference's 8-bit scalar codes: Euclidian_Point<uint8_t>
+ *                  affine codes for L2 (euclidian_point.h:98-107, 188-243),
+ *                  Quantized_Mips_Point<8, true> for MIPS (mips_point.h:292-470)
+ * so `PointRange<FP8_Point<Mips_Point<float>>> Q(Base)` stands where the
+ * reference writes `PointRange<Quantized_Mips_Point<8, true>> Q(Base)`.
+ *
+ * Errors the reference reports with abort() throw std::runtime_error carrying
+ * pann_last_error().  Not supported (they throw): use_filtering = true (a
+ * third, coarser store), random start points in qsearchAll (parlay's random
+ * generator cannot be reproduced), and early-stopping functors other than the
+ * always-false ones.
+ *
+ * Device-wide choices that the reference's signatures have no room for sit in
+ * search_options(): the device, the FP32 summation order (in-order by
+ * default: bit-identical to the reference's float loop) and whether
+ * QueryStats.distances counts the reference's dist_cmps exactly
+ * (exact_counts = true: the reference's seen set, a slower kernel) or the
+ * distance evaluations made (the default: ids, distances and visited counts
+ * are the reference's either way).
  *
  * Header-only; link with libpann.so (parlayann_amd/libpann.so).
  */
@@ -20,10 +56,17 @@
 #define PANN_BEAM_SEARCH_HPP
 
 #include <algorithm>
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -93,35 +136,32 @@ struct stats {
   }
 };
 
-/* Row-major points on the host (the data of a PointRange, without its 64-byte
- * row padding): n rows of dims elements of T (float, uint8_t or int8_t). */
-template <typename T>
-struct PointRange {
-  const T *values = nullptr;
-  size_t n = 0;
-  unsigned dims = 0;
-  PointRange() {}
-  PointRange(const T *values, size_t n, unsigned dims) : values(values), n(n), dims(dims) {}
-  size_t size() const { return n; }
-  unsigned dimension() const { return dims; }
-  unsigned get_dims() const { return dims; }
-  const T *operator[](size_t i) const { return values + i * dims; }
+/* The always-false early-stopping functors (beamSearch.h:23-29, :411-417). */
+struct EarlyStopping {
+  template <typename PointInfo>
+  bool operator()(const PointInfo &, const PointInfo &, const PointInfo &,
+                  const QueryParams &) {
+    return false;
+  }
+};
+struct EStop {
+  template <typename PointInfo>
+  bool operator()(const PointInfo &, const PointInfo &, const PointInfo &,
+                  const QueryParams &) {
+    return false;
+  }
 };
 
-/* Graph<indexType> storage (graph.h:46-64): n rows of maxDeg + 1 entries,
- * slot 0 the degree. */
-template <typename indexType>
-struct Graph {
-  const indexType *adj = nullptr;
-  size_t n = 0;
-  unsigned maxDeg = 0;
-  Graph() {}
-  Graph(const indexType *adj, size_t n, unsigned maxDeg) : adj(adj), n(n), maxDeg(maxDeg) {}
-  size_t size() const { return n; }
-  unsigned max_degree() const { return maxDeg; }
+/* Device-wide choices the reference's signatures have no room for. */
+struct SearchOptions {
+  int device = 0;
+  int32_t summation = PANN_SUM_INORDER;  /* PANN_SUM_TREE: faster, not bit-identical */
+  bool exact_counts = false;             /* stats distances = the reference's dist_cmps */
 };
-
-enum class Metric { Euclidian, Mips };
+inline SearchOptions &search_options() {
+  static SearchOptions o;
+  return o;
+}
 
 namespace detail {
 inline void check(pann_status st) {
@@ -135,137 +175,470 @@ constexpr int32_t dtype_of() {
   return std::is_same<T, float>::value ? PANN_DT_F32
          : std::is_same<T, uint8_t>::value ? PANN_DT_U8 : PANN_DT_I8;
 }
+inline uint64_t next_serial() {
+  static std::atomic<uint64_t> s{1};
+  return s.fetch_add(1);
+}
+}  // namespace detail
+
+/* Row views (euclidian_point.h:92-248, mips_point.h:68-144): the values of
+ * one row, its dimension and its id.  same_as is pointer equality, as in the
+ * reference.  Distances are computed on the device only. */
+template <typename T_, int Metric>
+struct RowPoint {
+  using T = T_;
+  using distanceType = float;
+  static constexpr int32_t pann_metric = Metric;
+  static constexpr int32_t pann_store = PANN_STORE_NATIVE;
+  static constexpr bool is_metric = Metric == PANN_METRIC_L2;
+  struct parameters {
+    int dims = 0;
+    parameters() {}
+    explicit parameters(int d) : dims(d) {}
+    int num_bytes() const { return dims * (int)sizeof(T); }
+  };
+  const T *values = nullptr;
+  unsigned d = 0;
+  long id_ = -1;
+  RowPoint() {}
+  RowPoint(const T *values, unsigned d, long id) : values(values), d(d), id_(id) {}
+  long id() const { return id_; }
+  bool same_as(const RowPoint &q) const { return values == q.values; }
+  void prefetch() const {}
+  const T &operator[](unsigned i) const { return values[i]; }
+};
+template <typename T>
+using Euclidian_Point = RowPoint<T, PANN_METRIC_L2>;
+template <typename T>
+using Mips_Point = RowPoint<T, PANN_METRIC_MIPS>;
+
+/* A reduced-precision copy of float points of type BasePoint.  A row view
+ * carries the float row: the device encodes queries itself. */
+template <typename BasePoint, int Store, int BytesPerDim, int ExtraBytes = 0>
+struct ReducedPoint : BasePoint {
+  static_assert(std::is_same<typename BasePoint::T, float>::value,
+                "reduced-precision copies are made of float points");
+  using Base = BasePoint;
+  static constexpr int32_t pann_store = Store;
+  struct parameters {
+    int dims = 0;
+    parameters() {}
+    explicit parameters(int d) : dims(d) {}
+    int num_bytes() const { return dims * BytesPerDim + ExtraBytes; }
+  };
+  ReducedPoint() {}
+  ReducedPoint(const float *values, unsigned d, long id) : BasePoint(values, d, id) {}
+};
+template <typename P>
+using BF16_Point = ReducedPoint<P, PANN_STORE_BF16, 2>;
+template <typename P>
+using FP8_Point = ReducedPoint<P, PANN_STORE_FP8, 1>;
+template <typename P>
+using Q8_Point = ReducedPoint<P, PANN_STORE_Q8, 1>;
+
+template <typename Point_>
+class PointRange;
+template <typename indexType_>
+class Graph;
+
+namespace detail {
+/* One device index per (graph version, float point set), kept while both
+ * objects live. */
+struct Handle {
+  pann_index_t h = nullptr;
+  std::set<int32_t> stores;
+  ~Handle() {
+    if (h) pann_close(h);
+  }
+};
+struct Registry {
+  std::mutex mu;
+  std::map<std::tuple<uint64_t, uint64_t, uint64_t>, std::shared_ptr<Handle>> m;
+  void forget(uint64_t serial) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto it = m.begin(); it != m.end();)
+      if (std::get<0>(it->first) == serial || std::get<2>(it->first) == serial) it = m.erase(it);
+      else ++it;
+  }
+};
+inline Registry &registry() {
+  static Registry r;
+  return r;
+}
+}  // namespace detail
+
+/* PointRange<Point> (point_range.h:43-141): n rows of dims values on the
+ * host, the source the device copy is made from.  Rows are stored without
+ * the reference's 64-byte padding (the device pads them to 128 bytes). */
+template <typename Point_>
+class PointRange {
+ public:
+  using Point = Point_;
+  using T = typename Point::T;
+  using parameters = typename Point::parameters;
+  parameters params;
+
+  PointRange() : serial_(detail::next_serial()) {}
+  /* point_range.h:74-117: u32 n, u32 dims, then n * dims values */
+  explicit PointRange(const char *filename) : serial_(detail::next_serial()) {
+    static_assert(Point::pann_store == PANN_STORE_NATIVE, "read the float points, then copy");
+    FILE *f = std::fopen(filename, "rb");
+    if (!f) throw std::runtime_error(std::string("data file ") + filename + " not found");
+    uint32_t h[2];
+    if (std::fread(h, 4, 2, f) != 2) {
+      std::fclose(f);
+      throw std::runtime_error("short data file");
+    }
+    n_ = h[0];
+    params = parameters((int)h[1]);
+    auto v = std::make_shared<std::vector<T>>((size_t)n_ * h[1]);
+    if (std::fread(v->data(), sizeof(T), v->size(), f) != v->size()) {
+      std::fclose(f);
+      throw std::runtime_error("short data file");
+    }
+    std::fclose(f);
+    own_ = v;
+    values_ = v->data();
+  }
+  /* n rows of dims values at `values`, borrowed (they must outlive this range) */
+  PointRange(const T *values, size_t n, unsigned dims)
+      : params((int)dims), values_(values), n_(n), serial_(detail::next_serial()) {}
+  /* the quantizing constructor (point_range.h:55-66): a reduced-precision copy
+   * of float points, made on the device at the first search */
+  template <typename PR, typename = typename PR::Point>
+  explicit PointRange(const PR &pr)
+      : params((int)pr.dimension()), own_(pr.own_), values_(pr.values_), n_(pr.size()),
+        serial_(detail::next_serial()), base_serial_(pr.base_serial()) {
+    static_assert(Point::pann_store != PANN_STORE_NATIVE,
+                  "the copy constructor makes a reduced-precision range");
+    static_assert(std::is_same<typename Point::Base, typename PR::Point>::value,
+                  "a copy of PointRange<Base> points");
+  }
+  PointRange(const PointRange &o)
+      : params(o.params), own_(o.own_), values_(o.values_), n_(o.n_),
+        serial_(detail::next_serial()), base_serial_(o.base_serial_) {}
+  PointRange &operator=(const PointRange &) = delete;
+  ~PointRange() { detail::registry().forget(serial_); }
+
+  size_t size() const { return n_; }
+  long dimension() const { return params.dims; }
+  long get_dims() const { return params.dims; }
+  Point operator[](long i) const {
+    return Point(values_ + (size_t)i * params.dims, (unsigned)params.dims, i);
+  }
+  const T *data() const { return values_; }
+  /* identity of the float rows a reduced-precision copy is made of */
+  uint64_t base_serial() const { return base_serial_ ? base_serial_ : serial_; }
+
+ private:
+  template <typename>
+  friend class PointRange;
+  std::shared_ptr<const std::vector<T>> own_;
+  const T *values_ = nullptr;
+  size_t n_ = 0;
+  uint64_t serial_;
+  uint64_t base_serial_ = 0;
+};
+
+/* Graph<indexType> (graph.h:46-255): n rows of maxDeg + 1 entries, slot 0 the
+ * degree.  update_neighbors marks the graph changed, so the next search
+ * uploads it again. */
+template <typename indexType_>
+class Graph {
+ public:
+  using indexType = indexType_;
+  static_assert(sizeof(indexType) == 4, "graph ids are 32-bit");
+
+  struct edgeRange {
+    indexType *row;
+    long maxDeg;
+    const Graph *g;
+    size_t size() const { return row[0]; }
+    indexType operator[](size_t j) const { return row[1 + j]; }
+    indexType *begin() const { return row + 1; }
+    indexType *end() const { return row + 1 + row[0]; }
+    void prefetch() const {}
+    template <typename Seq>
+    void update_neighbors(const Seq &nbhs) {  // graph.h:77-88
+      if ((long)nbhs.size() > maxDeg) throw std::runtime_error("neighbor list exceeds max degree");
+      row[0] = (indexType)nbhs.size();
+      for (size_t i = 0; i < nbhs.size(); i++) row[1 + i] = nbhs[i];
+      g->version_++;
+    }
+    void clear_neighbors() {
+      row[0] = 0;
+      g->version_++;
+    }
+  };
+
+  Graph() : serial_(detail::next_serial()) {}
+  Graph(long maxDeg, size_t n)
+      : adj_(std::make_shared<std::vector<indexType>>(n * (maxDeg + 1), 0)), n_(n),
+        maxDeg_(maxDeg), serial_(detail::next_serial()) {}
+  /* n rows of maxDeg + 1 entries at `adj` (slot 0 the degree), copied */
+  Graph(const indexType *adj, size_t n, long maxDeg)
+      : adj_(std::make_shared<std::vector<indexType>>(adj, adj + n * (maxDeg + 1))), n_(n),
+        maxDeg_(maxDeg), serial_(detail::next_serial()) {}
+  /* graph.h:151-208: u32 n, u32 maxDeg, u32 degrees[n], then the ids */
+  explicit Graph(const char *gFile) : serial_(detail::next_serial()) {
+    FILE *f = std::fopen(gFile, "rb");
+    if (!f) throw std::runtime_error(std::string("graph file ") + gFile + " not found");
+    uint32_t h[2];
+    if (std::fread(h, 4, 2, f) != 2) {
+      std::fclose(f);
+      throw std::runtime_error("short graph file");
+    }
+    n_ = h[0];
+    maxDeg_ = h[1];
+    adj_ = std::make_shared<std::vector<indexType>>(n_ * (maxDeg_ + 1), 0);
+    std::vector<uint32_t> deg(n_);
+    bool ok = std::fread(deg.data(), 4, n_, f) == n_;
+    for (size_t i = 0; ok && i < n_; i++) {
+      if (deg[i] > (uint32_t)maxDeg_) ok = false;
+      indexType *row = adj_->data() + i * (maxDeg_ + 1);
+      row[0] = deg[i];
+      if (ok && deg[i]) ok = std::fread(row + 1, 4, deg[i], f) == deg[i];
+    }
+    std::fclose(f);
+    if (!ok) throw std::runtime_error("bad graph file");
+  }
+  Graph(const Graph &o)
+      : adj_(o.adj_), n_(o.n_), maxDeg_(o.maxDeg_), serial_(detail::next_serial()) {}
+  Graph &operator=(const Graph &) = delete;
+  ~Graph() { detail::registry().forget(serial_); }
+
+  long max_degree() const { return maxDeg_; }
+  size_t size() const { return n_; }
+  edgeRange operator[](indexType i) const {
+    if (i >= n_) throw std::out_of_range("graph index out of range");
+    return edgeRange{adj_->data() + (size_t)i * (maxDeg_ + 1), maxDeg_, this};
+  }
+  /* graph.h:210-237 */
+  void save(const char *oFile) const {
+    FILE *f = std::fopen(oFile, "wb");
+    if (!f) throw std::runtime_error(std::string("cannot write ") + oFile);
+    uint32_t h[2] = {(uint32_t)n_, (uint32_t)maxDeg_};
+    std::fwrite(h, 4, 2, f);
+    for (size_t i = 0; i < n_; i++) std::fwrite(row(i), 4, 1, f);
+    for (size_t i = 0; i < n_; i++) std::fwrite(row(i) + 1, 4, row(i)[0], f);
+    std::fclose(f);
+  }
+  const indexType *data() const { return adj_->data(); }
+  uint64_t serial() const { return serial_; }
+  uint64_t version() const { return version_; }
+
+ private:
+  const indexType *row(size_t i) const { return adj_->data() + i * (maxDeg_ + 1); }
+  std::shared_ptr<std::vector<indexType>> adj_;
+  size_t n_ = 0;
+  long maxDeg_ = 0;
+  uint64_t serial_;
+  mutable uint64_t version_ = 0;
+};
+
+namespace detail {
+template <typename P>
+struct BaseOf {
+  using type = P;
+};
+template <typename B, int S, int BPD, int X>
+struct BaseOf<ReducedPoint<B, S, BPD, X>> {
+  using type = B;
+};
+
+/* The device index of (G, the float rows of Points) with Points' store built. */
+template <typename indexType, typename PR>
+pann_index_t device_index(const Graph<indexType> &G, const PR &Points) {
+  using Point = typename PR::Point;
+  using BP = typename BaseOf<Point>::type;
+  if (G.size() != Points.size()) throw std::runtime_error("graph size and point size do not match");
+  Registry &r = registry();
+  std::lock_guard<std::mutex> lk(r.mu);
+  auto key = std::make_tuple(G.serial(), G.version(), Points.base_serial());
+  auto it = r.m.find(key);
+  if (it == r.m.end()) {
+    for (auto j = r.m.begin(); j != r.m.end();)  // an older version of this graph
+      if (std::get<0>(j->first) == G.serial() && std::get<2>(j->first) == Points.base_serial())
+        j = r.m.erase(j);
+      else ++j;
+    auto hd = std::make_shared<Handle>();
+    pann_index_desc d{(uint32_t)G.size(), (uint32_t)Points.dimension(), (uint32_t)G.max_degree(),
+                      dtype_of<typename BP::T>(), BP::pann_metric, Points.data(), G.data(),
+                      search_options().device};
+    check(pann_open(&d, &hd->h));
+    it = r.m.emplace(key, hd).first;
+  }
+  Handle &hd = *it->second;
+  if (Point::pann_store != PANN_STORE_NATIVE && !hd.stores.count(Point::pann_store)) {
+    check(pann_build_store(hd.h, Point::pann_store));
+    hd.stores.insert(Point::pann_store);
+  }
+  return hd.h;
+}
+
 inline pann_qparams qparams(const QueryParams &QP) {
   return pann_qparams{QP.k, QP.beamSize, QP.limit, QP.degree_limit, QP.rerank_factor,
                       QP.batch_factor};
 }
+inline pann_search_opts opts(int32_t store, bool rerank) {
+  pann_search_opts o{};
+  o.store = store;
+  o.summation = search_options().summation;
+  o.rerank = rerank ? 1 : 0;
+  o.exact_counts = search_options().exact_counts ? 1 : 0;
+  return o;
+}
+template <typename ES>
+constexpr void check_early_stop() {
+  static_assert(std::is_same<ES, EarlyStopping>::value || std::is_same<ES, EStop>::value,
+                "only the always-false early-stopping functors are supported");
+}
 }  // namespace detail
 
-/* Points and graph resident on one GPU.  Movable, not copyable. */
-template <typename T>
-class DeviceIndex {
- public:
-  DeviceIndex(const PointRange<T> &Points, const Graph<uint32_t> &G, Metric metric,
-              int device = 0)
-      : n_(Points.size()), dims_(Points.dimension()), maxDeg_(G.max_degree()) {
-    if (G.size() != Points.size())
-      throw std::runtime_error("graph size and point size do not match");
-    pann_index_desc d{(uint32_t)n_, dims_, maxDeg_, detail::dtype_of<T>(),
-                      metric == Metric::Euclidian ? PANN_METRIC_L2 : PANN_METRIC_MIPS,
-                      Points.values, G.adj, device};
-    detail::check(pann_open(&d, &h_));
-  }
-  DeviceIndex(const DeviceIndex &) = delete;
-  DeviceIndex &operator=(const DeviceIndex &) = delete;
-  DeviceIndex(DeviceIndex &&o) noexcept : h_(o.h_), n_(o.n_), dims_(o.dims_), maxDeg_(o.maxDeg_) {
-    o.h_ = nullptr;
-  }
-  ~DeviceIndex() {
-    if (h_) pann_close(h_);
-  }
-  /* reduced-precision gather copy of float points: PANN_STORE_BF16 / _FP8 */
-  void build_store(int32_t store) { detail::check(pann_build_store(h_, store)); }
-  pann_index_t handle() const { return h_; }
-  size_t size() const { return n_; }
-  unsigned dimension() const { return dims_; }
-  unsigned max_degree() const { return maxDeg_; }
-
- private:
-  pann_index_t h_ = nullptr;
-  size_t n_;
-  unsigned dims_, maxDeg_;
-};
-
-/* Search options beyond QueryParams: which store the search gathers, FP32
- * summation order (in-order = the reference's bit pattern), and whether
- * results are re-scored against the float points (the rerank branch of
- * beam_search_rerank, beamSearch.h:569-587). */
-struct SearchOptions {
-  int32_t store = PANN_STORE_NATIVE;
-  int32_t summation = PANN_SUM_INORDER;
-  bool rerank = false;
-  bool exact_counts = true;  /* QueryStats.distances = the reference's dist_cmps */
-};
-
-/* qsearchAll (beamSearch.h:599-646) from one starting point: the k nearest
- * ids per query (beam_search_rerank per query, :528-597), with visited and
- * distance counts in QueryStats when it is sized for the queries. */
-template <typename T>
-std::vector<std::vector<uint32_t>> qsearchAll(const PointRange<T> &Query_Points,
-                                              DeviceIndex<T> &index,
-                                              stats<uint32_t> &QueryStats,
-                                              const uint32_t starting_point,
-                                              const QueryParams &QP,
-                                              const SearchOptions &so = SearchOptions()) {
-  const size_t nq = Query_Points.size();
-  const size_t k = (size_t)std::max<long>(QP.k, 0);
-  std::vector<uint32_t> ids(nq * k);
-  std::vector<float> dists(nq * k);
-  std::vector<uint32_t> vis(nq), cmps(nq);
-  pann_qparams qp = detail::qparams(QP);
-  pann_search_opts o{so.store, so.summation, so.rerank ? 1 : 0, 0, 0, 0, so.exact_counts ? 1 : 0};
-  uint32_t start = starting_point;
-  if (nq)
-    detail::check(pann_search_host(index.handle(), Query_Points.values, (uint32_t)nq, &qp,
-                                   &start, 1, &o, ids.data(), dists.data(), vis.data(),
-                                   cmps.data()));
-  std::vector<std::vector<uint32_t>> all(nq);
-  for (size_t i = 0; i < nq; i++) all[i].assign(ids.begin() + i * k, ids.begin() + (i + 1) * k);
-  if (QueryStats.visited.size() == nq) {
-    for (size_t i = 0; i < nq; i++) {
-      QueryStats.increment_visited((uint32_t)i, vis[i]);
-      QueryStats.increment_dist((uint32_t)i, cmps[i]);
-    }
-  }
-  return all;
+/* filtered_beam_search (beamSearch.h:33-228): ((frontier, visited), number
+ * of distance comparisons), both lists sorted by (distance, id).  The search
+ * gathers Points' rows (a reduced-precision range searches its store); p is a
+ * row of a range of the same kind.  use_filtering must be false. */
+template <typename indexType, typename Point, typename PointRange_, typename QPoint,
+          typename QPointRange, typename GT, typename ES = EarlyStopping>
+std::pair<std::pair<std::vector<std::pair<indexType, typename Point::distanceType>>,
+                    std::vector<std::pair<indexType, typename Point::distanceType>>>,
+          size_t>
+filtered_beam_search(const GT &G, const Point p, const PointRange_ &Points, const QPoint qp,
+                     const QPointRange &Q_Points, const std::vector<indexType> starting_points,
+                     const QueryParams &QP, bool use_filtering = false, ES early_stop = ES{}) {
+  (void)qp;
+  (void)Q_Points;
+  (void)early_stop;
+  detail::check_early_stop<ES>();
+  if (use_filtering)
+    throw std::runtime_error("filtered_beam_search: use_filtering is not supported");
+  pann_index_t h = detail::device_index(G, Points);
+  const size_t B = (size_t)std::max<long>(QP.beamSize, 1);
+  // the visited list is never longer than `limit` visits
+  const uint32_t vis_cap = (uint32_t)std::min<long>(std::max<long>(QP.limit, 1), 4096);
+  std::vector<uint32_t> fid(B), vid(vis_cap);
+  std::vector<float> fd(B), vd(vis_cap);
+  uint32_t fc = 0, vc = 0, cmps = 0;
+  pann_qparams q = detail::qparams(QP);
+  q.k = QP.beamSize;
+  pann_search_opts o = detail::opts(Point::pann_store, false);
+  detail::check(pann_beam_search_visited_host(
+      h, p.values, 1, &q, (const uint32_t *)starting_points.data(),
+      (uint32_t)starting_points.size(), &o, fid.data(), fd.data(), &fc, vis_cap, vid.data(),
+      vd.data(), &vc, &cmps));
+  using id_dist = std::pair<indexType, typename Point::distanceType>;
+  std::vector<id_dist> frontier(fc), visited(vc);
+  for (uint32_t i = 0; i < fc; i++) frontier[i] = id_dist((indexType)fid[i], fd[i]);
+  for (uint32_t i = 0; i < vc; i++) visited[i] = id_dist((indexType)vid[i], vd[i]);
+  return {{std::move(frontier), std::move(visited)}, (size_t)cmps};
 }
 
-/* beam_search_rerank (beamSearch.h:528-597) for one query: k (id, distance)
- * pairs.  Batch queries through qsearchAll: one call launches one kernel. */
-template <typename T>
-std::vector<std::pair<uint32_t, float>> beam_search_rerank(
-    const T *p, DeviceIndex<T> &index, stats<uint32_t> &QueryStats, uint32_t query_id,
-    const std::vector<uint32_t> &starting_points, const QueryParams &QP,
-    const SearchOptions &so = SearchOptions()) {
+/* beam_search without filtering (beamSearch.h:419-427) */
+template <typename Point, typename PointRange_, typename indexType>
+std::pair<std::pair<std::vector<std::pair<indexType, typename Point::distanceType>>,
+                    std::vector<std::pair<indexType, typename Point::distanceType>>>,
+          size_t>
+beam_search(const Point p, const Graph<indexType> &G, const PointRange_ &Points,
+            const std::vector<indexType> starting_points, const QueryParams &QP) {
+  return filtered_beam_search(G, p, Points, p, Points, starting_points, QP, false);
+}
+
+/* one start point (beamSearch.h:437-445) */
+template <typename Point, typename PointRange_, typename indexType>
+std::pair<std::pair<std::vector<std::pair<indexType, typename Point::distanceType>>,
+                    std::vector<std::pair<indexType, typename Point::distanceType>>>,
+          indexType>
+beam_search(const Point p, const Graph<indexType> &G, const PointRange_ &Points,
+            const indexType starting_point, const QueryParams &QP) {
+  std::vector<indexType> start_points = {starting_point};
+  auto r = beam_search(p, G, Points, start_points, QP);
+  return {std::move(r.first), (indexType)r.second};
+}
+
+/* beam_search_rerank (beamSearch.h:528-597): the k nearest (id, distance)
+ * pairs.  The search gathers Q_Base_Points; when that is a reduced-precision
+ * copy, the first min(k * rerank_factor, |beam|) entries are re-scored
+ * against the float Base_Points and sorted.  QQ_Base_Points must be
+ * Q_Base_Points' kind (no filtering).  Batch queries through qsearchAll: one
+ * call is one kernel launch. */
+template <typename Point, typename QPoint, typename QQPoint, typename PointRange_,
+          typename QPointRange, typename QQPointRange, typename indexType>
+std::vector<std::pair<indexType, typename Point::distanceType>> beam_search_rerank(
+    const Point &p, const QPoint &qp, const QQPoint &qqp, const Graph<indexType> &G,
+    const PointRange_ &Base_Points, const QPointRange &Q_Base_Points,
+    const QQPointRange &QQ_Base_Points, stats<indexType> &QueryStats,
+    const std::vector<indexType> starting_points, const QueryParams &QP, bool stats = true) {
+  (void)qqp;
+  (void)QQ_Base_Points;
+  static_assert(QPoint::pann_store == QQPoint::pann_store,
+                "filtering with a third store is not supported");
+  if (Base_Points.base_serial() != Q_Base_Points.base_serial())
+    throw std::runtime_error("Q_Base_Points must be made of Base_Points");
+  const bool use_rerank = QPoint::pann_store != PANN_STORE_NATIVE;
+  pann_index_t h = detail::device_index(G, Q_Base_Points);
   const size_t k = (size_t)std::max<long>(QP.k, 0);
-  std::vector<uint32_t> ids(k);
-  std::vector<float> dists(k);
+  std::vector<uint32_t> ids(std::max<size_t>(k, 1));
+  std::vector<float> dists(std::max<size_t>(k, 1));
   uint32_t vis = 0, cmps = 0;
-  pann_qparams qp = detail::qparams(QP);
-  pann_search_opts o{so.store, so.summation, so.rerank ? 1 : 0, 0, 0, 0, so.exact_counts ? 1 : 0};
-  detail::check(pann_search_host(index.handle(), p, 1, &qp, starting_points.data(),
+  pann_qparams q = detail::qparams(QP);
+  pann_search_opts o = detail::opts(QPoint::pann_store, use_rerank);
+  detail::check(pann_search_host(h, qp.values, 1, &q, (const uint32_t *)starting_points.data(),
                                  (uint32_t)starting_points.size(), &o, ids.data(), dists.data(),
                                  &vis, &cmps));
-  if (query_id < QueryStats.visited.size()) {
-    QueryStats.increment_visited(query_id, vis);
-    QueryStats.increment_dist(query_id, cmps);
+  if (stats) {
+    QueryStats.increment_visited((indexType)p.id(), vis);
+    QueryStats.increment_dist((indexType)p.id(), cmps);
   }
-  std::vector<std::pair<uint32_t, float>> r(k);
-  for (size_t i = 0; i < k; i++) r[i] = {ids[i], dists[i]};
+  std::vector<std::pair<indexType, typename Point::distanceType>> r(k);
+  for (size_t i = 0; i < k; i++) r[i] = {(indexType)ids[i], dists[i]};
   return r;
 }
 
-/* beam_search (beamSearch.h:419-445) for one query: ((frontier, visited),
- * dist_cmps).  The frontier is the final beam, sorted by (distance, id); the
- * visited list is not materialised on this path and comes back empty. */
-template <typename T>
-std::pair<std::pair<std::vector<std::pair<uint32_t, float>>,
-                    std::vector<std::pair<uint32_t, float>>>,
-          size_t>
-beam_search(const T *p, DeviceIndex<T> &index, const std::vector<uint32_t> &starting_points,
-            const QueryParams &QP, const SearchOptions &so = SearchOptions()) {
-  const size_t B = (size_t)std::max<long>(QP.beamSize, 0);
-  std::vector<uint32_t> ids(B);
-  std::vector<float> dists(B);
-  uint32_t cnt = 0, vis = 0, cmps = 0;
-  pann_qparams qp = detail::qparams(QP);
-  pann_search_opts o{so.store, so.summation, 0, 0, 0, 0, so.exact_counts ? 1 : 0};
-  detail::check(pann_beam_search_host(index.handle(), p, 1, &qp, starting_points.data(),
-                                      (uint32_t)starting_points.size(), &o, ids.data(),
-                                      dists.data(), &cnt, &vis, &cmps));
-  std::vector<std::pair<uint32_t, float>> frontier(cnt);
-  for (uint32_t i = 0; i < cnt; i++) frontier[i] = {ids[i], dists[i]};
-  return {{std::move(frontier), {}}, (size_t)cmps};
+/* qsearchAll (beamSearch.h:599-646): every query from one start point, in one
+ * device launch; the k nearest ids per query, with visited and distance
+ * counts added to QueryStats when it is sized for the queries. */
+template <typename PointRange_, typename QPointRange, typename QQPointRange, typename indexType>
+std::vector<std::vector<indexType>> qsearchAll(
+    const PointRange_ &Query_Points, const QPointRange &Q_Query_Points,
+    const QQPointRange &QQ_Query_Points, const Graph<indexType> &G,
+    const PointRange_ &Base_Points, const QPointRange &Q_Base_Points,
+    const QQPointRange &QQ_Base_Points, stats<indexType> &QueryStats,
+    const indexType starting_point, const QueryParams &QP, bool random = false) {
+  (void)Q_Query_Points;
+  (void)QQ_Query_Points;
+  (void)QQ_Base_Points;
+  using QP_t = typename QPointRange::Point;
+  static_assert(QP_t::pann_store == QQPointRange::Point::pann_store,
+                "filtering with a third store is not supported");
+  if (QP.k > QP.beamSize)  // beamSearch.h:612-616
+    throw std::runtime_error("beam search parameter Q = " + std::to_string(QP.beamSize) +
+                             " same size or smaller than k = " + std::to_string(QP.k));
+  if (random)
+    throw std::runtime_error("qsearchAll: random start points are not supported");
+  if (Base_Points.base_serial() != Q_Base_Points.base_serial())
+    throw std::runtime_error("Q_Base_Points must be made of Base_Points");
+  const size_t nq = Query_Points.size();
+  const size_t k = (size_t)std::max<long>(QP.k, 0);
+  std::vector<std::vector<indexType>> all(nq);
+  if (nq == 0) return all;
+  pann_index_t h = detail::device_index(G, Q_Base_Points);
+  std::vector<uint32_t> ids(nq * std::max<size_t>(k, 1));
+  std::vector<float> dists(ids.size());
+  std::vector<uint32_t> vis(nq), cmps(nq);
+  pann_qparams q = detail::qparams(QP);
+  pann_search_opts o = detail::opts(QP_t::pann_store, QP_t::pann_store != PANN_STORE_NATIVE);
+  uint32_t start = (uint32_t)starting_point;
+  detail::check(pann_search_host(h, Query_Points.data(), (uint32_t)nq, &q, &start, 1, &o,
+                                 ids.data(), dists.data(), vis.data(), cmps.data()));
+  for (size_t i = 0; i < nq; i++) all[i].assign(ids.begin() + i * k, ids.begin() + (i + 1) * k);
+  if (QueryStats.visited.size() == nq) {
+    for (size_t i = 0; i < nq; i++) {
+      QueryStats.increment_visited((indexType)i, vis[i]);
+      QueryStats.increment_dist((indexType)i, cmps[i]);
+    }
+  }
+  return all;
 }
 
 }  // namespace parlayANN

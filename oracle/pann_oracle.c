@@ -2,10 +2,15 @@
  * pann_oracle.c -- CPU restatement of ParlayANN's query hot path.
  *
  * TEST INFRASTRUCTURE ONLY (see pann_oracle.h).  Built with
- * -O3 -march=native -ffp-contract=off so that every float sum is the
- * in-order, separately rounded sum that the reference's own build produces
- * for dimensions that are multiples of 8 (euclidian_point.h:83-90 and
- * mips_point.h:60-66 compiled by g++ -O3 -march=native).
+ * -ffp-contract=off, so every float operation below rounds where it is
+ * written: the float distances are the arithmetic that g++ 11.4 -O3
+ * -march=<any FMA target> makes of the reference's loops
+ * (euclidian_point.h:83-90, mips_point.h:60-66).  That compiler keeps the
+ * loop's order (no -fassociative-math): an 8-wide then a 4-wide vector body
+ * forms separately rounded products and adds them one at a time, and the
+ * last d % 4 terms run as scalar code whose multiply and add it contracts
+ * into one fused multiply-add.  tests/test_f32_reference_loop.py compiles
+ * the reference loop with those flags and checks this model at every d.
  */
 #define _GNU_SOURCE
 #include "pann_oracle.h"
@@ -17,17 +22,41 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* ORACLE_MUTANT plants one plausible mistake in the graph builders, so that
+ * the builder pin tests can show each one is caught (tests/test_builder_pins.py
+ * builds every variant with `make mutant`).  0, the default, is the oracle.
+ *   1  Vamana reverse edges: append only while newsize < R (index.h:295 has <=)
+ *   2  robustPrune: prune only when alpha * d(p*, p') < d(p, p') (index.h:111 has <=)
+ *   3  Vamana: every point in one batch (index.h:223-234 doubles the batches)
+ *   4  Vamana: batches of one point
+ *   5  robustPrune ignores p's current out-neighbours (index.h:72-77 adds them)
+ *   6  Vamana reverse edges: the current neighbours first, then the new ones
+ *      (add_neighbors_without_repeats appends to the new ones, index.h:140-146)
+ *   7  HNSW prune_heuristic: prune when d(c, r) <= alpha * d(c, u) (HNSW.hpp:473 has <)
+ *   8  HNSW reverse edges: cut when the total reaches m (HNSW.hpp:966 has >)
+ *   9  HNSW reverse edges: the cut keeps the farthest m (HNSW.hpp:975 keeps the nearest)
+ *   10 HNSW batches end at ceil(begin * base), without the + 1 (HNSW.hpp:781) */
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+
 /* ------------------------------------------------------------------------ */
 /* distances: euclidian_point.h:45-90, mips_point.h:44-66                    */
 /* ------------------------------------------------------------------------ */
 
-/* euclidian_point.h:83-90: result += (q[i]-p[i])*(q[i]-p[i]), in order. */
+/* euclidian_point.h:83-90: result += (q[i]-p[i])*(q[i]-p[i]), in order;
+ * separately rounded terms up to the last multiple of 4, fused ones after. */
 static float l2_f32(const float *p, const float *q, unsigned d) {
   float r = 0.0f;
-  for (unsigned i = 0; i < d; i++) {
+  const unsigned body = d - d % 4;
+  for (unsigned i = 0; i < body; i++) {
     float t = q[i] - p[i];
     float s = t * t;
     r = r + s;
+  }
+  for (unsigned i = body; i < d; i++) {
+    float t = q[i] - p[i];
+    r = fmaf(t, t, r);
   }
   return r;
 }
@@ -52,13 +81,16 @@ static float l2_i8(const int8_t *p, const int8_t *q, unsigned d) {
   return (float)r;
 }
 
-/* mips_point.h:60-66: result += q[i]*p[i] in order; return -result. */
+/* mips_point.h:60-66: result += q[i]*p[i] in order; return -result.  The
+ * last d % 4 terms fused, as for l2_f32. */
 static float ip_f32(const float *p, const float *q, unsigned d) {
   float r = 0.0f;
-  for (unsigned i = 0; i < d; i++) {
+  const unsigned body = d - d % 4;
+  for (unsigned i = 0; i < body; i++) {
     float s = q[i] * p[i];
     r = r + s;
   }
+  for (unsigned i = body; i < d; i++) r = fmaf(q[i], p[i], r);
   return -r;
 }
 
@@ -307,6 +339,12 @@ typedef struct {
   int grew;
 } bs_result;
 
+/* The reference's software prefetches (graph.h:109-113, euclidian_point.h:135-139,
+ * mips_point.h:94-98): every 64-byte line of a graph row or point row. */
+static inline void prefetch_lines(const void *p, size_t bytes) {
+  for (size_t o = 0; o < bytes; o += 64) __builtin_prefetch((const char *)p + o);
+}
+
 static int bs_core(bs_scratch *S, const oracle_graph *G, const oracle_points *P, const void *q,
                    long self_id, const uint32_t *starts, int nstarts, const oracle_qparams *QP,
                    bs_result *res) {
@@ -342,6 +380,8 @@ static int bs_core(bs_scratch *S, const oracle_graph *G, const oracle_points *P,
 
   while (remain > offset && num_visited < QP->limit) { /* :116 */
     idd current = S->unvisited[offset]; /* :119 */
+    prefetch_lines(G->adj + (size_t)current.id * (G->max_degree + 1),
+                   ((size_t)G->max_degree + 1) * 4); /* :124 */
     /* visited.insert at upper_bound (:125-126) */
     S->visited = (idd *)grow(S->visited, &S->cap_v, vsize + 1, sizeof(idd));
     long lo = 0, hi = vsize;
@@ -364,6 +404,7 @@ static int bs_core(bs_scratch *S, const oracle_graph *G, const oracle_points *P,
     for (long i = 0; i < num_elts; i++) {
       uint32_t a = row[1 + i];
       if (hs_test(&hs, a) || (self_id >= 0 && (long)a == self_id)) continue;
+      prefetch_lines(row_of(P, a), (size_t)P->dim * (P->dtype == OR_F32 ? 4 : 1)); /* :157 */
       S->pruned[npruned++] = a;
     }
     dist_cmps += (uint64_t)npruned; /* :160 */
@@ -595,7 +636,7 @@ static int robust_prune_core(const uint32_t *adj, unsigned maxdeg, const oracle_
                              uint32_t *out, long *dist_comps) {
   long comps = 0;
   const uint32_t *row = adj + (size_t)p * (maxdeg + 1);
-  if (add) { /* :72-77 */
+  if (add && ORACLE_MUTANT != 5) { /* :72-77 */
     for (uint32_t i = 0; i < row[0]; i++) {
       comps++;
       cands[nc].id = row[1 + i];
@@ -620,7 +661,9 @@ static int robust_prune_core(const uint32_t *adj, unsigned maxdeg, const oracle_
         float d_sp = oracle_distance(row_of(P, p_star), row_of(P, p_prime), P->dim, P->dtype,
                                      P->metric);
         float d_pp = cands[i].dist;
-        if (alpha * (double)d_sp <= (double)d_pp) cands[i].id = 0xFFFFFFFFu; /* :111 */
+        const int prune = ORACLE_MUTANT == 2 ? alpha * (double)d_sp < (double)d_pp
+                                             : alpha * (double)d_sp <= (double)d_pp; /* :111 */
+        if (prune) cands[i].id = 0xFFFFFFFFu;
       }
     }
   }
@@ -699,19 +742,29 @@ static void bi_group_body(void *vctx, long g, int tid) {
   long nc = e - s;
   uint32_t *row = c->adj + (size_t)index * (c->R + 1);
   long newsize = nc + (long)row[0];
-  if (newsize <= (long)c->R) { /* :295-297 add_neighbors_without_repeats */
+  if (ORACLE_MUTANT == 1 ? newsize < (long)c->R
+                         : newsize <= (long)c->R) { /* :295-297 add_neighbors_without_repeats */
     uint32_t tmp[1024];
     long t = 0;
-    for (long i = 0; i < nc; i++) tmp[t++] = c->gvals[s + i];
-    for (uint32_t i = 0; i < row[0]; i++) {
-      uint32_t x = row[1 + i];
-      int found = 0;
-      for (long j = 0; j < nc; j++)
-        if (c->gvals[s + j] == x) {
-          found = 1;
-          break;
-        }
-      if (!found) tmp[t++] = x;
+    if (ORACLE_MUTANT == 6) {
+      for (uint32_t i = 0; i < row[0]; i++) tmp[t++] = row[1 + i];
+      for (long i = 0; i < nc; i++) {
+        int found = 0;
+        for (uint32_t j = 0; j < row[0]; j++) found |= row[1 + j] == c->gvals[s + i];
+        if (!found) tmp[t++] = c->gvals[s + i];
+      }
+    } else {
+      for (long i = 0; i < nc; i++) tmp[t++] = c->gvals[s + i];
+      for (uint32_t i = 0; i < row[0]; i++) {
+        uint32_t x = row[1 + i];
+        int found = 0;
+        for (long j = 0; j < nc; j++)
+          if (c->gvals[s + j] == x) {
+            found = 1;
+            break;
+          }
+        if (!found) tmp[t++] = x;
+      }
     }
     row[0] = (uint32_t)t;
     for (long i = 0; i < t; i++) row[1 + i] = tmp[i];
@@ -751,6 +804,15 @@ static int batch_insert(bi_ctx *c, size_t m, int nthreads) {
   if (max_batch_size == 0) max_batch_size = n;
   while (count < m) { /* :223-234 */
     size_t floor_, ceiling;
+    if (ORACLE_MUTANT == 3) {
+      floor_ = 0;
+      ceiling = m;
+      count = m;
+    } else if (ORACLE_MUTANT == 4) {
+      floor_ = count;
+      ceiling = count + 1;
+      count += 1;
+    } else
     if (pow(base, (double)inc) <= (double)max_batch_size) {
       floor_ = (size_t)pow(base, (double)inc) - 1;
       size_t c2 = (size_t)pow(base, (double)(inc + 1)) - 1;
@@ -1115,7 +1177,7 @@ static int hnsw_prune(const oracle_points *P, const idd *cand, long nc, unsigned
     for (int s = 0; s < nsel; s++) {
       float d_cr = oracle_distance(row_of(P, out[s]), row_of(P, cand[t].id), P->dim, P->dtype,
                                    P->metric);
-      if (d_cr < d_cu) {
+      if (ORACLE_MUTANT == 7 ? d_cr <= d_cu : d_cr < d_cu) { /* :473 */
         pruned = 1;
         break;
       }
@@ -1164,7 +1226,7 @@ static void hr_body(void *vctx, long g, int tid) {
   long s = c->gstart[g], e = c->gstart[g + 1];
   uint32_t *row = H->adj[lc] + (size_t)pv * (md + 1);
   long deg = row[0], nadd = e - s;
-  if (deg + nadd > (long)md) {
+  if (ORACLE_MUTANT == 8 ? deg + nadd >= (long)md : deg + nadd > (long)md) { /* :966 */
     c->buf[tid] = (idd *)grow(c->buf[tid], &c->cap[tid], deg + nadd, sizeof(idd));
     idd *cand = c->buf[tid];
     for (long k = 0; k < deg; k++) {
@@ -1179,7 +1241,8 @@ static void hr_body(void *vctx, long g, int tid) {
     }
     idd_sort(cand, deg + nadd);
     row[0] = md;
-    for (unsigned k = 0; k < md; k++) row[1 + k] = cand[k].id;
+    for (unsigned k = 0; k < md; k++)
+      row[1 + k] = ORACLE_MUTANT == 9 ? cand[deg + nadd - 1 - k].id : cand[k].id;
   } else {
     for (long k = 0; k < nadd; k++) row[1 + deg + k] = c->gvals[s + k];
     row[0] = (uint32_t)(deg + nadd);
@@ -1217,7 +1280,7 @@ int oracle_build_hnsw(const oracle_hnsw *H, const oracle_points *P, const uint32
   size_t cap_edges = 0;
   while (be < n) { /* :777-792 */
     bb = be;
-    uint32_t b1 = (uint32_t)ceilf((float)bb * batch_base) + 1;
+    uint32_t b1 = (uint32_t)ceilf((float)bb * batch_base) + (ORACLE_MUTANT == 10 ? 0 : 1);
     be = n;
     if (b1 < be) be = b1;
     if (bb + size_limit < be) be = bb + size_limit;

@@ -8,9 +8,11 @@
  *
  * Every function cites the reference file:line it restates (paths relative to
  * the ParlayANN source tree).  Arithmetic follows the reference exactly:
- *   - float L2/IP: strictly in-order left-to-right sum of separately rounded
- *     terms (the codegen of g++ -O3 -march=native for d % 8 == 0, without
- *     fp contraction), built here with -ffp-contract=off;
+ *   - float L2/IP: strictly in-order left-to-right sum; the terms up to the
+ *     last multiple of 4 are separately rounded products added one at a
+ *     time, the last d % 4 terms are fused multiply-adds -- the codegen of
+ *     g++ 11.4 -O3 -march=<FMA target> for the reference's loops (built here
+ *     with -ffp-contract=off, the fused terms written as fmaf);
  *   - uint8/int8: int32 accumulate then (float) conversion.
  */
 #ifndef PANN_ORACLE_H

@@ -10,6 +10,7 @@ vamana/index.h:63-319); see ``pann_oracle.h`` for the per-function citations.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -65,7 +66,25 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
+        _lib = _configure(C.CDLL(_LIB_PATH))
+    return _lib
+
+
+@contextlib.contextmanager
+def use_library(path: str):
+    """Runs the functions of this module on another build of pann_oracle.c
+    (a variant with a planted mistake, built by `make -C oracle mutant`)."""
+    global _lib
+    saved = lib()
+    _lib = _configure(C.CDLL(path))
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def _configure(L):
+    if True:
         vp, sz, u32p, f32p = C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p
         L.oracle_distance.restype = C.c_float
         L.oracle_distance.argtypes = [vp, vp, C.c_uint, C.c_int, C.c_int]
@@ -112,8 +131,7 @@ def lib():
         L.oracle_build_hnsw.restype = C.c_int
         L.oracle_build_hnsw.argtypes = [C.POINTER(_HNSW), C.POINTER(_Points), u32p, C.c_uint,
                                         C.c_float, C.c_float, u32p, C.c_int]
-        _lib = L
-    return _lib
+    return L
 
 
 def _ptr(a: np.ndarray):

@@ -31,6 +31,7 @@
 #include <float.h>
 
 #include <atomic>
+#include <mutex>
 
 namespace pann {
 
@@ -776,13 +777,19 @@ static hipError_t launch_k(const SearchArgs& a, uint32_t grid, hipStream_t s) {
   uint32_t lds = search_lds_bytes(a);
   auto kern = k_beam_search<ST, MT, NCH, SEQ, EX, FB>;
   // raise the kernel's dynamic-LDS limit only when a launch needs more, so a
-  // repeated launch makes no attribute call (it may be under graph capture)
+  // repeated launch makes no attribute call (it may be under graph capture);
+  // the check, the call and the record are one step under a lock, so two host
+  // threads cannot leave the record above the attribute
   static std::atomic<uint32_t> lds_set{0};
   if (lds > lds_set.load()) {
-    hipError_t e = hipFuncSetAttribute((const void*)kern,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    lds_set.store(lds);
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    if (lds > lds_set.load()) {
+      hipError_t e = hipFuncSetAttribute((const void*)kern,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      lds_set.store(lds);
+    }
   }
   hipLaunchKernelGGL(kern, dim3(grid), dim3(PANN_WAVE), lds, s, a);
   return hipGetLastError();

@@ -6,13 +6,15 @@
 //    per instruction; partial sums are reduced with DPP inside the 8-lane
 //    group.  Integer (uint8/int8) sums are exact in any order, so group mode is
 //    bit-exact for them (euclidian_point.h:45-81, mips_point.h:44-58).
-//  * in-order: terms summed strictly in index order with separately rounded
-//    multiply and add -- the arithmetic of the reference's float loop
-//    (euclidian_point.h:83-90, mips_point.h:60-66), so FP32 results are
-//    bit-identical to it.  Rows of up to 8 lines keep the group gather and
-//    carry the running sum from lane to lane; longer rows go 8 lines at a
-//    time with the query in LDS.  One lane per row remains for the rerank of
-//    the FP32 store's kernels and of the FP8 L2 kernels.
+//  * in-order: terms summed strictly in index order, the arithmetic of the
+//    reference's float loop (euclidian_point.h:83-90, mips_point.h:60-66) as
+//    g++ -O3 -march=<FMA target> compiles it: separately rounded products
+//    added one at a time up to the last multiple of 4 terms, the last d % 4
+//    terms as fused multiply-adds.  So FP32 results are bit-identical to
+//    it.  Rows of up to 8 lines keep the group gather and carry the running
+//    sum from lane to lane; longer rows go 8 lines at a time with the query
+//    in LDS.  One lane per row remains for the rerank of the FP32 store's
+//    kernels and of the FP8 L2 kernels.
 #pragma once
 #include "device_common.h"
 
@@ -340,54 +342,78 @@ __device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, cons
 // the lanes in element order: at step (c, s) lane s takes the sum from lane
 // s - 1 (DPP row_shr:1) or, at s = 0, from lane 7 of its own group, which
 // ended chunk c - 1 (row_ror:9 maps lane 7 to lane 0 and lane 15 to lane 8 of
-// each 16-lane row), and adds its four terms one rounding at a time.  Lane 7
-// ends with the row's sum.  Every other lane computes a discarded value.  The
-// terms and their order are those of seq_f32_row below, and the zero padding
-// past dim adds exact zeros to a sum that is never -0, so the result is
-// bit-identical to it and to the reference's float loop.
+// each 16-lane row), and adds its four terms in order.  Lane 7 ends with the
+// row's sum.  Every other lane computes a discarded value.  Each term is a
+// separately rounded product added with one rounding, except in the group of
+// four that holds the last dim % 4 elements (group dim / 4, when dim % 4 != 0),
+// whose terms are fused multiply-adds -- the reference loop's compiled
+// arithmetic.  The zero padding past dim adds exact zeros to a sum that is
+// never -0, so the result is bit-identical to seq_f32_row below and to the
+// reference's float loop.
 // ---------------------------------------------------------------------------
-// The four separately rounded terms of one 16-byte chunk (no fused multiply-add).
+// The operands (a, b) of the four terms a * b of one 16-byte chunk.
 template <int MT>
-__device__ __forceinline__ void inorder_terms(const float* q, const uint4& raw, float* t) {
+__device__ __forceinline__ void inorder_ops(const float* q, const uint4& raw, float* a, float* b) {
 #pragma clang fp contract(off)
   const float p[4] = {__uint_as_float(raw.x), __uint_as_float(raw.y), __uint_as_float(raw.z),
                       __uint_as_float(raw.w)};
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     if (MT == M_L2) {
-      const float df = q[e] - p[e];
-      t[e] = df * df;
+      a[e] = q[e] - p[e];
+      b[e] = a[e];
     } else {
-      t[e] = q[e] * p[e];
+      a[e] = q[e];
+      b[e] = p[e];
     }
   }
 }
 
-// The eight steps of one 128-byte chunk; `first` starts the sum at zero.
-__device__ __forceinline__ float inorder_chunk(float acc, const float* t, bool first) {
+// The fused group of a row of `dim` floats, as (chunk, step): dim / 4 = c * 8
+// + s; (-1, -1) when dim % 4 == 0.
+struct FusedGroup {
+  int c, s;
+  __host__ __device__ explicit FusedGroup(int dim)
+      : c(dim % 4 ? (dim / 4) / 8 : -1), s(dim % 4 ? (dim / 4) % 8 : -1) {}
+};
+
+// The eight steps of one 128-byte chunk; `first` starts the sum at zero, and
+// step `fs` (uniform, -1 for none) adds its terms fused.
+__device__ __forceinline__ float inorder_chunk(float acc, const float* a, const float* b,
+                                               bool first, int fs) {
 #pragma clang fp contract(off)
+  float t[4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) t[e] = a[e] * b[e];
 #pragma unroll
   for (int s = 0; s < 8; s++) {
     float in;
     if (s == 0) in = first ? 0.0f : __int_as_float(dpp_mov<0x129>(__float_as_int(acc)));
     else in = __int_as_float(dpp_mov<0x111>(__float_as_int(acc)));
-    acc = in + t[0];
-    acc = acc + t[1];
-    acc = acc + t[2];
-    acc = acc + t[3];
+    if (s == fs) {
+      acc = fmaf(a[0], b[0], in);
+      acc = fmaf(a[1], b[1], acc);
+      acc = fmaf(a[2], b[2], acc);
+      acc = fmaf(a[3], b[3], acc);
+    } else {
+      acc = in + t[0];
+      acc = acc + t[1];
+      acc = acc + t[2];
+      acc = acc + t[3];
+    }
   }
   return acc;
 }
 
 template <int MT, int NCH>
 __device__ __forceinline__ float group_inorder_sum(const GroupDist<ST_F32, MT, NCH>& D,
-                                                   const uint4* raw) {
-  float t[NCH][4];
+                                                   const uint4* raw, FusedGroup fg) {
+  float a[NCH][4], b[NCH][4];
 #pragma unroll
-  for (int c = 0; c < NCH; c++) inorder_terms<MT>(D.q[c], raw[c], t[c]);
+  for (int c = 0; c < NCH; c++) inorder_ops<MT>(D.q[c], raw[c], a[c], b[c]);
   float acc = 0.0f;
 #pragma unroll
-  for (int c = 0; c < NCH; c++) acc = inorder_chunk(acc, t[c], c == 0);
+  for (int c = 0; c < NCH; c++) acc = inorder_chunk(acc, a[c], b[c], c == 0, c == fg.c ? fg.s : -1);
   return MT == M_L2 ? acc : -acc;
 }
 
@@ -402,6 +428,7 @@ __device__ __forceinline__ void eval_group_inorder_lds(const float* QL, int dim,
                                                        int ncmp, float* DL, int lane) {
   const int sub = lane & 7, grp = lane >> 3;
   const int nl = (dim * 4 + 127) >> 7;  // lines per row, taken MAXC at a time
+  const FusedGroup fg(dim);
   for (int r0 = 0; r0 < ncmp; r0 += 8 * UP) {
     const char* row[UP];
 #pragma unroll
@@ -428,11 +455,12 @@ __device__ __forceinline__ void eval_group_inorder_lds(const float* QL, int dim,
             q[e] = j < dim ? QL[j] : 0.0f;
           }
           const bool first = c0 + c == 0;
+          const int fs = c0 + c == fg.c ? fg.s : -1;
 #pragma unroll
           for (int u = 0; u < UP; u++) {
-            float t[4];
-            inorder_terms<MT>(q, raw[u][c], t);
-            acc[u] = inorder_chunk(first ? 0.0f : acc[u], t, first);
+            float a[4], b[4];
+            inorder_ops<MT>(q, raw[u][c], a, b);
+            acc[u] = inorder_chunk(first ? 0.0f : acc[u], a, b, first, fs);
           }
         }
       }
@@ -455,6 +483,7 @@ __device__ __forceinline__ void eval_group_inorder(const GroupDist<ST_F32, MT, N
   constexpr int UP = NCH <= 2 ? 8 : (NCH <= 4 ? 4 : 2);
   const int sub = lane & 7, grp = lane >> 3;
   const int nl = (row_bytes + 127) >> 7;
+  const FusedGroup fg(row_bytes >> 2);
   for (int r0 = 0; r0 < ncmp; r0 += 8 * UP) {
     uint4 raw[UP][NCH];
 #pragma unroll
@@ -472,15 +501,16 @@ __device__ __forceinline__ void eval_group_inorder(const GroupDist<ST_F32, MT, N
 #pragma unroll
     for (int u = 0; u < UP; u++) {
       const int r = r0 + u * 8 + grp;
-      const float s = group_inorder_sum<MT, NCH>(D, raw[u]);
+      const float s = group_inorder_sum<MT, NCH>(D, raw[u], fg);
       if (sub == 7 && r < ncmp) DL[r] = s;
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Sequential FP32 (one lane per row, in-order sum of separately rounded terms;
-// the reference's float loop).  The query sits in LDS (QL, dim floats).
+// Sequential FP32 (one lane per row, in-order sum; the reference's float loop
+// as compiled: separately rounded terms up to the last multiple of 4, the
+// last dim % 4 terms fused).  The query sits in LDS (QL, dim floats).
 // ---------------------------------------------------------------------------
 template <int MT>
 __device__ __forceinline__ float seq_f32_row(const float* QL, const float* row, int dim) {
@@ -505,15 +535,13 @@ __device__ __forceinline__ float seq_f32_row(const float* QL, const float* row, 
     acc = acc + t2;
     acc = acc + t3;
   }
-  for (; j < dim; j++) {
-    float t;
+  for (; j < dim; j++) {  // the last dim % 4 terms, fused
     if (MT == M_L2) {
-      t = QL[j] - row[j];
-      t = t * t;
+      const float t = QL[j] - row[j];
+      acc = fmaf(t, t, acc);
     } else {
-      t = QL[j] * row[j];
+      acc = fmaf(QL[j], row[j], acc);
     }
-    acc = acc + t;
   }
   return MT == M_L2 ? acc : -acc;
 }

@@ -10,6 +10,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -40,6 +41,15 @@ pann_status hip_fail(hipError_t e, const char* where) {
     hipError_t _e = (expr);                  \
     if (_e != hipSuccess) return hip_fail(_e, where); \
   } while (0)
+
+// Every device allocation the library makes goes through here, so that
+// pann_device_alloc_count can show a caller that steady-state calls make none.
+std::atomic<uint64_t> g_device_allocs{0};
+template <typename T>
+hipError_t dev_malloc(T** p, size_t bytes) {
+  g_device_allocs.fetch_add(1, std::memory_order_relaxed);
+  return hipMalloc((void**)p, bytes);
+}
 
 uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
 uint32_t ceil_log2(uint64_t x) {
@@ -157,6 +167,10 @@ struct pann_index_s {
     size_t qtable_cap = 0;
     rocblas_handle blas = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the host-buffer entry points' device staging (queries in, results out),
+    // grown geometrically so that repeated calls allocate nothing
+    char* d_stage = nullptr;
+    size_t stage_cap = 0;
   };
   std::unordered_map<hipStream_t, StreamCtx> streams;
   StreamCtx* last_timed = nullptr;
@@ -168,6 +182,9 @@ struct pann_index_s {
   int64_t lds_per_cu = 0;
   int64_t max_lds_block = 0;
   std::mutex mu;
+  // serialises the host-buffer entry points, which share the default stream's
+  // staging buffer
+  std::mutex host_mu;
 };
 
 namespace {
@@ -189,9 +206,9 @@ pann_status stream_ctx(pann_index_t idx, hipStream_t s, StreamCtx** out) {
       return fail(PANN_ERR_ARG, "first use of a stream under graph capture: run the same "
                                 "search once on that stream before capturing it");
     StreamCtx c;
-    hipError_t e = hipMalloc(&c.d_counter, 256);
-    if (e == hipSuccess) e = hipMalloc(&c.d_status, 256);
-    if (e == hipSuccess) e = hipMalloc(&c.d_starts, 64 * 4);
+    hipError_t e = dev_malloc(&c.d_counter, 256);
+    if (e == hipSuccess) e = dev_malloc(&c.d_status, 256);
+    if (e == hipSuccess) e = dev_malloc(&c.d_starts, 64 * 4);
     if (e == hipSuccess) e = hipEventCreate(&c.ev0);
     if (e == hipSuccess) e = hipEventCreate(&c.ev1);
     if (e != hipSuccess) {
@@ -214,7 +231,7 @@ pann_status ensure_ghash(StreamCtx& C, size_t words, uint32_t nq) {
     if (C.d_ghash) (void)hipFree(C.d_ghash);
     C.d_ghash = nullptr;
     C.ghash_words = 0;
-    HIP_TRY(hipMalloc(&C.d_ghash, words * 8), "hipMalloc(visited overflow table)");
+    HIP_TRY(dev_malloc(&C.d_ghash, words * 8), "hipMalloc(visited overflow table)");
     HIP_TRY(hipMemset(C.d_ghash, 0, words * 8), "hipMemset(visited overflow table)");
     HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
     C.ghash_words = words;
@@ -228,12 +245,26 @@ pann_status ensure_ghash(StreamCtx& C, size_t words, uint32_t nq) {
   return PANN_OK;
 }
 
+// Staging of at least `bytes` for the host-buffer entry points: grown to
+// twice the old size at least, never shrunk.  The caller holds idx->host_mu.
+pann_status ensure_stage(StreamCtx& C, size_t bytes) {
+  if (C.stage_cap >= bytes) return PANN_OK;
+  const size_t cap = std::max<size_t>(bytes, 2 * C.stage_cap);
+  HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");  // the old buffer may be in use
+  if (C.d_stage) (void)hipFree(C.d_stage);
+  C.d_stage = nullptr;
+  C.stage_cap = 0;
+  HIP_TRY(dev_malloc(&C.d_stage, cap), "hipMalloc(host staging)");
+  C.stage_cap = cap;
+  return PANN_OK;
+}
+
 pann_status ensure_vis(StreamCtx& C, size_t words) {
   if (C.vis_words >= words) return PANN_OK;
   if (C.d_vis) (void)hipFree(C.d_vis);
   C.d_vis = nullptr;
   C.vis_words = 0;
-  HIP_TRY(hipMalloc(&C.d_vis, words * 8), "hipMalloc(visited lists)");
+  HIP_TRY(dev_malloc(&C.d_vis, words * 8), "hipMalloc(visited lists)");
   C.vis_words = words;
   return PANN_OK;
 }
@@ -245,6 +276,12 @@ __global__ void k_zero_status(uint32_t* p) { p[threadIdx.x] = 0u; }
 extern "C" {
 
 const char* pann_last_error(void) { return g_err.c_str(); }
+
+pann_status pann_device_alloc_count(uint64_t* count) {
+  if (!count) return fail(PANN_ERR_ARG, "null argument");
+  *count = g_device_allocs.load(std::memory_order_relaxed);
+  return PANN_OK;
+}
 
 pann_status pann_device_info(int32_t device, int32_t* num_cus, int64_t* lds_per_cu,
                              int64_t* hbm_bytes) {
@@ -301,7 +338,7 @@ pann_status pann_open(const pann_index_desc* desc, pann_index_t* out) {
     return st;
   };
   pann_status st;
-  e = hipMalloc(&idx->d_pts, (size_t)n * idx->pts_stride);
+  e = dev_malloc(&idx->d_pts, (size_t)n * idx->pts_stride);
   if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(points)"));
   {
     // rows padded with zeros on the host, uploaded in chunks with one plain
@@ -320,7 +357,7 @@ pann_status pann_open(const pann_index_desc* desc, pann_index_t* out) {
       if (e != hipSuccess) return cleanup(hip_fail(e, "hipMemcpy(points)"));
     }
   }
-  e = hipMalloc(&idx->d_nbrs, (size_t)n * idx->nbr_stride * 4);
+  e = dev_malloc(&idx->d_nbrs, (size_t)n * idx->nbr_stride * 4);
   if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(graph)"));
   st = upload_graph(n, md, idx->nbr_stride, desc->graph, idx->d_nbrs);
   if (st != PANN_OK) return cleanup(st);
@@ -328,7 +365,7 @@ pann_status pann_open(const pann_index_desc* desc, pann_index_t* out) {
   idx->levels[0].d_nbrs = idx->d_nbrs;
   idx->levels[0].max_degree = md;
   idx->levels[0].nbr_stride = idx->nbr_stride;
-  e = hipMalloc(&idx->d_dummy, 256);
+  e = dev_malloc(&idx->d_dummy, 256);
   if (e != hipSuccess) return cleanup(hip_fail(e, "scratch allocation"));
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return cleanup(hip_fail(e, "hipDeviceSynchronize"));
@@ -349,9 +386,9 @@ pann_status build_q8(pann_index_t idx, uint64_t fstride) {
     for (void* p : {(void*)d_mins, (void*)d_maxs, (void*)d_flag})
       if (p) (void)hipFree(p);
   };
-  hipError_t e = hipMalloc(&d_mins, (size_t)n * 4);
-  if (e == hipSuccess) e = hipMalloc(&d_maxs, (size_t)n * 4);
-  if (e == hipSuccess) e = hipMalloc(&d_flag, 256);
+  hipError_t e = dev_malloc(&d_mins, (size_t)n * 4);
+  if (e == hipSuccess) e = dev_malloc(&d_maxs, (size_t)n * 4);
+  if (e == hipSuccess) e = dev_malloc(&d_flag, 256);
   if (e == hipSuccess)
     e = pann::launch_row_minmax((const float*)idx->d_pts, fstride, n, idx->dim, d_mins, d_maxs,
                                 d_flag, nullptr);
@@ -385,7 +422,7 @@ pann_status build_q8(pann_index_t idx, uint64_t fstride) {
   }
   if (!idx->d_q8) {
     idx->q8_stride = round_up(idx->dim, 128);
-    HIP_TRY(hipMalloc(&idx->d_q8, (size_t)n * idx->q8_stride), "hipMalloc(q8)");
+    HIP_TRY(dev_malloc(&idx->d_q8, (size_t)n * idx->q8_stride), "hipMalloc(q8)");
     HIP_TRY(hipMemset(idx->d_q8, 0, (size_t)n * idx->q8_stride), "hipMemset(q8)");
     HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
   }
@@ -415,7 +452,7 @@ pann_status stage_queries(pann_index_t idx, StreamCtx& C, int32_t store, const v
       }
       C.d_qtable = nullptr;
       C.qtable_cap = 0;
-      HIP_TRY(hipMalloc(&C.d_qtable, need * 4), "hipMalloc(centroid table)");
+      HIP_TRY(dev_malloc(&C.d_qtable, need * 4), "hipMalloc(centroid table)");
       C.qtable_cap = need;
     }
     if (!C.blas && rocblas_create_handle(&C.blas) != rocblas_status_success) {
@@ -444,7 +481,7 @@ pann_status stage_queries(pann_index_t idx, StreamCtx& C, int32_t store, const v
     }
     C.d_qcodes = nullptr;
     C.qcodes_cap = 0;
-    HIP_TRY(hipMalloc(&C.d_qcodes, need), "hipMalloc(query codes)");
+    HIP_TRY(dev_malloc(&C.d_qcodes, need), "hipMalloc(query codes)");
     C.qcodes_cap = need;
   }
   HIP_TRY(pann::launch_encode_q8((const float*)queries, idx->dim, nq, idx->dim,
@@ -488,7 +525,7 @@ pann_status pann_build_store(pann_index_t idx, int32_t store) {
   } else if (store == PANN_STORE_BF16) {
     if (!idx->d_bf16) {
       idx->bf16_stride = round_up((uint64_t)idx->dim * 2, 128);
-      HIP_TRY(hipMalloc(&idx->d_bf16, (size_t)idx->n * idx->bf16_stride), "hipMalloc(bf16)");
+      HIP_TRY(dev_malloc(&idx->d_bf16, (size_t)idx->n * idx->bf16_stride), "hipMalloc(bf16)");
       HIP_TRY(hipMemset(idx->d_bf16, 0, (size_t)idx->n * idx->bf16_stride), "hipMemset(bf16)");
       HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
     }
@@ -498,11 +535,11 @@ pann_status pann_build_store(pann_index_t idx, int32_t store) {
   } else {
     if (!idx->d_fp8) {
       idx->fp8_stride = round_up((uint64_t)idx->dim, 128);
-      HIP_TRY(hipMalloc(&idx->d_fp8, (size_t)idx->n * idx->fp8_stride), "hipMalloc(fp8)");
+      HIP_TRY(dev_malloc(&idx->d_fp8, (size_t)idx->n * idx->fp8_stride), "hipMalloc(fp8)");
       HIP_TRY(hipMemset(idx->d_fp8, 0, (size_t)idx->n * idx->fp8_stride), "hipMemset(fp8)");
       HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
-      HIP_TRY(hipMalloc(&idx->d_scales, idx->dim * 4), "hipMalloc(scales)");
-      HIP_TRY(hipMalloc(&idx->d_absmax, idx->dim * 4), "hipMalloc(absmax)");
+      HIP_TRY(dev_malloc(&idx->d_scales, idx->dim * 4), "hipMalloc(scales)");
+      HIP_TRY(dev_malloc(&idx->d_absmax, idx->dim * 4), "hipMalloc(absmax)");
     }
     HIP_TRY(pann::launch_fp8_store((const float*)idx->d_pts, fstride, idx->n, idx->dim,
                                    idx->d_absmax, idx->d_scales, (uint8_t*)idx->d_fp8,
@@ -603,6 +640,11 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   }
   if (o.rerank && idx->dtype != PANN_DT_F32)
     return fail(PANN_ERR_ARG, "rerank needs float points");
+  // the rerank keeps the first k of min(k * rerank_factor, |beam|) re-scored
+  // entries (beamSearch.h:571-584): fewer than k would leave result slots
+  // unwritten (the reference reads past its vector there)
+  if (o.rerank && qp->rerank_factor < 1.0)
+    return fail(PANN_ERR_ARG, "rerank needs rerank_factor >= 1");
   if (!sequential && row_bytes > 8 * 128)
     return fail(PANN_ERR_UNSUPPORTED, "rows wider than 1024 bytes need the in-order f32 path");
 
@@ -749,14 +791,21 @@ pann_status check_status(StreamCtx& C, hipStream_t stream, bool allow_short = fa
 
 namespace {
 
-// pann_search with two extras: the frontier size per query (out_nfront) and
-// whether a frontier shorter than k is a result rather than an error.
+// What the frontier and visited-list entry points add to pann_search.
+struct SearchExtras {
+  uint32_t* out_nfront = nullptr;  // device: frontier size per query
+  bool allow_short = false;        // a frontier shorter than k is a result, not an error
+  uint32_t vis_cap = 0;            // > 0: write each query's visited list, sorted by
+  uint32_t* vis_ids = nullptr;     //   (dist, id), to vis_ids / vis_dists (device,
+  float* vis_dists = nullptr;      //   nq x vis_cap; counts in out_visited)
+};
+
 pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
                         const pann_qparams* qp, const int32_t* k_pq, const int32_t* beam_pq,
                         const uint32_t* starts, uint32_t nstarts, const pann_search_opts* opts,
                         uint32_t* out_ids, float* out_dists, uint32_t out_stride,
-                        uint32_t* out_visited, uint32_t* out_cmps, uint32_t* out_nfront,
-                        bool allow_short, void* stream_) {
+                        uint32_t* out_visited, uint32_t* out_cmps, const SearchExtras& ex,
+                        void* stream_) {
   if (!idx || !qp || !starts) return fail(PANN_ERR_ARG, "null argument");
   {
     pann_status qst = check_qparams(qp);
@@ -790,8 +839,16 @@ pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
   a.out_stride = out_stride;
   a.out_visited = out_visited;
   a.out_cmps = out_cmps;
-  a.out_nfront = out_nfront;
-  a.short_ok = allow_short ? 1u : 0u;
+  a.out_nfront = ex.out_nfront;
+  a.short_ok = ex.allow_short ? 1u : 0u;
+  if (ex.vis_cap) {
+    if (!out_visited || !ex.vis_ids || !ex.vis_dists)
+      return fail(PANN_ERR_ARG, "visited lists need out_visited, ids and distances");
+    st = ensure_vis(*C, (size_t)nq * ex.vis_cap);
+    if (st != PANN_OK) return st;
+    a.vis_out = C->d_vis;
+    a.vis_cap = ex.vis_cap;
+  }
   st = stage_queries(idx, *C, opts ? opts->store : PANN_STORE_NATIVE, queries, nq, a, stream);
   if (st != PANN_OK) return st;
   grid = std::min<uint32_t>(grid, nq);
@@ -805,8 +862,12 @@ pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
     HIP_TRY(hipEventRecord(C->ev1, stream), "hipEventRecord");
     idx->last_timed = C;
   }
+  if (ex.vis_cap)
+    HIP_TRY(pann::launch_sort_visited(C->d_vis, nq, ex.vis_cap, out_visited, ex.vis_ids,
+                                      ex.vis_dists, stream),
+            "visited sort launch");
   if (opts && opts->async) return PANN_OK;
-  return check_status(*C, stream, allow_short);
+  return check_status(*C, stream, ex.allow_short);
 }
 
 }  // namespace
@@ -817,7 +878,7 @@ pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
                         uint32_t* out_ids, float* out_dists, uint32_t out_stride,
                         uint32_t* out_visited, uint32_t* out_cmps, void* stream) {
   return search_impl(idx, queries, nq, qp, k_pq, beam_pq, starts, nstarts, opts, out_ids,
-                     out_dists, out_stride, out_visited, out_cmps, nullptr, false, stream);
+                     out_dists, out_stride, out_visited, out_cmps, SearchExtras(), stream);
 }
 
 pann_status pann_search_status(pann_index_t idx, void* stream) {
@@ -841,6 +902,101 @@ pann_status pann_last_kernel_ms(pann_index_t idx, float* ms) {
   return PANN_OK;
 }
 
+namespace {
+
+// Carves the staging buffer of the default stream into consecutive regions
+// (256-byte aligned), growing it first.  The caller holds idx->host_mu.
+struct Stage {
+  size_t off = 0;
+  std::vector<std::pair<void**, size_t>> regions;
+  void add(void** p, size_t bytes) {
+    regions.push_back({p, off});
+    off += round_up(std::max<size_t>(bytes, 1), 256);
+  }
+  pann_status commit(pann_index_t idx) {
+    StreamCtx* C = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(idx->mu);
+      pann_status st = stream_ctx(idx, nullptr, &C);
+      if (st != PANN_OK) return st;
+      st = ensure_stage(*C, off);
+      if (st != PANN_OK) return st;
+    }
+    for (auto& r : regions) *r.first = C->d_stage + r.second;
+    return PANN_OK;
+  }
+};
+
+// pann_beam_search_host and pann_beam_search_visited_host: the whole final
+// frontier, and optionally the sorted visited list.
+pann_status beam_search_host(pann_index_t idx, const void* queries, uint32_t nq,
+                             const pann_qparams* qp, const uint32_t* starts, uint32_t nstarts,
+                             const pann_search_opts* opts, uint32_t* out_ids, float* out_dists,
+                             uint32_t* out_count, uint32_t* out_visited, uint32_t vis_cap,
+                             uint32_t* out_vis_ids, float* out_vis_dists, uint32_t* out_cmps) {
+  if (!idx || !qp || !queries || !out_ids || !out_dists || !out_count)
+    return fail(PANN_ERR_ARG, "null argument");
+  const bool want_vis = out_vis_ids || out_vis_dists;
+  if (want_vis && !(out_vis_ids && out_vis_dists && out_visited))
+    return fail(PANN_ERR_ARG, "the visited outputs come together (ids, distances, counts)");
+  if (want_vis && (vis_cap == 0 || vis_cap > 4096))
+    return fail(PANN_ERR_ARG, "vis_cap must be in [1, 4096]");
+  if (nq == 0) return PANN_OK;
+  if (qp->beam_size < 1 || qp->beam_size > 4096)
+    return fail(PANN_ERR_ARG, "beam_size must be in [1, 4096]");
+  std::lock_guard<std::mutex> hl(idx->host_mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  const size_t qbytes = (size_t)nq * idx->dim * (idx->dtype == PANN_DT_F32 ? 4 : 1);
+  const size_t B = (size_t)qp->beam_size;
+  pann_qparams q2 = *qp;
+  q2.k = qp->beam_size;  // the whole frontier comes back
+  void* dq = nullptr;
+  uint32_t *di = nullptr, *dn = nullptr, *dv = nullptr, *dc = nullptr, *dvi = nullptr;
+  float *dd = nullptr, *dvd = nullptr;
+  Stage sg;
+  sg.add((void**)&dq, qbytes);
+  sg.add((void**)&di, nq * B * 4);
+  sg.add((void**)&dd, nq * B * 4);
+  sg.add((void**)&dn, nq * 4);
+  sg.add((void**)&dv, nq * 4);
+  sg.add((void**)&dc, nq * 4);
+  if (want_vis) {
+    sg.add((void**)&dvi, (size_t)nq * vis_cap * 4);
+    sg.add((void**)&dvd, (size_t)nq * vis_cap * 4);
+  }
+  pann_status st = sg.commit(idx);
+  if (st != PANN_OK) return st;
+  HIP_TRY(hipMemcpy(dq, queries, qbytes, hipMemcpyHostToDevice), "beam_search_host staging");
+  pann_search_opts o{};
+  if (opts) o = *opts;
+  o.async = 0;
+  o.rerank = 0;
+  SearchExtras ex;
+  ex.out_nfront = dn;
+  ex.allow_short = true;
+  ex.vis_cap = want_vis ? vis_cap : 0u;
+  ex.vis_ids = dvi;
+  ex.vis_dists = dvd;
+  st = search_impl(idx, dq, nq, &q2, nullptr, nullptr, starts, nstarts, &o, di, dd, (uint32_t)B,
+                   dv, dc, ex, nullptr);
+  if (st != PANN_OK) return st;
+  hipError_t e = hipMemcpy(out_ids, di, nq * B * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(out_dists, dd, nq * B * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(out_count, dn, nq * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && out_cmps) e = hipMemcpy(out_cmps, dc, nq * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && out_visited)
+    e = hipMemcpy(out_visited, dv, nq * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && want_vis) {
+    e = hipMemcpy(out_vis_ids, dvi, (size_t)nq * vis_cap * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(out_vis_dists, dvd, (size_t)nq * vis_cap * 4, hipMemcpyDeviceToHost);
+  }
+  if (e != hipSuccess) return hip_fail(e, "beam_search_host copy-out");
+  return PANN_OK;
+}
+
+}  // namespace
+
 pann_status pann_search_host(pann_index_t idx, const void* queries, uint32_t nq,
                              const pann_qparams* qp, const uint32_t* starts, uint32_t nstarts,
                              const pann_search_opts* opts, uint32_t* out_ids, float* out_dists,
@@ -852,43 +1008,34 @@ pann_status pann_search_host(pann_index_t idx, const void* queries, uint32_t nq,
     if (qst != PANN_OK) return qst;
   }
   if (nq == 0) return PANN_OK;
+  std::lock_guard<std::mutex> hl(idx->host_mu);
   HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
   const size_t qbytes = (size_t)nq * idx->dim * (idx->dtype == PANN_DT_F32 ? 4 : 1);
   const size_t k = (size_t)qp->k;
   void* dq = nullptr;
   uint32_t *di = nullptr, *dv = nullptr, *dc = nullptr;
   float* dd = nullptr;
-  auto release = [&]() {
-    if (dq) (void)hipFree(dq);
-    if (di) (void)hipFree(di);
-    if (dd) (void)hipFree(dd);
-    if (dv) (void)hipFree(dv);
-    if (dc) (void)hipFree(dc);
-  };
-  hipError_t e = hipMalloc(&dq, qbytes);
-  if (e == hipSuccess) e = hipMalloc(&di, nq * k * 4);
-  if (e == hipSuccess) e = hipMalloc(&dd, nq * k * 4);
-  if (e == hipSuccess) e = hipMalloc(&dv, nq * 4);
-  if (e == hipSuccess) e = hipMalloc(&dc, nq * 4);
-  if (e == hipSuccess) e = hipMemcpy(dq, queries, qbytes, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    release();
-    return hip_fail(e, "search_host staging");
-  }
+  Stage sg;
+  sg.add((void**)&dq, qbytes);
+  sg.add((void**)&di, nq * k * 4);
+  sg.add((void**)&dd, nq * k * 4);
+  sg.add((void**)&dv, nq * 4);
+  sg.add((void**)&dc, nq * 4);
+  pann_status st = sg.commit(idx);
+  if (st != PANN_OK) return st;
+  HIP_TRY(hipMemcpy(dq, queries, qbytes, hipMemcpyHostToDevice), "search_host staging");
   pann_search_opts o{};
   if (opts) o = *opts;
   o.async = 0;
-  pann_status st = pann_search(idx, dq, nq, qp, nullptr, nullptr, starts, nstarts, &o, di, dd,
-                               (uint32_t)k, dv, dc, nullptr);
-  if (st == PANN_OK) {
-    e = hipMemcpy(out_ids, di, nq * k * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(out_dists, dd, nq * k * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && out_visited) e = hipMemcpy(out_visited, dv, nq * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && out_cmps) e = hipMemcpy(out_cmps, dc, nq * 4, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) st = hip_fail(e, "search_host copy-out");
-  }
-  release();
-  return st;
+  st = pann_search(idx, dq, nq, qp, nullptr, nullptr, starts, nstarts, &o, di, dd, (uint32_t)k,
+                   dv, dc, nullptr);
+  if (st != PANN_OK) return st;
+  hipError_t e = hipMemcpy(out_ids, di, nq * k * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(out_dists, dd, nq * k * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && out_visited) e = hipMemcpy(out_visited, dv, nq * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && out_cmps) e = hipMemcpy(out_cmps, dc, nq * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "search_host copy-out");
+  return PANN_OK;
 }
 
 pann_status pann_beam_search_host(pann_index_t idx, const void* queries, uint32_t nq,
@@ -896,50 +1043,22 @@ pann_status pann_beam_search_host(pann_index_t idx, const void* queries, uint32_
                                   uint32_t nstarts, const pann_search_opts* opts,
                                   uint32_t* out_ids, float* out_dists, uint32_t* out_count,
                                   uint32_t* out_visited, uint32_t* out_cmps) {
-  if (!idx || !qp || !queries || !out_ids || !out_dists || !out_count)
-    return fail(PANN_ERR_ARG, "null argument");
-  if (nq == 0) return PANN_OK;
-  if (qp->beam_size < 1 || qp->beam_size > 4096)
-    return fail(PANN_ERR_ARG, "beam_size must be in [1, 4096]");
-  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
-  const size_t qbytes = (size_t)nq * idx->dim * (idx->dtype == PANN_DT_F32 ? 4 : 1);
-  const size_t B = (size_t)qp->beam_size;
-  pann_qparams q2 = *qp;
-  q2.k = qp->beam_size;  // the whole frontier comes back
-  void* dq = nullptr;
-  uint32_t *di = nullptr, *dn = nullptr, *dv = nullptr, *dc = nullptr;
-  float* dd = nullptr;
-  auto release = [&]() {
-    for (void* p : {(void*)dq, (void*)di, (void*)dn, (void*)dv, (void*)dc, (void*)dd})
-      if (p) (void)hipFree(p);
-  };
-  hipError_t e = hipMalloc(&dq, qbytes);
-  if (e == hipSuccess) e = hipMalloc(&di, nq * B * 4);
-  if (e == hipSuccess) e = hipMalloc(&dd, nq * B * 4);
-  if (e == hipSuccess) e = hipMalloc(&dn, nq * 4);
-  if (e == hipSuccess) e = hipMalloc(&dv, nq * 4);
-  if (e == hipSuccess) e = hipMalloc(&dc, nq * 4);
-  if (e == hipSuccess) e = hipMemcpy(dq, queries, qbytes, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    release();
-    return hip_fail(e, "beam_search_host staging");
-  }
-  pann_search_opts o{};
-  if (opts) o = *opts;
-  o.async = 0;
-  o.rerank = 0;
-  pann_status st = search_impl(idx, dq, nq, &q2, nullptr, nullptr, starts, nstarts, &o, di, dd,
-                                (uint32_t)B, dv, dc, dn, true, nullptr);
-  if (st == PANN_OK) {
-    e = hipMemcpy(out_ids, di, nq * B * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(out_dists, dd, nq * B * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(out_count, dn, nq * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && out_visited) e = hipMemcpy(out_visited, dv, nq * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && out_cmps) e = hipMemcpy(out_cmps, dc, nq * 4, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) st = hip_fail(e, "beam_search_host copy-out");
-  }
-  release();
-  return st;
+  return beam_search_host(idx, queries, nq, qp, starts, nstarts, opts, out_ids, out_dists,
+                          out_count, out_visited, 0, nullptr, nullptr, out_cmps);
+}
+
+pann_status pann_beam_search_visited_host(pann_index_t idx, const void* queries, uint32_t nq,
+                                          const pann_qparams* qp, const uint32_t* starts,
+                                          uint32_t nstarts, const pann_search_opts* opts,
+                                          uint32_t* out_ids, float* out_dists,
+                                          uint32_t* out_count, uint32_t vis_cap,
+                                          uint32_t* out_vis_ids, float* out_vis_dists,
+                                          uint32_t* out_vis_count, uint32_t* out_cmps) {
+  if (!out_vis_ids || !out_vis_dists || !out_vis_count)
+    return fail(PANN_ERR_ARG, "null visited output");
+  return beam_search_host(idx, queries, nq, qp, starts, nstarts, opts, out_ids, out_dists,
+                          out_count, out_vis_count, vis_cap, out_vis_ids, out_vis_dists,
+                          out_cmps);
 }
 
 pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint32_t nq,
@@ -1040,8 +1159,8 @@ pann_status run_prune(pann_index_t idx, uint32_t level, const uint32_t* pts, uin
       idx->d_pkeys = nullptr;
       idx->d_pflags = nullptr;
       idx->pscratch_entries = 0;
-      HIP_TRY(hipMalloc(&idx->d_pkeys, need * 8), "hipMalloc(prune scratch)");
-      HIP_TRY(hipMalloc(&idx->d_pflags, need), "hipMalloc(prune scratch)");
+      HIP_TRY(dev_malloc(&idx->d_pkeys, need * 8), "hipMalloc(prune scratch)");
+      HIP_TRY(dev_malloc(&idx->d_pflags, need), "hipMalloc(prune scratch)");
       idx->pscratch_entries = need;
     }
     a.gkeys = idx->d_pkeys;
@@ -1168,7 +1287,7 @@ pann_status pann_hnsw_set_level(pann_index_t idx, uint32_t level, uint32_t max_d
   G = pann_index_s::Level{};
   const uint32_t stride = (uint32_t)round_up(max_degree, 16);
   uint32_t* d = nullptr;
-  HIP_TRY(hipMalloc(&d, (size_t)idx->n * stride * 4), "hipMalloc(level graph)");
+  HIP_TRY(dev_malloc(&d, (size_t)idx->n * stride * 4), "hipMalloc(level graph)");
   pann_status st = upload_graph(idx->n, max_degree, stride, graph_host, d);
   if (st != PANN_OK) {
     (void)hipFree(d);
@@ -1273,8 +1392,8 @@ pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq,
     C->d_eps = nullptr;
     C->d_epsd = nullptr;
     C->eps_cap = 0;
-    HIP_TRY(hipMalloc(&C->d_eps, (size_t)nq * 4), "hipMalloc(entry points)");
-    HIP_TRY(hipMalloc(&C->d_epsd, (size_t)nq * 4), "hipMalloc(entry points)");
+    HIP_TRY(dev_malloc(&C->d_eps, (size_t)nq * 4), "hipMalloc(entry points)");
+    HIP_TRY(dev_malloc(&C->d_epsd, (size_t)nq * 4), "hipMalloc(entry points)");
     C->eps_cap = nq;
   }
   pann_search_opts o{};
@@ -1473,8 +1592,8 @@ static pann_status run_bruteforce(const pann_index_s* idx, const void* X, int pd
   uint64_t* buf = nullptr;
   uint32_t* counts = nullptr;
   const uint32_t cq = std::min(nq, chunk);
-  hipError_t e = hipMalloc(&buf, (size_t)cq * a.nranges * a.cap * 8);
-  if (e == hipSuccess) e = hipMalloc(&counts, (size_t)cq * a.nranges * 4);
+  hipError_t e = dev_malloc(&buf, (size_t)cq * a.nranges * a.cap * 8);
+  if (e == hipSuccess) e = dev_malloc(&counts, (size_t)cq * a.nranges * 4);
   if (e != hipSuccess) {
     if (buf) (void)hipFree(buf);
     return hip_fail(e, "hipMalloc(brute-force scratch)");
@@ -1508,7 +1627,7 @@ pann_status pann_bruteforce_candidates(pann_index_t idx, const float* queries, u
   const int pdt = idx->dtype == PANN_DT_F32 ? 0 : (idx->dtype == PANN_DT_U8 ? 1 : 2);
   const uint64_t xs = idx->pts_stride / elem_bytes(idx->dtype);
   if (idx->metric == PANN_METRIC_L2 && !idx->d_xnorm) {
-    HIP_TRY(hipMalloc(&idx->d_xnorm, (size_t)idx->n * 4), "hipMalloc(norms)");
+    HIP_TRY(dev_malloc(&idx->d_xnorm, (size_t)idx->n * 4), "hipMalloc(norms)");
     HIP_TRY(pann::launch_row_sqnorm(idx->d_pts, pdt, xs, idx->n, idx->dim, idx->d_xnorm, stream),
             "norms launch");
   }
@@ -1543,13 +1662,13 @@ pann_status pann_build_fp8_residual(pann_index_t idx, uint32_t ncent) {
   const uint64_t ps = idx->pts_stride, fs = ps / 4;
   const uint32_t n = idx->n, dim = idx->dim;
   idx->fp8r_stride = round_up((uint64_t)dim + 4, 128);
-  HIP_TRY(hipMalloc(&idx->d_fp8r, (size_t)n * idx->fp8r_stride), "hipMalloc(fp8 residual)");
+  HIP_TRY(dev_malloc(&idx->d_fp8r, (size_t)n * idx->fp8r_stride), "hipMalloc(fp8 residual)");
   HIP_TRY(hipMemset(idx->d_fp8r, 0, (size_t)n * idx->fp8r_stride), "hipMemset(fp8 residual)");
-  HIP_TRY(hipMalloc(&idx->d_rscales, dim * 4), "hipMalloc(scales)");
-  HIP_TRY(hipMalloc(&idx->d_rabsmax, dim * 4), "hipMalloc(absmax)");
-  HIP_TRY(hipMalloc(&idx->d_cent, (size_t)ncent * ps), "hipMalloc(centroids)");
-  HIP_TRY(hipMalloc(&idx->d_cent_t, (size_t)ncent * dim * 4), "hipMalloc(centroids)");
-  HIP_TRY(hipMalloc(&idx->d_assign, (size_t)n * 4), "hipMalloc(assignment)");
+  HIP_TRY(dev_malloc(&idx->d_rscales, dim * 4), "hipMalloc(scales)");
+  HIP_TRY(dev_malloc(&idx->d_rabsmax, dim * 4), "hipMalloc(absmax)");
+  HIP_TRY(dev_malloc(&idx->d_cent, (size_t)ncent * ps), "hipMalloc(centroids)");
+  HIP_TRY(dev_malloc(&idx->d_cent_t, (size_t)ncent * dim * 4), "hipMalloc(centroids)");
+  HIP_TRY(dev_malloc(&idx->d_assign, (size_t)n * 4), "hipMalloc(assignment)");
   // centroids: every (n / ncent)-th point
   const uint64_t step = n / ncent;
   HIP_TRY(hipMemcpy2D(idx->d_cent, ps, idx->d_pts, ps * step, ps, ncent, hipMemcpyDeviceToDevice),
@@ -1564,8 +1683,8 @@ pann_status pann_build_fp8_residual(pann_index_t idx, uint32_t ncent) {
   // each point's nearest centroid in L2, on the brute-force kernel
   float* cnorm = nullptr;
   float* cd = nullptr;
-  hipError_t e = hipMalloc(&cnorm, (size_t)ncent * 4);
-  if (e == hipSuccess) e = hipMalloc(&cd, (size_t)n * 4);
+  hipError_t e = dev_malloc(&cnorm, (size_t)ncent * 4);
+  if (e == hipSuccess) e = dev_malloc(&cd, (size_t)n * 4);
   if (e == hipSuccess) e = pann::launch_row_sqnorm(idx->d_cent, 0, fs, ncent, dim, cnorm, nullptr);
   pann_status st = e == hipSuccess ? PANN_OK : hip_fail(e, "centroid norms");
   if (st == PANN_OK)
@@ -1618,7 +1737,7 @@ pann_status pann_close(pann_index_t idx) {
   for (auto& kv : idx->streams) {
     StreamCtx& C = kv.second;
     void* cp[] = {C.d_counter, C.d_status, C.d_starts, C.d_ghash, C.d_vis, C.d_eps, C.d_epsd,
-                  C.d_qcodes, C.d_qtable};
+                  C.d_qcodes, C.d_qtable, C.d_stage};
     for (void* p : cp)
       if (p) (void)hipFree(p);
     if (C.blas) (void)rocblas_destroy_handle(C.blas);

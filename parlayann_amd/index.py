@@ -161,6 +161,44 @@ class DeviceIndex:
                                          dists.ctypes.data, vis.ctypes.data, cmps.ctypes.data))
         return {"ids": ids, "dists": dists, "visited": vis, "dist_cmps": cmps}
 
+    def beam_search(self, queries, beam: int, *, limit=None, degree_limit=None,
+                    batch_factor: float = 0.125, starts: Sequence[int] = (0,),
+                    summation: str = "inorder", exact_counts: bool = False, vis_cap: int = 0):
+        """beam_search / filtered_beam_search without filtering (beamSearch.h:33-228,
+        :419-445) for host queries: each query's whole final frontier sorted by
+        (dist, id), and with vis_cap > 0 its visited list sorted the same way.
+        Returns dict(frontier_ids, frontier_dists, frontier_count, visited,
+        dist_cmps[, visited_ids, visited_dists])."""
+        qdt = np.float32 if (self.dtype == np.float32) else self.dtype
+        Q = np.ascontiguousarray(queries, dtype=qdt)
+        if Q.ndim != 2 or Q.shape[1] != self.dim:
+            raise ValueError(f"queries must be (nq, {self.dim})")
+        nq = Q.shape[0]
+        fi = np.zeros((nq, beam), np.uint32)
+        fd = np.zeros((nq, beam), np.float32)
+        fc = np.zeros(nq, np.uint32)
+        vis = np.zeros(nq, np.uint32)
+        cmps = np.zeros(nq, np.uint32)
+        st = np.ascontiguousarray(starts, dtype=np.uint32)
+        qp = self._qparams(beam, beam, limit, degree_limit, 100.0, batch_factor)
+        op = self._opts("native", summation, False, 0, False, 0, exact_counts)
+        out = {"frontier_ids": fi, "frontier_dists": fd, "frontier_count": fc, "visited": vis,
+               "dist_cmps": cmps}
+        if vis_cap > 0:
+            vi = np.zeros((nq, vis_cap), np.uint32)
+            vd = np.zeros((nq, vis_cap), np.float32)
+            L.check(L.lib().pann_beam_search_visited_host(
+                self._h, Q.ctypes.data, nq, C.byref(qp), st.ctypes.data, len(st), C.byref(op),
+                fi.ctypes.data, fd.ctypes.data, fc.ctypes.data, int(vis_cap), vi.ctypes.data,
+                vd.ctypes.data, vis.ctypes.data, cmps.ctypes.data))
+            out["visited_ids"], out["visited_dists"] = vi, vd
+        else:
+            L.check(L.lib().pann_beam_search_host(
+                self._h, Q.ctypes.data, nq, C.byref(qp), st.ctypes.data, len(st), C.byref(op),
+                fi.ctypes.data, fd.ctypes.data, fc.ctypes.data, vis.ctypes.data,
+                cmps.ctypes.data))
+        return out
+
     def search_device(self, queries, k: int, beam: int, out_ids, out_dists, *, out_visited=None,
                       out_cmps=None, k_per_query=None, beam_per_query=None, limit=None,
                       degree_limit=None, batch_factor: float = 0.125,
@@ -196,6 +234,13 @@ class DeviceIndex:
         ms = C.c_float(0)
         L.check(L.lib().pann_last_kernel_ms(self._h, C.byref(ms)))
         return float(ms.value)
+
+
+def device_alloc_count() -> int:
+    """Device allocations libpann has made since it was loaded."""
+    c = C.c_uint64(0)
+    L.check(L.lib().pann_device_alloc_count(C.byref(c)))
+    return int(c.value)
 
 
 def device_info(device: int = 0):

@@ -110,12 +110,15 @@ def test_f32_inorder_bit_exact(oracle, f32_case, ip_case, metric, B):
     assert np.array_equal(f["visited"], o["visited"])
 
 
-@pytest.mark.parametrize("dim", [5, 32, 40, 100, 200, 256, 300, 600])
+@pytest.mark.parametrize("dim", [5, 13, 32, 40, 99, 100, 101, 130, 200, 203, 256, 300, 301, 600,
+                                 601])
 @pytest.mark.parametrize("metric", ["l2", "ip"])
 def test_f32_inorder_bit_exact_dims(oracle, metric, dim):
-    """In-order FP32 at every row shape: rows of 1, 1, 2, 4, 7 and 8 128-byte
-    lines (8-lane gather, sum carried lane to lane in element order) and rows
-    of 10 and 19 lines (8 lines at a time, query in LDS)."""
+    """In-order FP32 at every row shape: rows of 1 to 8 128-byte lines (8-lane
+    gather, sum carried lane to lane in element order) and rows of 10 and 19
+    lines (8 lines at a time, query in LDS); dimensions that are not multiples
+    of 4 end with the fused terms of the reference loop's compiled tail, in
+    every lane position of the group (dim / 4 % 8 = 1, 3, 0, 0, 2, 5, 3, 2)."""
     rng = np.random.default_rng(1000 + dim)
     X = rng.standard_normal((3000, dim)).astype(np.float32)
     Q = rng.standard_normal((100, dim)).astype(np.float32)
@@ -258,21 +261,24 @@ def test_reduced_precision_rerank(oracle, ip_case, store):
     assert np.all(np.diff(red["dists"], axis=1) >= 0)
 
 
-@pytest.mark.parametrize("dim", [20, 96, 128, 200, 256, 300])
+@pytest.mark.parametrize("dim", [20, 23, 96, 99, 128, 200, 201, 256, 300, 301])
 @pytest.mark.parametrize("metric", ["l2", "ip"])
-def test_rerank_distances_bit_exact_dims(oracle, metric, dim):
-    """The fused FP32 rerank at rows of 1 to 4 lines (two rows per 8-lane
-    group in flight), 5 to 8 lines and past 8 lines (one, 8 lines at a time):
-    every returned distance is the oracle's in-order FP32 distance."""
+@pytest.mark.parametrize("store", ["bf16", "fp8"])
+def test_rerank_distances_bit_exact_dims(oracle, store, metric, dim):
+    """The fused FP32 rerank: the 8-lane in-order chain (BF16, FP8 inner
+    product) at rows of 1 to 4 lines, of 5 to 8 lines and past 8 lines (8
+    lines at a time), and one lane per row (FP8 L2); dimensions that are not
+    multiples of 4 included.  Every returned distance is the oracle's FP32
+    distance."""
     rng = np.random.default_rng(2000 + dim)
     X = rng.standard_normal((3000, dim)).astype(np.float32)
     Q = rng.standard_normal((100, dim)).astype(np.float32)
-    _say(f"bf16 + rerank {metric} dim={dim}")
+    _say(f"{store} + rerank {metric} dim={dim}")
     perm = datagen.insertion_orders(len(X), 1, 22)
     adj = oracle.build_vamana(X, metric, 16, 32, 1.2 if metric == "l2" else 1.0, perm)
     idx = DeviceIndex(X, adj, metric)
-    idx.build_store("bf16")
-    r = idx.search(Q, 10, 40, store="bf16", rerank=True)
+    idx.build_store(store)
+    r = idx.search(Q, 10, 40, store=store, rerank=True)
     for i in range(len(Q)):
         for a, d in zip(r["ids"][i], r["dists"][i]):
             assert d == oracle.distance(X[a], Q[i], metric), (i, a)
