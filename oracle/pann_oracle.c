@@ -794,14 +794,17 @@ static int kv_cmp(const void *a, const void *b) {
   return x->pos < y->pos ? -1 : (x->pos > y->pos ? 1 : 0);
 }
 
-static int batch_insert(bi_ctx *c, size_t m, int nthreads) {
-  size_t n = c->P->n;
+/* batch_insert's schedule (index.h:202-234, base 2, max_fraction 0.02):
+ * batches [floors[i], ceilings[i]) of the m inserts into n points.  Returns
+ * the number of batches; writes at most cap of them. */
+long oracle_vamana_batches(size_t n, size_t m, size_t *floors, size_t *ceilings, long cap) {
   size_t count = 0;
   size_t inc = 0;
+  long nb = 0;
   const double base = 2.0, max_fraction = 0.02;
   size_t max_batch_size = (size_t)(max_fraction * (double)(float)n); /* :206-207 */
   if (max_batch_size > 1000000ul) max_batch_size = 1000000ul;
-  if (max_batch_size == 0) max_batch_size = n;
+  if (max_batch_size == 0) max_batch_size = n; /* :208-209 */
   while (count < m) { /* :223-234 */
     size_t floor_, ceiling;
     if (ORACLE_MUTANT == 3) {
@@ -812,8 +815,7 @@ static int batch_insert(bi_ctx *c, size_t m, int nthreads) {
       floor_ = count;
       ceiling = count + 1;
       count += 1;
-    } else
-    if (pow(base, (double)inc) <= (double)max_batch_size) {
+    } else if (pow(base, (double)inc) <= (double)max_batch_size) {
       floor_ = (size_t)pow(base, (double)inc) - 1;
       size_t c2 = (size_t)pow(base, (double)(inc + 1)) - 1;
       ceiling = c2 < m ? c2 : m;
@@ -823,12 +825,35 @@ static int batch_insert(bi_ctx *c, size_t m, int nthreads) {
       ceiling = count + max_batch_size < m ? count + max_batch_size : m;
       count += max_batch_size;
     }
+    if (nb < cap) {
+      floors[nb] = floor_;
+      ceilings[nb] = ceiling;
+    }
+    nb++;
+    inc += 1;
+  }
+  return nb;
+}
+
+static int batch_insert(bi_ctx *c, size_t m, int nthreads) {
+  const long nbat = oracle_vamana_batches(c->P->n, m, NULL, NULL, 0);
+  size_t *floors = (size_t *)malloc((size_t)(nbat + 1) * sizeof(size_t));
+  size_t *ceilings = (size_t *)malloc((size_t)(nbat + 1) * sizeof(size_t));
+  oracle_vamana_batches(c->P->n, m, floors, ceilings, nbat);
+  int rc = 0;
+  for (long b = 0; b < nbat && rc == 0; b++) {
+    const size_t floor_ = floors[b], ceiling = ceilings[b];
     long bsz = (long)(ceiling - floor_);
     c->floor_ = floor_;
     c->new_out = (uint32_t *)malloc((size_t)bsz * c->R * sizeof(uint32_t));
     c->new_cnt = (int *)calloc((size_t)bsz, sizeof(int));
     parallel_for(bsz, 4, nthreads, bi_search_body, c); /* :247-269 */
-    if (atomic_load(&c->failed)) return -1;
+    if (atomic_load(&c->failed)) {
+      free(c->new_out);
+      free(c->new_cnt);
+      rc = -1;
+      break;
+    }
     for (long bi = 0; bi < bsz; bi++) { /* :271-273 update_neighbors */
       uint32_t *row = c->adj + (size_t)c->order[floor_ + bi] * (c->R + 1);
       row[0] = (uint32_t)c->new_cnt[bi];
@@ -871,9 +896,10 @@ static int batch_insert(bi_ctx *c, size_t m, int nthreads) {
     free(c->gvals);
     free(c->new_out);
     free(c->new_cnt);
-    inc += 1;
   }
-  return 0;
+  free(floors);
+  free(ceilings);
+  return rc;
 }
 
 typedef struct {
@@ -1249,6 +1275,29 @@ static void hr_body(void *vctx, long g, int tid) {
   }
 }
 
+/* HNSW::HNSW's insert batches (HNSW.hpp:776-792): [begins[i], ends[i]) after
+ * the first node.  Returns the number of batches; writes at most cap. */
+long oracle_hnsw_batches(uint32_t n, float batch_base, uint32_t *begins, uint32_t *ends,
+                         long cap) {
+  uint32_t size_limit = (uint32_t)((double)n * 0.02); /* n*0.02 (:778) */
+  if (size_limit == 0) size_limit = 1; /* the reference never ends a batch of 0 */
+  uint32_t bb = 0, be = 1;
+  long nb = 0;
+  while (be < n) { /* :779-782 */
+    bb = be;
+    uint32_t b1 = (uint32_t)ceilf((float)bb * batch_base) + (ORACLE_MUTANT == 10 ? 0 : 1);
+    be = n;
+    if (b1 < be) be = b1;
+    if (bb + size_limit < be) be = bb + size_limit;
+    if (nb < cap) {
+      begins[nb] = bb;
+      ends[nb] = be;
+    }
+    nb++;
+  }
+  return nb;
+}
+
 int oracle_build_hnsw(const oracle_hnsw *H, const oracle_points *P, const uint32_t *levels,
                       unsigned efc, float alpha, float batch_base, uint32_t *entrance_out,
                       int nthreads) {
@@ -1259,8 +1308,6 @@ int oracle_build_hnsw(const oracle_hnsw *H, const oracle_points *P, const uint32
     if ((int)levels[i] > H->top) return -1;
   uint32_t entrance = 0; /* node 0 starts the index (:763-774) */
   uint32_t bb = 0, be = 1;
-  uint32_t size_limit = (uint32_t)((double)n * 0.02); /* n*0.02 (:778) */
-  if (size_limit == 0) size_limit = 1; /* the reference never ends a batch of 0 */
   hb_ctx c;
   memset(&c, 0, sizeof(c));
   c.H = H;
@@ -1278,12 +1325,13 @@ int oracle_build_hnsw(const oracle_hnsw *H, const oracle_points *P, const uint32
   rc.cap = (long *)calloc((size_t)nthreads, sizeof(long));
   kv *edges = NULL;
   size_t cap_edges = 0;
-  while (be < n) { /* :777-792 */
-    bb = be;
-    uint32_t b1 = (uint32_t)ceilf((float)bb * batch_base) + (ORACLE_MUTANT == 10 ? 0 : 1);
-    be = n;
-    if (b1 < be) be = b1;
-    if (bb + size_limit < be) be = bb + size_limit;
+  const long nbat = oracle_hnsw_batches(n, batch_base, NULL, NULL, 0);
+  uint32_t *begins = (uint32_t *)malloc((size_t)(nbat + 1) * sizeof(uint32_t));
+  uint32_t *ends = (uint32_t *)malloc((size_t)(nbat + 1) * sizeof(uint32_t));
+  oracle_hnsw_batches(n, batch_base, begins, ends, nbat);
+  for (long b = 0; b < nbat; b++) { /* :777-792 */
+    bb = begins[b];
+    be = ends[b];
     size_t sz = be - bb;
     if (sz > cap_batch) {
       cap_batch = sz;
@@ -1363,6 +1411,8 @@ int oracle_build_hnsw(const oracle_hnsw *H, const oracle_points *P, const uint32
   free(rc.gkeys);
   free(rc.gvals);
   free(rc.gstart);
+  free(begins);
+  free(ends);
   return atomic_load(&c.failed) ? -1 : 0;
 }
 

@@ -166,6 +166,13 @@ void oracle_q8_translate(const oracle_q8_params *prm, int metric, const float *x
 /* The (dist, id) sort used everywhere above, exposed for its property test. */
 void oracle_sort_pairs(uint32_t *ids, float *dists, long n);
 
+/* The insert schedules of the two builders, exposed for the pin tests:
+ * batch_insert (vamana/index.h:202-234) and HNSW::HNSW (HNSW.hpp:776-792).
+ * Each returns the number of batches and writes at most cap of them. */
+long oracle_vamana_batches(size_t n, size_t m, size_t *floors, size_t *ceilings, long cap);
+long oracle_hnsw_batches(uint32_t n, float batch_base, uint32_t *begins, uint32_t *ends,
+                         long cap);
+
 #ifdef __cplusplus
 }
 #endif

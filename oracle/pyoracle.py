@@ -131,7 +131,30 @@ def _configure(L):
         L.oracle_build_hnsw.restype = C.c_int
         L.oracle_build_hnsw.argtypes = [C.POINTER(_HNSW), C.POINTER(_Points), u32p, C.c_uint,
                                         C.c_float, C.c_float, u32p, C.c_int]
+        L.oracle_vamana_batches.restype = C.c_long
+        L.oracle_vamana_batches.argtypes = [sz, sz, vp, vp, C.c_long]
+        L.oracle_hnsw_batches.restype = C.c_long
+        L.oracle_hnsw_batches.argtypes = [C.c_uint32, C.c_float, vp, vp, C.c_long]
     return L
+
+
+def vamana_batches(n, m=None):
+    """batch_insert's [(floor, ceiling)] (vamana/index.h:202-234)."""
+    m = n if m is None else m
+    nb = lib().oracle_vamana_batches(n, m, None, None, 0)
+    f = np.zeros(max(nb, 1), np.uint64)
+    c = np.zeros(max(nb, 1), np.uint64)
+    lib().oracle_vamana_batches(n, m, _ptr(f), _ptr(c), nb)
+    return [(int(a), int(b)) for a, b in zip(f[:nb], c[:nb])]
+
+
+def hnsw_batches(n, batch_base=2.0):
+    """HNSW::HNSW's insert batches [(begin, end)] (HNSW.hpp:776-792)."""
+    nb = lib().oracle_hnsw_batches(n, batch_base, None, None, 0)
+    b = np.zeros(max(nb, 1), np.uint32)
+    e = np.zeros(max(nb, 1), np.uint32)
+    lib().oracle_hnsw_batches(n, batch_base, _ptr(b), _ptr(e), nb)
+    return [(int(x), int(y)) for x, y in zip(b[:nb], e[:nb])]
 
 
 def _ptr(a: np.ndarray):
