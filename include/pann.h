@@ -166,6 +166,25 @@ pann_status pann_search(pann_index_t idx, const void *queries, uint32_t nq,
                         uint32_t *out_ids, float *out_dists, uint32_t out_stride,
                         uint32_t *out_visited, uint32_t *out_cmps, void *stream);
 
+/* pann_search without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results as pann_search; the memory the launch writes besides the outputs
+ * (work counter, status word, start points, PANN_STORE_Q8 query codes) is a
+ * file-scope device array of 8 MiB per GPU, held until the process ends, so
+ * the index's stream contexts are neither created nor used.  Limits:
+ * synchronous only (opts->async = 0); exact_counts = 0; no
+ * PANN_STORE_FP8_RESIDUAL; PANN_STORE_Q8 needs nq * dim <= 8 MiB - 1 KiB
+ * (PANN_ERR_CAPACITY beyond).  Calls share each GPU's array: this library
+ * serialises the calls of one process, and calls that overlap on one GPU in
+ * any other way race. */
+pann_status pann_search_noalloc(pann_index_t idx, const void *queries, uint32_t nq,
+                                const pann_qparams *qp, const int32_t *k_pq,
+                                const int32_t *beam_pq, const uint32_t *starts,
+                                uint32_t nstarts, const pann_search_opts *opts,
+                                uint32_t *out_ids, float *out_dists, uint32_t out_stride,
+                                uint32_t *out_visited, uint32_t *out_cmps, void *stream);
+
 /* Same search with host buffers (copies in and out, synchronises).  This is
  * what a C++ qsearchAll caller binds directly.  The device staging for the
  * copies is kept per index and grown geometrically, so repeated calls of the

@@ -18,7 +18,7 @@ SUM_INORDER, SUM_TREE = 0, 1
 # every symbol include/pann.h declares
 EXPORTS = ["pann_open", "pann_build_store", "pann_get_fp8_scales", "pann_get_q8_params",
            "pann_build_fp8_residual", "pann_get_fp8_residual",
-           "pann_search",
+           "pann_search", "pann_search_noalloc",
            "pann_search_host", "pann_beam_search_host", "pann_beam_search_visited_host",
            "pann_device_alloc_count", "pann_search_status", "pann_last_kernel_ms",
            "pann_search_visited", "pann_robust_prune_batch", "pann_robust_prune_groups",
@@ -76,6 +76,9 @@ def lib():
         "pann_get_q8_params": [vp, vp, vp, vp, vp],
         "pann_search": [vp, vp, C.c_uint32, C.POINTER(QParams), vp, vp, vp, C.c_uint32,
                         C.POINTER(SearchOpts), vp, vp, C.c_uint32, vp, vp, vp],
+        "pann_search_noalloc": [vp, vp, C.c_uint32, C.POINTER(QParams), vp, vp, vp,
+                                C.c_uint32, C.POINTER(SearchOpts), vp, vp, C.c_uint32, vp, vp,
+                                vp],
         "pann_search_host": [vp, vp, C.c_uint32, C.POINTER(QParams), vp, C.c_uint32,
                              C.POINTER(SearchOpts), vp, vp, vp, vp],
         "pann_beam_search_host": [vp, vp, C.c_uint32, C.POINTER(QParams), vp, C.c_uint32,

@@ -273,6 +273,38 @@ __global__ void k_zero_status(uint32_t* p) { p[threadIdx.x] = 0u; }
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// pann_search_noalloc's memory: one file-scope device array per GPU, held
+// until the process ends.  Bytes [0, 256) hold the persistent grid's work
+// counter, [256, 512) the status word, [512, 768) the start points, and
+// [1024, 8 MiB) the PANN_STORE_Q8 query codes (nq * dim bytes).  Host code
+// never uses the array's name as an address: each device's address comes
+// from hipGetSymbolAddress, taken once per device with that device current.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoallocScratchBytes = 8u << 20;
+constexpr size_t kNoallocCodesOffset = 1024;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+std::mutex g_noalloc_mu;              // the calls of this process share each device's array
+void* g_noalloc_ptr[64] = {nullptr};  // its address per device
+
+// The caller holds g_noalloc_mu and has made `device` current.
+pann_status noalloc_scratch(int device, char** out) {
+  if (device < 0 || device >= 64) return fail(PANN_ERR_ARG, "device ordinal out of range");
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_noalloc_ptr[device] = p;
+  }
+  *out = (char*)g_noalloc_ptr[device];
+  return PANN_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* pann_last_error(void) { return g_err.c_str(); }
@@ -569,14 +601,26 @@ struct StartSets {
   uint32_t stride = 0;
 };
 
+// Device memory a launch writes besides its outputs, when the caller brings
+// it instead of the stream's context (pann_search_noalloc).
+struct LaunchMem {
+  uint32_t* counter;  // persistent grid's work counter
+  uint32_t* status;   // status bits
+  uint32_t* starts;   // start points (64 ids)
+};
+
 // Fills the launch block shared by every search entry point: graph `level`,
 // host start points (starts, nstarts) or per-query device start sets (ss).
+// With `fixed` the launch uses that memory and the stream's context is neither
+// created nor used (C stays null): nothing is allocated.
 pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams* qp,
                            const uint32_t* starts, uint32_t nstarts, const StartSets* ss,
                            const pann_search_opts* opts, uint32_t nq, pann::SearchArgs& a,
                            int& store_kind, bool& sequential, uint32_t& grid,
-                           hipStream_t stream, StreamCtx*& C, uint32_t passes = 1) {
-  {
+                           hipStream_t stream, StreamCtx*& C, uint32_t passes = 1,
+                           const LaunchMem* fixed = nullptr) {
+  C = nullptr;
+  if (!fixed) {
     pann_status cst = stream_ctx(idx, stream, &C);
     if (cst != PANN_OK) return cst;
   }
@@ -667,10 +711,10 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   a.degree_limit = (int32_t)std::min<int64_t>(qp->degree_limit, G.max_degree);
   a.batch_factor = qp->batch_factor;
   a.rerank_factor = qp->rerank_factor;
-  a.status = C->d_status;
-  a.work_counter = C->d_counter;
+  a.status = fixed ? fixed->status : C->d_status;
+  a.work_counter = fixed ? fixed->counter : C->d_counter;
   a.nstarts = ss ? 0u : nstarts;
-  a.starts = C->d_starts;
+  a.starts = fixed ? fixed->starts : C->d_starts;
   if (ss) {
     a.starts_pq = ss->starts;
     a.nstarts_pq = ss->counts;
@@ -735,6 +779,9 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   a.ghash = nullptr;
   a.ghash_log2 = 0;
   a.ghash_tag_base = 0;
+  if (o.exact_counts && fixed)
+    return fail(PANN_ERR_UNSUPPORTED, "exact_counts needs the overflow table, which is "
+                                      "allocated: not available without allocation");
   if (o.exact_counts) {
     // per-wave global overflow table: twice the reference's initial hashset
     // size 2*(10+B)*degree_limit (beamSearch.h:66), at least 4x the LDS table
@@ -754,8 +801,11 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   const bool cap = capturing(stream);
   if (cap && o.exact_counts)
     return fail(PANN_ERR_ARG, "exact_counts searches cannot be captured into a graph");
-  if (!ss && (C->starts_host.size() != nstarts ||
-              !std::equal(starts, starts + nstarts, C->starts_host.begin()))) {
+  if (fixed && !ss) {  // no per-stream record of what the memory holds: copy each time
+    HIP_TRY(hipMemcpyAsync(fixed->starts, starts, nstarts * 4, hipMemcpyHostToDevice, stream),
+            "hipMemcpyAsync(starts)");
+  } else if (!ss && (C->starts_host.size() != nstarts ||
+                     !std::equal(starts, starts + nstarts, C->starts_host.begin()))) {
     if (cap)
       return fail(PANN_ERR_ARG, "start points changed under graph capture: run the search "
                                 "once with these start points before capturing it");
@@ -764,17 +814,18 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     C->starts_host.assign(starts, starts + nstarts);
   }
-  hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(1), 0, stream, C->d_status);
+  hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(1), 0, stream, a.status);
   HIP_TRY(hipGetLastError(), "status reset");
   return PANN_OK;
 }
 
 // allow_short: a frontier shorter than k is a result, not an error (HNSW
 // layers return whatever frontier they reach)
-pann_status check_status(StreamCtx& C, hipStream_t stream, bool allow_short = false) {
+pann_status check_status_word(const uint32_t* d_status, hipStream_t stream,
+                              bool allow_short = false) {
   HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
   uint32_t st = 0;
-  HIP_TRY(hipMemcpy(&st, C.d_status, 4, hipMemcpyDeviceToHost), "hipMemcpy(status)");
+  HIP_TRY(hipMemcpy(&st, d_status, 4, hipMemcpyDeviceToHost), "hipMemcpy(status)");
   if (st & pann::PANN_ST_BAD_PARAMS)  // beamSearch.h:612-616 (k > beam) per query
     return fail(PANN_ERR_ARG, "a per-query k or beam is invalid (k > beam, beam > qp->beam_size, "
                               "or k > out_stride)");
@@ -785,6 +836,10 @@ pann_status check_status(StreamCtx& C, hipStream_t stream, bool allow_short = fa
   if (st & pann::PANN_ST_VISITED_OVERFLOW)
     return fail(PANN_ERR_CAPACITY, "a visited list outgrew vis_cap");
   return PANN_OK;
+}
+
+pann_status check_status(StreamCtx& C, hipStream_t stream, bool allow_short = false) {
+  return check_status_word(C.d_status, stream, allow_short);
 }
 
 }  // namespace
@@ -879,6 +934,80 @@ pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
                         uint32_t* out_visited, uint32_t* out_cmps, void* stream) {
   return search_impl(idx, queries, nq, qp, k_pq, beam_pq, starts, nstarts, opts, out_ids,
                      out_dists, out_stride, out_visited, out_cmps, SearchExtras(), stream);
+}
+
+pann_status pann_search_noalloc(pann_index_t idx, const void* queries, uint32_t nq,
+                                const pann_qparams* qp, const int32_t* k_pq,
+                                const int32_t* beam_pq, const uint32_t* starts, uint32_t nstarts,
+                                const pann_search_opts* opts, uint32_t* out_ids, float* out_dists,
+                                uint32_t out_stride, uint32_t* out_visited, uint32_t* out_cmps,
+                                void* stream_) {
+  if (!idx || !qp || !starts) return fail(PANN_ERR_ARG, "null argument");
+  {
+    pann_status qst = check_qparams(qp);
+    if (qst != PANN_OK) return qst;
+  }
+  if (nq == 0) return PANN_OK;
+  if (!queries || !out_ids || !out_dists) return fail(PANN_ERR_ARG, "null buffer");
+  if (out_stride < (uint32_t)std::max<int64_t>(qp->k, 0))
+    return fail(PANN_ERR_ARG, "out_stride smaller than k");
+  pann_search_opts o{};
+  if (opts) o = *opts;
+  if (o.async) return fail(PANN_ERR_ARG, "pann_search_noalloc is synchronous (async = 0)");
+  if (o.store == PANN_STORE_FP8_RESIDUAL)
+    return fail(PANN_ERR_UNSUPPORTED, "the FP8 residual store's per-query table is allocated: "
+                                      "not available without allocation");
+  if (o.store == PANN_STORE_Q8 &&
+      (uint64_t)nq * idx->dim > kNoallocScratchBytes - kNoallocCodesOffset)
+    return fail(PANN_ERR_CAPACITY,
+                "PANN_STORE_Q8 without allocation: nq * dim = " +
+                    std::to_string((uint64_t)nq * idx->dim) + " exceeds " +
+                    std::to_string(kNoallocScratchBytes - kNoallocCodesOffset) + " bytes");
+  std::lock_guard<std::mutex> gl(g_noalloc_mu);
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  char* p = nullptr;
+  {
+    pann_status st = noalloc_scratch(idx->device, &p);
+    if (st != PANN_OK) return st;
+  }
+  const LaunchMem lm{(uint32_t*)p, (uint32_t*)(p + 256), (uint32_t*)(p + 512)};
+  hipStream_t stream = (hipStream_t)stream_;
+  pann::SearchArgs a;
+  int store_kind;
+  bool sequential;
+  uint32_t grid;
+  StreamCtx* C = nullptr;
+  pann_status st = prepare_search(idx, 0, qp, starts, nstarts, nullptr, &o, nq, a, store_kind,
+                                  sequential, grid, stream, C, 1, &lm);
+  if (st != PANN_OK) return st;
+  const bool f32_queries = idx->dtype == PANN_DT_F32;
+  a.queries = (const char*)queries;
+  a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
+  a.fqueries = (const char*)queries;
+  a.fq_stride = (uint64_t)idx->dim * 4;
+  a.nq = nq;
+  a.k_pq = k_pq;
+  a.beam_pq = beam_pq;
+  a.out_ids = out_ids;
+  a.out_dists = out_dists;
+  a.out_stride = out_stride;
+  a.out_visited = out_visited;
+  a.out_cmps = out_cmps;
+  if (o.store == PANN_STORE_Q8) {  // the query codes go to the array past its first KiB
+    uint8_t* codes = (uint8_t*)(p + kNoallocCodesOffset);
+    HIP_TRY(pann::launch_encode_q8((const float*)queries, idx->dim, nq, idx->dim,
+                                   idx->metric == PANN_METRIC_L2 ? 0 : 1, idx->q8_slope,
+                                   idx->q8_offset, idx->q8_max, idx->q8_scale, codes, idx->dim,
+                                   stream),
+            "query encode");
+    a.queries = (const char*)codes;
+    a.q_stride = idx->dim;
+  }
+  grid = std::min<uint32_t>(grid, nq);
+  HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
+          "beam search launch");
+  return check_status_word(lm.status, stream);
 }
 
 pann_status pann_search_status(pann_index_t idx, void* stream) {
