@@ -19,8 +19,10 @@ Prints one JSON line on stdout; progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import copy
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -183,7 +185,7 @@ def measured_traffic(workload, w, leg, beam):
     return None
 
 
-def main():
+def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -214,27 +216,80 @@ def main():
     ap.add_argument("--cache", default="",
                     help="directory holding the built graph and groundtruth of this workload "
                          "(written on the first run, loaded by later ones, e.g. profiler passes)")
-    args = ap.parse_args()
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="run only --workload (by default the 10M inner-product headline is "
+                         "followed by the 100k x 128 FP32, 1M x 128 uint8 and HNSW 1M x 200 "
+                         "workloads, reported under config.secondary)")
+    return ap.parse_args(argv)
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), reported with every CPU number."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+_NATIVE_ORACLE = []
+
+
+def cpu_oracle():
+    """The oracle for the host-CPU leg, built with -march=native on this host (the
+    reference's flags, parallelDefsANN:8) into oracle/liboracle_native.so; the
+    portable build if that compile fails.  Returns (context manager, flags)."""
+    import contextlib
+    from oracle import pyoracle
+    if not _NATIVE_ORACLE:
+        path = os.path.join(ROOT, "oracle", "liboracle_native.so")
+        try:
+            subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"),
+                            "liboracle_native.so"], check=True, capture_output=True, timeout=120)
+            _NATIVE_ORACLE.append((path, "-O3 -march=native"))
+        except Exception as e:  # noqa: BLE001 -- report and use the portable build
+            log(f"[bench] -march=native oracle build failed ({e}); portable build used")
+            _NATIVE_ORACLE.append((None, "-O3 -march=x86-64-v3"))
+    path, flags = _NATIVE_ORACLE[0]
+    return (pyoracle.use_library(path) if path else contextlib.nullcontext()), flags
+
+
+def main():
+    args = parse_args()
     if args.cache:  # the autotuner's table lives beside the graph / groundtruth cache
         os.environ.setdefault("PANN_TUNE_CACHE", os.path.join(args.cache, "tune.json"))
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
                                    "of it (--gpus must be 1)"}))
         sys.exit(2)
+    out = run_workload(args, args.workload)
+    if args.workload == "aniso-ip-10M" and not args.no_secondary and not (args.n or args.nq
+                                                                         or args.beam):
+        out["config"]["secondary"] = run_secondary(args)
+    print(json.dumps(out))
 
-    import torch
-    from parlayann_amd import DeviceIndex, autotune, datagen, device_info
-    from parlayann_amd.groundtruth import groundtruth_device
-    from parlayann_amd.recall import recall_plain, recall_with_ties
-    from parlayann_amd.vamana import build_vamana
 
-    w = dict(WORKLOADS[args.workload])
+def run_workload(args, name, w=None):
+    w = dict(WORKLOADS[name]) if w is None else w
     if args.n:
         w["n"] = args.n
     if args.nq:
         w["nq"] = args.nq
     if w["kind"] == "hnsw":
-        return run_hnsw(args, w)
+        return run_hnsw(args, w, name)
+    return run_vamana(args, w, name)
+
+
+def run_vamana(args, w, name):
+    import torch
+    from parlayann_amd import DeviceIndex, autotune, datagen
+    from parlayann_amd.groundtruth import groundtruth_device
+    from parlayann_amd.recall import recall_plain, recall_with_ties
+    from parlayann_amd.vamana import build_vamana
+
     legs = args.legs.split(",") if args.legs else w["legs"]
     beams = [args.beam] if args.beam else w.get("beams", BEAMS)
     dev = torch.device("cuda", 0)
@@ -251,7 +306,7 @@ def main():
             log(f"[bench] build pass {p}: {hi:,}/{len(X):,} points inserted "
                 f"({time.time() - t0:.0f}s)")
 
-    tag = f"{args.workload}-n{w['n']}-q{w['nq']}"
+    tag = f"{name}-n{w['n']}-q{w['nq']}"
     cg = os.path.join(args.cache, tag + "-graph.npy") if args.cache else ""
     cgt = os.path.join(args.cache, tag + "-gt.npy") if args.cache else ""
     t_build = None
@@ -380,6 +435,48 @@ def main():
             f"{gbs:,.0f} GB/s visited {vis.mean():.1f} cmps {cmps.mean():.1f} "
             f"evals {evals.mean():.1f}")
 
+    # the full beam sweep of the first leg: QPS and recall at every beam, and a
+    # bit-exact check of a query sample against the oracle at each
+    if w.get("sweep"):
+        from oracle import pyoracle
+        leg = legs[0]
+        opts = LEG_OPTS[leg]
+        shape = {k_: v for k_, v in results[leg]["launch_shape"].items() if k_ != "tuned"}
+        m_chk = min(nq, 500)
+        sweep = []
+        for B in beams:
+            if B < k:
+                continue
+            run = lambda a=True: idx.search_device(
+                tq, k, B, oi, od, out_visited=ov, out_cmps=oc, async_=a,
+                rerank_factor=args.rerank_factor, **opts, **shape)
+            run(False)
+            ids = oi.cpu().numpy().view(np.uint32)
+            dists = od.cpu().numpy()
+            vis = ov.cpu().numpy().view(np.uint32)
+            o = pyoracle.search_batch(adj, X, w["metric"], Q[:m_chk], k, B)
+            exact = bool(np.array_equal(ids[:m_chk], o["ids"]) and
+                         np.array_equal(dists[:m_chk], o["dists"]) and
+                         np.array_equal(vis[:m_chk], o["visited"]))
+            rec = recall_plain(gt, ids, k)
+            for _ in range(args.warmup):
+                run()
+            torch.cuda.synchronize()
+            kms = []
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                run()
+                kms.append(idx.last_kernel_ms())
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            idx.search_status()
+            sweep.append(dict(beam=B, recall_at_10=round(rec, 5), qps=nq * args.steps / dt,
+                              kernel_ms=float(np.mean(kms)), oracle_sample=m_chk,
+                              bit_exact_vs_oracle=exact))
+            log(f"[bench] {leg} sweep beam {B}: recall {rec:.4f} QPS {nq * args.steps / dt:,.0f}"
+                f" oracle sample of {m_chk} bit-exact: {exact}")
+        results[leg]["beam_sweep"] = sweep
+
     # batched uneven search: per-query k in {1, 10, 100}, beam ~ U[max(16, k), 256],
     # the whole query set in one persistent launch
     if w.get("uneven") and not args.no_uneven:
@@ -470,19 +567,22 @@ def main():
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         threads = min(threads, 64)
         # bounded sample: size one repetition to about --cpu-seconds
-        m = min(nq, 1000)
-        t0 = time.perf_counter()
-        pyoracle.search_batch(adj, X, w["metric"], Q[:m], k, B, nthreads=threads)
-        rate = m / (time.perf_counter() - t0)
-        m = int(min(nq, max(1000, rate * args.cpu_seconds)))
-        times = []
-        for rep in range(5):
-            _ = np.random.default_rng(rep).permutation(5_000_000)  # cache flush
+        ctx, flags = cpu_oracle()
+        with ctx:
+            m = min(nq, 1000)
             t0 = time.perf_counter()
             pyoracle.search_batch(adj, X, w["metric"], Q[:m], k, B, nthreads=threads)
-            times.append(time.perf_counter() - t0)
+            rate = m / (time.perf_counter() - t0)
+            m = int(min(nq, max(1000, rate * args.cpu_seconds)))
+            times = []
+            for rep in range(5):
+                _ = np.random.default_rng(rep).permutation(5_000_000)  # cache flush
+                t0 = time.perf_counter()
+                pyoracle.search_batch(adj, X, w["metric"], Q[:m], k, B, nthreads=threads)
+                times.append(time.perf_counter() - t0)
         med = float(np.median(times))
         cpu = {"value": m / med, "unit": "queries/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(), "build": f"gcc {flags}",
                "sample": f"first {m} of the {nq} queries at beam {B} (the in-order FP32 "
                          f"operating point), 5 repetitions, median; oracle restatement of "
                          f"beamSearch.h"}
@@ -504,19 +604,20 @@ def main():
         "dtype": {"fp8-rerank": "fp8", "fp8r-rerank": "fp8", "bf16-rerank": "bf16", "u8": "u8",
                   "q8-rerank": "i8" if w["metric"] == "ip" else "u8"}.get(head, "f32"),
         "data": "synthetic (seeded generator; graph built on the GPU by this library)",
-        "config": {"workload": args.workload, "n": w["n"], "dim": w["d"], "nq": nq, "k": k,
+        "config": {"workload": name, "n": w["n"], "dim": w["d"], "nq": nq, "k": k,
                    "graph": f"vamana R={w['R']} L={w['L']} alpha={w['alpha']}",
                    "headline_leg": head, "legs": results, "graph_build_s": t_build,
                    "gather_ceiling": gceil},
         "roofline": {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": h["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, w, head, h["beam"])},
+                     "traffic": measured_traffic(name, w, head, h["beam"])},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    idx.close()
+    return out
 
 
-def run_hnsw(args, w):
+def run_hnsw(args, w, name):
     """HNSW serving workload: calibrate ef per leg, closed-loop batches of all
     queries (inputs resident), then open-loop Poisson arrivals through the
     continuous-batching server at 50% and 80% of the closed-loop rate."""
@@ -531,7 +632,7 @@ def run_hnsw(args, w):
     t0 = time.time()
     X, Q = make_data(w)
     log(f"[bench] data {X.shape} + {Q.shape} in {time.time() - t0:.1f}s")
-    tag = f"{args.workload}-n{w['n']}-q{w['nq']}"
+    tag = f"{name}-n{w['n']}-q{w['nq']}"
     cm = os.path.join(args.cache, tag + ".hnsw") if args.cache else ""
     cgt = os.path.join(args.cache, tag + "-gt.npy") if args.cache else ""
     t_build = None
@@ -672,19 +773,21 @@ def run_hnsw(args, w):
                 sel = np.nonzero(ks[:m] == kk)[0]
                 pyoracle.hnsw_search(adj, X, w["metric"], Q[sel], kk, max(ef, kk), ent,
                                      nthreads=threads)
-        m = min(nq, 500)
-        t0 = time.perf_counter()
-        cpu_run(m)
-        rate = m / (time.perf_counter() - t0)
-        m = int(min(nq, max(500, rate * args.cpu_seconds)))
-        times = []
-        for rep in range(5):
-            _ = np.random.default_rng(rep).permutation(5_000_000)  # cache flush
+        ctx, flags = cpu_oracle()
+        with ctx:
+            m = min(nq, 500)
             t0 = time.perf_counter()
             cpu_run(m)
-            times.append(time.perf_counter() - t0)
+            rate = m / (time.perf_counter() - t0)
+            m = int(min(nq, max(500, rate * args.cpu_seconds)))
+            times = []
+            for rep in range(5):
+                _ = np.random.default_rng(rep).permutation(5_000_000)  # cache flush
+                t0 = time.perf_counter()
+                cpu_run(m)
+                times.append(time.perf_counter() - t0)
         cpu = {"value": m / float(np.median(times)), "unit": "queries/s", "cores": threads,
-               "kind": "port",
+               "kind": "port", "cpu_model": cpu_model(), "build": f"gcc {flags}",
                "sample": f"first {m} of the {nq} queries (k in {{10,100}}, ef max(ef, k) at the "
                          f"FP32 operating ef {ef}), 5 repetitions, median; oracle restatement "
                          f"of HNSW.hpp search"}
@@ -702,16 +805,56 @@ def run_hnsw(args, w):
         "dtype": {"fp8-rerank": "fp8", "bf16-rerank": "bf16",
                   "q8-rerank": "i8" if w["metric"] == "ip" else "u8"}.get(head, "f32"),
         "data": "synthetic (seeded generator; HNSW built on the GPU by this library)",
-        "config": {"workload": args.workload, "n": w["n"], "dim": w["d"], "nq": nq,
+        "config": {"workload": name, "n": w["n"], "dim": w["d"], "nq": nq,
                    "k": "U{10,100}", "graph": f"hnsw m={w['m']} efc={w['efc']}",
                    "headline_leg": head, "legs": results, "graph_build_s": t_build,
                    "gather_ceiling": gceil},
         "roofline": {"bound": "hbm", "achieved": h["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": h["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, w, head, h["ef"])},
+                     "traffic": measured_traffic(name, w, head, h["ef"])},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    idx.close()
+    return out
+
+
+# The workloads reported under config.secondary of the default run, with the
+# legs each reports: the 100k x 128 FP32 L2 graph (recall saturates below 0.95,
+# reported at its best beam), the exact-integer 1M x 128 uint8 beam sweep, and
+# the HNSW serving workload (closed loop and Poisson arrivals, p50/p99).
+SECONDARY = [
+    ("f32-l2-100k", dict(legs=["fp32-tree", "fp32-inorder", "fp8-rerank"])),
+    ("u8-l2-1M", dict(sweep=True)),
+    ("hnsw-ip-1M", dict(legs=["fp32-tree", "fp8-rerank", "fp8r-rerank", "q8-rerank"])),
+]
+
+
+def run_secondary(args):
+    import gc
+
+    import torch
+    out = {}
+    for name, over in SECONDARY:
+        a2 = copy.copy(args)
+        a2.legs = ""
+        a2.steps = min(args.steps, 10)
+        a2.warmup = min(args.warmup, 2)
+        a2.cpu_seconds = min(args.cpu_seconds, 3.0)
+        a2.calib_copy = False
+        a2.no_calib_gather = True
+        w = dict(WORKLOADS[name])
+        w.update(over)
+        t0 = time.time()
+        log(f"[bench] secondary workload {name}")
+        r = run_workload(a2, name, w)
+        r["config"].pop("gather_ceiling", None)
+        r["config"]["wall_s"] = time.time() - t0
+        for key in ("n_gpus", "higher_is_better", "scaling", "vs_baseline"):
+            r.pop(key, None)
+        out[name] = r
+        gc.collect()
+        torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

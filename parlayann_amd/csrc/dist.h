@@ -377,8 +377,12 @@ struct FusedGroup {
       : c(dim % 4 ? (dim / 4) / 8 : -1), s(dim % 4 ? (dim / 4) % 8 : -1) {}
 };
 
-// The eight steps of one 128-byte chunk; `first` starts the sum at zero, and
-// step `fs` (uniform, -1 for none) adds its terms fused.
+// The eight steps of one 128-byte chunk; `first` starts the sum at zero.  With
+// FUSE, step `fs` (uniform, -1 for none) adds its terms fused: both sums are
+// formed and a select keeps one, so the chains of the rows a lane interleaves
+// stay in one basic block.  Rows whose dimension is a multiple of 4 run the
+// FUSE = false body, which has no fused step at all.
+template <bool FUSE>
 __device__ __forceinline__ float inorder_chunk(float acc, const float* a, const float* b,
                                                bool first, int fs) {
 #pragma clang fp contract(off)
@@ -390,22 +394,23 @@ __device__ __forceinline__ float inorder_chunk(float acc, const float* a, const 
     float in;
     if (s == 0) in = first ? 0.0f : __int_as_float(dpp_mov<0x129>(__float_as_int(acc)));
     else in = __int_as_float(dpp_mov<0x111>(__float_as_int(acc)));
-    if (s == fs) {
-      acc = fmaf(a[0], b[0], in);
-      acc = fmaf(a[1], b[1], acc);
-      acc = fmaf(a[2], b[2], acc);
-      acc = fmaf(a[3], b[3], acc);
-    } else {
-      acc = in + t[0];
-      acc = acc + t[1];
-      acc = acc + t[2];
-      acc = acc + t[3];
+    float x = in + t[0];
+    x = x + t[1];
+    x = x + t[2];
+    x = x + t[3];
+    if constexpr (FUSE) {
+      float y = fmaf(a[0], b[0], in);
+      y = fmaf(a[1], b[1], y);
+      y = fmaf(a[2], b[2], y);
+      y = fmaf(a[3], b[3], y);
+      x = s == fs ? y : x;
     }
+    acc = x;
   }
   return acc;
 }
 
-template <int MT, int NCH>
+template <int MT, int NCH, bool FUSE>
 __device__ __forceinline__ float group_inorder_sum(const GroupDist<ST_F32, MT, NCH>& D,
                                                    const uint4* raw, FusedGroup fg) {
   float a[NCH][4], b[NCH][4];
@@ -413,7 +418,8 @@ __device__ __forceinline__ float group_inorder_sum(const GroupDist<ST_F32, MT, N
   for (int c = 0; c < NCH; c++) inorder_ops<MT>(D.q[c], raw[c], a[c], b[c]);
   float acc = 0.0f;
 #pragma unroll
-  for (int c = 0; c < NCH; c++) acc = inorder_chunk(acc, a[c], b[c], c == 0, c == fg.c ? fg.s : -1);
+  for (int c = 0; c < NCH; c++)
+    acc = inorder_chunk<FUSE>(acc, a[c], b[c], c == 0, c == fg.c ? fg.s : -1);
   return MT == M_L2 ? acc : -acc;
 }
 
@@ -422,10 +428,11 @@ __device__ __forceinline__ float group_inorder_sum(const GroupDist<ST_F32, MT, N
 // rows are not the search store's, and rows past 8 lines.  Rows are gathered
 // MAXC lines at a time, UP rows per group in flight; the sum carries over from
 // one block of lines to the next.
-template <int MT, int MAXC, int UP>
-__device__ __forceinline__ void eval_group_inorder_lds(const float* QL, int dim, const char* base,
-                                                       uint64_t stride, const uint32_t* NL,
-                                                       int ncmp, float* DL, int lane) {
+template <int MT, int MAXC, int UP, bool FUSE>
+__device__ __forceinline__ void eval_group_inorder_lds_t(const float* QL, int dim,
+                                                         const char* base, uint64_t stride,
+                                                         const uint32_t* NL, int ncmp, float* DL,
+                                                         int lane) {
   const int sub = lane & 7, grp = lane >> 3;
   const int nl = (dim * 4 + 127) >> 7;  // lines per row, taken MAXC at a time
   const FusedGroup fg(dim);
@@ -460,7 +467,7 @@ __device__ __forceinline__ void eval_group_inorder_lds(const float* QL, int dim,
           for (int u = 0; u < UP; u++) {
             float a[4], b[4];
             inorder_ops<MT>(q, raw[u][c], a, b);
-            acc[u] = inorder_chunk(first ? 0.0f : acc[u], a, b, first, fs);
+            acc[u] = inorder_chunk<FUSE>(first ? 0.0f : acc[u], a, b, first, fs);
           }
         }
       }
@@ -473,13 +480,23 @@ __device__ __forceinline__ void eval_group_inorder_lds(const float* QL, int dim,
   }
 }
 
+template <int MT, int MAXC, int UP>
+__device__ __forceinline__ void eval_group_inorder_lds(const float* QL, int dim, const char* base,
+                                                       uint64_t stride, const uint32_t* NL,
+                                                       int ncmp, float* DL, int lane) {
+  if (dim % 4)  // uniform
+    eval_group_inorder_lds_t<MT, MAXC, UP, true>(QL, dim, base, stride, NL, ncmp, DL, lane);
+  else
+    eval_group_inorder_lds_t<MT, MAXC, UP, false>(QL, dim, base, stride, NL, ncmp, DL, lane);
+}
+
 // eval_group's gather with the in-order sum: DL[r] comes from lane 7 of the
 // row's group.
-template <int MT, int NCH>
-__device__ __forceinline__ void eval_group_inorder(const GroupDist<ST_F32, MT, NCH>& D,
-                                                   const char* base, uint64_t stride,
-                                                   int row_bytes, const uint32_t* NL, int ncmp,
-                                                   float* DL, int lane) {
+template <int MT, int NCH, bool FUSE>
+__device__ __forceinline__ void eval_group_inorder_t(const GroupDist<ST_F32, MT, NCH>& D,
+                                                     const char* base, uint64_t stride,
+                                                     int row_bytes, const uint32_t* NL,
+                                                     int ncmp, float* DL, int lane) {
   constexpr int UP = NCH <= 2 ? 8 : (NCH <= 4 ? 4 : 2);
   const int sub = lane & 7, grp = lane >> 3;
   const int nl = (row_bytes + 127) >> 7;
@@ -501,10 +518,21 @@ __device__ __forceinline__ void eval_group_inorder(const GroupDist<ST_F32, MT, N
 #pragma unroll
     for (int u = 0; u < UP; u++) {
       const int r = r0 + u * 8 + grp;
-      const float s = group_inorder_sum<MT, NCH>(D, raw[u], fg);
+      const float s = group_inorder_sum<MT, NCH, FUSE>(D, raw[u], fg);
       if (sub == 7 && r < ncmp) DL[r] = s;
     }
   }
+}
+
+template <int MT, int NCH>
+__device__ __forceinline__ void eval_group_inorder(const GroupDist<ST_F32, MT, NCH>& D,
+                                                   const char* base, uint64_t stride,
+                                                   int row_bytes, const uint32_t* NL, int ncmp,
+                                                   float* DL, int lane) {
+  if (row_bytes % 16)  // uniform: a dimension that is not a multiple of 4
+    eval_group_inorder_t<MT, NCH, true>(D, base, stride, row_bytes, NL, ncmp, DL, lane);
+  else
+    eval_group_inorder_t<MT, NCH, false>(D, base, stride, row_bytes, NL, ncmp, DL, lane);
 }
 
 // ---------------------------------------------------------------------------
