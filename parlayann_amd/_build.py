@@ -15,7 +15,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpann.so")
-SOURCES = ["beam_search.hip", "quantize.hip", "prune.hip", "hnsw.hip", "bruteforce.hip", "pann_capi.cpp"]
+# the beam-search kernels are spread over one translation unit per store (bs_*.hip)
+SOURCES = ["beam_search.hip", "bs_f32.hip", "bs_f32_inorder.hip", "bs_u8.hip", "bs_i8.hip",
+           "bs_bf16.hip", "bs_fp8.hip", "bs_fp8r.hip", "quantize.hip", "prune.hip", "hnsw.hip",
+           "bruteforce.hip", "pann_capi.cpp"]
 # the device's random row-gather ceiling, for bench.py's roofline (not part of libpann)
 CALIB_SRC = "calib.hip"
 CALIB_LIB = os.path.join(HERE, "libpann_calib.so")

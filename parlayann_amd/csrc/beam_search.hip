@@ -1,807 +1,18 @@
-// beam_search.hip -- wave-per-query beam search over a fixed-degree graph.
-//
-// Restates filtered_beam_search (beamSearch.h:33-228, use_filtering = false)
-// for one query per 64-lane wave, replaying the reference's sequential rules
-// exactly:
-//   * one visit per step: the closest unvisited entry of the (possibly stale)
-//     frontier (beamSearch.h:116-127);
-//   * neighbours G[cur][0 .. min(deg, degree_limit)) not seen before and not
-//     the query's own row are distance-compared (:151-160, :176-182); the seen
-//     set is exact (LDS hash with a per-wave global overflow table), like the
-//     reference's hashset before it ever grows (hashset.h:12-55);
-//   * candidates with d >= cutoff (last frontier distance when the frontier is
-//     full) are dropped (:173-181);
-//   * the merge is deferred while |candidates| < batch_factor * beam and more
-//     unvisited entries remain (:185-192), |candidates| counted before dedup;
-//   * merge = sort by (dist, id), union with the frontier, trim to beam
-//     (:197-222), done here as a rank merge in LDS.
-// Results: the first k frontier entries (beam_search_rerank :588-596), or with
-// a full-precision FP32 store the rerank branch (:569-587): FP32 distances of
-// the first min(k * rerank_factor, |frontier|) entries, sorted, first k.
-//
-// Memory: one wave per workgroup, persistent grid pulling query indices from
-// a device counter.  LDS per wave: two frontier buffers (u64 keys + visited
-// flags), candidate buffers, the gather list and the visited hash.  The FP32
-// query row for the rerank is staged in the candidate buffers once the search
-// is over; it has a region of its own only for the in-order
-// sum of rows past 8 lines, or when it is longer than those buffers.
-#include "kernels.h"
-#include "dist.h"
-
-#include <float.h>
-
-#include <atomic>
-#include <mutex>
+// beam_search.hip -- host-side launch of the beam-search kernels
+// (beam_search_kernel.h), whose instantiations are spread over one
+// translation unit per store (bs_*.hip).
+#include "beam_search_kernel.h"
 
 namespace pann {
-
-struct WaveLds {
-  uint64_t *F0, *F1;
-  uint8_t *FV0, *FV1;
-  uint64_t* C;
-  uint64_t* Cs;
-  uint32_t* NL;
-  float* DL;
-  float* QL;
-  uint32_t* H;
-  uint32_t* RC;
-  uint32_t* FB;
-  float* CT;  // FP8 residual store: q . c for every centroid c
-};
-
-__host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
-
-__host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap, uint32_t dim,
-                                                   uint32_t hslots_log2, uint32_t rc_log2,
-                                                   uint32_t fb_log2, uint32_t ncent,
-                                                   uint32_t q_own) {
-  uint32_t b = 0;
-  b += 2 * align16(fcap * 8);
-  b += 2 * align16(fcap);
-  b += 2 * align16(ccap * 8);
-  b += align16(128 * 4) * 2;
-  b += q_own ? align16(dim * 4) : 0u;
-  b += (1u << hslots_log2) * 4;
-  b += rc_log2 ? (1u << rc_log2) * 4 : 0u;
-  b += fb_log2 ? (1u << fb_log2) / 8 : 0u;
-  b += align16(ncent * 4);
-  return b;
-}
-
-__device__ inline WaveLds carve(char* p, const SearchArgs& a) {
-  WaveLds L;
-  uint32_t o = 0;
-  L.F0 = (uint64_t*)(p + o); o += align16(a.fcap * 8);
-  L.F1 = (uint64_t*)(p + o); o += align16(a.fcap * 8);
-  L.FV0 = (uint8_t*)(p + o); o += align16(a.fcap);
-  L.FV1 = (uint8_t*)(p + o); o += align16(a.fcap);
-  L.C = (uint64_t*)(p + o); o += align16(a.ccap * 8);
-  L.Cs = (uint64_t*)(p + o); o += align16(a.ccap * 8);
-  L.NL = (uint32_t*)(p + o); o += align16(128 * 4);
-  L.DL = (float*)(p + o); o += align16(128 * 4);
-  // without a region of its own the query row shares the candidate buffers C, Cs
-  L.QL = a.q_own ? (float*)(p + o) : (float*)L.C;
-  o += a.q_own ? align16(a.dim * 4) : 0u;
-  L.H = (uint32_t*)(p + o);
-  o += (1u << a.hslots_log2) * 4;
-  L.RC = (uint32_t*)(p + o);
-  o += a.rc_log2 ? (1u << a.rc_log2) * 4 : 0u;
-  L.FB = (uint32_t*)(p + o);
-  o += a.fb_log2 ? (1u << a.fb_log2) / 8 : 0u;
-  L.CT = (float*)(p + o);
-  return L;
-}
-
-
-// The LDS seen set: open addressing over buckets of 4 slots, probed bucket
-// after bucket; one ds_read_b128 reads a whole bucket, so a lookup almost
-// always resolves in one LDS access even for the slowest lane of the wave.
-// Slots only ever go from empty to a key, so a key inserted into bucket b
-// found every bucket before b full, and they stay full: a lookup that meets
-// a bucket with an empty slot can stop.
-__device__ __forceinline__ uint32_t bucket_of(uint32_t key, uint32_t log2_slots) {
-  return (key * 0x9E3779B1u) >> (32u - (log2_slots - 2u));
-}
-
-__device__ __forceinline__ bool lds_contains(const uint32_t* H, uint32_t log2n, uint32_t key) {
-  const uint32_t bmask = (1u << (log2n - 2u)) - 1u;
-  uint32_t b = bucket_of(key, log2n);
-  for (;;) {
-    const uint4 v = *(const uint4*)(H + 4u * b);
-    if (v.x == key || v.y == key || v.z == key || v.w == key) return true;
-    if (v.x == PANN_SENTINEL || v.y == PANN_SENTINEL || v.z == PANN_SENTINEL ||
-        v.w == PANN_SENTINEL)
-      return false;
-    b = (b + 1u) & bmask;
-  }
-}
-
-// Exact insert-or-test.  Returns true when `key` was absent and is now in.
-__device__ __forceinline__ bool lds_insert(uint32_t* H, uint32_t log2n, uint32_t key) {
-  const uint32_t bmask = (1u << (log2n - 2u)) - 1u;
-  uint32_t b = bucket_of(key, log2n);
-  for (;;) {
-    const uint4 v = *(const uint4*)(H + 4u * b);
-    if (v.x == key || v.y == key || v.z == key || v.w == key) return false;
-    int j = v.x == PANN_SENTINEL ? 0
-            : v.y == PANN_SENTINEL ? 1
-            : v.z == PANN_SENTINEL ? 2
-            : v.w == PANN_SENTINEL ? 3 : -1;
-    if (j < 0) {
-      b = (b + 1u) & bmask;
-      continue;
-    }
-    const uint32_t old = atomicCAS(&H[4u * b + (uint32_t)j], PANN_SENTINEL, key);
-    if (old == PANN_SENTINEL) return true;
-    if (old == key) return false;
-    // another lane took this slot for a different key: read the bucket again
-  }
-}
-
-// Open addressing with linear probing (the global overflow table).
-__device__ __forceinline__ uint32_t hash_slot(uint32_t key, uint32_t log2n) {
-  return (key * 0x9E3779B1u) >> (32u - log2n);
-}
-
-// Per-wave global table of 64-bit slots (tag << 32 | id).  A slot whose tag
-// differs from this query's tag is empty, so the table is never cleared:
-// every query of every launch carries a distinct tag.
-__device__ __forceinline__ bool global_insert(unsigned long long* G, uint32_t log2n, uint32_t tag,
-                                              uint32_t key) {
-  const uint32_t mask = (1u << log2n) - 1u;
-  const unsigned long long mine = ((unsigned long long)tag << 32) | key;
-  uint32_t h = hash_slot(key, log2n);
-  unsigned long long old = __hip_atomic_load(&G[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (;;) {
-    if ((uint32_t)(old >> 32) != tag) {
-      unsigned long long prev = atomicCAS(&G[h], old, mine);
-      if (prev == old) return true;
-      old = prev;  // another lane won this slot: look at it again
-      continue;
-    }
-    if ((uint32_t)old == key) return false;
-    h = (h + 1u) & mask;
-    old = __hip_atomic_load(&G[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-// index of the first entry of FV[start..f) that is 0 (unvisited), or -1
-__device__ __forceinline__ int first_unvisited(const uint8_t* FV, int f, int start, int lane) {
-  for (int t = start; t < f; t += PANN_WAVE) {
-    int i = t + lane;
-    uint64_t m = ballot(i < f && FV[i] == 0);
-    if (m) return t + (int)__builtin_ctzll(m);
-  }
-  return -1;
-}
-
-__device__ __forceinline__ int wave_min_i32(int v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    int w = __shfl_xor(v, o);
-    v = w < v ? w : v;
-  }
-  return v;
-}
-
-__device__ __forceinline__ bool global_contains(const unsigned long long* G, uint32_t log2n,
-                                                uint32_t tag, uint32_t key) {
-  const uint32_t mask = (1u << log2n) - 1u;
-  uint32_t h = hash_slot(key, log2n);
-  for (;;) {
-    unsigned long long v = __hip_atomic_load(&G[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((uint32_t)(v >> 32) != tag) return false;
-    if ((uint32_t)v == key) return true;
-    h = (h + 1u) & mask;
-  }
-}
-
-// number of entries of sorted A[0..n) that are < x
-__device__ __forceinline__ int lower_bound_u64(const uint64_t* A, int n, uint64_t x) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (A[mid] < x) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ uint32_t rc_slot(uint32_t id, uint32_t log2n) {
-  return (id * 0x9E3779B1u) >> (32u - log2n);
-}
-
-// Frontier-id Bloom filter: two bits per id, from two multiplicative hashes
-// other than the tables' own.
-__device__ __forceinline__ uint32_t fb_h1(uint32_t id, uint32_t log2n) {
-  return (id * 0x85EBCA77u) >> (32u - log2n);
-}
-__device__ __forceinline__ uint32_t fb_h2(uint32_t id, uint32_t log2n) {
-  return ((id ^ 0x5BD1E995u) * 0xC2B2AE3Du) >> (32u - log2n);
-}
-__device__ __forceinline__ bool fb_test(const uint32_t* FB, uint32_t log2n, uint32_t id) {
-  const uint32_t a = fb_h1(id, log2n), b = fb_h2(id, log2n);
-  return ((FB[a >> 5] >> (a & 31u)) & (FB[b >> 5] >> (b & 31u)) & 1u) != 0u;
-}
-
-// whether sorted A[0..n) holds x
-__device__ __forceinline__ bool sorted_contains(const uint64_t* A, int n, uint64_t x) {
-  int lo = 0, len = n;
-  while (len > 0) {
-    const int half = len >> 1;
-    if (A[lo + half] < x) {
-      lo += half + 1;
-      len -= half + 1;
-    } else {
-      len = half;
-    }
-  }
-  return lo < n && A[lo] == x;
-}
-
-template <int ST, int MT, int NCH, bool SEQ>
-__device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
-                                          const WaveLds& L, int ncmp, int lane) {
-  if constexpr (SEQ) {
-    // rows of up to 8 lines: the group gather with the in-order lane chain and the
-    // query in registers; longer rows (NCH 8 only): 8 lines at a time, query in LDS
-    if (NCH < 8 || a.row_bytes <= 8u * 128u)
-      eval_group_inorder<MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, L.NL, ncmp, L.DL,
-                                  lane);
-    else
-      eval_group_inorder_lds<MT, 8, 1>(L.QL, (int)a.dim, a.pts, a.pts_stride, L.NL, ncmp, L.DL,
-                                       lane);
-  } else {
-    eval_group<ST, MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, L.NL, ncmp, L.DL, lane);
-  }
-}
-
-// One beam search of a query: the graph it walks, its beam, k, limit and
-// start points.  A launch runs the HNSW upper levels as passes at beam 1
-// (not final: the nearest id is handed on, nothing is written) and then the
-// main search (final: results and counters written).
-struct Pass {
-  const uint32_t* nbrs;
-  uint32_t nbr_stride;
-  int dl;              // neighbours read per row: min(degree_limit, max degree)
-  int B, k;
-  int64_t limit;
-  const uint32_t* sp;  // start ids, or nullptr: the single id start1
-  int ns;
-  uint32_t start1;
-  uint32_t pass;       // index of this pass within the query (seen-table tag)
-  bool final;
-  uint32_t vis0, cmps0;  // visits and evaluations of the earlier passes
-};
-struct PassOut {
-  uint32_t best;  // id of the nearest frontier entry
-  uint32_t vis, cmps;
-  bool ok;
-};
-
-template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
-__device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, int lane,
-                              unsigned long long* ghash, const Pass& P) {
-  const uint32_t gtag = a.ghash_tag_base + q * (a.nup + 1u) + P.pass + 1u;
-  using QT = typename QueryElem<ST>::T;
-  const int B = P.B;
-  const int k = P.k;
-  const uint32_t self = a.self_ids ? a.self_ids[q] : PANN_SENTINEL;
-  // |C| < batch_factor * B  <=>  |C| < ceil(batch_factor * B) for integer |C|
-  const int thr = (int)ceil(a.batch_factor * (double)B);
-  const bool defer_ok = P.limit >= 2 * (int64_t)B;
-  const int dl = P.dl;
-  const uint32_t hlog2 = a.hslots_log2;
-  const uint32_t hslots = 1u << hlog2;
-  const uint32_t* sp = P.sp;
-  const int ns = P.ns;
-  if (B < 1 || B > (int)a.fcap || k < 0 || k > B || (P.final && k > (int)a.out_stride) ||
-      ns < 1 || ns > B) {  // per-query values outside what this launch was sized for
-    if (lane == 0) atomicOr(a.status, PANN_ST_BAD_PARAMS);
-    return PassOut{PANN_SENTINEL, P.vis0, P.cmps0, false};
-  }
-
-  // ---- query into registers (group mode) and LDS (sequential / rerank) ----
-  GroupDist<ST, MT, NCH> D;
-  const uint64_t qr = a.self_ids ? (uint64_t)self : (uint64_t)q;  // build mode: the point's own row
-  const char* qrow = a.queries + qr * a.q_stride;
-  D.load_query((const QT*)qrow, a.scales, (int)a.dim, lane & 7);
-  const float* fq = SEQ ? (const float*)qrow : (const float*)(a.fqueries + qr * a.fq_stride);
-  if (a.q_own && (SEQ || a.full))
-    for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
-  // ---- clear the visited hash and the cache of evaluated ids ----
-  for (uint32_t i = lane * 4; i < hslots; i += PANN_WAVE * 4)
-    *(uint4*)(L.H + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
-  const uint32_t rcl = EX ? 0u : a.rc_log2;
-  // the filter is a separate instantiation (FB), so that kernels without it
-  // keep their register count and occupancy
-  const uint32_t fbl = FB ? a.fb_log2 : 0u;
-  if (rcl)
-    for (uint32_t i = lane * 4; i < (1u << rcl); i += PANN_WAVE * 4)
-      *(uint4*)(L.RC + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
-  wave_lds_sync();
-  if constexpr (ST == ST_FP8R) {
-    // the query's row of q . c (one per centroid, ncent a multiple of 4),
-    // into LDS once per query (its first pass)
-    if (P.pass == 0) {
-      const float* row = a.qtable + qr * a.ncent;
-      for (uint32_t c0 = 4u * lane; c0 < a.ncent; c0 += 4u * PANN_WAVE)
-        *(float4*)(L.CT + c0) = *(const float4*)(row + c0);
-      wave_lds_sync();
-    }
-    D.T = L.CT;
-  }
-
-  bool ovf = false;
-  uint32_t lds_count = 0, g_count = 0;
-  const uint32_t glog2 = a.ghash_log2;
-  uint32_t status = 0;
-
-  // ---- starting points (beamSearch.h:71-93) ----
-  // evaluated 64 at a time into F1, then rank-sorted into F0 (equal keys keep
-  // their order, so repeated starts stay repeated as in the reference)
-  for (int t = 0; t < ns; t += PANN_WAVE) {
-    const int m = ns - t < PANN_WAVE ? ns - t : PANN_WAVE;
-    if (lane < m) {
-      const uint32_t s = sp ? sp[t + lane] : P.start1;
-      L.NL[lane] = s;
-      if constexpr (EX) lds_insert(L.H, hlog2, s);
-    }
-    wave_lds_sync();
-    eval_rows<ST, MT, NCH, SEQ>(D, a, L, m, lane);
-    wave_lds_sync();
-    if (lane < m) L.F1[t + lane] = make_key(L.DL[lane], L.NL[lane]);
-    wave_lds_sync();
-  }
-  lds_count = (uint32_t)ns;
-  int cur_buf = 0;
-  for (int j = lane; j < ns; j += PANN_WAVE) {
-    const uint64_t x = L.F1[j];
-    int r = 0;
-    for (int i = 0; i < ns; i++) {
-      const uint64_t y = L.F1[i];
-      r += (y < x) || (y == x && i < j);
-    }
-    L.F0[r] = x;
-    L.FV0[r] = 0;
-  }
-  wave_lds_sync();
-  int f = ns;
-  uint32_t nvis = 0;
-  uint32_t cmps = (uint32_t)ns;
-
-  // The frontier-id filter: a Bloom filter of the ids in the frontier,
-  // rebuilt whenever the frontier changes (here and after every merge).
-  // A neighbour it reports is looked up exactly in F before its row is
-  // gathered; one in F was seen before, so the reference's seen set skips it
-  // and so does this search (its evaluation would only be discarded).
-  auto fb_rebuild = [&](const uint64_t* Fx, int fx) {
-    for (uint32_t i = lane; i < ((1u << fbl) >> 5); i += PANN_WAVE) L.FB[i] = 0u;
-    wave_lds_sync();
-    for (int i = lane; i < fx; i += PANN_WAVE) {
-      const uint32_t id = key_id(Fx[i]);
-      const uint32_t h1 = fb_h1(id, fbl), h2 = fb_h2(id, fbl);
-      atomicOr(&L.FB[h1 >> 5], 1u << (h1 & 31u));
-      atomicOr(&L.FB[h2 >> 5], 1u << (h2 & 31u));
-    }
-    wave_lds_sync();
-  };
-  // Which of two candidate lists (ids a where ca, ids b where cb) are in
-  // F[0..f).  A wave-wide operation, called by all 64 lanes: only when the
-  // filter reports something, each lane loads the ids of F[c + lane + 64 i]
-  // (i < 8) for each 512-entry chunk c, and each filter hit still unresolved
-  // costs one compare per register and a ballot.
-  auto in_frontier2 = [&](const uint64_t* F, uint32_t ida, bool ca, uint32_t idb, bool cb,
-                          bool& fa, bool& fbb) {
-    uint64_t ma = ballot(ca && fb_test(L.FB, fbl, ida));
-    uint64_t mb = ballot(cb && fb_test(L.FB, fbl, idb));
-    fa = fbb = false;
-    const uint32_t* Fw = (const uint32_t*)F;  // key_id = the low word of a key
-    for (int c0 = 0; c0 < f && (ma | mb) != 0; c0 += 8 * PANN_WAVE) {
-      uint32_t fid[8];
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-        const int t = c0 + lane + i * PANN_WAVE;
-        fid[i] = t < f ? Fw[2 * t] : PANN_SENTINEL;
-      }
-      for (uint64_t m = ma; m; m &= m - 1) {
-        const int j = __ffsll((long long)m) - 1;
-        const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)ida, j);
-        bool any = false;
-#pragma unroll
-        for (int i = 0; i < 8; i++) any |= fid[i] == x;
-        if (ballot(any) != 0 && lane == j) fa = true;
-      }
-      for (uint64_t m = mb; m; m &= m - 1) {
-        const int j = __ffsll((long long)m) - 1;
-        const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)idb, j);
-        bool any = false;
-#pragma unroll
-        for (int i = 0; i < 8; i++) any |= fid[i] == x;
-        if (ballot(any) != 0 && lane == j) fbb = true;
-      }
-      ma &= ~ballot(fa);  // found: no later chunk needs to look
-      mb &= ~ballot(fbb);
-    }
-  };
-  if (fbl) fb_rebuild(L.F0, f);
-  int c = 0;
-
-  // ---- main loop (beamSearch.h:116-223) ----
-  // u = the first unvisited frontier entry (unvisited_frontier[offset] of the
-  // reference: entries before it are all visited).  The adjacency row of the
-  // next candidate for the following step is loaded while this step gathers.
-  int u = first_unvisited(L.FV0, f, 0, lane);
-  uint32_t pref_id = PANN_SENTINEL;
-  uint32_t nb_pref = PANN_SENTINEL;
-  // Frontier-and-pending seen set: rows [lo, hi) of NL/DL with d < cutoff
-  // join C unless their key is already in the frontier F (binary search on
-  // the exact (dist, id) key: a row's distance is computed the same way every
-  // time) or already pending in C (the small LDS table H, emptied at every
-  // merge).  An id the reference's seen set would also drop is dropped here.
-  auto accept_range = [&](const uint64_t* F, int lo, int hi, float cutoff) {
-    const int r = lo + lane;
-    bool ins = false;
-    uint64_t key = 0;
-    if (r < hi) {
-      const float d = L.DL[r];
-      key = make_key(d, L.NL[r]);
-      // (with the frontier-id filter on no evaluated id is in F; the exact
-      // check stays, at the cost of one binary search per id under the cutoff)
-      if (!(d >= cutoff) && !sorted_contains(F, f, key)) ins = lds_insert(L.H, hlog2, key_id(key));
-      if (rcl) L.RC[rc_slot(key_id(key), rcl)] = key_id(key);  // evaluated and committed
-    }
-    const uint64_t mi = ballot(ins);
-    if (ins) L.C[c + (int)mbcnt(mi)] = key;
-    c += __popcll(mi);
-  };
-  auto record_visit = [&](uint64_t k) {
-    if (P.final && a.vis_out && lane == 0) {
-      if (nvis < a.vis_cap) a.vis_out[(uint64_t)q * a.vis_cap + nvis] = k;
-      else status |= PANN_ST_VISITED_OVERFLOW;
-    }
-    nvis++;
-  };
-  auto adjacency = [&](uint32_t id) -> uint32_t {
-    return lane < dl ? P.nbrs[(uint64_t)id * P.nbr_stride + lane] : PANN_SENTINEL;
-  };
-
-  for (;;) {
-    uint64_t* F = cur_buf ? L.F1 : L.F0;
-    uint8_t* FV = cur_buf ? L.FV1 : L.FV0;
-    if (u < 0 || (int64_t)nvis >= P.limit) break;
-    const uint64_t cur_key = F[u];
-    const uint32_t cur = key_id(cur_key);
-    if (lane == 0) FV[u] = 1;
-    record_visit(cur_key);
-    const bool full = (f == B);
-    const float cutoff = full ? key_dist(F[f - 1]) : FLT_MAX;
-
-    // neighbours of cur (graph rows are padded with the sentinel past deg)
-    const uint32_t nb = cur == pref_id ? nb_pref : adjacency(cur);
-    // next unvisited entry of this (stale) frontier
-    const int nu = first_unvisited(FV, f, u + 1, lane);
-
-    if constexpr (EX) {
-      // reference counting: every id enters the seen set at first encounter
-      // (has_been_seen, :156), so dist_cmps equals the reference's
-      pref_id = nu >= 0 ? key_id(F[nu]) : PANN_SENTINEL;
-      nb_pref = nu >= 0 ? adjacency(pref_id) : PANN_SENTINEL;
-      const bool valid = nb != PANN_SENTINEL;
-      const int nvalid = __popcll(ballot(valid));
-      if (!ovf && lds_count + (uint32_t)nvalid > (hslots >> 1)) ovf = true;
-      bool isnew = false;
-      if (valid) {
-        if (!ovf) {
-          isnew = lds_insert(L.H, hlog2, nb);
-        } else if (!lds_contains(L.H, hlog2, nb)) {
-          if (g_count + PANN_WAVE < (1u << glog2)) isnew = global_insert(ghash, glog2, gtag, nb);
-          else isnew = true;  // table exhausted: flagged below
-        }
-      }
-      const uint64_t mins = ballot(isnew);
-      if (!ovf) lds_count += (uint32_t)__popcll(mins);
-      else {
-        g_count += (uint32_t)__popcll(mins);
-        if (g_count + PANN_WAVE >= (1u << glog2)) status |= PANN_ST_HASH_CAPACITY;
-      }
-      const bool evalme = isnew && nb != self;  // Points[a].same_as(p) (:156)
-      const uint64_t mnew = ballot(evalme);
-      const int ncmp = __popcll(mnew);
-      cmps += (uint32_t)ncmp;
-      if (evalme) L.NL[mbcnt(mnew)] = nb;
-      wave_lds_sync();
-      eval_rows<ST, MT, NCH, SEQ>(D, a, L, ncmp, lane);
-      wave_lds_sync();
-      bool keep = false;
-      uint64_t key = 0;
-      if (lane < ncmp) {
-        const float d = L.DL[lane];
-        keep = !(d >= cutoff);
-        key = make_key(d, L.NL[lane]);
-      }
-      const uint64_t mk = ballot(keep);
-      if (keep) L.C[c + (int)mbcnt(mk)] = key;
-      c += __popcll(mk);
-      wave_lds_sync();
-      // defer while the candidates are few and the stale frontier still has
-      // unvisited entries (:185-192)
-      if (c == 0 || (defer_ok && c < thr && nu >= 0)) {
-        u = nu;
-        continue;
-      }
-    } else {
-      // Every neighbour is evaluated; the seen test happens after the
-      // distance (accept_range).  Which ids the reference's seen set would
-      // skip, and this kernel evaluates:
-      //   * ids rejected earlier: the cutoff never rises once the frontier is
-      //     full (and nothing is rejected before), so they are rejected again;
-      //   * ids once accepted and since evicted from the frontier: eviction
-      //     leaves the frontier full of B smaller keys, so they now fail the
-      //     cutoff too;
-      //   * ids in the frontier: skipped before their gather when the
-      //     frontier-id filter is on, else found by accept_range;
-      //   * ids pending in C: found by accept_range.
-      // Candidates, merges and visits are therefore the reference's; only the
-      // count of distance evaluations exceeds its dist_cmps.
-      //
-      // Two visits per step: while merges are deferred the next visit is nu
-      // with the same stale frontier and cutoff, so its neighbours are
-      // gathered together with cur's.  Its results are committed only if the
-      // first visit defers; its accepted ids are inserted after cur's, so an
-      // id both lists share is taken once, for cur, as in the reference.
-      const bool have2 = a.lookahead && nu >= 0 && (int64_t)nvis < P.limit;
-      const uint64_t key2 = have2 ? F[nu] : 0;
-      const uint32_t id2 = key_id(key2);
-      const uint32_t nb2 = !have2 ? PANN_SENTINEL : (id2 == pref_id ? nb_pref : adjacency(id2));
-      const int nnu = have2 ? first_unvisited(FV, f, nu + 1, lane) : -1;
-      const int nxt = have2 ? nnu : nu;
-      pref_id = nxt >= 0 ? key_id(F[nxt]) : PANN_SENTINEL;
-      nb_pref = nxt >= 0 ? adjacency(pref_id) : PANN_SENTINEL;
-      // an id found in the cache of committed evaluations was seen before:
-      // the reference's seen set skips it, and its outcome here would be
-      // the same (rejected again, or already in F or C), so it is not
-      // evaluated again
-      bool e1 = nb != PANN_SENTINEL && nb != self && !(rcl && L.RC[rc_slot(nb, rcl)] == nb);
-      bool e2 = have2 && nb2 != PANN_SENTINEL && nb2 != self &&
-                !(rcl && L.RC[rc_slot(nb2, rcl)] == nb2);
-      if (fbl) {  // ids in the frontier were seen: not gathered again
-        // a wave-wide call: every lane takes part, whatever its own flags
-        bool f1, f2;
-        in_frontier2(F, nb, e1, nb2, e2, f1, f2);
-        e1 = e1 && !f1;
-        e2 = e2 && !f2;
-      }
-      const uint64_t m1 = ballot(e1), m2 = ballot(e2);
-      const int n1 = __popcll(m1), n2 = __popcll(m2);
-      if (e1) L.NL[mbcnt(m1)] = nb;
-      if (e2) L.NL[n1 + (int)mbcnt(m2)] = nb2;
-      wave_lds_sync();
-      eval_rows<ST, MT, NCH, SEQ>(D, a, L, n1 + n2, lane);
-      wave_lds_sync();
-      accept_range(F, 0, n1, cutoff);
-      cmps += (uint32_t)n1;
-      wave_lds_sync();
-      if (c == 0 || (defer_ok && c < thr && nu >= 0)) {
-        if (!have2) {
-          u = nu;
-          continue;
-        }
-        if (lane == 0) FV[nu] = 1;
-        record_visit(key2);
-        accept_range(F, n1, n1 + n2, cutoff);
-        cmps += (uint32_t)n2;
-        wave_lds_sync();
-        if (c == 0 || (defer_ok && c < thr && nnu >= 0)) {
-          u = nnu;
-          continue;
-        }
-      }
-    }
-
-    // ---- merge: rank-sort C, then place C and F by rank into the other buffer
-    for (int j = lane; j < c; j += PANN_WAVE) {
-      uint64_t x = L.C[j];
-      int r = 0;
-      for (int i = 0; i < c; i++) r += (L.C[i] < x);
-      L.Cs[r] = x;
-    }
-    wave_lds_sync();
-    uint64_t* F2 = cur_buf ? L.F0 : L.F1;
-    uint8_t* FV2 = cur_buf ? L.FV0 : L.FV1;
-    int first = 0x7FFFFFFF;  // lowest new position of an unvisited entry
-    for (int j = lane; j < c; j += PANN_WAVE) {
-      uint64_t x = L.Cs[j];
-      int pos = j + lower_bound_u64(F, f, x);
-      if (pos < B) {
-        F2[pos] = x;
-        FV2[pos] = 0;
-        first = pos < first ? pos : first;
-      }
-    }
-    for (int i = lane; i < f; i += PANN_WAVE) {
-      uint64_t x = F[i];
-      int pos = i + lower_bound_u64(L.Cs, c, x);
-      if (pos < B) {
-        F2[pos] = x;
-        uint8_t v = FV[i];
-        FV2[pos] = v;
-        if (!v) first = pos < first ? pos : first;
-      }
-    }
-    f = f + c < B ? f + c : B;
-    c = 0;
-    cur_buf ^= 1;
-    if constexpr (!EX) {  // C is empty again: so is its table
-      for (uint32_t i = lane * 4; i < hslots; i += PANN_WAVE * 4)
-        *(uint4*)(L.H + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
-    }
-    u = wave_min_i32(first);
-    if (u == 0x7FFFFFFF) u = -1;
-    wave_lds_sync();
-    if (fbl) fb_rebuild(cur_buf ? L.F1 : L.F0, f);
-  }
-
-  const uint64_t* F = cur_buf ? L.F1 : L.F0;
-  if (!P.final) {  // an upper HNSW level: W[0], the nearest (HNSW.hpp:1557-1558)
-    const uint32_t best = key_id(F[0]);
-    if (lane == 0 && status) atomicOr(a.status, status);
-    wave_lds_sync();
-    return PassOut{best, P.vis0 + nvis, P.cmps0 + cmps, true};
-  }
-  // ---- results (beam_search_rerank :557-596) ----
-  uint32_t* oi = a.out_ids + (uint64_t)q * a.out_stride;
-  float* od = a.out_dists + (uint64_t)q * a.out_stride;
-  if (f < k && !a.short_ok) status |= PANN_ST_SHORT_BEAM;
-  if (a.full) {
-    double nc = (double)k * a.rerank_factor;
-    int nchk = nc < (double)f ? (int)nc : f;
-    uint64_t* R = cur_buf ? L.F0 : L.F1;
-    if (!a.q_own) {  // into the candidate buffers: no candidate is pending after the search
-      for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
-      wave_lds_sync();
-    }
-    for (int t = 0; t < nchk; t += PANN_WAVE) {
-      int m = nchk - t < PANN_WAVE ? nchk - t : PANN_WAVE;
-      if (lane < m) L.NL[lane] = key_id(F[t + lane]);
-      wave_lds_sync();
-      // 8-lane gather and in-order lane chain in the kernels of the reduced-precision
-      // stores.  One lane per row for the FP8 L2 kernels (their query scales leave no
-      // registers for it) and for the FP32 store's own kernels, whose rerank is rare and
-      // whose search slowed by 1.5 % with it compiled in.
-      constexpr bool group_rr = ST != ST_F32 && !(ST == ST_FP8 && MT == M_L2);
-      if (group_rr && a.dim <= 128u)
-        eval_group_inorder_lds<MT, 4, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
-                                         lane);
-      else if (group_rr)
-        eval_group_inorder_lds<MT, 8, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
-                                         lane);
-      else
-        eval_seq_f32<MT>(L.QL, a.full, a.full_stride, (int)a.dim, L.NL, m, L.DL, lane);
-      wave_lds_sync();
-      if (lane < m) R[t + lane] = make_key(L.DL[lane], L.NL[lane]);
-      wave_lds_sync();
-    }
-    for (int j = lane; j < nchk; j += PANN_WAVE) {
-      uint64_t x = R[j];
-      int r = 0;
-      for (int i = 0; i < nchk; i++) r += (R[i] < x);
-      if (r < k) {
-        oi[r] = key_id(x);
-        od[r] = key_dist(x);
-      }
-    }
-  } else {
-    for (int j = lane; j < k && j < f; j += PANN_WAVE) {
-      oi[j] = key_id(F[j]);
-      od[j] = key_dist(F[j]);
-    }
-  }
-  if (lane == 0) {
-    if (a.out_visited)
-      a.out_visited[q] = (a.accumulate ? a.out_visited[q] : 0u) + P.vis0 + nvis;
-    if (a.out_cmps) a.out_cmps[q] = (a.accumulate ? a.out_cmps[q] : 0u) + P.cmps0 + cmps;
-    if (a.out_nfront) a.out_nfront[q] = (uint32_t)f;
-    if (status) atomicOr(a.status, status);
-  }
-  wave_lds_sync();
-  return PassOut{PANN_SENTINEL, P.vis0 + nvis, P.cmps0 + cmps, true};
-}
-
-// The one-line FP8 inner-product kernel is held to 128 VGPRs (4 waves per SIMD, what
-// its LDS allows at the beams it serves); left alone, the 8-lane rerank takes it to 132.
-template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
-__global__ __launch_bounds__(PANN_WAVE, (ST == ST_FP8 && MT == M_IP && NCH == 1 && !EX) ? 4 : 1)
-void k_beam_search(SearchArgs a) {
-  extern __shared__ __attribute__((aligned(16))) char smem[];
-  const int lane = threadIdx.x;
-  WaveLds L = carve(smem, a);
-  unsigned long long* ghash = EX ? a.ghash + ((uint64_t)blockIdx.x << a.ghash_log2) : nullptr;
-  for (;;) {
-    uint32_t q = 0;
-    if (lane == 0) q = atomicAdd(a.work_counter, 1u);
-    q = readfirstlane(q);
-    if (q >= a.nq) break;
-    // passes 0 .. nup-1: the HNSW upper levels at beam 1 (search_layer_to,
-    // HNSW.hpp:1545-1566), the first from the start points, each next one
-    // from the last one's nearest id; pass nup: the search itself.  One call
-    // site, so the search is inlined once.
-    PassOut po{PANN_SENTINEL, 0u, 0u, true};
-    for (uint32_t l = 0; l <= a.nup; l++) {
-      const bool up = l < a.nup;
-      Pass P;
-      P.nbrs = up ? a.up[l].nbrs : a.nbrs;
-      P.nbr_stride = up ? a.up[l].nbr_stride : a.nbr_stride;
-      // an upper level: degree_limit = its max degree, limit = n
-      P.dl = up ? (int)a.up[l].max_degree
-                : (a.degree_limit < (int)a.max_degree ? a.degree_limit : (int)a.max_degree);
-      P.B = readfirstlane(up ? 1 : (a.beam_pq ? a.beam_pq[q] : a.beam));
-      P.k = readfirstlane(up ? 1 : (a.k_pq ? a.k_pq[q] : a.k));
-      P.limit = up ? (int64_t)a.n : a.limit;
-      if (l == 0) {
-        P.sp = a.starts_pq ? a.starts_pq + (uint64_t)q * a.starts_stride : a.starts;
-        P.ns = a.starts_pq ? (a.nstarts_pq ? (int)a.nstarts_pq[q] : 1) : (int)a.nstarts;
-      } else {
-        P.sp = nullptr;
-        P.ns = 1;
-      }
-      P.start1 = po.best;
-      P.pass = l;
-      P.final = !up;
-      P.vis0 = po.vis;
-      P.cmps0 = po.cmps;
-      po = search_one<ST, MT, NCH, SEQ, EX, FB>(a, L, q, lane, ghash, P);
-      po.best = readfirstlane(po.best);  // the same in every lane: keep it scalar
-      po.vis = readfirstlane(po.vis);
-      po.cmps = readfirstlane(po.cmps);
-      if (!po.ok) break;
-    }
-  }
-}
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
 }
 
-// ---------------------------------------------------------------------------
-// host-side launch
-// ---------------------------------------------------------------------------
 uint32_t search_lds_bytes(const SearchArgs& a) {
   return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2, a.rc_log2, a.fb_log2, a.ncent,
                         a.q_own);
-}
-
-template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
-static hipError_t launch_k(const SearchArgs& a, uint32_t grid, hipStream_t s) {
-  uint32_t lds = search_lds_bytes(a);
-  auto kern = k_beam_search<ST, MT, NCH, SEQ, EX, FB>;
-  // raise the kernel's dynamic-LDS limit only when a launch needs more, so a
-  // repeated launch makes no attribute call (it may be under graph capture);
-  // the check, the call and the record are one step under a lock, so two host
-  // threads cannot leave the record above the attribute
-  static std::atomic<uint32_t> lds_set{0};
-  if (lds > lds_set.load()) {
-    static std::mutex mu;
-    std::lock_guard<std::mutex> lk(mu);
-    if (lds > lds_set.load()) {
-      hipError_t e = hipFuncSetAttribute((const void*)kern,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      if (e != hipSuccess) return e;
-      lds_set.store(lds);
-    }
-  }
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(PANN_WAVE), lds, s, a);
-  return hipGetLastError();
-}
-
-// the exact-counts seen set is a separate instantiation, so the default
-// kernel carries none of its state
-template <int ST, int MT, int NCH, bool SEQ>
-static hipError_t launch_t(const SearchArgs& a, uint32_t grid, hipStream_t s) {
-  if (a.exact_seen) return launch_k<ST, MT, NCH, SEQ, true, false>(a, grid, s);
-  return a.fb_log2 ? launch_k<ST, MT, NCH, SEQ, false, true>(a, grid, s)
-                   : launch_k<ST, MT, NCH, SEQ, false, false>(a, grid, s);
 }
 
 static int nch_for(int row_bytes) {
@@ -814,16 +25,6 @@ static int nch_for(int row_bytes) {
   return -1;
 }
 
-#define PANN_NCH_CASES(ST, MT, SEQ)                              \
-  switch (nch) {                                                 \
-    case 1: return launch_t<ST, MT, 1, SEQ>(a, grid, s);         \
-    case 2: return launch_t<ST, MT, 2, SEQ>(a, grid, s);         \
-    case 3: return launch_t<ST, MT, 3, SEQ>(a, grid, s);         \
-    case 4: return launch_t<ST, MT, 4, SEQ>(a, grid, s);         \
-    case 8: return launch_t<ST, MT, 8, SEQ>(a, grid, s);         \
-    default: return hipErrorInvalidValue;                        \
-  }
-
 hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool sequential,
                               uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, a.work_counter, 1u);
@@ -831,34 +32,19 @@ hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool s
   if (sequential) {
     if (store != ST_F32) return hipErrorInvalidValue;
     if (nch < 0) nch = 8;  // rows past 8 lines: 8 lines at a time, query in LDS
-    if (metric == M_L2) { PANN_NCH_CASES(ST_F32, M_L2, true) }
-    else { PANN_NCH_CASES(ST_F32, M_IP, true) }
+    return launch_bs_f32_inorder(a, metric, nch, grid, s);
   }
-  if (store == ST_F32) {
-    if (metric == M_L2) { PANN_NCH_CASES(ST_F32, M_L2, false) }
-    else { PANN_NCH_CASES(ST_F32, M_IP, false) }
-  } else if (store == ST_U8) {
-    if (metric == M_L2) { PANN_NCH_CASES(ST_U8, M_L2, false) }
-    else { PANN_NCH_CASES(ST_U8, M_IP, false) }
-  } else if (store == ST_I8) {
-    if (metric == M_L2) { PANN_NCH_CASES(ST_I8, M_L2, false) }
-    else { PANN_NCH_CASES(ST_I8, M_IP, false) }
-  } else if (store == ST_BF16) {
-    if (metric == M_L2) { PANN_NCH_CASES(ST_BF16, M_L2, false) }
-    else { PANN_NCH_CASES(ST_BF16, M_IP, false) }
-  } else if (store == ST_FP8) {
-    if (metric == M_L2) { PANN_NCH_CASES(ST_FP8, M_L2, false) }
-    else { PANN_NCH_CASES(ST_FP8, M_IP, false) }
-  } else if (store == ST_FP8R && metric == M_IP) {  // rows of dim + 4 <= 384 bytes
-    if (!a.qtable || a.ncent == 0) return hipErrorInvalidValue;
-    switch (nch) {
-      case 1: return launch_t<ST_FP8R, M_IP, 1, false>(a, grid, s);
-      case 2: return launch_t<ST_FP8R, M_IP, 2, false>(a, grid, s);
-      case 3: return launch_t<ST_FP8R, M_IP, 3, false>(a, grid, s);
-      default: return hipErrorInvalidValue;
-    }
+  switch (store) {
+    case ST_F32: return launch_bs_f32(a, metric, nch, grid, s);
+    case ST_U8: return launch_bs_u8(a, metric, nch, grid, s);
+    case ST_I8: return launch_bs_i8(a, metric, nch, grid, s);
+    case ST_BF16: return launch_bs_bf16(a, metric, nch, grid, s);
+    case ST_FP8: return launch_bs_fp8(a, metric, nch, grid, s);
+    case ST_FP8R:  // rows of dim + 4 <= 384 bytes, inner product only
+      if (metric != M_IP || !a.qtable || a.ncent == 0) return hipErrorInvalidValue;
+      return launch_bs_fp8r(a, metric, nch, grid, s);
+    default: return hipErrorInvalidValue;
   }
-  return hipErrorInvalidValue;
 }
 
 }  // namespace pann

@@ -108,6 +108,15 @@ struct SearchArgs {
 };
 
 uint32_t search_lds_bytes(const SearchArgs& a);
+// per-store launchers (bs_*.hip), called by launch_beam_search
+hipError_t launch_bs_f32(const SearchArgs& a, int metric, int nch, uint32_t grid, hipStream_t s);
+hipError_t launch_bs_f32_inorder(const SearchArgs& a, int metric, int nch, uint32_t grid,
+                                 hipStream_t s);
+hipError_t launch_bs_u8(const SearchArgs& a, int metric, int nch, uint32_t grid, hipStream_t s);
+hipError_t launch_bs_i8(const SearchArgs& a, int metric, int nch, uint32_t grid, hipStream_t s);
+hipError_t launch_bs_bf16(const SearchArgs& a, int metric, int nch, uint32_t grid, hipStream_t s);
+hipError_t launch_bs_fp8(const SearchArgs& a, int metric, int nch, uint32_t grid, hipStream_t s);
+hipError_t launch_bs_fp8r(const SearchArgs& a, int metric, int nch, uint32_t grid, hipStream_t s);
 hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool sequential,
                               uint32_t grid, hipStream_t s);
 
