@@ -380,10 +380,12 @@ def run_vamana(args, w, name):
         rec_ties = recall_with_ties(X, Q, gt, ids, k, w["metric"])
         mre = max_rel_dist_error(X, Q, ids, dists, w["metric"] if w["kind"] != "u8" else "l2")
         tuned = {"waves_per_cu": 0, "visits_per_round": 0, "eval_cache_log2": 0,
-                 "frontier_filter": 0, "exact_counts": False, "hash_slots_log2": 0}
+                 "frontier_filter": 0, "exact_counts": False, "hash_slots_log2": 0,
+                 "waves_per_query": 1}
         if not args.no_tune:
             tuned = autotune.tune(idx, tq, k, B, rerank_factor=args.rerank_factor, **opts)
-            log(f"[bench] {leg}: tuned waves/CU {tuned['waves_per_cu']}, " +
+            log(f"[bench] {leg}: tuned waves/CU {tuned['waves_per_cu']}, waves/query "
+                f"{tuned['waves_per_query']}, " +
                 (f"exact seen set (LDS slots 2^{tuned['hash_slots_log2'] or 'auto'})"
                  if tuned["exact_counts"] else
                  f"visits/round {tuned['visits_per_round']}, eval cache "
@@ -392,7 +394,8 @@ def run_vamana(args, w, name):
                 (" (cached)" if tuned["cached"] else "") +
                 f" [{autotune.cache_path()} {tuned['key']}]")
         shape = {k_: tuned[k_] for k_ in ("waves_per_cu", "visits_per_round", "eval_cache_log2",
-                                          "frontier_filter", "exact_counts", "hash_slots_log2")}
+                                          "frontier_filter", "exact_counts", "hash_slots_log2",
+                                          "waves_per_query")}
         for _ in range(args.warmup):
             idx.search_device(tq, k, B, oi, od, out_visited=ov, out_cmps=oc, async_=True,
                               rerank_factor=args.rerank_factor, **opts, **shape)
@@ -492,14 +495,15 @@ def run_vamana(args, w, name):
         for leg in [l for l in ("fp32-tree", "fp8-rerank") if l in legs]:
             opts = LEG_OPTS[leg]
             shape = {"waves_per_cu": 0, "visits_per_round": 0, "eval_cache_log2": 0,
-                     "frontier_filter": 0}
+                     "frontier_filter": 0, "waves_per_query": 1}
             if not args.no_tune:
                 # replayed from a captured hipGraph: the default seen set
                 t = autotune.tune(idx, tq, 100, 256, k_per_query=tk, beam_per_query=tb,
                                   out_stride=100, rerank_factor=args.rerank_factor,
                                   allow_exact=False, **opts)
                 shape = {k_: t[k_] for k_ in ("waves_per_cu", "visits_per_round",
-                                              "eval_cache_log2", "frontier_filter")}
+                                              "eval_cache_log2", "frontier_filter",
+                                              "waves_per_query")}
             run = lambda a=False, st=None: idx.search_device(
                 tq, 100, 256, ui, ud, out_visited=ov, out_cmps=oc, k_per_query=tk,
                 beam_per_query=tb, async_=a, stream=st, rerank_factor=args.rerank_factor,

@@ -118,6 +118,11 @@ typedef struct {
                                gathered again.  0 = automatic (on for float32 rows wider than
                                one 128-byte line), 1 = on, 2 = off.  Results do not depend
                                on it (a tuning knob) */
+  int32_t waves_per_query;  /* 0 or 1: one 64-lane wave per query; 2 or 4: a workgroup of that
+                               many waves per query, which share each visit's row gathers,
+                               the merges and the rerank (for large beams, where one query's
+                               work is long and few queries fit a CU).  waves_per_cu then
+                               counts waves.  Results do not depend on it (a tuning knob) */
 } pann_search_opts;
 
 /* Creates the device index: uploads the points (rows padded to 128 bytes)

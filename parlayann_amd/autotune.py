@@ -10,7 +10,9 @@ evaluated ids (fewer rows gathered again, against fewer waves per CU) -- and
 keeps the four fastest.  The reference's exact seen set is timed on the
 sample too, at a few LDS sizes and occupancies (an id seen once is never
 evaluated again, at the cost of an insert per neighbour and, past LDS, a
-global table).  The four, the best exact shape, and the frontier-id filter
+global table).  Workgroups of 2 and 4 waves per query
+(each visit's gathers, the merges and the rerank shared) are timed too.
+The four, the best exact shape, and the frontier-id filter
 forced on and off for the best (fewer rows gathered again, against a filter
 test before every gather) are then timed on the whole query batch, whose
 throughput a small sample's tail can misstate.  Winners are cached in a JSON file keyed by
@@ -30,6 +32,9 @@ FRONTIER_FILTER = (0, 1, 2)  # automatic, on, off
 # (LDS slots log2 (0 = automatic), waves per CU); slower where the set
 # overflows LDS, much faster where the default set re-evaluates many rows
 EXACT_SHAPES = ((0, 0), (12, 0), (13, 0), (0, 8), (12, 8), (13, 8))
+# several waves per query (a workgroup shares each visit's gathers, the merges
+# and the rerank; no frontier filter): (waves per query, waves per CU)
+COOP_SHAPES = ((2, 16), (2, 32), (4, 16), (4, 32))
 
 
 def cache_path() -> str:
@@ -63,7 +68,7 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
          reps: int = 2, use_cache: bool = True, rerank_factor: float = 100.0,
          confirm: int = 4, confirm_max: int = 131072, allow_exact: bool = True) -> dict:
     """Fastest {waves_per_cu, visits_per_round, eval_cache_log2,
-    frontier_filter, exact_counts, hash_slots_log2} for DeviceIndex `index`
+    frontier_filter, exact_counts, hash_slots_log2, waves_per_query} for DeviceIndex `index`
     on device queries (a torch tensor).  allow_exact=False keeps the default
     seen set (a search captured into a hipGraph needs it).  Returns the
     choice and the timings."""
@@ -78,17 +83,17 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
         str(index.dtype), index.metric, store, summation, int(rerank), k, beam,
         "uneven" if uneven else "uniform", "exact" if allow_exact else "noexact"))
     table = _load() if use_cache else {}
-    if use_cache and key in table and "exact_counts" in table[key]:
+    if use_cache and key in table and "waves_per_query" in table[key]:
         return dict(table[key], cached=True, key=key)
     stride = out_stride or k
     mfull = min(int(queries.shape[0]), confirm_max)
     oi = torch.empty((mfull, max(stride, 1)), dtype=torch.int32, device=dev)
     od = torch.empty((mfull, max(stride, 1)), dtype=torch.float32, device=dev)
 
-    def time_shape(m, wpc, vpr, ec, ff, ex=False, hs=0):
+    def time_shape(m, wpc, vpr, ec, ff, ex=False, hs=0, wpq=1):
         kw = dict(store=store, summation=summation, rerank=rerank, waves_per_cu=wpc,
                   visits_per_round=vpr, eval_cache_log2=ec, frontier_filter=ff,
-                  exact_counts=ex, hash_slots_log2=hs,
+                  exact_counts=ex, hash_slots_log2=hs, waves_per_query=wpq,
                   k_per_query=k_per_query[:m] if k_per_query is not None else None,
                   beam_per_query=beam_per_query[:m] if beam_per_query is not None else None,
                   rerank_factor=rerank_factor)
@@ -105,13 +110,18 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
     for wpc in WAVES_PER_CU:
         for vpr in VISITS_PER_ROUND:
             for ec in EVAL_CACHE_LOG2:
-                timings.append((time_shape(m, wpc, vpr, ec, 0), wpc, vpr, ec, 0, False, 0))
+                timings.append((time_shape(m, wpc, vpr, ec, 0), wpc, vpr, ec, 0, False, 0, 1))
+    for wpq, wpc in COOP_SHAPES:
+        for vpr in VISITS_PER_ROUND:
+            # the cooperative launch takes no frontier filter (2 = off)
+            timings.append((time_shape(m, wpc, vpr, 0, 2, wpq=wpq), wpc, vpr, 0, 2, False, 0,
+                            wpq))
     timings.sort()
-    _, bw, bv, be, _, _, _ = timings[0]
     shapes = [t[1:] for t in timings[:confirm]]
-    shapes += [(bw, bv, be, ff, False, 0) for ff in FRONTIER_FILTER[1:]]
+    bw, bv, be = next((t[1], t[2], t[3]) for t in timings if t[7] == 1)
+    shapes += [(bw, bv, be, ff, False, 0, 1) for ff in FRONTIER_FILTER[1:]]
     if allow_exact:
-        ex_t = sorted((time_shape(m, wpc, 1, 0, 0, True, hs), wpc, 1, 0, 0, True, hs)
+        ex_t = sorted((time_shape(m, wpc, 1, 0, 0, True, hs), wpc, 1, 0, 0, True, hs, 1)
                       for hs, wpc in EXACT_SHAPES)
         timings += ex_t
         shapes.append(ex_t[0][1:])
@@ -119,12 +129,12 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
     best = min(full)
     res = {"waves_per_cu": best[1], "visits_per_round": best[2], "eval_cache_log2": best[3],
            "frontier_filter": best[4], "exact_counts": best[5], "hash_slots_log2": best[6],
-           "ms": best[0], "queries": mfull, "sample": m,
+           "waves_per_query": best[7], "ms": best[0], "queries": mfull, "sample": m,
            "timings": [{"waves_per_cu": w, "visits_per_round": v, "eval_cache_log2": e,
                         "frontier_filter": f_, "exact_counts": x, "hash_slots_log2": h,
-                        "ms": t, "queries": n_}
+                        "waves_per_query": q_, "ms": t, "queries": n_}
                        for n_, rows in ((m, timings), (mfull, full))
-                       for t, w, v, e, f_, x, h in rows]}
+                       for t, w, v, e, f_, x, h, q_ in rows]}
     if use_cache:
         table[key] = res
         _store(table)

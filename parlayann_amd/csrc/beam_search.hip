@@ -12,7 +12,7 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 
 uint32_t search_lds_bytes(const SearchArgs& a) {
   return wave_lds_bytes(a.fcap, a.ccap, a.dim, a.hslots_log2, a.rc_log2, a.fb_log2, a.ncent,
-                        a.q_own);
+                        a.q_own, a.wpq);
 }
 
 static int nch_for(int row_bytes) {

@@ -47,7 +47,10 @@ struct WaveLds {
   uint32_t* H;
   uint32_t* RC;
   uint32_t* FB;
-  float* CT;  // FP8 residual store: q . c for every centroid c
+  float* CT;  // FP8 residual store: q . c for every centroid
+  // several waves per query: the command words of the cooperative launch
+  // (command, three arguments, each helper wave's partial minimum)
+  uint32_t* CO;
 };
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
@@ -55,7 +58,7 @@ __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15
 __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap, uint32_t dim,
                                                    uint32_t hslots_log2, uint32_t rc_log2,
                                                    uint32_t fb_log2, uint32_t ncent,
-                                                   uint32_t q_own) {
+                                                   uint32_t q_own, uint32_t wpq) {
   uint32_t b = 0;
   b += 2 * align16(fcap * 8);
   b += 2 * align16(fcap);
@@ -66,6 +69,7 @@ __host__ __device__ inline uint32_t wave_lds_bytes(uint32_t fcap, uint32_t ccap,
   b += rc_log2 ? (1u << rc_log2) * 4 : 0u;
   b += fb_log2 ? (1u << fb_log2) / 8 : 0u;
   b += align16(ncent * 4);
+  b += wpq > 1 ? 64u : 0u;
   return b;
 }
 
@@ -90,6 +94,8 @@ __device__ inline WaveLds carve(char* p, const SearchArgs& a) {
   L.FB = (uint32_t*)(p + o);
   o += a.fb_log2 ? (1u << a.fb_log2) / 8 : 0u;
   L.CT = (float*)(p + o);
+  o += align16(a.ncent * 4);
+  L.CO = (uint32_t*)(p + o);
   return L;
 }
 
@@ -240,20 +246,160 @@ __device__ __forceinline__ bool sorted_contains(const uint64_t* A, int n, uint64
   return lo < n && A[lo] == x;
 }
 
+// Distances of rows NL[0..ncmp) into DL[0..ncmp) with the search's store.
 template <int ST, int MT, int NCH, bool SEQ>
-__device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
-                                          const WaveLds& L, int ncmp, int lane) {
+__device__ __forceinline__ void eval_rows_at(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
+                                             const float* QL, const uint32_t* NL, float* DL,
+                                             int ncmp, int lane) {
   if constexpr (SEQ) {
     // rows of up to 8 lines: the group gather with the in-order lane chain and the
     // query in registers; longer rows (NCH 8 only): 8 lines at a time, query in LDS
     if (NCH < 8 || a.row_bytes <= 8u * 128u)
-      eval_group_inorder<MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, L.NL, ncmp, L.DL,
-                                  lane);
+      eval_group_inorder<MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, NL, ncmp, DL, lane);
     else
-      eval_group_inorder_lds<MT, 8, 1>(L.QL, (int)a.dim, a.pts, a.pts_stride, L.NL, ncmp, L.DL,
-                                       lane);
+      eval_group_inorder_lds<MT, 8, 1>(QL, (int)a.dim, a.pts, a.pts_stride, NL, ncmp, DL, lane);
   } else {
-    eval_group<ST, MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, L.NL, ncmp, L.DL, lane);
+    eval_group<ST, MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, NL, ncmp, DL, lane);
+  }
+}
+
+template <int ST, int MT, int NCH, bool SEQ>
+__device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
+                                          const WaveLds& L, int ncmp, int lane) {
+  eval_rows_at<ST, MT, NCH, SEQ>(D, a, L.QL, L.NL, L.DL, ncmp, lane);
+}
+
+// FP32 distances of rows NL[0..m) of the full-precision store (the rerank),
+// the reference's arithmetic.  The 8-lane gather and in-order lane chain in the
+// kernels of the reduced-precision stores; one lane per row for the FP8 L2
+// kernels (their query scales leave no registers for it) and for the FP32
+// store's own kernels, whose rerank is rare and whose search slowed by 1.5 %
+// with it compiled in.
+template <int ST, int MT>
+__device__ __forceinline__ void eval_rerank_rows(const SearchArgs& a, const float* QL,
+                                                 const uint32_t* NL, float* DL, int m, int lane) {
+  constexpr bool group_rr = ST != ST_F32 && !(ST == ST_FP8 && MT == M_L2);
+  if (group_rr && a.dim <= 128u)
+    eval_group_inorder_lds<MT, 4, 1>(QL, (int)a.dim, a.full, a.full_stride, NL, m, DL, lane);
+  else if (group_rr)
+    eval_group_inorder_lds<MT, 8, 1>(QL, (int)a.dim, a.full, a.full_stride, NL, m, DL, lane);
+  else
+    eval_seq_f32<MT>(QL, a.full, a.full_stride, (int)a.dim, NL, m, DL, lane);
+}
+
+// ---------------------------------------------------------------------------
+// The merge (beamSearch.h:197-222) as two phases over threads tid of nt, so
+// that the waves of a cooperative launch share it: rank-sort the candidates
+// C[0..c) into Cs, then place Cs and the frontier by rank into the other
+// frontier buffer (first min(B, f + c) positions).  Keys are unique, so every
+// rank is a distinct position.  merge_place returns the lowest new position
+// of an unvisited entry among this thread's entries (INT_MAX if none).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void merge_rank_sort(const WaveLds& L, int c, int tid, int nt) {
+  for (int j = tid; j < c; j += nt) {
+    const uint64_t x = L.C[j];
+    int r = 0;
+    for (int i = 0; i < c; i++) r += (L.C[i] < x);
+    L.Cs[r] = x;
+  }
+}
+
+__device__ __forceinline__ int merge_place(const WaveLds& L, int cur_buf, int c, int f, int B,
+                                           int tid, int nt) {
+  const uint64_t* F = cur_buf ? L.F1 : L.F0;
+  const uint8_t* FV = cur_buf ? L.FV1 : L.FV0;
+  uint64_t* F2 = cur_buf ? L.F0 : L.F1;
+  uint8_t* FV2 = cur_buf ? L.FV0 : L.FV1;
+  int first = 0x7FFFFFFF;
+  for (int j = tid; j < c; j += nt) {
+    const uint64_t x = L.Cs[j];
+    const int pos = j + lower_bound_u64(F, f, x);
+    if (pos < B) {
+      F2[pos] = x;
+      FV2[pos] = 0;
+      first = pos < first ? pos : first;
+    }
+  }
+  for (int i = tid; i < f; i += nt) {
+    const uint64_t x = F[i];
+    const int pos = i + lower_bound_u64(L.Cs, c, x);
+    if (pos < B) {
+      F2[pos] = x;
+      const uint8_t v = FV[i];
+      FV2[pos] = v;
+      if (!v) first = pos < first ? pos : first;
+    }
+  }
+  return first;
+}
+
+// The final rank of the re-scored rows R[0..nchk) (beamSearch.h:579-584): the
+// first k go to oi / od.
+__device__ __forceinline__ void rerank_rank(const uint64_t* R, int nchk, int k, uint32_t* oi,
+                                            float* od, int tid, int nt) {
+  for (int j = tid; j < nchk; j += nt) {
+    const uint64_t x = R[j];
+    int r = 0;
+    for (int i = 0; i < nchk; i++) r += (R[i] < x);
+    if (r < k) {
+      oi[r] = key_id(x);
+      od[r] = key_dist(x);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Several waves per query (SearchArgs::wpq = W > 1): a workgroup of W waves
+// takes one query.  Wave 0 runs the search as a single wave would; at each
+// row gather, merge and rerank it posts a command in LDS (CO[0], arguments
+// CO[1..3]) and every wave of the workgroup does a share of it between two
+// workgroup barriers.  The other waves loop on commands until CO_EXIT, so
+// every wave passes the same number of barriers.
+// ---------------------------------------------------------------------------
+enum : uint32_t {
+  CO_QUERY = 1,        // load query row CO[1] into the gather registers
+  CO_EVAL = 2,         // distances of NL[0..CO[1]) with the search's store
+  CO_EVAL_RR = 3,      // FP32 distances of NL[0..CO[1]) (the rerank)
+  CO_MERGE_SORT = 4,   // merge phase 1 over CO[1] candidates
+  CO_MERGE_PLACE = 5,  // merge phase 2: c = CO[1], f = CO[2], cur_buf | B << 1 = CO[3]
+  CO_RR_RANK = 6,      // rerank rank: nchk = CO[1], cur_buf | k << 1 = CO[2], query CO[3]
+  CO_EXIT = 7
+};
+
+// rows of a gather taken by each wave: a multiple of the 8 rows one gather
+// instruction covers
+__device__ __forceinline__ int coop_share(int n, int W) { return ((n + W - 1) / W + 7) & ~7; }
+
+// wave 0 posts a command; the first barrier of its two
+__device__ __forceinline__ void coop_post(const WaveLds& L, int lane, uint32_t cmd, uint32_t a0,
+                                          uint32_t a1, uint32_t a2) {
+  if (lane == 0) {
+    L.CO[0] = cmd;
+    L.CO[1] = a0;
+    L.CO[2] = a1;
+    L.CO[3] = a2;
+  }
+  __syncthreads();
+}
+
+// Distances of NL[0..n) into DL[0..n): by this wave alone (W = 1), or shared
+// by the workgroup's W waves, this wave taking the first share; rr: the FP32
+// rerank.  The cooperative launch is an instantiation of its own (CO), so that
+// the one-wave kernels carry none of it.
+template <int ST, int MT, int NCH, bool SEQ, bool CO>
+__device__ __forceinline__ void eval_n(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
+                                       const WaveLds& L, int W, int n, bool rr, int lane) {
+  if (CO && W > 1) {
+    coop_post(L, lane, rr ? CO_EVAL_RR : CO_EVAL, (uint32_t)n, 0u, 0u);
+    const int per = coop_share(n, W);
+    const int cnt = n < per ? n : per;
+    if (rr) eval_rerank_rows<ST, MT>(a, L.QL, L.NL, L.DL, cnt, lane);
+    else eval_rows_at<ST, MT, NCH, SEQ>(D, a, L.QL, L.NL, L.DL, cnt, lane);
+    __syncthreads();
+  } else if (rr) {
+    eval_rerank_rows<ST, MT>(a, L.QL, L.NL, L.DL, n, lane);
+  } else {
+    eval_rows<ST, MT, NCH, SEQ>(D, a, L, n, lane);
   }
 }
 
@@ -280,8 +426,8 @@ struct PassOut {
   bool ok;
 };
 
-template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
-__device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, int lane,
+template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB, bool CO>
+__device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, int lane,
                               unsigned long long* ghash, const Pass& P) {
   const uint32_t gtag = a.ghash_tag_base + q * (a.nup + 1u) + P.pass + 1u;
   using QT = typename QueryElem<ST>::T;
@@ -333,6 +479,8 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
     D.T = L.CT;
   }
 
+  const int W = CO ? (int)a.wpq : 1;
+
   bool ovf = false;
   uint32_t lds_count = 0, g_count = 0;
   const uint32_t glog2 = a.ghash_log2;
@@ -349,7 +497,7 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
       if constexpr (EX) lds_insert(L.H, hlog2, s);
     }
     wave_lds_sync();
-    eval_rows<ST, MT, NCH, SEQ>(D, a, L, m, lane);
+    eval_n<ST, MT, NCH, SEQ, CO>(D, a, L, W, m, false, lane);
     wave_lds_sync();
     if (lane < m) L.F1[t + lane] = make_key(L.DL[lane], L.NL[lane]);
     wave_lds_sync();
@@ -490,6 +638,9 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
       nb_pref = nu >= 0 ? adjacency(pref_id) : PANN_SENTINEL;
       const bool valid = nb != PANN_SENTINEL;
       const int nvalid = __popcll(ballot(valid));
+      // the LDS table takes ids up to half its slots, ids past it go to the
+      // global table (at 7/8 load the longer probes made an HNSW 1M x 200
+      // search 1.33x slower than at 1/2)
       if (!ovf && lds_count + (uint32_t)nvalid > (hslots >> 1)) ovf = true;
       bool isnew = false;
       if (valid) {
@@ -512,7 +663,7 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
       cmps += (uint32_t)ncmp;
       if (evalme) L.NL[mbcnt(mnew)] = nb;
       wave_lds_sync();
-      eval_rows<ST, MT, NCH, SEQ>(D, a, L, ncmp, lane);
+      eval_n<ST, MT, NCH, SEQ, CO>(D, a, L, W, ncmp, false, lane);
       wave_lds_sync();
       bool keep = false;
       uint64_t key = 0;
@@ -578,7 +729,7 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
       if (e1) L.NL[mbcnt(m1)] = nb;
       if (e2) L.NL[n1 + (int)mbcnt(m2)] = nb2;
       wave_lds_sync();
-      eval_rows<ST, MT, NCH, SEQ>(D, a, L, n1 + n2, lane);
+      eval_n<ST, MT, NCH, SEQ, CO>(D, a, L, W, n1 + n2, false, lane);
       wave_lds_sync();
       accept_range(F, 0, n1, cutoff);
       cmps += (uint32_t)n1;
@@ -601,34 +752,23 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
     }
 
     // ---- merge: rank-sort C, then place C and F by rank into the other buffer
-    for (int j = lane; j < c; j += PANN_WAVE) {
-      uint64_t x = L.C[j];
-      int r = 0;
-      for (int i = 0; i < c; i++) r += (L.C[i] < x);
-      L.Cs[r] = x;
-    }
-    wave_lds_sync();
-    uint64_t* F2 = cur_buf ? L.F0 : L.F1;
-    uint8_t* FV2 = cur_buf ? L.FV0 : L.FV1;
-    int first = 0x7FFFFFFF;  // lowest new position of an unvisited entry
-    for (int j = lane; j < c; j += PANN_WAVE) {
-      uint64_t x = L.Cs[j];
-      int pos = j + lower_bound_u64(F, f, x);
-      if (pos < B) {
-        F2[pos] = x;
-        FV2[pos] = 0;
-        first = pos < first ? pos : first;
+    int first;  // lowest new position of an unvisited entry
+    if (W > 1) {
+      coop_post(L, lane, CO_MERGE_SORT, (uint32_t)c, 0u, 0u);
+      merge_rank_sort(L, c, lane, PANN_WAVE * W);
+      __syncthreads();
+      coop_post(L, lane, CO_MERGE_PLACE, (uint32_t)c, (uint32_t)f,
+                (uint32_t)cur_buf | ((uint32_t)B << 1));
+      first = merge_place(L, cur_buf, c, f, B, lane, PANN_WAVE * W);
+      __syncthreads();
+      for (int w = 1; w < W; w++) {
+        const int o = (int)L.CO[8 + w];
+        first = o < first ? o : first;
       }
-    }
-    for (int i = lane; i < f; i += PANN_WAVE) {
-      uint64_t x = F[i];
-      int pos = i + lower_bound_u64(L.Cs, c, x);
-      if (pos < B) {
-        F2[pos] = x;
-        uint8_t v = FV[i];
-        FV2[pos] = v;
-        if (!v) first = pos < first ? pos : first;
-      }
+    } else {
+      merge_rank_sort(L, c, lane, PANN_WAVE);
+      wave_lds_sync();
+      first = merge_place(L, cur_buf, c, f, B, lane, PANN_WAVE);
     }
     f = f + c < B ? f + c : B;
     c = 0;
@@ -666,31 +806,17 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
       int m = nchk - t < PANN_WAVE ? nchk - t : PANN_WAVE;
       if (lane < m) L.NL[lane] = key_id(F[t + lane]);
       wave_lds_sync();
-      // 8-lane gather and in-order lane chain in the kernels of the reduced-precision
-      // stores.  One lane per row for the FP8 L2 kernels (their query scales leave no
-      // registers for it) and for the FP32 store's own kernels, whose rerank is rare and
-      // whose search slowed by 1.5 % with it compiled in.
-      constexpr bool group_rr = ST != ST_F32 && !(ST == ST_FP8 && MT == M_L2);
-      if (group_rr && a.dim <= 128u)
-        eval_group_inorder_lds<MT, 4, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
-                                         lane);
-      else if (group_rr)
-        eval_group_inorder_lds<MT, 8, 1>(L.QL, (int)a.dim, a.full, a.full_stride, L.NL, m, L.DL,
-                                         lane);
-      else
-        eval_seq_f32<MT>(L.QL, a.full, a.full_stride, (int)a.dim, L.NL, m, L.DL, lane);
+      eval_n<ST, MT, NCH, SEQ, CO>(D, a, L, W, m, true, lane);
       wave_lds_sync();
       if (lane < m) R[t + lane] = make_key(L.DL[lane], L.NL[lane]);
       wave_lds_sync();
     }
-    for (int j = lane; j < nchk; j += PANN_WAVE) {
-      uint64_t x = R[j];
-      int r = 0;
-      for (int i = 0; i < nchk; i++) r += (R[i] < x);
-      if (r < k) {
-        oi[r] = key_id(x);
-        od[r] = key_dist(x);
-      }
+    if (W > 1) {
+      coop_post(L, lane, CO_RR_RANK, (uint32_t)nchk, (uint32_t)cur_buf | ((uint32_t)k << 1), q);
+      rerank_rank(R, nchk, k, oi, od, lane, PANN_WAVE * W);
+      __syncthreads();
+    } else {
+      rerank_rank(R, nchk, k, oi, od, lane, PANN_WAVE);
     }
   } else {
     for (int j = lane; j < k && j < f; j += PANN_WAVE) {
@@ -709,20 +835,76 @@ __device__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q,
   return PassOut{PANN_SENTINEL, P.vis0 + nvis, P.cmps0 + cmps, true};
 }
 
+// A helper wave of a cooperative launch: its share of each command wave 0
+// posts, until CO_EXIT.
+template <int ST, int MT, int NCH, bool SEQ>
+__device__ __forceinline__ void coop_helper(const SearchArgs& a, const WaveLds& L, int lane, int wid,
+                                            int W) {
+  using QT = typename QueryElem<ST>::T;
+  GroupDist<ST, MT, NCH> D;
+  const int tid = wid * PANN_WAVE + lane, nt = PANN_WAVE * W;
+  for (;;) {
+    __syncthreads();
+    const uint32_t cmd = L.CO[0], a0 = L.CO[1], a1 = L.CO[2], a2 = L.CO[3];
+    if (cmd == CO_EXIT) break;
+    if (cmd == CO_QUERY) {
+      D.load_query((const QT*)(a.queries + (uint64_t)a0 * a.q_stride), a.scales, (int)a.dim,
+                   lane & 7);
+      if constexpr (ST == ST_FP8R) D.T = L.CT;
+    } else if (cmd == CO_EVAL || cmd == CO_EVAL_RR) {
+      const int n = (int)a0, per = coop_share(n, W), lo = wid * per;
+      const int cnt = n - lo < per ? n - lo : per;
+      if (cnt > 0) {
+        if (cmd == CO_EVAL)
+          eval_rows_at<ST, MT, NCH, SEQ>(D, a, L.QL, L.NL + lo, L.DL + lo, cnt, lane);
+        else
+          eval_rerank_rows<ST, MT>(a, L.QL, L.NL + lo, L.DL + lo, cnt, lane);
+      }
+    } else if (cmd == CO_MERGE_SORT) {
+      merge_rank_sort(L, (int)a0, tid, nt);
+    } else if (cmd == CO_MERGE_PLACE) {
+      int first = merge_place(L, (int)(a2 & 1u), (int)a0, (int)a1, (int)(a2 >> 1), tid, nt);
+      first = wave_min_i32(first);
+      if (lane == 0) L.CO[8 + wid] = (uint32_t)first;
+    } else if (cmd == CO_RR_RANK) {
+      const uint64_t* R = (a1 & 1u) ? L.F0 : L.F1;
+      rerank_rank(R, (int)a0, (int)(a1 >> 1), a.out_ids + (uint64_t)a2 * a.out_stride,
+                  a.out_dists + (uint64_t)a2 * a.out_stride, tid, nt);
+    }
+    __syncthreads();
+  }
+}
+
 // The one-line FP8 inner-product kernel is held to 128 VGPRs (4 waves per SIMD, what
 // its LDS allows at the beams it serves); left alone, the 8-lane rerank takes it to 132.
-template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
-__global__ __launch_bounds__(PANN_WAVE, (ST == ST_FP8 && MT == M_IP && NCH == 1 && !EX) ? 4 : 1)
+// CO: workgroups of 2 or 4 waves per query (SearchArgs::wpq); otherwise one.
+template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB, bool CO>
+__global__ __launch_bounds__(CO ? 4 * PANN_WAVE : PANN_WAVE,
+                             (!CO && ST == ST_FP8 && MT == M_IP && NCH == 1 && !EX) ? 4 : 1)
 void k_beam_search(SearchArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
-  const int lane = threadIdx.x;
+  // (the one-wave kernels keep threadIdx.x itself: its known range [0, 64)
+  // saves registers and scratch over the masked form)
+  const int lane = CO ? ((int)threadIdx.x & (PANN_WAVE - 1)) : (int)threadIdx.x;
+  const int W = CO ? (int)a.wpq : 1;
   WaveLds L = carve(smem, a);
+  if constexpr (CO) {
+    const int wid = (int)threadIdx.x / PANN_WAVE;
+    if (wid > 0) {  // a helper wave of the cooperative launch
+      coop_helper<ST, MT, NCH, SEQ>(a, L, lane, wid, W);
+      return;
+    }
+  }
   unsigned long long* ghash = EX ? a.ghash + ((uint64_t)blockIdx.x << a.ghash_log2) : nullptr;
   for (;;) {
     uint32_t q = 0;
     if (lane == 0) q = atomicAdd(a.work_counter, 1u);
     q = readfirstlane(q);
     if (q >= a.nq) break;
+    if (W > 1) {  // the helpers load the query row into their gather registers
+      coop_post(L, lane, CO_QUERY, a.self_ids ? a.self_ids[q] : q, 0u, 0u);
+      __syncthreads();
+    }
     // passes 0 .. nup-1: the HNSW upper levels at beam 1 (search_layer_to,
     // HNSW.hpp:1545-1566), the first from the start points, each next one
     // from the last one's nearest id; pass nup: the search itself.  One call
@@ -751,19 +933,20 @@ void k_beam_search(SearchArgs a) {
       P.final = !up;
       P.vis0 = po.vis;
       P.cmps0 = po.cmps;
-      po = search_one<ST, MT, NCH, SEQ, EX, FB>(a, L, q, lane, ghash, P);
+      po = search_one<ST, MT, NCH, SEQ, EX, FB, CO>(a, L, q, lane, ghash, P);
       po.best = readfirstlane(po.best);  // the same in every lane: keep it scalar
       po.vis = readfirstlane(po.vis);
       po.cmps = readfirstlane(po.cmps);
       if (!po.ok) break;
     }
   }
+  if (W > 1) coop_post(L, lane, CO_EXIT, 0u, 0u, 0u);
 }
 
-template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB>
+template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB, bool CO>
 static hipError_t launch_k(const SearchArgs& a, uint32_t grid, hipStream_t s) {
   uint32_t lds = search_lds_bytes(a);
-  auto kern = k_beam_search<ST, MT, NCH, SEQ, EX, FB>;
+  auto kern = k_beam_search<ST, MT, NCH, SEQ, EX, FB, CO>;
   // raise the kernel's dynamic-LDS limit only when a launch needs more, so a
   // repeated launch makes no attribute call (it may be under graph capture);
   // the check, the call and the record are one step under a lock, so two host
@@ -779,17 +962,22 @@ static hipError_t launch_k(const SearchArgs& a, uint32_t grid, hipStream_t s) {
       lds_set.store(lds);
     }
   }
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(PANN_WAVE), lds, s, a);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(PANN_WAVE * (CO ? a.wpq : 1u)), lds, s, a);
   return hipGetLastError();
 }
 
-// the exact-counts seen set is a separate instantiation, so the default
-// kernel carries none of its state
+// the exact-counts seen set and the frontier filter are instantiations of their
+// own, so the default kernel carries none of their state; so is the
+// cooperative launch (several waves per query), which takes neither
 template <int ST, int MT, int NCH, bool SEQ>
 static hipError_t launch_t(const SearchArgs& a, uint32_t grid, hipStream_t s) {
-  if (a.exact_seen) return launch_k<ST, MT, NCH, SEQ, true, false>(a, grid, s);
-  return a.fb_log2 ? launch_k<ST, MT, NCH, SEQ, false, true>(a, grid, s)
-                   : launch_k<ST, MT, NCH, SEQ, false, false>(a, grid, s);
+  if (a.wpq > 1) {
+    if (a.exact_seen || a.fb_log2) return hipErrorInvalidValue;
+    return launch_k<ST, MT, NCH, SEQ, false, false, true>(a, grid, s);
+  }
+  if (a.exact_seen) return launch_k<ST, MT, NCH, SEQ, true, false, false>(a, grid, s);
+  return a.fb_log2 ? launch_k<ST, MT, NCH, SEQ, false, true, false>(a, grid, s)
+                   : launch_k<ST, MT, NCH, SEQ, false, false, false>(a, grid, s);
 }
 
 #define PANN_NCH_CASES(ST, MT, SEQ)                              \

@@ -57,7 +57,7 @@ struct GroupDist;
 template <int MT, int NCH>
 struct GroupDist<ST_F32, MT, NCH> {
   float q[NCH][4];
-  __device__ void load_query(const float* qv, const float* /*scales*/, int dim, int sub) {
+  __device__ __forceinline__ void load_query(const float* qv, const float* /*scales*/, int dim, int sub) {
 #pragma unroll
     for (int c = 0; c < NCH; c++)
 #pragma unroll
@@ -66,7 +66,7 @@ struct GroupDist<ST_F32, MT, NCH> {
         q[c][e] = j < dim ? qv[j] : 0.0f;
       }
   }
-  __device__ float partial(const uint4* raw) const {
+  __device__ __forceinline__ float partial(const uint4* raw) const {
     float acc = 0.0f;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
@@ -84,14 +84,14 @@ struct GroupDist<ST_F32, MT, NCH> {
     }
     return acc;
   }
-  __device__ float finish(float s) const { return MT == M_L2 ? s : -s; }
+  __device__ __forceinline__ float finish(float s) const { return MT == M_L2 ? s : -s; }
 };
 
 // BF16 rows: 8 values per chunk, f32 arithmetic.
 template <int MT, int NCH>
 struct GroupDist<ST_BF16, MT, NCH> {
   float q[NCH][8];
-  __device__ void load_query(const float* qv, const float* /*scales*/, int dim, int sub) {
+  __device__ __forceinline__ void load_query(const float* qv, const float* /*scales*/, int dim, int sub) {
 #pragma unroll
     for (int c = 0; c < NCH; c++)
 #pragma unroll
@@ -100,7 +100,7 @@ struct GroupDist<ST_BF16, MT, NCH> {
         q[c][e] = j < dim ? qv[j] : 0.0f;
       }
   }
-  __device__ float partial(const uint4* raw) const {
+  __device__ __forceinline__ float partial(const uint4* raw) const {
     float acc = 0.0f;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
@@ -120,7 +120,7 @@ struct GroupDist<ST_BF16, MT, NCH> {
     }
     return acc;
   }
-  __device__ float finish(float s) const { return MT == M_L2 ? s : -s; }
+  __device__ __forceinline__ float finish(float s) const { return MT == M_L2 ? s : -s; }
 };
 
 // FP8 e4m3 rows with per-dimension scales s_j: x_j ~ s_j * fp8_j.
@@ -129,7 +129,7 @@ template <int MT, int NCH>
 struct GroupDist<ST_FP8, MT, NCH> {
   float q[NCH][16];
   float s[MT == M_L2 ? NCH : 1][16];
-  __device__ void load_query(const float* qv, const float* scales, int dim, int sub) {
+  __device__ __forceinline__ void load_query(const float* qv, const float* scales, int dim, int sub) {
 #pragma unroll
     for (int c = 0; c < NCH; c++)
 #pragma unroll
@@ -145,7 +145,7 @@ struct GroupDist<ST_FP8, MT, NCH> {
         }
       }
   }
-  __device__ float partial(const uint4* raw) const {
+  __device__ __forceinline__ float partial(const uint4* raw) const {
     float acc = 0.0f;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
@@ -173,7 +173,7 @@ struct GroupDist<ST_FP8, MT, NCH> {
     }
     return acc;
   }
-  __device__ float finish(float v) const { return MT == M_L2 ? v : -v; }
+  __device__ __forceinline__ float finish(float v) const { return MT == M_L2 ? v : -v; }
 };
 
 // FP8 residual rows (inner product only): x ~ c + s * fp8, c the row's
@@ -190,11 +190,11 @@ struct GroupDist<ST_FP8R, MT, NCH> : GroupDist<ST_FP8, M_IP, NCH> {
   static_assert(MT == M_IP, "the FP8 residual store is inner product only");
   const float* T;
   bool idl;
-  __device__ void load_query(const float* qv, const float* scales, int dim, int sub) {
+  __device__ __forceinline__ void load_query(const float* qv, const float* scales, int dim, int sub) {
     GroupDist<ST_FP8, M_IP, NCH>::load_query(qv, scales, dim, sub);
     idl = sub == 7;
   }
-  __device__ float partial(const uint4* raw) const {
+  __device__ __forceinline__ float partial(const uint4* raw) const {
     float acc = GroupDist<ST_FP8, M_IP, NCH>::partial(raw);
     if (idl) acc += T[centroid_of(raw[NCH - 1].w)];
     return acc;
@@ -207,7 +207,7 @@ template <int MT, int NCH>
 struct GroupDist<ST_U8, MT, NCH> {
   uint32_t q[NCH][4];
   uint32_t qq;  // this lane's share of sum q^2
-  __device__ void load_query(const uint8_t* qv, const float*, int dim, int sub) {
+  __device__ __forceinline__ void load_query(const uint8_t* qv, const float*, int dim, int sub) {
     qq = 0u;
 #pragma unroll
     for (int c = 0; c < NCH; c++)
@@ -224,7 +224,7 @@ struct GroupDist<ST_U8, MT, NCH> {
         qq = __builtin_amdgcn_udot4(v, v, qq, false);
       }
   }
-  __device__ int partial(const uint4* raw) const {
+  __device__ __forceinline__ int partial(const uint4* raw) const {
     uint32_t pp = MT == M_L2 ? qq : 0u, pq = 0u;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
@@ -237,7 +237,7 @@ struct GroupDist<ST_U8, MT, NCH> {
     }
     return MT == M_L2 ? (int)(pp - 2u * pq) : (int)pq;
   }
-  __device__ float finish(int s) const { return MT == M_L2 ? (float)s : -((float)s); }
+  __device__ __forceinline__ float finish(int s) const { return MT == M_L2 ? (float)s : -((float)s); }
 };
 
 // int8 rows: exact int32 via v_dot4_i32_i8.
@@ -245,7 +245,7 @@ template <int MT, int NCH>
 struct GroupDist<ST_I8, MT, NCH> {
   int32_t q[NCH][4];
   int32_t qq;
-  __device__ void load_query(const int8_t* qv, const float*, int dim, int sub) {
+  __device__ __forceinline__ void load_query(const int8_t* qv, const float*, int dim, int sub) {
     qq = 0;
 #pragma unroll
     for (int c = 0; c < NCH; c++)
@@ -262,7 +262,7 @@ struct GroupDist<ST_I8, MT, NCH> {
         qq = __builtin_amdgcn_sdot4((int)v, (int)v, qq, false);
       }
   }
-  __device__ int partial(const uint4* raw) const {
+  __device__ __forceinline__ int partial(const uint4* raw) const {
     int pp = MT == M_L2 ? qq : 0, pq = 0;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
@@ -275,7 +275,7 @@ struct GroupDist<ST_I8, MT, NCH> {
     }
     return MT == M_L2 ? (int)((uint32_t)pp - 2u * (uint32_t)pq) : pq;
   }
-  __device__ float finish(int s) const { return MT == M_L2 ? (float)s : -((float)s); }
+  __device__ __forceinline__ float finish(int s) const { return MT == M_L2 ? (float)s : -((float)s); }
 };
 
 template <int ST>

@@ -105,6 +105,10 @@ struct SearchArgs {
   // from the previous pass's nearest id; visits and evaluations add up
   uint32_t nup;
   UpperLevel up[PANN_MAX_UP];
+  // waves per query (1, 2 or 4): a workgroup of wpq waves takes each query,
+  // wave 0 running the search and every wave sharing its gathers, merges and
+  // rerank (the launch's block size is 64 * wpq)
+  uint32_t wpq;
 };
 
 uint32_t search_lds_bytes(const SearchArgs& a);

@@ -743,9 +743,17 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   // more to that test than they save.
   if (o.frontier_filter < 0 || o.frontier_filter > 2)
     return fail(PANN_ERR_ARG, "frontier_filter must be 0, 1 or 2");
+  if (o.waves_per_query != 0 && o.waves_per_query != 1 && o.waves_per_query != 2 &&
+      o.waves_per_query != 4)
+    return fail(PANN_ERR_ARG, "waves_per_query must be 0, 1, 2 or 4");
+  a.wpq = o.waves_per_query > 1 ? (uint32_t)o.waves_per_query : 1u;
+  if (a.wpq > 1 && (o.exact_counts || o.frontier_filter == 1))
+    return fail(PANN_ERR_UNSUPPORTED, "waves_per_query > 1 runs the default seen set without "
+                                      "the frontier filter (exact_counts = 0, frontier_filter "
+                                      "0 or 2)");
   const bool fb_on =
-      o.frontier_filter == 1 ||
-      (o.frontier_filter == 0 && store_kind == pann::ST_F32 && a.row_bytes > 128);
+      a.wpq == 1 && (o.frontier_filter == 1 ||
+                     (o.frontier_filter == 0 && store_kind == pann::ST_F32 && a.row_bytes > 128));
   a.fb_log2 = (o.exact_counts || !fb_on)
                   ? 0u
                   : std::min<uint32_t>(std::max<uint32_t>(ceil_log2((uint64_t)8 * a.fcap), 9), 13);
@@ -772,10 +780,13 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   const uint32_t lds = pann::search_lds_bytes(a);
   if ((int64_t)lds > idx->max_lds_block || (int64_t)lds > idx->lds_per_cu)
     return fail(PANN_ERR_UNSUPPORTED, "beam too large for LDS (" + std::to_string(lds) + " bytes)");
+  // queries per CU: waves_per_cu waves (16 by default) in workgroups of wpq
+  // waves, and no more workgroups than the CU's LDS holds
   int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
-  wpc = (int)std::min<int64_t>(wpc, idx->lds_per_cu / lds);
-  if (wpc < 1) wpc = 1;
-  grid = (uint32_t)(idx->num_cus * wpc);
+  int bpc = std::max<int>(1, wpc / (int)a.wpq);
+  bpc = (int)std::min<int64_t>(bpc, idx->lds_per_cu / lds);
+  if (bpc < 1) bpc = 1;
+  grid = (uint32_t)(idx->num_cus * bpc);
   a.ghash = nullptr;
   a.ghash_log2 = 0;
   a.ghash_tag_base = 0;

@@ -127,18 +127,18 @@ class DeviceIndex:
 
     @staticmethod
     def _opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu, exact_counts,
-              visits_per_round=0, eval_cache_log2=0, frontier_filter=0):
+              visits_per_round=0, eval_cache_log2=0, frontier_filter=0, waves_per_query=0):
         return L.SearchOpts(_STORE[store], _SUM[summation], int(bool(rerank)),
                             int(hash_slots_log2), int(bool(async_)), int(waves_per_cu),
                             int(bool(exact_counts)), int(visits_per_round),
-                            int(eval_cache_log2), int(frontier_filter))
+                            int(eval_cache_log2), int(frontier_filter), int(waves_per_query))
 
     def search(self, queries, k: int, beam: int, *, limit=None, degree_limit=None,
                batch_factor: float = 0.125, rerank_factor: float = 100.0,
                store: str = "native", summation: str = "inorder", rerank: bool = False,
                starts: Sequence[int] = (0,), hash_slots_log2: int = 0, waves_per_cu: int = 0,
                exact_counts: bool = False, visits_per_round: int = 0,
-               eval_cache_log2: int = 0, frontier_filter: int = 0):
+               eval_cache_log2: int = 0, frontier_filter: int = 0, waves_per_query: int = 0):
         """Host-buffer batched search (qsearchAll, beamSearch.h:599-646).
         Returns dict(ids uint32[nq,k], dists float32[nq,k], visited, dist_cmps);
         dist_cmps is the reference's count with exact_counts, else the number
@@ -155,7 +155,8 @@ class DeviceIndex:
         st = np.ascontiguousarray(starts, dtype=np.uint32)
         qp = self._qparams(k, beam, limit, degree_limit, rerank_factor, batch_factor)
         op = self._opts(store, summation, rerank, hash_slots_log2, False, waves_per_cu,
-                        exact_counts, visits_per_round, eval_cache_log2, frontier_filter)
+                        exact_counts, visits_per_round, eval_cache_log2, frontier_filter,
+                        waves_per_query)
         L.check(L.lib().pann_search_host(self._h, Q.ctypes.data, nq, C.byref(qp), st.ctypes.data,
                                          len(st), C.byref(op), ids.ctypes.data,
                                          dists.ctypes.data, vis.ctypes.data, cmps.ctypes.data))
@@ -207,7 +208,8 @@ class DeviceIndex:
                       starts: Sequence[int] = (0,), hash_slots_log2: int = 0,
                       waves_per_cu: int = 0, stream=None, async_: bool = False,
                       exact_counts: bool = False, visits_per_round: int = 0,
-                      eval_cache_log2: int = 0, frontier_filter: int = 0):
+                      eval_cache_log2: int = 0, frontier_filter: int = 0,
+                      waves_per_query: int = 0):
         """Device-buffer search: queries/outputs are torch tensors on the
         index's device.  `beam` is the maximum beam when beam_per_query is
         given.  Launches on `stream` (torch stream or raw handle).  With
@@ -217,7 +219,8 @@ class DeviceIndex:
         st = np.ascontiguousarray(starts, dtype=np.uint32)
         qp = self._qparams(k, beam, limit, degree_limit, rerank_factor, batch_factor)
         op = self._opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu,
-                        exact_counts, visits_per_round, eval_cache_log2, frontier_filter)
+                        exact_counts, visits_per_round, eval_cache_log2, frontier_filter,
+                        waves_per_query)
         if stream is not None and hasattr(stream, "cuda_stream"):
             stream = stream.cuda_stream
         L.check(L.lib().pann_search(self._h, _ptr(queries), nq, C.byref(qp), _ptr(k_per_query),

@@ -161,8 +161,11 @@ def test_gpu_autotune_returns_a_valid_shape(oracle, tmp_path, monkeypatch):
     res = autotune.tune(idx, tq, 10, 48, sample=128)
     assert not res["cached"] and res["queries"] == len(Q) and res["sample"] == 128
     shape = {k: res[k] for k in ("waves_per_cu", "visits_per_round", "eval_cache_log2",
-                                 "frontier_filter", "exact_counts", "hash_slots_log2")}
-    assert shape["waves_per_cu"] in autotune.WAVES_PER_CU + (0,)
+                                 "frontier_filter", "exact_counts", "hash_slots_log2",
+                                 "waves_per_query")}
+    assert shape["waves_per_cu"] in autotune.WAVES_PER_CU + (0,) + tuple(
+        w for _, w in autotune.COOP_SHAPES)
+    assert shape["waves_per_query"] in (1,) + tuple(q for q, _ in autotune.COOP_SHAPES)
     assert shape["frontier_filter"] in autotune.FRONTIER_FILTER
     assert not shape["exact_counts"] or (shape["hash_slots_log2"], shape["waves_per_cu"]) \
         in autotune.EXACT_SHAPES
@@ -171,4 +174,91 @@ def test_gpu_autotune_returns_a_valid_shape(oracle, tmp_path, monkeypatch):
         assert np.array_equal(r[key], o[key]), key
     again = autotune.tune(idx, tq, 10, 48, sample=128)
     assert again["cached"] and {k: again[k] for k in shape} == shape
+    idx.close()
+
+
+@pytest.mark.parametrize("store,metric,dim,beam", [("native", "l2", 40, 48), ("native", "l2", 40, 700),
+                                                   ("fp8", "ip", 96, 300), ("bf16", "l2", 150, 90),
+                                                   ("q8", "ip", 40, 200)])
+def test_gpu_waves_per_query_equal_one_wave(oracle, store, metric, dim, beam):
+    """Several waves per query (a workgroup shares each visit's gathers, the
+    merges and the rerank) return exactly what one wave returns: ids,
+    distances, visit counts and evaluations, for the in-order FP32 sum (also
+    bit-exact against the oracle) and the reduced-precision stores with rerank,
+    at beams that fill several merge rounds, and with per-query k and beam."""
+    import torch
+    from parlayann_amd import DeviceIndex, datagen
+    rng = np.random.default_rng(dim + beam)
+    X = rng.standard_normal((6000, dim)).astype(np.float32)
+    if metric == "ip":
+        X /= np.linalg.norm(X, axis=1, keepdims=True)
+    Q = X[rng.integers(0, len(X), 150)] + 0.05 * rng.standard_normal((150, dim)).astype(np.float32)
+    Q = Q.astype(np.float32)
+    print(f"[waves/query] store={store} metric={metric} dim={dim} beam={beam}", flush=True)
+    adj = oracle.build_vamana(X, metric, 24, 48, 1.2 if metric == "l2" else 1.0,
+                              datagen.insertion_orders(len(X), 1, 17))
+    idx = DeviceIndex(X, adj, metric)
+    rr = store != "native"
+    if rr:
+        idx.build_store(store)
+    # the cooperative launch runs without the frontier-id filter: so does the
+    # one-wave reference here, whose evaluation count then matches too
+    kw = dict(store=store, rerank=rr, summation="inorder", frontier_filter=2)
+    one = idx.search(Q, 10, beam, **kw)
+    if store == "native":
+        o = oracle.search_batch(adj, X, metric, Q, 10, beam)
+        for key in ("ids", "dists", "visited"):
+            assert np.array_equal(one[key], o[key]), key
+    for wpq in (2, 4):
+        for wpc in (0, 8):
+            r = idx.search(Q, 10, beam, waves_per_query=wpq, waves_per_cu=wpc, **kw)
+            for key in ("ids", "dists", "visited", "dist_cmps"):
+                assert np.array_equal(r[key], one[key]), (key, wpq, wpc)
+    # per-query k and beam in one cooperative launch
+    dev = torch.device("cuda", 0)
+    kmax = min(50, beam)
+    ks = rng.choice([1, 10, kmax], len(Q)).astype(np.int32)
+    bs = np.array([rng.integers(max(16, k), beam + 1) for k in ks], np.int32)
+    tq = torch.from_numpy(Q).to(dev)
+    tk, tb = torch.from_numpy(ks).to(dev), torch.from_numpy(bs).to(dev)
+    outs = []
+    for wpq in (1, 2, 4):
+        oi = torch.full((len(Q), kmax), -1, dtype=torch.int32, device=dev)
+        od = torch.zeros((len(Q), kmax), dtype=torch.float32, device=dev)
+        idx.search_device(tq, kmax, beam, oi, od, k_per_query=tk, beam_per_query=tb,
+                          waves_per_query=wpq, **kw)
+        outs.append((oi.cpu().numpy(), od.cpu().numpy()))
+    for oi, od in outs[1:]:
+        assert np.array_equal(oi, outs[0][0]) and np.array_equal(od, outs[0][1])
+    idx.close()
+
+
+def test_gpu_waves_per_query_hnsw_and_limits(oracle):
+    """The HNSW search (descent and layer 0 in one launch) with several waves
+    per query equals one wave; the options it does not take are errors."""
+    from parlayann_amd import hnsw
+    from parlayann_amd._lib import PannError
+    from tests.hnsw_data import build_small, model_of
+    X, lev, adj, ent = build_small(oracle, "l2", m=8, efc=48)
+    idx = hnsw.HNSWIndex(X, model_of(X, lev, adj, ent, 8, 48), "l2")
+    rng = np.random.default_rng(3)
+    Q = (X[rng.integers(0, len(X), 200)] + 0.2 * rng.standard_normal((200, X.shape[1]))).astype(
+        np.float32)
+    import torch
+    dev = torch.device("cuda", 0)
+    tq = torch.from_numpy(Q).to(dev)
+    res = []
+    for wpq in (1, 2, 4):
+        oi = torch.zeros((len(Q), 64), dtype=torch.int32, device=dev)
+        od = torch.zeros((len(Q), 64), dtype=torch.float32, device=dev)
+        oc = torch.zeros(len(Q), dtype=torch.int32, device=dev)
+        idx.search_device(tq, 10, 64, oi, od, oc, waves_per_query=wpq)
+        res.append((oi.cpu().numpy(), od.cpu().numpy(), oc.cpu().numpy()))
+    for r in res[1:]:
+        for a, b in zip(r, res[0]):
+            assert np.array_equal(a, b)
+    with pytest.raises(PannError):
+        idx.search_device(tq, 10, 64, oi, od, oc, waves_per_query=2, exact_counts=True)
+    with pytest.raises(PannError):
+        idx.search_device(tq, 10, 64, oi, od, oc, waves_per_query=3)
     idx.close()
