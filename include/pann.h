@@ -226,6 +226,16 @@ pann_status pann_beam_search_visited_host(pann_index_t idx, const void *queries,
                                           uint32_t *out_vis_ids, float *out_vis_dists,
                                           uint32_t *out_vis_count, uint32_t *out_cmps);
 
+/* Distance evaluations made by this index's build steps since it was created
+ * (or since the last call with reset != 0): *search_evals by the build-mode
+ * searches (pann_search_visited), *prune_evals by robustPrune
+ * (pann_robust_prune_batch / pann_robust_prune_groups, each row distance it
+ * computes).  Either pointer may be NULL.  Waits for the device.  No
+ * reference counterpart: times rows per evaluation give the build's
+ * algorithmic bytes. */
+pann_status pann_build_eval_counts(pann_index_t idx, uint64_t *search_evals,
+                                   uint64_t *prune_evals, int32_t reset);
+
 /* Number of device allocations this library has made since it was loaded
  * (every hipMalloc of the index, its stores and its per-stream scratch).  No
  * reference counterpart: it lets a caller check that steady-state searches

@@ -20,7 +20,8 @@ EXPORTS = ["pann_open", "pann_build_store", "pann_get_fp8_scales", "pann_get_q8_
            "pann_build_fp8_residual", "pann_get_fp8_residual",
            "pann_search", "pann_search_noalloc",
            "pann_search_host", "pann_beam_search_host", "pann_beam_search_visited_host",
-           "pann_device_alloc_count", "pann_search_status", "pann_last_kernel_ms",
+           "pann_device_alloc_count", "pann_build_eval_counts", "pann_search_status",
+           "pann_last_kernel_ms",
            "pann_search_visited", "pann_robust_prune_batch", "pann_robust_prune_groups",
            "pann_update_neighbors",
            "pann_download_graph", "pann_close", "pann_last_error", "pann_device_info",
@@ -87,6 +88,8 @@ def lib():
                                           C.c_uint32, C.POINTER(SearchOpts), vp, vp, vp,
                                           C.c_uint32, vp, vp, vp, vp],
         "pann_device_alloc_count": [C.POINTER(C.c_uint64)],
+        "pann_build_eval_counts": [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                   C.c_int32],
         "pann_search_status": [vp, vp],
         "pann_last_kernel_ms": [vp, C.POINTER(C.c_float)],
         "pann_search_visited": [vp, vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32,

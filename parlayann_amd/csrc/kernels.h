@@ -190,6 +190,9 @@ struct PruneArgs {
   uint32_t* out_ids;
   uint32_t* out_count;
   uint32_t sort_cap;  // power of two >= cand_stride + max_degree
+  // nullptr, or a counter to which each workgroup adds the number of row
+  // distances it evaluated
+  unsigned long long* evals;
 };
 
 uint32_t prune_lds_bytes(uint32_t sort_cap, uint32_t dim, int dtype);
@@ -203,6 +206,8 @@ hipError_t launch_sort_visited(const uint64_t* vis, uint32_t nq, uint32_t cap,
 hipError_t launch_hnsw_cut(uint32_t* ids, float* dists, uint32_t stride, uint32_t* count,
                            uint32_t nq, uint32_t k, const int32_t* k_pq,
                            const uint32_t* node_ids, hipStream_t s);
+// acc[0] += v[0] + .. + v[n - 1]
+hipError_t launch_sum_u32(const uint32_t* v, uint32_t n, unsigned long long* acc, hipStream_t s);
 hipError_t launch_update_neighbors(uint32_t* nbrs, uint32_t nbr_stride, uint32_t max_degree,
                                    const uint32_t* points, uint32_t np, const uint32_t* lists,
                                    const uint32_t* counts, uint32_t R, hipStream_t s);

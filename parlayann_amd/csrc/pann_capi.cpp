@@ -171,10 +171,15 @@ struct pann_index_s {
     // grown geometrically so that repeated calls allocate nothing
     char* d_stage = nullptr;
     size_t stage_cap = 0;
+    uint32_t* d_cmps = nullptr;  // build-mode search: evaluations per query
+    size_t cmps_cap = 0;
   };
   std::unordered_map<hipStream_t, StreamCtx> streams;
   StreamCtx* last_timed = nullptr;
   uint32_t* d_dummy = nullptr;
+  // distance evaluations of the build steps: [0] build-mode searches, [1]
+  // robustPrune (pann_build_eval_counts)
+  unsigned long long* d_evals = nullptr;
   uint64_t* d_pkeys = nullptr;   // prune sort scratch for groups too large for LDS
   uint8_t* d_pflags = nullptr;
   size_t pscratch_entries = 0;
@@ -309,6 +314,20 @@ extern "C" {
 
 const char* pann_last_error(void) { return g_err.c_str(); }
 
+pann_status pann_build_eval_counts(pann_index_t idx, uint64_t* search_evals,
+                                   uint64_t* prune_evals, int32_t reset) {
+  if (!idx) return fail(PANN_ERR_ARG, "null index");
+  std::lock_guard<std::mutex> lk(idx->mu);
+  HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
+  HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  unsigned long long v[2];
+  HIP_TRY(hipMemcpy(v, idx->d_evals, sizeof(v), hipMemcpyDeviceToHost), "hipMemcpy(counters)");
+  if (search_evals) *search_evals = v[0];
+  if (prune_evals) *prune_evals = v[1];
+  if (reset) HIP_TRY(hipMemset(idx->d_evals, 0, sizeof(v)), "hipMemset(counters)");
+  return PANN_OK;
+}
+
 pann_status pann_device_alloc_count(uint64_t* count) {
   if (!count) return fail(PANN_ERR_ARG, "null argument");
   *count = g_device_allocs.load(std::memory_order_relaxed);
@@ -399,6 +418,9 @@ pann_status pann_open(const pann_index_desc* desc, pann_index_t* out) {
   idx->levels[0].nbr_stride = idx->nbr_stride;
   e = dev_malloc(&idx->d_dummy, 256);
   if (e != hipSuccess) return cleanup(hip_fail(e, "scratch allocation"));
+  e = dev_malloc(&idx->d_evals, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(idx->d_evals, 0, 2 * sizeof(unsigned long long));
+  if (e != hipSuccess) return cleanup(hip_fail(e, "counter allocation"));
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return cleanup(hip_fail(e, "hipDeviceSynchronize"));
   (void)st;
@@ -1228,6 +1250,14 @@ pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint
   if (st != PANN_OK) return st;
   st = ensure_vis(*C, (size_t)nq * vis_cap);
   if (st != PANN_OK) return st;
+  if (C->cmps_cap < nq) {  // per-query evaluations, summed into the build counter
+    const size_t cap = std::max<size_t>((size_t)nq, 2 * C->cmps_cap);
+    if (C->d_cmps) (void)hipFree(C->d_cmps);
+    C->d_cmps = nullptr;
+    C->cmps_cap = 0;
+    HIP_TRY(dev_malloc(&C->d_cmps, cap * 4), "hipMalloc(evaluation counts)");
+    C->cmps_cap = cap;
+  }
   a.queries = idx->d_pts;  // each query is the point's own row
   a.q_stride = idx->pts_stride;
   a.fqueries = idx->d_pts;
@@ -1240,10 +1270,13 @@ pann_status pann_search_visited(pann_index_t idx, const uint32_t* self_ids, uint
   a.out_dists = (float*)idx->d_dummy;
   a.out_stride = 0;
   a.out_visited = vis_count;
+  a.out_cmps = C->d_cmps;
+  a.accumulate = 0u;
   // with self_ids set the kernel reads query q from row self_ids[q]
   grid = std::min<uint32_t>(grid, nq);
   HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
           "build-mode search launch");
+  HIP_TRY(pann::launch_sum_u32(C->d_cmps, nq, idx->d_evals, stream), "evaluation count launch");
   HIP_TRY(pann::launch_sort_visited(C->d_vis, nq, vis_cap, vis_count, vis_ids, vis_dists, stream),
           "visited sort launch");
   return check_status(*C, stream);
@@ -1281,6 +1314,7 @@ pann_status run_prune(pann_index_t idx, uint32_t level, const uint32_t* pts, uin
   a.add = add;
   a.out_ids = out_ids;
   a.out_count = out_count;
+  a.evals = idx->d_evals + 1;
   uint64_t cap = 2;
   while (cap < (uint64_t)max_count + G.max_degree) cap <<= 1;
   if (cap > (1u << 24)) return fail(PANN_ERR_UNSUPPORTED, "candidate set too large");
@@ -1869,7 +1903,7 @@ pann_status pann_close(pann_index_t idx) {
   void* ptrs[] = {idx->d_pts,    idx->d_bf16,    idx->d_fp8,   idx->d_scales, idx->d_absmax,
                   idx->d_nbrs,   idx->d_q8,      idx->d_xnorm, idx->d_dummy,  idx->d_pkeys,
                   idx->d_pflags, idx->d_fp8r,    idx->d_rscales, idx->d_rabsmax, idx->d_cent,
-                  idx->d_cent_t, idx->d_assign};
+                  idx->d_cent_t, idx->d_assign, idx->d_evals};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (size_t l = 1; l < idx->levels.size(); l++)
@@ -1877,7 +1911,7 @@ pann_status pann_close(pann_index_t idx) {
   for (auto& kv : idx->streams) {
     StreamCtx& C = kv.second;
     void* cp[] = {C.d_counter, C.d_status, C.d_starts, C.d_ghash, C.d_vis, C.d_eps, C.d_epsd,
-                  C.d_qcodes, C.d_qtable, C.d_stage};
+                  C.d_qcodes, C.d_qtable, C.d_stage, C.d_cmps};
     for (void* p : cp)
       if (p) (void)hipFree(p);
     if (C.blas) (void)rocblas_destroy_handle(C.blas);

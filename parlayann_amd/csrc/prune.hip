@@ -18,32 +18,89 @@
 //   add = 5: the reverse-edge step of insert (:952-983): the added ids are
 //     appended to the current list when both fit in R, otherwise the union is
 //     sorted by distance to the point and cut to R.
+#include <algorithm>
+
 #include "kernels.h"
 #include "dist.h"
 
 namespace pann {
 
-// distance between two stored rows, the reference's arithmetic
+// Rows evaluated per block of the pruning steps: their ids, distances and
+// sort positions stay in LDS.
+constexpr int PRUNE_BLK = 128;
+
+// Integer rows (uint8 / int8): distances of rows NL[0..n) to the row QL (LDS,
+// zero-padded to whole 128-byte lines) into DL, exact int32 as in
+// euclidian_point.h:45-56 / mips_point.h:44-50.  Rows are gathered 8 at a
+// time, one 128-byte line per 8-lane group and instruction, as the search
+// gathers them; L2 is sum q^2 + sum p^2 - 2 sum q.p (mod 2^32: the exact
+// value, which is below 2^31).
 template <int DT, int MT>
-__device__ __forceinline__ float row_dist(const char* qrow_lds, const char* row, int dim) {
-  if constexpr (DT == ST_F32) {
-    return seq_f32_row<MT>((const float*)qrow_lds, (const float*)row, dim);
-  } else {
-    int32_t acc = 0;
-    for (int j = 0; j < dim; j++) {
-      int32_t a, b;
-      if constexpr (DT == ST_U8) {
-        a = (int32_t)((const uint8_t*)qrow_lds)[j];
-        b = (int32_t)((const uint8_t*)row)[j];
-      } else {
-        a = (int32_t)((const int8_t*)qrow_lds)[j];
-        b = (int32_t)((const int8_t*)row)[j];
-      }
-      if constexpr (MT == M_L2) acc += (a - b) * (a - b);
-      else acc += a * b;
+__device__ __forceinline__ void eval_int_lds(const char* QL, int row_bytes, const char* base,
+                                             uint64_t stride, const uint32_t* NL, int n,
+                                             float* DL, int lane) {
+  constexpr int UP = 4;
+  const int sub = lane & 7, grp = lane >> 3;
+  const int nl = (row_bytes + 127) >> 7;
+  for (int r0 = 0; r0 < n; r0 += 8 * UP) {
+    const char* row[UP];
+#pragma unroll
+    for (int u = 0; u < UP; u++) {
+      const int r = r0 + u * 8 + grp;
+      row[u] = base + (uint64_t)NL[r < n ? r : 0] * stride;
     }
-    return MT == M_L2 ? (float)acc : -((float)acc);
+    uint32_t pq[UP] = {}, pp[UP] = {}, qq = 0u;
+    for (int c = 0; c < nl; c++) {  // uniform
+      const uint4 qv = *(const uint4*)(QL + c * 128 + sub * 16);
+      const uint32_t qw[4] = {qv.x, qv.y, qv.z, qv.w};
+      uint4 raw[UP];
+#pragma unroll
+      for (int u = 0; u < UP; u++) raw[u] = ld16(row[u] + c * 128 + sub * 16);
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        if constexpr (DT == ST_U8) {
+          if (MT == M_L2) qq = __builtin_amdgcn_udot4(qw[e], qw[e], qq, false);
+        } else {
+          if (MT == M_L2) qq = (uint32_t)__builtin_amdgcn_sdot4((int)qw[e], (int)qw[e], (int)qq, false);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UP; u++) {
+        const uint32_t w[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          if constexpr (DT == ST_U8) {
+            if (MT == M_L2) pp[u] = __builtin_amdgcn_udot4(w[e], w[e], pp[u], false);
+            pq[u] = __builtin_amdgcn_udot4(w[e], qw[e], pq[u], false);
+          } else {
+            if (MT == M_L2)
+              pp[u] = (uint32_t)__builtin_amdgcn_sdot4((int)w[e], (int)w[e], (int)pp[u], false);
+            pq[u] = (uint32_t)__builtin_amdgcn_sdot4((int)w[e], (int)qw[e], (int)pq[u], false);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UP; u++) {
+      const int r = r0 + u * 8 + grp;
+      const int v = MT == M_L2 ? (int)(qq + pp[u] - 2u * pq[u]) : (int)pq[u];
+      const int s = group8_sum(v);
+      if (sub == 0 && r < n) DL[r] = MT == M_L2 ? (float)s : -((float)s);
+    }
   }
+}
+
+// Distances of the stored rows NL[0..n) to the row QL (LDS) into DL, the
+// reference's arithmetic: in-order FP32 sums (the search's 8-lane gather and
+// DPP chain, bit-identical to the reference's float loop) or exact int32.
+template <int DT, int MT>
+__device__ __forceinline__ void eval_rows_lds(const char* QL, int dim, const char* base,
+                                              uint64_t stride, const uint32_t* NL, int n,
+                                              float* DL, int lane) {
+  if constexpr (DT == ST_F32)
+    eval_group_inorder_lds<MT, 4, 2>((const float*)QL, dim, base, stride, NL, n, DL, lane);
+  else
+    eval_int_lds<DT, MT>(QL, dim * store_elem_bytes(DT), base, stride, NL, n, DL, lane);
 }
 
 // Orders this wave's accesses to its sort buffer (LDS, or global scratch
@@ -82,15 +139,23 @@ __device__ void wave_bitonic_sort(uint64_t* K, int np, int lane) {
 
 
 
+// One stored row (whole 128-byte lines: its zero padding included) into LDS.
+__device__ __forceinline__ void row_to_lds(char* dst, const char* row, uint32_t row_lines,
+                                           int lane) {
+  for (uint32_t j = lane; j < row_lines * 8u; j += PANN_WAVE)
+    *(uint4*)(dst + j * 16u) = ld16(row + j * 16u);
+}
+
 template <int DT, int MT, bool KG>
 __device__ void prune_one(const PruneArgs& a, uint32_t i, int lane, char* QP, char* QS,
-                          uint64_t* K, uint8_t* PR) {
-  const int esz = store_elem_bytes(DT);
+                          uint64_t* K, uint8_t* PR, uint32_t* NL, float* DL, uint32_t* TI,
+                          uint64_t& evals) {
   const uint32_t p = a.points[i];
+  const int dim = (int)a.dim;
+  const uint32_t row_lines = (a.dim * (uint32_t)store_elem_bytes(DT) + 127u) >> 7;
 
-  const char* prow = a.pts + (uint64_t)p * a.stride;
-  for (uint32_t j = lane; j < a.dim * esz; j += PANN_WAVE) QP[j] = prow[j];
-  wave_sync_k<KG>();
+  row_to_lds(QP, a.pts + (uint64_t)p * a.stride, row_lines, lane);
+  wave_lds_sync();
 
   uint32_t nc;
   const uint32_t* cid;
@@ -156,25 +221,46 @@ __device__ void prune_one(const PruneArgs& a, uint32_t i, int lane, char* QP, ch
       return;
     }
   }
-  for (uint32_t t = lane; t < nc; t += PANN_WAVE) {
-    uint32_t c = cid[t];
-    float d = cdist ? cdist[t] : row_dist<DT, MT>(QP, a.pts + (uint64_t)c * a.stride, (int)a.dim);
-    K[t] = make_key(d, c);
-  }
+  // keys (d(p, c), c) of the candidates and, with `add`, of p's current
+  // out-neighbours (:72-77), the distances evaluated PRUNE_BLK rows at a time
   uint32_t total = nc;
+  if (cdist) {
+    for (uint32_t t = lane; t < nc; t += PANN_WAVE) K[t] = make_key(cdist[t], cid[t]);
+  } else {
+    for (uint32_t t0 = 0; t0 < nc; t0 += PRUNE_BLK) {
+      const int m = (int)(nc - t0 < (uint32_t)PRUNE_BLK ? nc - t0 : (uint32_t)PRUNE_BLK);
+      for (int t = lane; t < m; t += PANN_WAVE) NL[t] = cid[t0 + t];
+      wave_lds_sync();
+      eval_rows_lds<DT, MT>(QP, dim, a.pts, a.stride, NL, m, DL, lane);
+      evals += (uint64_t)m;
+      wave_lds_sync();
+      for (int t = lane; t < m; t += PANN_WAVE) K[t0 + t] = make_key(DL[t], NL[t]);
+      wave_lds_sync();
+    }
+  }
   if (a.add != 0 && a.add != 4) {  // :72-77 (robustPrune's add, also the reverse-edge rules)
     const uint32_t* row = a.nbrs + (uint64_t)p * a.nbr_stride;
-    for (uint32_t t = lane; t < a.max_degree; t += PANN_WAVE) {
-      uint32_t c = row[t];
-      if (c != PANN_SENTINEL)
-        K[nc + t] = make_key(row_dist<DT, MT>(QP, a.pts + (uint64_t)c * a.stride, (int)a.dim), c);
+    // the row's ids before its first sentinel, in order
+    for (uint32_t t0 = 0; t0 < a.max_degree; t0 += PRUNE_BLK) {
+      const int m = (int)(a.max_degree - t0 < (uint32_t)PRUNE_BLK ? a.max_degree - t0
+                                                                   : (uint32_t)PRUNE_BLK);
+      int cnt = 0;
+      for (int b = 0; b < m; b += PANN_WAVE) {
+        const int t = b + lane;
+        const uint32_t c = t < m ? row[t0 + t] : PANN_SENTINEL;
+        const bool v = c != PANN_SENTINEL;
+        const uint64_t bm = ballot(v);
+        if (v) NL[cnt + (int)mbcnt(bm)] = c;
+        cnt += __popcll(bm);
+      }
+      wave_lds_sync();
+      eval_rows_lds<DT, MT>(QP, dim, a.pts, a.stride, NL, cnt, DL, lane);
+      evals += (uint64_t)cnt;
+      wave_lds_sync();
+      for (int t = lane; t < cnt; t += PANN_WAVE) K[total + t] = make_key(DL[t], NL[t]);
+      total += (uint32_t)cnt;
+      wave_lds_sync();
     }
-    uint32_t deg = 0;
-    for (uint32_t t = 0; t < a.max_degree; t += PANN_WAVE) {
-      uint32_t tt = t + lane;
-      deg += __popcll(ballot(tt < a.max_degree && row[tt] != PANN_SENTINEL));
-    }
-    total = nc + deg;
   }
   int np2 = 1;
   while ((uint32_t)np2 < total) np2 <<= 1;
@@ -206,22 +292,40 @@ __device__ void prune_one(const PruneArgs& a, uint32_t i, int lane, char* QP, ch
     if (!hnsw && ps == p) continue;
     if (lane == 0) out[nout] = ps;
     nout++;
-    const char* srow = a.pts + (uint64_t)ps * a.stride;
-    for (uint32_t j = lane; j < a.dim * esz; j += PANN_WAVE) QS[j] = srow[j];
+    row_to_lds(QS, a.pts + (uint64_t)ps * a.stride, row_lines, lane);
     wave_sync_k<KG>();
-    for (uint32_t t = idx + 1 + lane; t < total; t += PANN_WAVE) {
-      if (!PR[t]) {
-        uint64_t kk = K[t];
-        float d_sp = row_dist<DT, MT>(QS, a.pts + (uint64_t)key_id(kk) * a.stride, (int)a.dim);
+    // the surviving candidates after idx, PRUNE_BLK at a time: their ids into
+    // NL (positions in TI), d(p*, p') into DL, then the pruning test (:108-113)
+    for (uint32_t tb = idx + 1; tb < total;) {
+      int cnt = 0;
+      while (tb < total && cnt <= PRUNE_BLK - PANN_WAVE) {
+        const uint32_t t = tb + lane;
+        const bool v = t < total && !PR[t];
+        const uint64_t bm = ballot(v);
+        if (v) {
+          const int o = cnt + (int)mbcnt(bm);
+          NL[o] = key_id(K[t]);
+          TI[o] = t;
+        }
+        cnt += __popcll(bm);
+        tb += PANN_WAVE;
+      }
+      wave_lds_sync();
+      eval_rows_lds<DT, MT>(QS, dim, a.pts, a.stride, NL, cnt, DL, lane);
+      evals += (uint64_t)cnt;
+      wave_lds_sync();
+      for (int j = lane; j < cnt; j += PANN_WAVE) {
+        const uint32_t t = TI[j];
+        const float d_sp = DL[j];
+        const float d_pc = key_dist(K[t]);
         if (hnsw) {
-          const float d_cu = key_dist(kk) * alpha_f;  // HNSW.hpp:460-470
-          if (d_sp < d_cu) PR[t] = 1;
-        } else if (a.alpha * (double)d_sp <= (double)key_dist(kk)) {
+          if (d_sp < d_pc * alpha_f) PR[t] = 1;  // HNSW.hpp:460-470
+        } else if (a.alpha * (double)d_sp <= (double)d_pc) {
           PR[t] = 1;
         }
       }
+      wave_sync_k<KG>();
     }
-    wave_sync_k<KG>();
   }
   if (lane == 0) a.out_count[i] = nout;
 }
@@ -233,16 +337,31 @@ template <int DT, int MT, bool KG>
 __global__ __launch_bounds__(PANN_WAVE) void k_robust_prune(PruneArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int lane = threadIdx.x;
-  const int esz = store_elem_bytes(DT);
-  char* QP = smem;                                             // row of p
-  char* QS = QP + ((a.dim * esz + 15) & ~15u);                 // row of p*
-  char* tail = QS + ((a.dim * esz + 15) & ~15u);
+  const uint32_t row = ((a.dim * (uint32_t)store_elem_bytes(DT) + 127u) >> 7) * 128u;
+  char* QP = smem;       // row of p (whole lines)
+  char* QS = QP + row;   // row of p*
+  uint32_t* NL = (uint32_t*)(QS + row);  // a block of rows to evaluate ...
+  float* DL = (float*)(NL + PRUNE_BLK);   // ... their distances ...
+  uint32_t* TI = (uint32_t*)(DL + PRUNE_BLK);  // ... and sort positions
+  char* tail = (char*)(TI + PRUNE_BLK);
   uint64_t* K = KG ? a.gkeys + (uint64_t)blockIdx.x * a.sort_cap : (uint64_t*)tail;
   uint8_t* PR = KG ? a.gflags + (uint64_t)blockIdx.x * a.sort_cap
                    : (uint8_t*)(tail + a.sort_cap * 8);
+  uint64_t evals = 0;  // uniform over the wave
   for (uint32_t i = blockIdx.x; i < a.np; i += gridDim.x) {
-    prune_one<DT, MT, KG>(a, i, lane, QP, QS, K, PR);
+    prune_one<DT, MT, KG>(a, i, lane, QP, QS, K, PR, NL, DL, TI, evals);
   }
+  if (a.evals && lane == 0 && evals) atomicAdd(a.evals, (unsigned long long)evals);
+}
+
+// acc[0] += the sum of v[0..n): one wave per block, one atomic per block
+__global__ __launch_bounds__(PANN_WAVE) void k_sum_u32(const uint32_t* v, uint32_t n,
+                                                       unsigned long long* acc) {
+  unsigned long long s = 0;
+  for (uint32_t i = blockIdx.x * PANN_WAVE + threadIdx.x; i < n; i += gridDim.x * PANN_WAVE)
+    s += v[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (threadIdx.x == 0 && s) atomicAdd(acc, s);
 }
 // Sorts each query's visited keys and splits them into ids / dists.
 __global__ __launch_bounds__(PANN_WAVE) void k_sort_visited(const uint64_t* vis, uint32_t nq,
@@ -279,8 +398,8 @@ __global__ void k_update_neighbors(uint32_t* nbrs, uint32_t nbr_stride, uint32_t
 }
 
 uint32_t prune_lds_bytes(uint32_t sort_cap, uint32_t dim, int dtype) {
-  uint32_t row = (dim * (uint32_t)store_elem_bytes(dtype) + 15u) & ~15u;
-  return 2u * row + (sort_cap ? sort_cap * 9u : 0u);
+  const uint32_t row = ((dim * (uint32_t)store_elem_bytes(dtype) + 127u) >> 7) * 128u;
+  return 2u * row + 12u * (uint32_t)PRUNE_BLK + (sort_cap ? sort_cap * 9u : 0u);
 }
 
 hipError_t launch_robust_prune(const PruneArgs& a, int dtype, int metric, bool keys_global,
@@ -319,6 +438,13 @@ hipError_t launch_sort_visited(const uint64_t* vis, uint32_t nq, uint32_t cap,
   if (nq == 0) return hipSuccess;
   hipLaunchKernelGGL(k_sort_visited, dim3(nq), dim3(PANN_WAVE), lds, s, vis, nq, cap, counts, ids,
                      dists);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_u32(const uint32_t* v, uint32_t n, unsigned long long* acc, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t grid = std::min<uint32_t>((n + PANN_WAVE - 1) / PANN_WAVE, 1024u);
+  hipLaunchKernelGGL(k_sum_u32, dim3(grid), dim3(PANN_WAVE), 0, s, v, n, acc);
   return hipGetLastError();
 }
 

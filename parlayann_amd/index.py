@@ -119,6 +119,13 @@ class DeviceIndex:
         L.check(L.lib().pann_download_graph(self._h, adj.ctypes.data))
         return adj
 
+    def build_eval_counts(self, reset: bool = False) -> dict:
+        """Distance evaluations of the build steps on this index so far:
+        {"search": build-mode searches, "prune": robustPrune row distances}."""
+        se, pe = C.c_uint64(0), C.c_uint64(0)
+        L.check(L.lib().pann_build_eval_counts(self._h, C.byref(se), C.byref(pe), int(reset)))
+        return {"search": int(se.value), "prune": int(pe.value)}
+
     # -- search --------------------------------------------------------------
     def _qparams(self, k, beam, limit, degree_limit, rerank_factor, batch_factor):
         return L.QParams(int(k), int(beam), int(self.n if limit is None else limit),

@@ -480,15 +480,58 @@ def test_build_mode_visited_and_prune(oracle, u8_case):
         assert np.array_equal(out_h[j, :cnt_h[j]], pr)
 
 
-@pytest.mark.parametrize("kind", ["u8", "f32"])
+def test_build_eval_counts(u8_case):
+    """pann_build_eval_counts: the build-mode search evaluates at least every
+    node it visits; robustPrune's sort-and-cut mode (add = 3) evaluates exactly
+    the candidates and the point's current neighbours; reset clears both."""
+    import torch
+    from parlayann_amd import _lib as L
+    X, Q, adj = u8_case
+    idx = DeviceIndex(X, adj, "l2")
+    dev = torch.device("cuda")
+    ids = np.arange(0, 20000, 997, dtype=np.int32)
+    nb, cap = len(ids), 512
+    tid = torch.from_numpy(ids).to(dev)
+    vid = torch.zeros((nb, cap), dtype=torch.int32, device=dev)
+    vd = torch.zeros((nb, cap), dtype=torch.float32, device=dev)
+    vc = torch.zeros(nb, dtype=torch.int32, device=dev)
+    idx.build_eval_counts(reset=True)
+    L.check(L.lib().pann_search_visited(idx._h, tid.data_ptr(), nb, 64, 0.125, 0, cap,
+                                        vid.data_ptr(), vd.data_ptr(), vc.data_ptr(), None))
+    ev = idx.build_eval_counts(reset=True)
+    nvis = int(vc.sum())
+    assert ev["search"] >= nvis > 0 and ev["prune"] == 0, ev
+    R = int(adj.shape[1] - 1)
+    out = torch.zeros((nb, R), dtype=torch.int32, device=dev)
+    cnt = torch.zeros(nb, dtype=torch.int32, device=dev)
+    L.check(L.lib().pann_robust_prune_batch(idx._h, tid.data_ptr(), nb, vid.data_ptr(), None,
+                                            vc.data_ptr(), cap, 1.2, R, 3, out.data_ptr(),
+                                            cnt.data_ptr(), None))
+    ev = idx.build_eval_counts()
+    assert ev == {"search": 0, "prune": nvis + int(adj[ids, 0].astype(np.int64).sum())}, ev
+    assert idx.build_eval_counts(reset=True) == ev
+    assert idx.build_eval_counts() == {"search": 0, "prune": 0}
+    idx.close()
+
+
+@pytest.mark.parametrize("kind", ["u8", "f32", "f32-ip-d99", "i8-d200"])
 def test_gpu_vamana_build_matches_oracle(oracle, kind):
+    """The whole build, bit-equal: search, robustPrune (its distances on the
+    8-lane row gathers: the in-order FP32 sum with the fused tail at d = 99,
+    rows of two lines for int8 at d = 200) and the reverse-edge rule."""
     from parlayann_amd.vamana import build_vamana
     _say(f"gpu vamana build {kind}")
     if kind == "u8":
         X, _ = datagen.workload_u8_l2(n=4000, nq=10)
         R, Lb, alpha, metric = 32, 64, 1.2, "l2"
-    else:
+    elif kind == "f32":
         X, _ = datagen.workload_f32_l2(n=4000, nq=10)
+        R, Lb, alpha, metric = 32, 64, 1.2, "l2"
+    elif kind == "f32-ip-d99":
+        X = datagen.anisotropic_unit(3000, datagen.anisotropic_sigmas(99, 5), 6)
+        R, Lb, alpha, metric = 32, 64, 1.0, "ip"
+    else:
+        X = (datagen.workload_u8_l2(n=3000, nq=10, d=200)[0].astype(np.int16) - 128).astype(np.int8)
         R, Lb, alpha, metric = 32, 64, 1.2, "l2"
     perm = datagen.insertion_orders(len(X), 2, 21)
     o = oracle.build_vamana(X, metric, R, Lb, alpha, perm, num_passes=2)
