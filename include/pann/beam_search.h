@@ -157,6 +157,8 @@ struct SearchOptions {
   int device = 0;
   int32_t summation = PANN_SUM_INORDER;  /* PANN_SUM_TREE: faster, not bit-identical */
   bool exact_counts = false;             /* stats distances = the reference's dist_cmps */
+  int32_t waves_per_query = 1;           /* 2 or 4: a workgroup of waves per query (large
+                                            beams over small batches); same results */
 };
 inline SearchOptions &search_options() {
   static SearchOptions o;
@@ -486,6 +488,7 @@ inline pann_search_opts opts(int32_t store, bool rerank) {
   o.summation = search_options().summation;
   o.rerank = rerank ? 1 : 0;
   o.exact_counts = search_options().exact_counts ? 1 : 0;
+  o.waves_per_query = search_options().waves_per_query;
   return o;
 }
 template <typename ES>

@@ -64,6 +64,16 @@ int main(int argc, char** argv) {
     fprintf(stderr, "a repeated qsearchAll returned other ids\n");
     return 3;
   }
+  // two waves per query: a launch option only, the same ids
+  search_options().waves_per_query = 2;
+  stats<unsigned> qs3(nq);
+  auto coop = qsearchAll(Query_Points, Q_Query, Q_Query, G, Points, Q_Points, Q_Points, qs3,
+                         start, QP);
+  search_options().waves_per_query = 1;
+  if (coop != qall) {
+    fprintf(stderr, "qsearchAll with two waves per query returned other ids\n");
+    return 4;
+  }
 
   FILE* out = fopen(argv[7], "wb");
   if (!out) return 2;
