@@ -141,7 +141,8 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
     return dict(res, cached=False, key=key)
 
 
-HNSW_SHAPES = ((False, 0), (True, 0), (True, 12))  # (exact seen set, its LDS slots log2)
+# (exact seen set, its LDS slots log2, evaluated-id cache log2 of the default set)
+HNSW_SHAPES = ((False, 0, 0), (False, 0, 10), (False, 0, 11), (True, 0, 0), (True, 12, 0))
 HNSW_WAVES = (0, 12, 8)
 
 
@@ -151,9 +152,10 @@ def tune_hnsw(index, queries, k_per_query, ef_per_query, max_ef: int, *, store: 
     """Fastest seen set and occupancy for HNSWIndex `index` on a batch of
     device queries with per-query k and ef: the default set (every
     neighbour evaluated, checked against the frontier and pending
-    candidates) or the reference's exact set, which at small ef and high
-    degree gathers far fewer rows (an id already seen is not evaluated
-    again); results are the same.  Returns {exact_counts, hash_slots_log2,
+    candidates), with or without an LDS cache of evaluated ids, or the
+    reference's exact set, which at small ef and high degree gathers far
+    fewer rows (an id already seen is not evaluated again); results are the
+    same.  Returns {exact_counts, hash_slots_log2, eval_cache_log2,
     waves_per_cu, ms}."""
     import torch
     from .index import device_info
@@ -165,29 +167,30 @@ def tune_hnsw(index, queries, k_per_query, ef_per_query, max_ef: int, *, store: 
         index.model.m, index.model.top_level, str(base.dtype), base.metric, store, summation,
         int(rerank), int(queries.shape[0]), max_ef))
     table = _load() if use_cache else {}
-    if use_cache and key in table:
+    if use_cache and key in table and "eval_cache_log2" in table[key]:
         return dict(table[key], cached=True, key=key)
     nq = int(queries.shape[0])
     oi = torch.empty((nq, max_ef), dtype=torch.int32, device=dev)
     od = torch.empty((nq, max_ef), dtype=torch.float32, device=dev)
     oc = torch.empty(nq, dtype=torch.int32, device=dev)
     timings = []
-    for ex, hs in HNSW_SHAPES:
+    for ex, hs, ec in HNSW_SHAPES:
         for wpc in HNSW_WAVES:
             kw = dict(store=store, summation=summation, rerank=rerank, exact_counts=ex,
-                      hash_slots_log2=hs, waves_per_cu=wpc, k_per_query=k_per_query,
-                      ef_per_query=ef_per_query)
+                      hash_slots_log2=hs, eval_cache_log2=ec, waves_per_cu=wpc,
+                      k_per_query=k_per_query, ef_per_query=ef_per_query)
             index.search_device(queries, 0, max_ef, oi, od, oc, **kw)  # warm
             ms = []
             for _ in range(reps):
                 index.search_device(queries, 0, max_ef, oi, od, oc, async_=True, **kw)
                 ms.append(base.last_kernel_ms())
             base.search_status()
-            timings.append((min(ms), ex, hs, wpc))
+            timings.append((min(ms), ex, hs, ec, wpc))
     best = min(timings)
-    res = {"exact_counts": best[1], "hash_slots_log2": best[2], "waves_per_cu": best[3],
-           "ms": best[0], "timings": [{"exact_counts": e, "hash_slots_log2": h,
-                                       "waves_per_cu": w, "ms": t} for t, e, h, w in timings]}
+    res = {"exact_counts": best[1], "hash_slots_log2": best[2], "eval_cache_log2": best[3],
+           "waves_per_cu": best[4], "ms": best[0],
+           "timings": [{"exact_counts": e, "hash_slots_log2": h, "eval_cache_log2": c,
+                        "waves_per_cu": w, "ms": t} for t, e, h, c, w in timings]}
     if use_cache:
         table[key] = res
         _store(table)

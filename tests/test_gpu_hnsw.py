@@ -91,7 +91,7 @@ def test_gpu_search_level_counts(oracle, small_l2, extra, exact):
 
 
 def test_gpu_tune_hnsw_choice_is_exact(oracle, small_l2, tmp_path, monkeypatch):
-    """autotune.tune_hnsw picks a seen set and occupancy; the chosen options
+    """autotune.tune_hnsw picks a seen set, eval cache and occupancy; the chosen options
     return the oracle's results through the serving path's call."""
     import torch
     from parlayann_amd import autotune, hnsw
@@ -105,13 +105,14 @@ def test_gpu_tune_hnsw_choice_is_exact(oracle, small_l2, tmp_path, monkeypatch):
     efs = np.maximum(ks, 32).astype(np.int32)
     tk, te = torch.from_numpy(ks).to(dev), torch.from_numpy(efs).to(dev)
     t = autotune.tune_hnsw(idx, tq, tk, te, 40, summation="inorder")
-    assert (t["exact_counts"], t["hash_slots_log2"]) in autotune.HNSW_SHAPES
+    assert (t["exact_counts"], t["hash_slots_log2"], t["eval_cache_log2"]) in autotune.HNSW_SHAPES
     oi = torch.zeros((len(Q), 40), dtype=torch.int32, device=dev)
     od = torch.zeros((len(Q), 40), dtype=torch.float32, device=dev)
     oc = torch.zeros(len(Q), dtype=torch.int32, device=dev)
     idx.search_device(tq, 0, 40, oi, od, oc, k_per_query=tk, ef_per_query=te,
                       summation="inorder", exact_counts=t["exact_counts"],
-                      hash_slots_log2=t["hash_slots_log2"], waves_per_cu=t["waves_per_cu"])
+                      hash_slots_log2=t["hash_slots_log2"], eval_cache_log2=t["eval_cache_log2"],
+                      waves_per_cu=t["waves_per_cu"])
     gi, gd, gc = oi.cpu().numpy().view(np.uint32), od.cpu().numpy(), oc.cpu().numpy()
     for k in (10, 40):
         sel = np.nonzero(ks == k)[0]
