@@ -43,9 +43,7 @@ def build(verbose: bool = False) -> str:
     srcs = SOURCES + [CALIB_SRC]
     with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, srcs))
-    # libpann links rocBLAS for one plain FP32 GEMM (the FP8 residual store's
-    # per-query centroid table)
-    for lib, o, libs in ((LIB, objs[:-1], ["-L/opt/rocm/lib", "-lrocblas"]),
+    for lib, o, libs in ((LIB, objs[:-1], []),
                          (CALIB_LIB, objs[-1:], [])):
         tmp = lib + ".tmp"
         cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *o, *libs]

@@ -322,4 +322,82 @@ hipError_t launch_row_sqnorm(const void* X, int point_dtype, uint64_t xs, uint32
   return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// The FP8 residual store's per-query centroid table T = Q . C^T (nq x ncent,
+// row-major; Q: nq rows of qs floats, dim used; C: ncent rows of cs floats,
+// zero past dim up to a multiple of 8, 16-byte aligned) on v_mfma_f32_32x32x2f32.  One wave
+// per tile of 32 queries x 32 * QC_TN centroids.  The K index is permuted as
+// in k_bf_topk: over a block of 8 dimensions, MFMA step s takes dimension
+// k0 + (l / 32) * 4 + s in lane l, so each lane loads the four it needs with
+// one 16-byte load of its query row (A) and one of each of its centroid rows
+// (B).  The QC_TN accumulators are independent, so their MFMAs overlap.
+// ---------------------------------------------------------------------------
+constexpr int QC_TN = 4;
+
+__global__ __launch_bounds__(PANN_WAVE) void k_qc_table(const float* Q, uint64_t qs, uint32_t nq,
+                                                       uint32_t dim, const float* C,
+                                                       uint64_t cs, uint32_t ncent, float* T) {
+  const int lane = threadIdx.x;
+  const uint32_t q0 = blockIdx.x * 32u, c0 = blockIdx.y * 32u * QC_TN;
+  const uint32_t qa = q0 + (uint32_t)(lane & 31);
+  const uint32_t half = ((uint32_t)lane >> 5) * 4u;
+  const float* qrow = Q + (uint64_t)(qa < nq ? qa : nq - 1) * qs;
+  const float* crow[QC_TN];
+  bool cok[QC_TN];
+#pragma unroll
+  for (int t = 0; t < QC_TN; t++) {
+    const uint32_t c = c0 + (uint32_t)t * 32u + (uint32_t)(lane & 31);
+    cok[t] = c < ncent;
+    crow[t] = C + (uint64_t)(cok[t] ? c : 0u) * cs;
+  }
+  f32x16 acc[QC_TN];
+#pragma unroll
+  for (int t = 0; t < QC_TN; t++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[t][r] = 0.0f;
+  // 16-byte query loads when every row starts 16-byte aligned (uniform)
+  const bool qvec = (qs & 3u) == 0 && ((uintptr_t)Q & 15u) == 0;
+  const uint32_t dfull = qvec ? dim & ~7u : 0u;
+  for (uint32_t k0 = 0; k0 < dim; k0 += 8) {
+    float a[4];
+    if (k0 < dfull) {  // uniform: a whole block of 8 inside the query row
+      const float4 v = *(const float4*)(qrow + k0 + half);
+      a[0] = v.x; a[1] = v.y; a[2] = v.z; a[3] = v.w;
+    } else {  // unaligned rows, or the last dim % 8 dimensions: the row ends at dim
+#pragma unroll
+      for (int e = 0; e < 4; e++) a[e] = k0 + half + e < dim ? qrow[k0 + half + e] : 0.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < QC_TN; t++) {
+      // centroid rows are zero from dim to the next multiple of 8
+      const float4 v = cok[t] ? *(const float4*)(crow[t] + k0 + half) : make_float4(0, 0, 0, 0);
+      const float b[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[e], b[e], acc[t], 0, 0, 0);
+    }
+  }
+  // register r of lane l holds C[(r / 4) * 8 + (l / 32) * 4 + r % 4][l % 32]
+#pragma unroll
+  for (int t = 0; t < QC_TN; t++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const uint32_t q = q0 + (uint32_t)((r >> 2) * 8 + (lane >> 5) * 4 + (r & 3));
+      const uint32_t c = c0 + (uint32_t)t * 32u + (uint32_t)(lane & 31);
+      if (q < nq && c < ncent) T[(uint64_t)q * ncent + c] = acc[t][r];
+    }
+}
+
+hipError_t launch_qc_table(const float* Q, uint64_t qs, uint32_t nq, uint32_t dim,
+                           const float* C, uint64_t cs, uint32_t ncent, float* T,
+                           hipStream_t s) {
+  if (nq == 0 || ncent == 0) return hipSuccess;
+  // the centroid rows' 16-byte loads reach dim rounded up to 8
+  if (cs < ((dim + 7u) & ~7u) || (cs & 3u) || ((uintptr_t)C & 15u)) return hipErrorInvalidValue;
+  const dim3 grid((nq + 31u) / 32u, (ncent + 32u * QC_TN - 1u) / (32u * QC_TN));
+  hipLaunchKernelGGL(k_qc_table, grid, dim3(PANN_WAVE), 0, s, Q, qs, nq, dim, C, cs, ncent, T);
+  return hipGetLastError();
+}
+
 }  // namespace pann

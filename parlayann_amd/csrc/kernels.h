@@ -155,6 +155,11 @@ uint32_t bf_lds_bytes(uint32_t cap);
 uint32_t bf_dim_pad(uint32_t dim, int point_dtype);  // dim rounded up to 32 bytes
 hipError_t launch_bf(const BFArgs& a, int point_dtype, uint32_t* out_ids, float* out_dists,
                      hipStream_t s);
+// T (nq x ncent, row-major) = Q (nq rows of qs floats, dim used) . C^T (C: ncent
+// rows of cs floats, zero from dim to the next multiple of 8; qs, cs multiples of 4)
+hipError_t launch_qc_table(const float* Q, uint64_t qs, uint32_t nq, uint32_t dim,
+                           const float* C, uint64_t cs, uint32_t ncent, float* T,
+                           hipStream_t s);
 hipError_t launch_row_sqnorm(const void* X, int point_dtype, uint64_t xs, uint32_t n,
                              uint32_t dim, float* out, hipStream_t s);
 

@@ -7,7 +7,6 @@
 #include "../../include/pann.h"
 
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <atomic>
@@ -165,7 +164,6 @@ struct pann_index_s {
     size_t qcodes_cap = 0;
     float* d_qtable = nullptr;    // PANN_STORE_FP8_RESIDUAL: nq x ncent q . c
     size_t qtable_cap = 0;
-    rocblas_handle blas = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // the host-buffer entry points' device staging (queries in, results out),
     // grown geometrically so that repeated calls allocate nothing
@@ -493,13 +491,17 @@ pann_status build_q8(pann_index_t idx, uint64_t fstride) {
 
 // For PANN_STORE_Q8 the kernel reads query codes: encode the float queries
 // into the index's scratch on `stream`.  For PANN_STORE_FP8_RESIDUAL it reads
-// q . c for every centroid: one FP32 GEMM (rocBLAS) into the scratch.  Other
-// stores read the queries as given.
+// q . c for every centroid: one FP32 product on the matrix cores (k_qc_table)
+// into the scratch.  Other stores read the queries as given.
 pann_status stage_queries(pann_index_t idx, StreamCtx& C, int32_t store, const void* queries,
                           uint32_t nq, pann::SearchArgs& a, hipStream_t stream) {
   if (store == PANN_STORE_FP8_RESIDUAL) {
     const size_t need = (size_t)nq * idx->ncent;
     if (C.qtable_cap < need) {
+      if (capturing(stream))
+        return fail(PANN_ERR_ARG, "the centroid table would grow under graph capture: run the "
+                                  "search once with at least this many queries before "
+                                  "capturing it");
       if (C.d_qtable) {
         HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
         (void)hipFree(C.d_qtable);
@@ -509,26 +511,21 @@ pann_status stage_queries(pann_index_t idx, StreamCtx& C, int32_t store, const v
       HIP_TRY(dev_malloc(&C.d_qtable, need * 4), "hipMalloc(centroid table)");
       C.qtable_cap = need;
     }
-    if (!C.blas && rocblas_create_handle(&C.blas) != rocblas_status_success) {
-      C.blas = nullptr;
-      return fail(PANN_ERR_DEVICE, "rocblas_create_handle failed");
-    }
-    if (rocblas_set_stream(C.blas, stream) != rocblas_status_success)
-      return fail(PANN_ERR_DEVICE, "rocblas_set_stream failed");
-    // row-major T (nq x ncent) = Q (nq x dim) . Ct (dim x ncent); in rocBLAS's
-    // column-major terms T^T (ncent x nq) = Ct^T (ncent x dim) . Q^T (dim x nq)
-    const float one = 1.0f, zero = 0.0f;
-    const rocblas_int m = (rocblas_int)idx->ncent, nn = (rocblas_int)nq, kd = (rocblas_int)idx->dim;
-    if (rocblas_sgemm(C.blas, rocblas_operation_none, rocblas_operation_none, m, nn, kd, &one,
-                      idx->d_cent_t, m, (const float*)queries, kd, &zero, C.d_qtable, m) !=
-        rocblas_status_success)
-      return fail(PANN_ERR_DEVICE, "rocblas_sgemm (centroid table) failed");
+    // row-major T (nq x ncent) = Q (nq x dim) . C^T, C the centroid rows
+    // (pts_stride bytes apart, zero padded like the points)
+    HIP_TRY(pann::launch_qc_table((const float*)queries, idx->dim, nq, idx->dim, idx->d_cent,
+                                  idx->pts_stride / 4, idx->ncent, C.d_qtable, stream),
+            "centroid table launch");
     a.qtable = C.d_qtable;
     return PANN_OK;
   }
   if (store != PANN_STORE_Q8) return PANN_OK;
   const size_t need = (size_t)nq * idx->dim;
   if (C.qcodes_cap < need) {
+    if (capturing(stream))
+      return fail(PANN_ERR_ARG, "the query codes would grow under graph capture: run the "
+                                "search once with at least this many queries before "
+                                "capturing it");
     if (C.d_qcodes) {
       HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
       (void)hipFree(C.d_qcodes);
@@ -1914,7 +1911,6 @@ pann_status pann_close(pann_index_t idx) {
                   C.d_qcodes, C.d_qtable, C.d_stage, C.d_cmps};
     for (void* p : cp)
       if (p) (void)hipFree(p);
-    if (C.blas) (void)rocblas_destroy_handle(C.blas);
     if (C.ev0) (void)hipEventDestroy(C.ev0);
     if (C.ev1) (void)hipEventDestroy(C.ev1);
   }

@@ -10,14 +10,19 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a HIP device")]
 
 
-@pytest.mark.parametrize("store", ["native", "q8"])
-def test_gpu_graph_replay_equals_eager(oracle, store):
+@pytest.mark.parametrize("store,metric", [("native", "l2"), ("q8", "l2"), ("fp8r", "ip")])
+def test_gpu_graph_replay_equals_eager(oracle, store, metric):
+    """fp8r: the per-query centroid table is the library's own MFMA kernel, so
+    that store's search is captured too."""
     import torch
     from parlayann_amd import DeviceIndex, datagen
-    print(f"[graph] store={store}", flush=True)
-    X, Q = datagen.workload_f32_l2(n=5000, nq=700, d=32)
-    adj = oracle.build_vamana(X, "l2", 24, 48, 1.2, datagen.insertion_orders(len(X), 1, 9))
-    idx = DeviceIndex(X, adj, "l2")
+    print(f"[graph] store={store} metric={metric}", flush=True)
+    if metric == "l2":
+        X, Q = datagen.workload_f32_l2(n=5000, nq=700, d=32)
+    else:
+        X, Q = datagen.workload_aniso_ip(n=5000, nq=700, d=40)
+    adj = oracle.build_vamana(X, metric, 24, 48, 1.2, datagen.insertion_orders(len(X), 1, 9))
+    idx = DeviceIndex(X, adj, metric)
     if store != "native":
         idx.build_store(store)
     dev = torch.device("cuda", 0)
