@@ -128,7 +128,6 @@ struct pann_index_s {
   float* d_rscales = nullptr;
   uint32_t* d_rabsmax = nullptr;
   float* d_cent = nullptr;     // ncent x pts_stride (the sampled rows)
-  float* d_cent_t = nullptr;   // dim x ncent (transposed, for the per-query table)
   uint32_t* d_assign = nullptr;  // n: each point's centroid
   uint32_t ncent = 0;
   char* d_q8 = nullptr;            // the reference's 8-bit codes (PANN_STORE_Q8)
@@ -1820,14 +1819,14 @@ pann_status pann_build_fp8_residual(pann_index_t idx, uint32_t ncent) {
   std::lock_guard<std::mutex> lk(idx->mu);
   HIP_TRY(hipSetDevice(idx->device), "hipSetDevice");
   HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
-  void* old[] = {idx->d_fp8r, idx->d_rscales, idx->d_rabsmax, idx->d_cent, idx->d_cent_t,
+  void* old[] = {idx->d_fp8r, idx->d_rscales, idx->d_rabsmax, idx->d_cent,
                  idx->d_assign};
   for (void* p : old)
     if (p) (void)hipFree(p);
   idx->d_fp8r = nullptr;
   idx->d_rscales = nullptr;
   idx->d_rabsmax = nullptr;
-  idx->d_cent = idx->d_cent_t = nullptr;
+  idx->d_cent = nullptr;
   idx->d_assign = nullptr;
   idx->ncent = 0;
   const uint64_t ps = idx->pts_stride, fs = ps / 4;
@@ -1838,19 +1837,11 @@ pann_status pann_build_fp8_residual(pann_index_t idx, uint32_t ncent) {
   HIP_TRY(dev_malloc(&idx->d_rscales, dim * 4), "hipMalloc(scales)");
   HIP_TRY(dev_malloc(&idx->d_rabsmax, dim * 4), "hipMalloc(absmax)");
   HIP_TRY(dev_malloc(&idx->d_cent, (size_t)ncent * ps), "hipMalloc(centroids)");
-  HIP_TRY(dev_malloc(&idx->d_cent_t, (size_t)ncent * dim * 4), "hipMalloc(centroids)");
   HIP_TRY(dev_malloc(&idx->d_assign, (size_t)n * 4), "hipMalloc(assignment)");
   // centroids: every (n / ncent)-th point
   const uint64_t step = n / ncent;
   HIP_TRY(hipMemcpy2D(idx->d_cent, ps, idx->d_pts, ps * step, ps, ncent, hipMemcpyDeviceToDevice),
           "hipMemcpy2D(centroids)");
-  std::vector<float> c((size_t)ncent * fs), ct((size_t)ncent * dim);
-  HIP_TRY(hipMemcpy(c.data(), idx->d_cent, c.size() * 4, hipMemcpyDeviceToHost),
-          "hipMemcpy(centroids)");
-  for (uint32_t i = 0; i < ncent; i++)
-    for (uint32_t j = 0; j < dim; j++) ct[(size_t)j * ncent + i] = c[(size_t)i * fs + j];
-  HIP_TRY(hipMemcpy(idx->d_cent_t, ct.data(), ct.size() * 4, hipMemcpyHostToDevice),
-          "hipMemcpy(centroids)");
   // each point's nearest centroid in L2, on the brute-force kernel
   float* cnorm = nullptr;
   float* cd = nullptr;
@@ -1900,7 +1891,7 @@ pann_status pann_close(pann_index_t idx) {
   void* ptrs[] = {idx->d_pts,    idx->d_bf16,    idx->d_fp8,   idx->d_scales, idx->d_absmax,
                   idx->d_nbrs,   idx->d_q8,      idx->d_xnorm, idx->d_dummy,  idx->d_pkeys,
                   idx->d_pflags, idx->d_fp8r,    idx->d_rscales, idx->d_rabsmax, idx->d_cent,
-                  idx->d_cent_t, idx->d_assign, idx->d_evals};
+                  idx->d_assign, idx->d_evals};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (size_t l = 1; l < idx->levels.size(); l++)
