@@ -108,8 +108,11 @@ typedef struct {
                           ids, distances and visited counts, out_cmps counts evaluations, no
                           capacity limit */
   int32_t visits_per_round; /* 0 or 2: while a merge is deferred, the next visit's rows are
-                               gathered with the current one's; 1: one visit per round.
-                               Results do not depend on it (a tuning knob) */
+                               gathered with the current one's; 3: the same, but only where
+                               the current visit is predicted to defer its merge (rows
+                               gathered for a visit that a merge cancels are discarded);
+                               1: one visit per round.  Results do not depend on it (a
+                               tuning knob) */
   int32_t eval_cache_log2;  /* exact_counts = 0: log2 of the slots of an LDS cache of evaluated
                                ids (0 = none, 6..13); a cached id is not evaluated again.
                                Results do not depend on it (a tuning knob) */

@@ -25,7 +25,7 @@ import os
 from typing import Optional
 
 WAVES_PER_CU = (4, 6, 8, 12, 16)
-VISITS_PER_ROUND = (1, 2)
+VISITS_PER_ROUND = (1, 2, 3)
 EVAL_CACHE_LOG2 = (0, 8, 10, 12)
 FRONTIER_FILTER = (0, 1, 2)  # automatic, on, off
 # the reference's exact seen set (an id seen once is not evaluated again):

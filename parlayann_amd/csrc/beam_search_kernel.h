@@ -583,6 +583,7 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
   int u = first_unvisited(L.FV0, f, 0, lane);
   uint32_t pref_id = PANN_SENTINEL;
   uint32_t nb_pref = PANN_SENTINEL;
+  int acc_last = 0;  // candidates the last committed visit added to C
   // Frontier-and-pending seen set: rows [lo, hi) of NL/DL with d < cutoff
   // join C unless their key is already in the frontier F (binary search on
   // the exact (dist, id) key: a row's distance is computed the same way every
@@ -702,7 +703,17 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       // gathered together with cur's.  Its results are committed only if the
       // first visit defers; its accepted ids are inserted after cur's, so an
       // id both lists share is taken once, for cur, as in the reference.
-      const bool have2 = a.lookahead && nu >= 0 && (int64_t)nvis < P.limit;
+      // When cur's visit merges, the rows gathered for nu are discarded:
+      // lookahead 2 gathers them only when the merge rule, applied to a
+      // prediction of |C| after cur's visit, defers -- every neighbour
+      // while the frontier is not full (there is no cutoff yet), else as
+      // many as the last committed visit added.
+      bool spec = a.lookahead == 1;
+      if (a.lookahead == 2) {
+        const int pred = c + (full ? acc_last : dl);
+        spec = pred == 0 || (defer_ok && pred < thr);
+      }
+      const bool have2 = spec && nu >= 0 && (int64_t)nvis < P.limit;
       const uint64_t key2 = have2 ? F[nu] : 0;
       const uint32_t id2 = key_id(key2);
       const uint32_t nb2 = !have2 ? PANN_SENTINEL : (id2 == pref_id ? nb_pref : adjacency(id2));
@@ -731,7 +742,9 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       wave_lds_sync();
       eval_n<ST, MT, NCH, SEQ, CO>(D, a, L, W, n1 + n2, false, lane);
       wave_lds_sync();
+      const int c0 = c;
       accept_range(F, 0, n1, cutoff);
+      acc_last = c - c0;
       cmps += (uint32_t)n1;
       wave_lds_sync();
       if (c == 0 || (defer_ok && c < thr && nu >= 0)) {
@@ -741,7 +754,9 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
         }
         if (lane == 0) FV[nu] = 1;
         record_visit(key2);
+        const int c1 = c;
         accept_range(F, n1, n1 + n2, cutoff);
+        acc_last = c - c1;
         cmps += (uint32_t)n2;
         wave_lds_sync();
         if (c == 0 || (defer_ok && c < thr && nnu >= 0)) {

@@ -58,7 +58,9 @@ struct SearchArgs {
   // 1: a frontier shorter than k is a result, not PANN_ST_SHORT_BEAM
   uint32_t short_ok;
   // 1: while a merge is deferred, gather the next visit's rows in the same
-  // round (two visits per round); 0: one visit per round
+  // round (two visits per round); 2: the same, but only in rounds where the
+  // merge rule applied to a prediction of the pending candidates defers;
+  // 0: one visit per round
   uint32_t lookahead;
   // QueryParams (types.h:233-268); per-query k / beam override the uniform ones
   int32_t k, beam;

@@ -98,7 +98,7 @@ def test_gpu_frontier_filter_exact_f32(oracle, beam):
     o = oracle.search_batch(adj, X, "l2", Q, 10, beam)
     idx = DeviceIndex(X, adj, "l2")
     for ff in (0, 1, 2):
-        for vpr in (1, 2):
+        for vpr in (1, 2, 3):
             r = idx.search(Q, 10, beam, summation="inorder", frontier_filter=ff,
                            visits_per_round=vpr)
             for key in ("ids", "dists", "visited"):

@@ -300,6 +300,17 @@ def _e4m3_table():
     return v
 
 
+def _fp16_query(q, s):
+    """The inner-product FP8 kernels' query: q~_j = q_j * s_j in FP32, times
+    2^e with max |q~| * 2^e in [2^14, 2^15), rounded to FP16 (RNE), as
+    float64 values of q~ (the 2^e taken back out, exactly)."""
+    qs = q.astype(np.float32) * s.astype(np.float32)
+    mx = float(np.abs(qs).max())
+    e = int(np.clip(15 - np.frexp(mx)[1], -120, 120)) if mx > 0 else 0
+    h = (qs * np.float32(2.0 ** e)).astype(np.float16)
+    return h.astype(np.float64) * 2.0 ** -e
+
+
 def _complete(n):
     adj = np.zeros((n, n), np.uint32)
     for i in range(n):
@@ -326,7 +337,8 @@ def test_fp8_store_codec():
     x/s (ties to even, saturating), decoded exactly.  Checked through the
     search itself: on a complete graph with beam = n every row's FP8 IP
     distance comes back, and must equal -sum_j q_j s_j dec(code_ij)
-    computed from a brute-force encoder over the 256-code table."""
+    computed from a brute-force encoder over the 256-code table, the query
+    term q_j s_j in the kernel's FP16 form (_fp16_query)."""
     _say("fp8 codec")
     rng = np.random.default_rng(3)
     n, d = 60, 16
@@ -339,11 +351,14 @@ def test_fp8_store_codec():
     tab = _e4m3_table()
     codes = _e4m3_encode_bruteforce((X / s[None, :]).astype(np.float32), tab)
     assert (codes & 0x7F).min() < 8  # subnormal codes are exercised
-    deq = tab[codes] * s[None, :].astype(np.float64)
     r = idx.search(Q, n, n, store="fp8")
     for i in range(len(Q)):
-        exp = -(deq[r["ids"][i]] * Q[i].astype(np.float64)).sum(1)
+        h = _fp16_query(Q[i], s)
+        exp = -(tab[codes[r["ids"][i]]] @ h)
         assert np.allclose(r["dists"][i], exp, rtol=1e-5, atol=1e-6 * np.abs(exp).max())
+        # the FP16 rounding shows at this tolerance: the FP32 query term misses it
+        full = -(tab[codes[r["ids"][i]]] @ (Q[i].astype(np.float64) * s.astype(np.float64)))
+        assert not np.allclose(r["dists"][i], full, rtol=1e-6, atol=1e-7 * np.abs(full).max())
 
 
 @pytest.mark.parametrize("d", [20, 200, 253])  # rows of 1, 2 and 3 128-byte chunks
@@ -352,7 +367,8 @@ def test_fp8_residual_store_codec(d):
     point's centroid is its nearest in L2; scales s_j = max|x_j - c_j| / 448;
     codes = nearest e4m3fn value of (x - c) / s.  Checked through the search:
     on a complete graph with beam = n every row's distance comes back and must
-    equal -(q.c + sum_j q_j s_j dec(code_ij)), from a brute-force encoder."""
+    equal -(q.c + sum_j q_j s_j dec(code_ij)), from a brute-force encoder, the
+    term q_j s_j in the kernel's FP16 form (_fp16_query)."""
     _say(f"fp8 residual codec d={d}")
     rng = np.random.default_rng(5)
     n, nc = 64, 8
@@ -378,12 +394,11 @@ def test_fp8_residual_store_codec(d):
     assert np.array_equal(s, (np.abs(R).max(0) / np.float32(448.0)).astype(np.float32))
     tab = _e4m3_table()
     codes = _e4m3_encode_bruteforce((R / s[None, :]).astype(np.float32), tab)
-    deq = tab[codes] * s[None, :].astype(np.float64)
     r = idx.search(Q, n, n, store="fp8r")
     for i in range(len(Q)):
         ids = r["ids"][i].astype(np.int64)
         q = Q[i].astype(np.float64)
-        exp = -(cent[a[ids]].astype(np.float64) @ q + deq[ids] @ q)
+        exp = -(cent[a[ids]].astype(np.float64) @ q + tab[codes[ids]] @ _fp16_query(Q[i], s))
         assert np.allclose(r["dists"][i], exp, rtol=1e-5, atol=1e-5 * np.abs(exp).max())
     # the store is inner product only
     from parlayann_amd._lib import PannError
