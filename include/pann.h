@@ -24,7 +24,8 @@ typedef enum {
   PANN_ERR_ARG = 1,         /* invalid argument (the reference aborts) */
   PANN_ERR_DEVICE = 2,      /* HIP runtime error */
   PANN_ERR_OOM = 3,         /* device allocation failed */
-  PANN_ERR_CAPACITY = 4,    /* a query outgrew a fixed capacity (visited set, visited list) */
+  PANN_ERR_CAPACITY = 4,    /* a query outgrew a fixed capacity (visited list; the exact seen
+                               set's table in an async search) */
   PANN_ERR_SHORT_BEAM = 5,  /* beam returned fewer than k entries (beamSearch.h:557-562) */
   PANN_ERR_UNSUPPORTED = 6  /* shape or option this build does not implement */
 } pann_status;
@@ -100,10 +101,12 @@ typedef struct {
   int32_t async;       /* 1: do not synchronise; call pann_search_status() later */
   int32_t waves_per_cu;/* persistent waves per CU (0 = automatic) */
   int32_t exact_counts;/* 1: out_cmps equals the reference's dist_cmps (every id seen is
-                          remembered in an LDS table with a per-wave global overflow table
-                          of fixed size: a query that fills it fails with
-                          PANN_ERR_CAPACITY, where the reference's hashset grows,
-                          hashset.h:25-38); 0: every neighbour is evaluated and checked
+                          remembered in an LDS table with a per-wave global overflow table,
+                          each filled to half its slots; a synchronous search that outgrows
+                          the global table grows it 4x and runs again, as the reference's
+                          hashset grows past half load, hashset.h:25-38, and the index
+                          keeps the larger table; an async search reports
+                          PANN_ERR_CAPACITY instead); 0: every neighbour is evaluated and checked
                           against the frontier and pending candidates afterwards -- same
                           ids, distances and visited counts, out_cmps counts evaluations, no
                           capacity limit */

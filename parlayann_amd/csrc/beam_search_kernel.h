@@ -643,21 +643,26 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       // global table (at 7/8 load the longer probes made an HNSW 1M x 200
       // search 1.33x slower than at 1/2)
       if (!ovf && lds_count + (uint32_t)nvalid > (hslots >> 1)) ovf = true;
-      bool isnew = false;
+      // the global table also takes ids up to half its slots (the reference's
+      // hashset grows past half load, hashset.h:25-38); an id it refuses
+      // flags the launch, whose host call grows the table and searches again
+      bool isnew = false, refused = false;
       if (valid) {
         if (!ovf) {
           isnew = lds_insert(L.H, hlog2, nb);
         } else if (!lds_contains(L.H, hlog2, nb)) {
-          if (g_count + PANN_WAVE < (1u << glog2)) isnew = global_insert(ghash, glog2, gtag, nb);
-          else isnew = true;  // table exhausted: flagged below
+          if (g_count + PANN_WAVE <= ((1u << glog2) >> 1)) {
+            isnew = global_insert(ghash, glog2, gtag, nb);
+          } else {
+            isnew = true;
+            refused = true;
+          }
         }
       }
       const uint64_t mins = ballot(isnew);
       if (!ovf) lds_count += (uint32_t)__popcll(mins);
-      else {
-        g_count += (uint32_t)__popcll(mins);
-        if (g_count + PANN_WAVE >= (1u << glog2)) status |= PANN_ST_HASH_CAPACITY;
-      }
+      else g_count += (uint32_t)__popcll(mins);
+      if (ballot(refused) != 0) status |= PANN_ST_HASH_CAPACITY;
       const bool evalme = isnew && nb != self;  // Points[a].same_as(p) (:156)
       const uint64_t mnew = ballot(evalme);
       const int ncmp = __popcll(mnew);

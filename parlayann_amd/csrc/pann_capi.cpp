@@ -12,6 +12,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -187,6 +188,11 @@ struct pann_index_s {
   // serialises the host-buffer entry points, which share the default stream's
   // staging buffer
   std::mutex host_mu;
+  // log2 growth of the exact seen set's global table: a synchronous search
+  // that outgrows the table doubles it twice and runs again, and later
+  // searches keep the larger table (ghash_log2_last: the last launch's size)
+  std::atomic<uint32_t> ghash_grow{0};
+  std::atomic<uint32_t> ghash_log2_last{0};
 };
 
 namespace {
@@ -817,6 +823,11 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     // size 2*(10+B)*degree_limit (beamSearch.h:66), at least 4x the LDS table
     uint32_t glog2 = std::max<uint32_t>(
         ceil_log2((uint64_t)2 * (10 + B) * a.degree_limit) + 1, hlog2 + 2);
+    // grown after a search outgrew it, up to twice the ids there are (a
+    // query sees each id once)
+    const uint32_t gmax = std::max<uint32_t>(glog2, ceil_log2(2 * ((uint64_t)idx->n + 64)) + 1);
+    glog2 = std::min<uint32_t>(glog2 + idx->ghash_grow.load(), std::min<uint32_t>(gmax, 31u));
+    idx->ghash_log2_last.store(glog2);
     a.ghash_log2 = glog2;
     // every pass of every query gets its own tag (the HNSW descent runs
     // passes * nq searches in one launch)
@@ -849,6 +860,10 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   return PANN_OK;
 }
 
+// set when a synchronous status check finds the exact seen set's table
+// outgrown (grow_retry below grows it and runs the search again)
+thread_local bool tl_seen_table_full = false;
+
 // allow_short: a frontier shorter than k is a result, not an error (HNSW
 // layers return whatever frontier they reach)
 pann_status check_status_word(const uint32_t* d_status, hipStream_t stream,
@@ -861,8 +876,11 @@ pann_status check_status_word(const uint32_t* d_status, hipStream_t stream,
                               "or k > out_stride)");
   if ((st & pann::PANN_ST_SHORT_BEAM) && !allow_short)  // beamSearch.h:557-562
     return fail(PANN_ERR_SHORT_BEAM, "beam search returned fewer than k elements");
-  if (st & pann::PANN_ST_HASH_CAPACITY)
-    return fail(PANN_ERR_CAPACITY, "a query outgrew the visited-set capacity");
+  if (st & pann::PANN_ST_HASH_CAPACITY) {
+    tl_seen_table_full = true;
+    return fail(PANN_ERR_CAPACITY, "a query outgrew the exact seen set's table (an async "
+                                   "search cannot grow it: run it synchronously once)");
+  }
   if (st & pann::PANN_ST_VISITED_OVERFLOW)
     return fail(PANN_ERR_CAPACITY, "a visited list outgrew vis_cap");
   return PANN_OK;
@@ -870,6 +888,24 @@ pann_status check_status_word(const uint32_t* d_status, hipStream_t stream,
 
 pann_status check_status(StreamCtx& C, hipStream_t stream, bool allow_short = false) {
   return check_status_word(C.d_status, stream, allow_short);
+}
+
+// Runs a synchronous search; when a query outgrew the exact seen set's
+// global table (the reference's hashset grows instead, hashset.h:25-38),
+// grows the table 4x and runs the whole search again -- its results do not
+// depend on the table's size -- until it fits or the table holds every id.
+pann_status grow_retry(pann_index_t idx, const std::function<pann_status()>& run) {
+  for (;;) {
+    tl_seen_table_full = false;
+    const pann_status st = run();
+    if (st != PANN_ERR_CAPACITY || !tl_seen_table_full || !idx) return st;
+    // the launch's table had 2^used slots per wave; at the size that holds
+    // every id of the index no query can outgrow it
+    const uint32_t used = idx->ghash_log2_last.load();
+    const uint32_t gmax = std::min<uint32_t>(ceil_log2(2 * ((uint64_t)idx->n + 64)) + 1, 31u);
+    if (used >= gmax) return st;
+    idx->ghash_grow.fetch_add(2);
+  }
 }
 
 }  // namespace
@@ -962,8 +998,10 @@ pann_status pann_search(pann_index_t idx, const void* queries, uint32_t nq,
                         const uint32_t* starts, uint32_t nstarts, const pann_search_opts* opts,
                         uint32_t* out_ids, float* out_dists, uint32_t out_stride,
                         uint32_t* out_visited, uint32_t* out_cmps, void* stream) {
-  return search_impl(idx, queries, nq, qp, k_pq, beam_pq, starts, nstarts, opts, out_ids,
-                     out_dists, out_stride, out_visited, out_cmps, SearchExtras(), stream);
+  return grow_retry(idx, [&] {
+    return search_impl(idx, queries, nq, qp, k_pq, beam_pq, starts, nstarts, opts, out_ids,
+                       out_dists, out_stride, out_visited, out_cmps, SearchExtras(), stream);
+  });
 }
 
 pann_status pann_search_noalloc(pann_index_t idx, const void* queries, uint32_t nq,
@@ -1136,8 +1174,10 @@ pann_status beam_search_host(pann_index_t idx, const void* queries, uint32_t nq,
   ex.vis_cap = want_vis ? vis_cap : 0u;
   ex.vis_ids = dvi;
   ex.vis_dists = dvd;
-  st = search_impl(idx, dq, nq, &q2, nullptr, nullptr, starts, nstarts, &o, di, dd, (uint32_t)B,
-                   dv, dc, ex, nullptr);
+  st = grow_retry(idx, [&] {
+    return search_impl(idx, dq, nq, &q2, nullptr, nullptr, starts, nstarts, &o, di, dd,
+                       (uint32_t)B, dv, dc, ex, nullptr);
+  });
   if (st != PANN_OK) return st;
   hipError_t e = hipMemcpy(out_ids, di, nq * B * 4, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(out_dists, dd, nq * B * 4, hipMemcpyDeviceToHost);
@@ -1470,13 +1510,14 @@ pann_status pann_hnsw_set_level(pann_index_t idx, uint32_t level, uint32_t max_d
   return PANN_OK;
 }
 
-pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void* queries,
-                                   const uint32_t* self_ids, uint32_t nq, uint32_t ef,
-                                   uint32_t degree_limit, int64_t limit, const uint32_t* starts,
-                                   const uint32_t* nstarts, uint32_t starts_stride,
-                                   const pann_search_opts* opts, uint32_t* out_ids,
-                                   float* out_dists, uint32_t out_stride, uint32_t* out_count,
-                                   uint32_t* out_visited, uint32_t* out_cmps, void* stream_) {
+static pann_status hnsw_search_layer_once(pann_index_t idx, uint32_t level, const void* queries,
+                                          const uint32_t* self_ids, uint32_t nq, uint32_t ef,
+                                          uint32_t degree_limit, int64_t limit,
+                                          const uint32_t* starts, const uint32_t* nstarts,
+                                          uint32_t starts_stride, const pann_search_opts* opts,
+                                          uint32_t* out_ids, float* out_dists, uint32_t out_stride,
+                                          uint32_t* out_count, uint32_t* out_visited,
+                                          uint32_t* out_cmps, void* stream_) {
   if (!idx || !starts || !out_ids || !out_dists || !out_count)
     return fail(PANN_ERR_ARG, "null argument");
   if ((queries == nullptr) == (self_ids == nullptr))
@@ -1531,13 +1572,27 @@ pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void*
   return check_status(*C, stream, true);
 }
 
-pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq, uint32_t k,
-                             uint32_t ef, const int32_t* k_pq, const int32_t* ef_pq,
-                             int64_t limit, const uint32_t* entrance,
-                             uint32_t nentrance, uint32_t top_level, const uint32_t* node_ids,
-                             const pann_search_opts* opts, uint32_t* out_ids, float* out_dists,
-                             uint32_t out_stride, uint32_t* out_count, uint32_t* out_visited,
-                             uint32_t* out_cmps, void* stream_) {
+pann_status pann_hnsw_search_layer(pann_index_t idx, uint32_t level, const void* queries,
+                                   const uint32_t* self_ids, uint32_t nq, uint32_t ef,
+                                   uint32_t degree_limit, int64_t limit, const uint32_t* starts,
+                                   const uint32_t* nstarts, uint32_t starts_stride,
+                                   const pann_search_opts* opts, uint32_t* out_ids,
+                                   float* out_dists, uint32_t out_stride, uint32_t* out_count,
+                                   uint32_t* out_visited, uint32_t* out_cmps, void* stream_) {
+  return grow_retry(idx, [&] {
+    return hnsw_search_layer_once(idx, level, queries, self_ids, nq, ef, degree_limit, limit,
+                                  starts, nstarts, starts_stride, opts, out_ids, out_dists,
+                                  out_stride, out_count, out_visited, out_cmps, stream_);
+  });
+}
+
+static pann_status hnsw_search_once(pann_index_t idx, const void* queries, uint32_t nq, uint32_t k,
+                                    uint32_t ef, const int32_t* k_pq, const int32_t* ef_pq,
+                                    int64_t limit, const uint32_t* entrance, uint32_t nentrance,
+                                    uint32_t top_level, const uint32_t* node_ids,
+                                    const pann_search_opts* opts, uint32_t* out_ids,
+                                    float* out_dists, uint32_t out_stride, uint32_t* out_count,
+                                    uint32_t* out_visited, uint32_t* out_cmps, void* stream_) {
   if (!idx || !queries || !entrance || !out_ids || !out_dists || !out_count)
     return fail(PANN_ERR_ARG, "null argument");
   if (nq == 0) return PANN_OK;
@@ -1679,6 +1734,20 @@ pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq,
   idx->last_timed = C;
   if (o.async) return PANN_OK;
   return check_status(*C, stream, true);
+}
+
+pann_status pann_hnsw_search(pann_index_t idx, const void* queries, uint32_t nq, uint32_t k,
+                             uint32_t ef, const int32_t* k_pq, const int32_t* ef_pq,
+                             int64_t limit, const uint32_t* entrance,
+                             uint32_t nentrance, uint32_t top_level, const uint32_t* node_ids,
+                             const pann_search_opts* opts, uint32_t* out_ids, float* out_dists,
+                             uint32_t out_stride, uint32_t* out_count, uint32_t* out_visited,
+                             uint32_t* out_cmps, void* stream_) {
+  return grow_retry(idx, [&] {
+    return hnsw_search_once(idx, queries, nq, k, ef, k_pq, ef_pq, limit, entrance, nentrance,
+                            top_level, node_ids, opts, out_ids, out_dists, out_stride, out_count,
+                            out_visited, out_cmps, stream_);
+  });
 }
 
 pann_status pann_hnsw_select_neighbors(pann_index_t idx, uint32_t level, const uint32_t* pts,

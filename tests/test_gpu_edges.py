@@ -150,3 +150,47 @@ def test_duplicate_points_and_base_queries(oracle, small):
     g = DeviceIndex(X2, adj, "l2").search(Qb, 10, 64, exact_counts=True)
     _assert_exact(g, o)
     assert np.all(g["dists"][:5, 0] == 0.0)
+
+
+def test_exact_seen_set_grows():
+    """The exact seen set outgrows its global table and grows, as the
+    reference's hashset does past half load (hashset.h:25-38).  On a path
+    graph (node i linked to i - 1 and i + 1; point i has coordinates
+    clip(i - 255 j, 0, 255), a u8 staircase whose distance to the last point
+    falls at every step) a query at the far end walks the whole path from
+    start 0: every one of the n ids is
+    seen exactly once and every node is visited, so dist_cmps = visited = n,
+    far past the table's first size (2^13 slots at beam 10, degree 2).  A
+    synchronous search grows the table and returns the default seen set's
+    ids and distances; an async one reports PANN_ERR_CAPACITY."""
+    import torch
+    _say("exact seen set growth (path graph)")
+    n = 20000
+    X = np.clip(np.arange(n)[:, None] - 255 * np.arange(80)[None, :], 0, 255).astype(np.uint8)
+    adj = np.zeros((n, 3), np.uint32)
+    for i in range(n):
+        nb = [j for j in (i - 1, i + 1) if 0 <= j < n]
+        adj[i, 0] = len(nb)
+        adj[i, 1:1 + len(nb)] = nb
+    Q = X[[n - 1, n - 1, n - 2]].copy()
+    Q[1, 79] = 3  # off the line
+    # async on a fresh index: the table cannot grow, the status says so
+    idx = DeviceIndex(X, adj, "l2")
+    dev = torch.device("cuda", 0)
+    tq = torch.from_numpy(Q).to(dev)
+    oi = torch.zeros((len(Q), 10), dtype=torch.int32, device=dev)
+    od = torch.zeros((len(Q), 10), dtype=torch.float32, device=dev)
+    idx.search_device(tq, 10, 10, oi, od, exact_counts=True, async_=True)
+    with pytest.raises(PannError):
+        idx.search_status()
+    g = idx.search(Q, 10, 10, exact_counts=True)
+    d = idx.search(Q, 10, 10)
+    for key in ("ids", "dists", "visited"):
+        assert np.array_equal(g[key], d[key]), key
+    assert np.all(g["visited"] == n) and np.all(g["dist_cmps"] == n)
+    assert np.all(g["ids"][:2, 0] == n - 1)
+    # the grown table stays: an async search now fits
+    idx.search_device(tq, 10, 10, oi, od, exact_counts=True, async_=True)
+    idx.search_status()
+    assert np.array_equal(oi.cpu().numpy().view(np.uint32), g["ids"])
+    idx.close()
