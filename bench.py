@@ -705,14 +705,15 @@ def run_hnsw(args, w, name):
         ov = torch.zeros(nq, dtype=torch.int32, device=dev)
         om = torch.zeros(nq, dtype=torch.int32, device=dev)
         launch = {"exact_counts": False, "hash_slots_log2": 0, "eval_cache_log2": 0,
-                  "waves_per_cu": 0}
+                  "waves_per_cu": 0, "visits_per_round": 0}
         if not args.no_tune:
             t = autotune.tune_hnsw(idx, tq, tk, te, mx, **kw)
             launch = {k_: t[k_] for k_ in launch}
             log(f"[bench] {leg}: tuned seen set {'exact' if t['exact_counts'] else 'default'}"
                 f" (LDS slots 2^{t['hash_slots_log2'] or 'auto'}, eval cache "
                 f"2^{t['eval_cache_log2']}), waves/CU "
-                f"{t['waves_per_cu'] or 'auto'}" + (" (cached)" if t["cached"] else ""))
+                f"{t['waves_per_cu'] or 'auto'}, visits/round {t['visits_per_round'] or 'auto'}"
+                + (" (cached)" if t["cached"] else ""))
         ex_t = launch["exact_counts"]
         shape = {k_: v for k_, v in launch.items() if k_ != "exact_counts"}
         run = lambda a=True, ex=ex_t: idx.search_device(

@@ -129,6 +129,10 @@ typedef struct {
                                the merges and the rerank (for large beams, where one query's
                                work is long and few queries fit a CU).  waves_per_cu then
                                counts waves.  Results do not depend on it (a tuning knob) */
+  int32_t query_order;      /* 0: a batch with per-query beams runs longest-first (a device
+                               counting sort by descending beam, so that long searches do not
+                               start last and set the batch's end); 1: the queries in the order
+                               given.  Results do not depend on it (a tuning knob) */
 } pann_search_opts;
 
 /* Creates the device index: uploads the points (rows padded to 128 bytes)

@@ -53,7 +53,8 @@ class SearchOpts(C.Structure):
                 ("hash_slots_log2", C.c_int32), ("async_", C.c_int32),
                 ("waves_per_cu", C.c_int32), ("exact_counts", C.c_int32),
                 ("visits_per_round", C.c_int32), ("eval_cache_log2", C.c_int32),
-                ("frontier_filter", C.c_int32), ("waves_per_query", C.c_int32)]
+                ("frontier_filter", C.c_int32), ("waves_per_query", C.c_int32),
+                ("query_order", C.c_int32)]
 
 
 _lib = None

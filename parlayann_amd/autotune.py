@@ -143,7 +143,10 @@ def tune(index, queries, k: int, beam: int, *, store: str = "native",
 
 # (exact seen set, its LDS slots log2, evaluated-id cache log2 of the default set)
 HNSW_SHAPES = ((False, 0, 0), (False, 0, 10), (False, 0, 11), (True, 0, 0), (True, 12, 0))
-HNSW_WAVES = (0, 12, 8)
+# (a serving batch's launch shares the device with the next one's, so the
+# HNSW shapes leave room: 16 waves per CU times a batch no faster than 12)
+HNSW_WAVES = (12, 10, 8)
+HNSW_VISITS_PER_ROUND = (2, 3)
 
 
 def tune_hnsw(index, queries, k_per_query, ef_per_query, max_ef: int, *, store: str = "native",
@@ -156,7 +159,7 @@ def tune_hnsw(index, queries, k_per_query, ef_per_query, max_ef: int, *, store: 
     reference's exact set, which at small ef and high degree gathers far
     fewer rows (an id already seen is not evaluated again); results are the
     same.  Returns {exact_counts, hash_slots_log2, eval_cache_log2,
-    waves_per_cu, ms}."""
+    waves_per_cu, visits_per_round, ms}."""
     import torch
     from .index import device_info
     dev = queries.device
@@ -167,7 +170,7 @@ def tune_hnsw(index, queries, k_per_query, ef_per_query, max_ef: int, *, store: 
         index.model.m, index.model.top_level, str(base.dtype), base.metric, store, summation,
         int(rerank), int(queries.shape[0]), max_ef))
     table = _load() if use_cache else {}
-    if use_cache and key in table and "eval_cache_log2" in table[key]:
+    if use_cache and key in table and "visits_per_round" in table[key]:
         return dict(table[key], cached=True, key=key)
     nq = int(queries.shape[0])
     oi = torch.empty((nq, max_ef), dtype=torch.int32, device=dev)
@@ -176,21 +179,25 @@ def tune_hnsw(index, queries, k_per_query, ef_per_query, max_ef: int, *, store: 
     timings = []
     for ex, hs, ec in HNSW_SHAPES:
         for wpc in HNSW_WAVES:
-            kw = dict(store=store, summation=summation, rerank=rerank, exact_counts=ex,
-                      hash_slots_log2=hs, eval_cache_log2=ec, waves_per_cu=wpc,
-                      k_per_query=k_per_query, ef_per_query=ef_per_query)
-            index.search_device(queries, 0, max_ef, oi, od, oc, **kw)  # warm
-            ms = []
-            for _ in range(reps):
-                index.search_device(queries, 0, max_ef, oi, od, oc, async_=True, **kw)
-                ms.append(base.last_kernel_ms())
-            base.search_status()
-            timings.append((min(ms), ex, hs, ec, wpc))
+            # the lookahead only applies to the default seen set
+            for vpr in ((0,) if ex else HNSW_VISITS_PER_ROUND):
+                kw = dict(store=store, summation=summation, rerank=rerank, exact_counts=ex,
+                          hash_slots_log2=hs, eval_cache_log2=ec, waves_per_cu=wpc,
+                          visits_per_round=vpr, k_per_query=k_per_query,
+                          ef_per_query=ef_per_query)
+                index.search_device(queries, 0, max_ef, oi, od, oc, **kw)  # warm
+                ms = []
+                for _ in range(reps):
+                    index.search_device(queries, 0, max_ef, oi, od, oc, async_=True, **kw)
+                    ms.append(base.last_kernel_ms())
+                base.search_status()
+                timings.append((min(ms), ex, hs, ec, wpc, vpr))
     best = min(timings)
     res = {"exact_counts": best[1], "hash_slots_log2": best[2], "eval_cache_log2": best[3],
-           "waves_per_cu": best[4], "ms": best[0],
+           "waves_per_cu": best[4], "visits_per_round": best[5], "ms": best[0],
            "timings": [{"exact_counts": e, "hash_slots_log2": h, "eval_cache_log2": c,
-                        "waves_per_cu": w, "ms": t} for t, e, h, c, w in timings]}
+                        "waves_per_cu": w, "visits_per_round": v, "ms": t}
+                       for t, e, h, c, w, v in timings]}
     if use_cache:
         table[key] = res
         _store(table)

@@ -3,11 +3,102 @@
 // translation unit per store (bs_*.hip).
 #include "beam_search_kernel.h"
 
+#include <algorithm>
+
 namespace pann {
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
+}
+
+// Longest-first order of a batch with per-query beams.  The persistent grid
+// takes queries in index order, so a long search that starts late sets the
+// batch's end; a counting sort by descending beam starts the long ones first
+// and leaves the short ones for the tail.  Bin b holds beam max_beam - b.
+__device__ __forceinline__ uint32_t order_bin(int32_t beam, uint32_t max_beam) {
+  const uint32_t b = beam < 0 ? 0u : ((uint32_t)beam > max_beam ? max_beam : (uint32_t)beam);
+  return max_beam - b;
+}
+
+// Adds `active` lanes to their bins, one atomic per distinct bin of the wave
+// (a batch often has a few beam values, whose bins every lane would hit);
+// returns each lane's position: its bin's old count plus its rank among the
+// wave's lanes with that bin.  Called by all 64 lanes.
+__device__ __forceinline__ uint32_t wave_bin_add(uint32_t* bins, uint32_t bin, bool active) {
+  uint64_t rem = ballot(active);
+  uint32_t pos = 0;
+  while (rem) {
+    const int leader = __ffsll((long long)rem) - 1;
+    const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)bin, leader);
+    const uint64_t m = ballot(active && bin == b) & rem;
+    uint32_t base = 0;
+    if ((int)threadIdx.x % PANN_WAVE == leader) base = atomicAdd(&bins[b], (uint32_t)__popcll(m));
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+    if ((m >> (threadIdx.x % PANN_WAVE)) & 1u) pos = base + mbcnt(m);
+    rem &= ~m;
+  }
+  return pos;
+}
+
+// bins[0 .. nbins) to exclusive prefix sums, in one workgroup of 1024
+// threads, each summing a contiguous run of bins
+__device__ __forceinline__ void order_scan(uint32_t* bins, uint32_t nbins, uint32_t* part) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (nbins + 1023u) / 1024u;
+  const uint32_t lo = t * per < nbins ? t * per : nbins;
+  const uint32_t hi = lo + per < nbins ? lo + per : nbins;
+  uint32_t s = 0;
+  for (uint32_t i = lo; i < hi; i++) s += bins[i];
+  part[t] = s;
+  __syncthreads();
+  for (uint32_t o = 1; o < 1024u; o <<= 1) {  // inclusive scan of the runs' sums
+    const uint32_t v = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t base = part[t] - s;
+  for (uint32_t i = lo; i < hi; i++) {
+    const uint32_t c = bins[i];
+    bins[i] = base;
+    base += c;
+  }
+  __syncthreads();
+}
+
+// The whole sort in one workgroup of 1024 threads, the bins in LDS.
+constexpr uint32_t kOrderLdsBins = 8192;
+__global__ __launch_bounds__(1024) void k_order_lds(const int32_t* beam_pq, uint32_t nq,
+                                                    uint32_t max_beam, uint32_t* order) {
+  __shared__ uint32_t bins[kOrderLdsBins];
+  __shared__ uint32_t part[1024];
+  const uint32_t nbins = max_beam + 1u;
+  for (uint32_t i = threadIdx.x; i < nbins; i += 1024u) bins[i] = 0u;
+  __syncthreads();
+  // every wave runs the same number of rounds (wave_bin_add is wave-wide)
+  for (uint32_t i0 = 0; i0 < nq; i0 += 1024u) {
+    const uint32_t i = i0 + threadIdx.x;
+    const bool act = i < nq;
+    wave_bin_add(bins, act ? order_bin(beam_pq[i], max_beam) : 0u, act);
+  }
+  __syncthreads();
+  order_scan(bins, nbins, part);
+  for (uint32_t i0 = 0; i0 < nq; i0 += 1024u) {
+    const uint32_t i = i0 + threadIdx.x;
+    const bool act = i < nq;
+    const uint32_t pos = wave_bin_add(bins, act ? order_bin(beam_pq[i], max_beam) : 0u, act);
+    if (act) order[pos] = i;
+  }
+}
+
+hipError_t launch_order_by_beam(const int32_t* beam_pq, uint32_t nq, uint32_t max_beam,
+                                uint32_t* order, hipStream_t s) {
+  if (nq == 0) return hipSuccess;
+  // (a beam past the bins would not fit a wave's LDS frontier either)
+  if (max_beam + 1u > kOrderLdsBins) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_order_lds, dim3(1), dim3(1024), 0, s, beam_pq, nq, max_beam, order);
+  return hipGetLastError();
 }
 
 uint32_t search_lds_bytes(const SearchArgs& a) {

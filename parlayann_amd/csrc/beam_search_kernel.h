@@ -921,6 +921,7 @@ void k_beam_search(SearchArgs a) {
     if (lane == 0) q = atomicAdd(a.work_counter, 1u);
     q = readfirstlane(q);
     if (q >= a.nq) break;
+    if (a.qorder) q = readfirstlane(a.qorder[q]);
     if (W > 1) {  // the helpers load the query row into their gather registers
       coop_post(L, lane, CO_QUERY, a.self_ids ? a.self_ids[q] : q, 0u, 0u);
       __syncthreads();

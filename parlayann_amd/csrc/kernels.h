@@ -66,6 +66,9 @@ struct SearchArgs {
   int32_t k, beam;
   const int32_t* k_pq;
   const int32_t* beam_pq;
+  // nullptr, or the order in which the persistent grid takes the queries (the
+  // longest beams first, launch_order_by_beam); results do not depend on it
+  const uint32_t* qorder;
   int64_t limit;
   int32_t degree_limit;
   double batch_factor;
@@ -213,6 +216,11 @@ hipError_t launch_sort_visited(const uint64_t* vis, uint32_t nq, uint32_t cap,
 hipError_t launch_hnsw_cut(uint32_t* ids, float* dists, uint32_t stride, uint32_t* count,
                            uint32_t nq, uint32_t k, const int32_t* k_pq,
                            const uint32_t* node_ids, hipStream_t s);
+// order[0 .. nq): the queries by descending beam_pq (beams clamped to
+// [0, max_beam], max_beam < 8192), a counting sort in one workgroup; the
+// order inside one beam value is unspecified
+hipError_t launch_order_by_beam(const int32_t* beam_pq, uint32_t nq, uint32_t max_beam,
+                                uint32_t* order, hipStream_t s);
 // acc[0] += v[0] + .. + v[n - 1]
 hipError_t launch_sum_u32(const uint32_t* v, uint32_t n, unsigned long long* acc, hipStream_t s);
 hipError_t launch_update_neighbors(uint32_t* nbrs, uint32_t nbr_stride, uint32_t max_degree,

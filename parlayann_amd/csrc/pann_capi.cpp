@@ -171,6 +171,9 @@ struct pann_index_s {
     size_t stage_cap = 0;
     uint32_t* d_cmps = nullptr;  // build-mode search: evaluations per query
     size_t cmps_cap = 0;
+    // per-query beams: the longest-first query order
+    uint32_t* d_order = nullptr;
+    size_t order_cap = 0;
   };
   std::unordered_map<hipStream_t, StreamCtx> streams;
   StreamCtx* last_timed = nullptr;
@@ -278,6 +281,35 @@ pann_status ensure_vis(StreamCtx& C, size_t words) {
 }
 
 __global__ void k_zero_status(uint32_t* p) { p[threadIdx.x] = 0u; }
+
+// A batch with per-query beams runs longest-first: a.qorder = the queries by
+// descending beam (clamped to max_beam), sorted on the device.  The persistent
+// grid takes queries in that order, so the long searches do not start last
+// and set the batch's end.  Results do not depend on the order.
+pann_status order_by_beam(StreamCtx& C, const int32_t* beam_pq, uint32_t nq, int64_t max_beam,
+                          pann::SearchArgs& a, hipStream_t stream) {
+  if (max_beam >= 8192) return PANN_OK;  // past the sort's bins: the order given
+  const size_t need = nq;
+  if (C.order_cap < need) {
+    if (capturing(stream))
+      return fail(PANN_ERR_ARG, "the query order scratch would grow under graph capture: run "
+                                "the search once with at least this many queries before "
+                                "capturing it");
+    const size_t cap = std::max(need, 2 * C.order_cap);
+    if (C.d_order) {
+      HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      (void)hipFree(C.d_order);
+    }
+    C.d_order = nullptr;
+    C.order_cap = 0;
+    HIP_TRY(dev_malloc(&C.d_order, cap * 4), "hipMalloc(query order)");
+    C.order_cap = cap;
+  }
+  HIP_TRY(pann::launch_order_by_beam(beam_pq, nq, (uint32_t)max_beam, C.d_order, stream),
+          "query order launch");
+  a.qorder = C.d_order;
+  return PANN_OK;
+}
 
 }  // namespace
 
@@ -771,6 +803,8 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   if (o.waves_per_query != 0 && o.waves_per_query != 1 && o.waves_per_query != 2 &&
       o.waves_per_query != 4)
     return fail(PANN_ERR_ARG, "waves_per_query must be 0, 1, 2 or 4");
+  if (o.query_order != 0 && o.query_order != 1)
+    return fail(PANN_ERR_ARG, "query_order must be 0 or 1");
   a.wpq = o.waves_per_query > 1 ? (uint32_t)o.waves_per_query : 1u;
   if (a.wpq > 1 && (o.exact_counts || o.frontier_filter == 1))
     return fail(PANN_ERR_UNSUPPORTED, "waves_per_query > 1 runs the default seen set without "
@@ -972,6 +1006,10 @@ pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
   }
   st = stage_queries(idx, *C, opts ? opts->store : PANN_STORE_NATIVE, queries, nq, a, stream);
   if (st != PANN_OK) return st;
+  if (beam_pq && !(opts && opts->query_order)) {
+    st = order_by_beam(*C, beam_pq, nq, qp->beam_size, a, stream);
+    if (st != PANN_OK) return st;
+  }
   grid = std::min<uint32_t>(grid, nq);
   const bool cap = capturing(stream);  // captured: no events, status read after replay
   if (cap && !(opts && opts->async))
@@ -1660,6 +1698,10 @@ static pann_status hnsw_search_once(pann_index_t idx, const void* queries, uint3
     if (ef_pq) {  // per-query ef: the whole frontier comes back
       a.k_pq = ef_pq;
       a.beam_pq = ef_pq;
+      if (!o.query_order) {
+        st = order_by_beam(*C, ef_pq, nq, ef, a, stream);
+        if (st != PANN_OK) return st;
+      }
     }
     a.nup = top_level;
     for (uint32_t i = 0; i < top_level; i++) {
@@ -1722,6 +1764,10 @@ static pann_status hnsw_search_once(pann_index_t idx, const void* queries, uint3
     if (last && ef_pq) {  // per-query ef: the whole frontier comes back
       a.k_pq = ef_pq;
       a.beam_pq = ef_pq;
+      if (!o.query_order) {
+        st = order_by_beam(*C, ef_pq, nq, ef, a, stream);
+        if (st != PANN_OK) return st;
+      }
     }
     grid = std::min<uint32_t>(grid, nq);
     HIP_TRY(pann::launch_beam_search(a, store_kind, idx->metric, sequential, grid, stream),
@@ -1969,7 +2015,7 @@ pann_status pann_close(pann_index_t idx) {
   for (auto& kv : idx->streams) {
     StreamCtx& C = kv.second;
     void* cp[] = {C.d_counter, C.d_status, C.d_starts, C.d_ghash, C.d_vis, C.d_eps, C.d_epsd,
-                  C.d_qcodes, C.d_qtable, C.d_stage, C.d_cmps};
+                  C.d_qcodes, C.d_qtable, C.d_stage, C.d_cmps, C.d_order};
     for (void* p : cp)
       if (p) (void)hipFree(p);
     if (C.ev0) (void)hipEventDestroy(C.ev0);

@@ -172,7 +172,7 @@ class HNSWIndex:
                       exact_counts: bool = False, hash_slots_log2: int = 0,
                       waves_per_cu: int = 0, visits_per_round: int = 0,
                       eval_cache_log2: int = 0, frontier_filter: int = 0,
-                      waves_per_query: int = 0):
+                      waves_per_query: int = 0, query_order: int = 0):
         """HNSW::search for a batch of device queries (torch tensors).  Rows of
         out_ids / out_dists (stride >= ef) hold out_count[q] results, point ids.
         k_per_query / ef_per_query (int32 tensors, ef_per_query <= ef) mix
@@ -188,7 +188,7 @@ class HNSWIndex:
         ent = np.ascontiguousarray(self.model.entrance, dtype=np.uint32)
         op = DeviceIndex._opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu,
                                exact_counts, visits_per_round, eval_cache_log2,
-                               frontier_filter, waves_per_query)
+                               frontier_filter, waves_per_query, query_order)
         if stream is not None and hasattr(stream, "cuda_stream"):
             stream = stream.cuda_stream
         L.check(L.lib().pann_hnsw_search(

@@ -134,11 +134,13 @@ class DeviceIndex:
 
     @staticmethod
     def _opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu, exact_counts,
-              visits_per_round=0, eval_cache_log2=0, frontier_filter=0, waves_per_query=0):
+              visits_per_round=0, eval_cache_log2=0, frontier_filter=0, waves_per_query=0,
+              query_order=0):
         return L.SearchOpts(_STORE[store], _SUM[summation], int(bool(rerank)),
                             int(hash_slots_log2), int(bool(async_)), int(waves_per_cu),
                             int(bool(exact_counts)), int(visits_per_round),
-                            int(eval_cache_log2), int(frontier_filter), int(waves_per_query))
+                            int(eval_cache_log2), int(frontier_filter), int(waves_per_query),
+                            int(query_order))
 
     def search(self, queries, k: int, beam: int, *, limit=None, degree_limit=None,
                batch_factor: float = 0.125, rerank_factor: float = 100.0,
@@ -216,7 +218,7 @@ class DeviceIndex:
                       waves_per_cu: int = 0, stream=None, async_: bool = False,
                       exact_counts: bool = False, visits_per_round: int = 0,
                       eval_cache_log2: int = 0, frontier_filter: int = 0,
-                      waves_per_query: int = 0):
+                      waves_per_query: int = 0, query_order: int = 0):
         """Device-buffer search: queries/outputs are torch tensors on the
         index's device.  `beam` is the maximum beam when beam_per_query is
         given.  Launches on `stream` (torch stream or raw handle).  With
@@ -227,7 +229,7 @@ class DeviceIndex:
         qp = self._qparams(k, beam, limit, degree_limit, rerank_factor, batch_factor)
         op = self._opts(store, summation, rerank, hash_slots_log2, async_, waves_per_cu,
                         exact_counts, visits_per_round, eval_cache_log2, frontier_filter,
-                        waves_per_query)
+                        waves_per_query, query_order)
         if stream is not None and hasattr(stream, "cuda_stream"):
             stream = stream.cuda_stream
         L.check(L.lib().pann_search(self._h, _ptr(queries), nq, C.byref(qp), _ptr(k_per_query),

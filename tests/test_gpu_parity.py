@@ -220,12 +220,22 @@ def test_per_query_k_and_beam(oracle, u8_case):
     idx = DeviceIndex(X, adj, "l2")
     dev = torch.device("cuda")
     tq = torch.from_numpy(Q).to(dev)
-    oi = torch.zeros((nq, 50), dtype=torch.int32, device=dev)
-    od = torch.zeros((nq, 50), dtype=torch.float32, device=dev)
-    idx.search_device(tq, 50, int(bs.max()), oi, od, k_per_query=torch.from_numpy(ks).to(dev),
-                      beam_per_query=torch.from_numpy(bs).to(dev))
-    torch.cuda.synchronize()
-    gi, gd = oi.cpu().numpy().view(np.uint32), od.cpu().numpy()
+    out = []
+    # query_order 0 runs the batch longest-first (a device sort by beam), 1 as
+    # given: the same results either way
+    for qo in (0, 1):
+        oi = torch.zeros((nq, 50), dtype=torch.int32, device=dev)
+        od = torch.zeros((nq, 50), dtype=torch.float32, device=dev)
+        ov = torch.zeros(nq, dtype=torch.int32, device=dev)
+        oc = torch.zeros(nq, dtype=torch.int32, device=dev)
+        idx.search_device(tq, 50, int(bs.max()), oi, od, out_visited=ov, out_cmps=oc,
+                          k_per_query=torch.from_numpy(ks).to(dev),
+                          beam_per_query=torch.from_numpy(bs).to(dev), query_order=qo)
+        torch.cuda.synchronize()
+        out.append([t.cpu().numpy() for t in (oi, od, ov, oc)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    gi, gd = out[0][0].view(np.uint32), out[0][1]
     for B in np.unique(bs):
         sel = np.nonzero(bs == B)[0]
         kmax = int(ks[sel].max())
