@@ -247,9 +247,11 @@ __device__ __forceinline__ bool sorted_contains(const uint64_t* A, int n, uint64
 }
 
 // Distances of rows NL[0..ncmp) into DL[0..ncmp) with the search's store.
-template <int ST, int MT, int NCH, bool SEQ>
+// PAD: NL is this wave's whole 128-entry list (not a cooperative share), so
+// the gather may fill its tail (eval_group_t).
+template <int ST, int MT, int NCH, bool SEQ, bool PAD = false>
 __device__ __forceinline__ void eval_rows_at(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
-                                             const float* QL, const uint32_t* NL, float* DL,
+                                             const float* QL, uint32_t* NL, float* DL,
                                              int ncmp, int lane) {
   if constexpr (SEQ) {
     // rows of up to 8 lines: the group gather with the in-order lane chain and the
@@ -259,14 +261,15 @@ __device__ __forceinline__ void eval_rows_at(const GroupDist<ST, MT, NCH>& D, co
     else
       eval_group_inorder_lds<MT, 8, 1>(QL, (int)a.dim, a.pts, a.pts_stride, NL, ncmp, DL, lane);
   } else {
-    eval_group<ST, MT, NCH>(D, a.pts, a.pts_stride, (int)a.row_bytes, NL, ncmp, DL, lane);
+    eval_group<ST, MT, NCH, PAD>(D, a.pts, a.pts_stride, (int)a.row_bytes,
+                                 a.lane_words == 3u ? 3 : 4, NL, ncmp, DL, lane);
   }
 }
 
 template <int ST, int MT, int NCH, bool SEQ>
 __device__ __forceinline__ void eval_rows(const GroupDist<ST, MT, NCH>& D, const SearchArgs& a,
                                           const WaveLds& L, int ncmp, int lane) {
-  eval_rows_at<ST, MT, NCH, SEQ>(D, a, L.QL, L.NL, L.DL, ncmp, lane);
+  eval_rows_at<ST, MT, NCH, SEQ, true>(D, a, L.QL, L.NL, L.DL, ncmp, lane);
 }
 
 // FP32 distances of rows NL[0..m) of the full-precision store (the rerank),
@@ -452,7 +455,8 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
   GroupDist<ST, MT, NCH> D;
   const uint64_t qr = a.self_ids ? (uint64_t)self : (uint64_t)q;  // build mode: the point's own row
   const char* qrow = a.queries + qr * a.q_stride;
-  D.load_query((const QT*)qrow, a.scales, (int)a.dim, lane & 7);
+  load_query_slices(D, (const QT*)qrow, a.scales, (int)a.dim, lane & 7,
+                    a.lane_words == 3u ? 3 : 4);
   const float* fq = SEQ ? (const float*)qrow : (const float*)(a.fqueries + qr * a.fq_stride);
   if (a.q_own && (SEQ || a.full))
     for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
@@ -868,8 +872,8 @@ __device__ __forceinline__ void coop_helper(const SearchArgs& a, const WaveLds& 
     const uint32_t cmd = L.CO[0], a0 = L.CO[1], a1 = L.CO[2], a2 = L.CO[3];
     if (cmd == CO_EXIT) break;
     if (cmd == CO_QUERY) {
-      D.load_query((const QT*)(a.queries + (uint64_t)a0 * a.q_stride), a.scales, (int)a.dim,
-                   lane & 7);
+      load_query_slices(D, (const QT*)(a.queries + (uint64_t)a0 * a.q_stride), a.scales,
+                        (int)a.dim, lane & 7, a.lane_words == 3u ? 3 : 4);
       if constexpr (ST == ST_FP8R) D.T = L.CT;
     } else if (cmd == CO_EVAL || cmd == CO_EVAL_RR) {
       const int n = (int)a0, per = coop_share(n, W), lo = wid * per;

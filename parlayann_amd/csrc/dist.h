@@ -57,6 +57,7 @@ struct GroupDist;
 template <int MT, int NCH>
 struct GroupDist<ST_F32, MT, NCH> {
   float q[NCH][4];
+  template <int LW = 4>  // whole 16-byte slices only
   __device__ __forceinline__ void load_query(const float* qv, const float* /*scales*/, int dim, int sub) {
 #pragma unroll
     for (int c = 0; c < NCH; c++)
@@ -91,6 +92,7 @@ struct GroupDist<ST_F32, MT, NCH> {
 template <int MT, int NCH>
 struct GroupDist<ST_BF16, MT, NCH> {
   float q[NCH][8];
+  template <int LW = 4>  // whole 16-byte slices only
   __device__ __forceinline__ void load_query(const float* qv, const float* /*scales*/, int dim, int sub) {
 #pragma unroll
     for (int c = 0; c < NCH; c++)
@@ -130,12 +132,14 @@ template <int MT, int NCH>
 struct GroupDist<ST_FP8, MT, NCH> {
   float q[NCH][16];
   float s[MT == M_L2 ? NCH : 1][16];
+  template <int LW = 4>
   __device__ __forceinline__ void load_query(const float* qv, const float* scales, int dim, int sub) {
+    constexpr int lw = LW;
 #pragma unroll
     for (int c = 0; c < NCH; c++)
 #pragma unroll
       for (int e = 0; e < 16; e++) {
-        int j = c * 128 + sub * 16 + e;
+        int j = e < 4 * lw ? c * 128 + sub * 4 * lw + e : dim;
         float qq = j < dim ? qv[j] : 0.0f;
         float sc = j < dim ? scales[j] : 0.0f;
         if (MT == M_L2) {
@@ -146,14 +150,16 @@ struct GroupDist<ST_FP8, MT, NCH> {
         }
       }
   }
-  __device__ __forceinline__ float partial(const uint4* raw) const {
+  __device__ __forceinline__ float partial(const uint4* raw) const { return partial_w<4>(raw); }
+  template <int LW>  // the lane's first LW words of each chunk
+  __device__ __forceinline__ float partial_w(const uint4* raw) const {
     float acc = 0.0f;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
       uint32_t w[4] = {raw[c].x, raw[c].y, raw[c].z, raw[c].w};
       float pv[16];
 #pragma unroll
-      for (int e = 0; e < 4; e++) {  // v_cvt_pk_f32_fp8: two e4m3 bytes per instruction
+      for (int e = 0; e < LW; e++) {  // v_cvt_pk_f32_fp8: two e4m3 bytes per instruction
         f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[e], false);
         f32x2 hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[e], true);
         pv[4 * e + 0] = lo[0];
@@ -162,7 +168,7 @@ struct GroupDist<ST_FP8, MT, NCH> {
         pv[4 * e + 3] = hi[1];
       }
 #pragma unroll
-      for (int e = 0; e < 16; e++) {
+      for (int e = 0; e < 4 * LW; e++) {
         float p = pv[e];
         if (MT == M_L2) {
           float t = q[c][e] - s[c][e] * p;
@@ -207,14 +213,16 @@ template <int NCH>
 struct GroupDist<ST_FP8, M_IP, NCH> {
   f16x2 q[NCH][8];
   float scl, inv;  // 2^e, 2^-e
+  template <int LW = 4>
   __device__ __forceinline__ void load_query(const float* qv, const float* scales, int dim, int sub) {
+    constexpr int lw = LW;
     float v[NCH][16];
     float mx = 0.0f;
 #pragma unroll
     for (int c = 0; c < NCH; c++)
 #pragma unroll
       for (int e = 0; e < 16; e++) {
-        int j = c * 128 + sub * 16 + e;
+        int j = e < 4 * lw ? c * 128 + sub * 4 * lw + e : dim;
         v[c][e] = j < dim ? qv[j] * scales[j] : 0.0f;
         mx = fmaxf(mx, fabsf(v[c][e]));
       }
@@ -230,13 +238,15 @@ struct GroupDist<ST_FP8, M_IP, NCH> {
         q[c][e] = h;
       }
   }
-  __device__ __forceinline__ float partial(const uint4* raw) const {
+  __device__ __forceinline__ float partial(const uint4* raw) const { return partial_w<4>(raw); }
+  template <int LW>  // the lane's first LW words of each chunk
+  __device__ __forceinline__ float partial_w(const uint4* raw) const {
     float acc = 0.0f;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
       uint32_t w[4] = {raw[c].x, raw[c].y, raw[c].z, raw[c].w};
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
+      for (int e = 0; e < LW; e++) {
         f16x2 lo = __builtin_amdgcn_cvt_scalef32_pk_f16_fp8(w[e], 1.0f, false);
         f16x2 hi = __builtin_amdgcn_cvt_scalef32_pk_f16_fp8(w[e], 1.0f, true);
         acc = __builtin_amdgcn_fdot2(lo, q[c][2 * e], acc, false);
@@ -262,6 +272,7 @@ struct GroupDist<ST_FP8R, MT, NCH> : GroupDist<ST_FP8, M_IP, NCH> {
   static_assert(MT == M_IP, "the FP8 residual store is inner product only");
   const float* T;
   bool idl;
+  template <int LW = 4>  // whole 16-byte slices only
   __device__ __forceinline__ void load_query(const float* qv, const float* scales, int dim, int sub) {
     GroupDist<ST_FP8, M_IP, NCH>::load_query(qv, scales, dim, sub);
     idl = sub == 7;
@@ -280,7 +291,9 @@ template <int MT, int NCH>
 struct GroupDist<ST_U8, MT, NCH> {
   uint32_t q[NCH][4];
   uint32_t qq;  // this lane's share of sum q^2
+  template <int LW = 4>
   __device__ __forceinline__ void load_query(const uint8_t* qv, const float*, int dim, int sub) {
+    constexpr int lw = LW;
     qq = 0u;
 #pragma unroll
     for (int c = 0; c < NCH; c++)
@@ -289,7 +302,7 @@ struct GroupDist<ST_U8, MT, NCH> {
         uint32_t v = 0u;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
-          int j = c * 128 + sub * 16 + 4 * w + b;
+          int j = w < lw ? c * 128 + sub * 4 * lw + 4 * w + b : dim;
           uint32_t x = j < dim ? (uint32_t)qv[j] : 0u;
           v |= x << (8 * b);
         }
@@ -297,13 +310,15 @@ struct GroupDist<ST_U8, MT, NCH> {
         qq = __builtin_amdgcn_udot4(v, v, qq, false);
       }
   }
-  __device__ __forceinline__ int partial(const uint4* raw) const {
+  __device__ __forceinline__ int partial(const uint4* raw) const { return partial_w<4>(raw); }
+  template <int LW>  // the lane's first LW words of each chunk
+  __device__ __forceinline__ int partial_w(const uint4* raw) const {
     uint32_t pp = MT == M_L2 ? qq : 0u, pq = 0u;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
       uint32_t w[4] = {raw[c].x, raw[c].y, raw[c].z, raw[c].w};
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
+      for (int e = 0; e < LW; e++) {
         if (MT == M_L2) pp = __builtin_amdgcn_udot4(w[e], w[e], pp, false);
         pq = __builtin_amdgcn_udot4(w[e], q[c][e], pq, false);
       }
@@ -318,7 +333,9 @@ template <int MT, int NCH>
 struct GroupDist<ST_I8, MT, NCH> {
   int32_t q[NCH][4];
   int32_t qq;
+  template <int LW = 4>
   __device__ __forceinline__ void load_query(const int8_t* qv, const float*, int dim, int sub) {
+    constexpr int lw = LW;
     qq = 0;
 #pragma unroll
     for (int c = 0; c < NCH; c++)
@@ -327,7 +344,7 @@ struct GroupDist<ST_I8, MT, NCH> {
         uint32_t v = 0u;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
-          int j = c * 128 + sub * 16 + 4 * w + b;
+          int j = w < lw ? c * 128 + sub * 4 * lw + 4 * w + b : dim;
           uint32_t x = j < dim ? (uint32_t)(uint8_t)qv[j] : 0u;
           v |= x << (8 * b);
         }
@@ -335,13 +352,15 @@ struct GroupDist<ST_I8, MT, NCH> {
         qq = __builtin_amdgcn_sdot4((int)v, (int)v, qq, false);
       }
   }
-  __device__ __forceinline__ int partial(const uint4* raw) const {
+  __device__ __forceinline__ int partial(const uint4* raw) const { return partial_w<4>(raw); }
+  template <int LW>  // the lane's first LW words of each chunk
+  __device__ __forceinline__ int partial_w(const uint4* raw) const {
     int pp = MT == M_L2 ? qq : 0, pq = 0;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
       int w[4] = {(int)raw[c].x, (int)raw[c].y, (int)raw[c].z, (int)raw[c].w};
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
+      for (int e = 0; e < LW; e++) {
         if (MT == M_L2) pp = __builtin_amdgcn_sdot4(w[e], w[e], pp, false);
         pq = __builtin_amdgcn_sdot4(w[e], q[c][e], pq, false);
       }
@@ -350,6 +369,28 @@ struct GroupDist<ST_I8, MT, NCH> {
   }
   __device__ __forceinline__ float finish(int s) const { return MT == M_L2 ? (float)s : -((float)s); }
 };
+
+// Whether a store's one-chunk rows may run on 12-byte lane slices: 1-byte
+// elements with no per-row extra word (the FP8 residual rows end in one).
+template <int ST, int NCH>
+__host__ __device__ constexpr bool lane_slices_ok() {
+  return NCH == 1 && (ST == ST_FP8 || ST == ST_U8 || ST == ST_I8);
+}
+
+// Loads the query in the lane slices the gather will read (lw = 3 or 4 words
+// per lane and chunk); two straight-line loads, so the slice width stays a
+// compile-time constant in each.
+template <int ST, int MT, int NCH, typename QT>
+__device__ __forceinline__ void load_query_slices(GroupDist<ST, MT, NCH>& D, const QT* qv,
+                                                  const float* scales, int dim, int sub, int lw) {
+  if constexpr (lane_slices_ok<ST, NCH>()) {
+    if (lw == 3) {
+      D.template load_query<3>(qv, scales, dim, sub);
+      return;
+    }
+  }
+  D.template load_query<4>(qv, scales, dim, sub);
+}
 
 template <int ST>
 struct QueryElem {
@@ -364,20 +405,88 @@ struct QueryElem<ST_I8> {
   using T = int8_t;
 };
 
+// DPP moves inside each group of 8 lanes: the mirror (lane i takes lane
+// 7 - i), xor 2 and xor 1 -- each pairs a lane with one whose bit 2, 1 or 0
+// differs.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(dpp_mov<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_f(int v) {
+  return dpp_mov<CTRL>(v);
+}
+
+// One step of the transpose reduction: each lane holds N partial sums (rows
+// 0..N-1 of its current set) and a partner across bit `b`; it keeps the
+// half of the rows whose index bit equals its own bit, adding the partner's
+// partials of those rows (the partner sends the half it does not keep).
+// With N = 1 both lanes keep the one row.
+template <int CTRL, int N, typename T>
+__device__ __forceinline__ void transpose_step(T (&p)[N], T (&q)[N > 1 ? N / 2 : 1], bool b) {
+  if constexpr (N == 1) {
+    q[0] = p[0] + dpp_f<CTRL>(p[0]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < N / 2; j++) {
+      const T send = b ? p[j] : p[j + N / 2];
+      const T keep = b ? p[j + N / 2] : p[j];
+      q[j] = keep + dpp_f<CTRL>(send);
+    }
+  }
+}
+
+// The row sums of a group's UP rows from the 8 lanes' partials p[0..UP):
+// three exchange-and-add steps across lane bits 2, 1 and 0 (7 DPP adds for
+// UP = 8 instead of 3 per row).  Lane `sub` ends with the sum of row
+// sub >> (3 - log2 UP); for UP < 8 two or four lanes hold each row.
+template <int UP, typename T>
+__device__ __forceinline__ T group8_transpose_sum(T (&p)[UP], int sub) {
+  constexpr int U1 = UP > 1 ? UP / 2 : 1;
+  constexpr int U2 = U1 > 1 ? U1 / 2 : 1;
+  T q1[U1], q2[U2], q3[1];
+  transpose_step<0x141, UP>(p, q1, (sub & 4) != 0);
+  transpose_step<0x4E, U1>(q1, q2, (sub & 2) != 0);
+  transpose_step<0xB1, U2>(q2, q3, (sub & 1) != 0);
+  return q3[0];
+}
+
 // Evaluates the distances of rows NL[0..ncmp) into DL[0..ncmp) (both LDS).
 // Rows are gathered 8 at a time (one per 8-lane group), UP rounds of loads in
-// flight per lane.
-template <int ST, int MT, int NCH>
-__device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, const char* base,
-                                           uint64_t stride, int row_bytes, const uint32_t* NL,
-                                           int ncmp, float* DL, int lane) {
+// flight per lane.  A row's address is the lane's base plus (id * lines) << 7,
+// lines = the row's 128-byte lines: the product fits 32 bits for any index
+// under 512 GB, and one 64-bit shift-and-add (v_lshl_add_u64) places it.
+// PAD: NL is this wave's own and holds at least round_up(ncmp, 8 * UP)
+// entries; those past ncmp are set to NL[0] first, so each round's slots
+// read ids without a bounds select (their distances are discarded).
+// LW: 4-byte words per lane and chunk; 3 for one-chunk rows of at most 96
+// bytes, whose 8 lanes then read 12 bytes each and leave the line's padding
+// unread (the query was loaded with the same slices, load_query's lw).
+template <int ST, int MT, int NCH, bool PAD, int LW>
+__device__ __forceinline__ void eval_group_t(const GroupDist<ST, MT, NCH>& D, const char* base,
+                                             uint64_t stride, int row_bytes, uint32_t* NL,
+                                             int ncmp, float* DL, int lane) {
   // rows in flight per lane: every 8-lane group gathers UP rows per round,
   // sized so the raw chunks (UP * NCH * 4 VGPRs) keep the kernel at or
   // under ~128 VGPRs, i.e. no fewer waves per CU than LDS already allows
-  // (FP8 at two chunks: a float per query element leaves room for only 4)
-  constexpr int UP = (ST == ST_FP8 || ST == ST_FP8R) && NCH == 2 ? 4
+  // (FP8 and BF16 rows keep UP * NCH <= 8: their query slices take 8 or 16
+  // registers per chunk, which leaves room for only that many raw chunks)
+  constexpr bool wideq = ST == ST_FP8 || ST == ST_FP8R || ST == ST_BF16;
+  constexpr int UP = wideq && NCH >= 2 && NCH <= 4 ? (NCH == 2 ? 4 : 2)
                      : NCH <= 2 ? 8 : (NCH <= 4 ? 4 : 2);
+  constexpr int RSH = UP == 8 ? 0 : (UP == 4 ? 1 : 2);  // lane sub holds row sub >> RSH
   const int sub = lane & 7, grp = lane >> 3;
+  // 128-byte lines per row: the chunk count itself up to 4 (the stride is the
+  // row rounded up to 128 bytes), so the multiply folds into shifts and adds
+  const uint32_t lines = NCH <= 4 ? (uint32_t)NCH : (uint32_t)(stride >> 7);
+  if (ncmp <= 0) return;
+  const char* lbase = base + sub * 4 * LW;  // this lane's slice of every row
+  if constexpr (PAD) {
+    const int np = (ncmp + 8 * UP - 1) / (8 * UP) * (8 * UP);
+    const uint32_t id0 = NL[0];
+    for (int r = ncmp + lane; r < np; r += PANN_WAVE) NL[r] = id0;
+    wave_lds_sync();
+  }
   // 128-byte lines a row occupies: NCH rounds 5..7 up to 8, and the chunks
   // past the row's last line are not loaded (they would be the next row's
   // line, or past the last row's end)
@@ -391,22 +500,46 @@ __device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, cons
 #pragma unroll
     for (int u = 0; u < UP; u++) {
       const int r = r0 + u * 8 + grp;
-      const char* row = base + (uint64_t)NL[r < ncmp ? r : 0] * stride;
+      const uint32_t id = PAD ? NL[r] : NL[r < ncmp ? r : 0];
+      const char* row = lbase + ((uint64_t)(id * lines) << 7);
+      if constexpr (LW == 3) {
+        const uint3 v = *(const uint3*)row;
+        raw[u][0] = make_uint4(v.x, v.y, v.z, 0u);
+      } else {
 #pragma unroll
-      for (int c = 0; c < NCH; c++) {
-        if (NCH <= 4 || c < nl)  // uniform: the same for every row of the launch
-          raw[u][c] = ld16(row + c * 128 + sub * 16);
-        else
-          raw[u][c] = make_uint4(0u, 0u, 0u, 0u);
+        for (int c = 0; c < NCH; c++) {
+          if (NCH <= 4 || c < nl)  // uniform: the same for every row of the launch
+            raw[u][c] = ld16(row + c * 128);
+          else
+            raw[u][c] = make_uint4(0u, 0u, 0u, 0u);
+        }
       }
     }
+    using PT = decltype(D.partial(raw[0]));
+    PT part[UP];
 #pragma unroll
     for (int u = 0; u < UP; u++) {
-      int r = r0 + u * 8 + grp;
-      auto s = group8_sum(D.partial(raw[u]));
-      if (sub == 0 && r < ncmp) DL[r] = D.finish(s);
+      if constexpr (LW == 4) part[u] = D.partial(raw[u]);
+      else part[u] = D.template partial_w<LW>(raw[u]);
+    }
+    const PT tot = group8_transpose_sum<UP>(part, sub);
+    const int r = r0 + (sub >> RSH) * 8 + grp;
+    if ((sub & ((1 << RSH) - 1)) == 0 && r < ncmp) DL[r] = D.finish(tot);
+  }
+}
+
+
+template <int ST, int MT, int NCH, bool PAD>
+__device__ __forceinline__ void eval_group(const GroupDist<ST, MT, NCH>& D, const char* base,
+                                           uint64_t stride, int row_bytes, int lw, uint32_t* NL,
+                                           int ncmp, float* DL, int lane) {
+  if constexpr (lane_slices_ok<ST, NCH>()) {
+    if (lw == 3) {
+      eval_group_t<ST, MT, NCH, PAD, 3>(D, base, stride, row_bytes, NL, ncmp, DL, lane);
+      return;
     }
   }
+  eval_group_t<ST, MT, NCH, PAD, 4>(D, base, stride, row_bytes, NL, ncmp, DL, lane);
 }
 
 // ---------------------------------------------------------------------------

@@ -26,6 +26,9 @@ struct SearchArgs {
   const char* pts;
   uint64_t pts_stride;
   uint32_t row_bytes;
+  // 4-byte words each of a row's 8 lanes reads per 128-byte chunk: 4, or 3
+  // for one-chunk rows of 1-byte elements of at most 96 bytes
+  uint32_t lane_words;
   const float* scales;  // FP8 per-dimension scales (nullptr otherwise)
   // FP8 residual store: q . c for every query and centroid (nq x ncent,
   // computed before the launch); each row's last 4 bytes name its centroid

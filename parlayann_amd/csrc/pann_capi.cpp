@@ -752,6 +752,11 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   a.pts = pts;
   a.pts_stride = stride;
   a.row_bytes = row_bytes;
+  // 12-byte lane slices for rows of 1-byte elements that fill at most 96 bytes
+  // of their line (the FP8 residual rows' last word names their centroid)
+  a.lane_words = (!sequential && row_bytes <= 96 && store_kind != pann::ST_FP8R &&
+                  (store_kind == pann::ST_FP8 || store_kind == pann::ST_U8 ||
+                   store_kind == pann::ST_I8)) ? 3u : 4u;
   a.scales = store_kind == pann::ST_FP8R ? idx->d_rscales : idx->d_scales;
   a.ncent = store_kind == pann::ST_FP8R ? idx->ncent : 0u;
   a.full = o.rerank ? idx->d_pts : nullptr;
