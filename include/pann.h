@@ -40,7 +40,10 @@ typedef enum { PANN_METRIC_L2 = 0, PANN_METRIC_MIPS = 1 } pann_metric;
 typedef enum {
   PANN_STORE_NATIVE = 0, /* the points as given (f32 / u8 / i8) */
   PANN_STORE_BF16 = 1,   /* f32 points rounded to bfloat16 (RNE) */
-  PANN_STORE_FP8 = 2,    /* f32 points as OCP FP8 e4m3 with per-dimension absmax scales */
+  PANN_STORE_FP8 = 2,    /* f32 points as OCP FP8 e4m3 with per-dimension absmax scales;
+                            inner product sums exact products of the codes and the query
+                            term q_j s_j rounded to FP16 (scaled per query by a power of
+                            two) in FP32 */
   PANN_STORE_Q8 = 3,     /* the reference's 8-bit scalar quantization of f32 points: uint8
                             affine codes for L2 (Euclidian_Point<uint8_t>, euclidian_point.h:
                             98-107, 188-243), int8 codes with a trimmed global scale for MIPS
