@@ -7,10 +7,6 @@
 
 namespace pann {
 
-__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0u;
-}
 
 // Longest-first order of a batch with per-query beams.  The persistent grid
 // takes queries in index order, so a long search that starts late sets the
@@ -67,8 +63,11 @@ __device__ __forceinline__ void order_scan(uint32_t* bins, uint32_t nbins, uint3
   __syncthreads();
 }
 
-// The whole sort in one workgroup of 1024 threads, the bins in LDS.
+// The whole sort in one workgroup of 1024 threads, the bins in LDS.  Each
+// thread loads its queries' beams 16 at a time before using any of them, so a
+// batch of up to 16k queries pays one load latency per phase.
 constexpr uint32_t kOrderLdsBins = 8192;
+constexpr int kOrderPerThread = 16;
 __global__ __launch_bounds__(1024) void k_order_lds(const int32_t* beam_pq, uint32_t nq,
                                                     uint32_t max_beam, uint32_t* order) {
   __shared__ uint32_t bins[kOrderLdsBins];
@@ -76,19 +75,36 @@ __global__ __launch_bounds__(1024) void k_order_lds(const int32_t* beam_pq, uint
   const uint32_t nbins = max_beam + 1u;
   for (uint32_t i = threadIdx.x; i < nbins; i += 1024u) bins[i] = 0u;
   __syncthreads();
-  // every wave runs the same number of rounds (wave_bin_add is wave-wide)
-  for (uint32_t i0 = 0; i0 < nq; i0 += 1024u) {
-    const uint32_t i = i0 + threadIdx.x;
-    const bool act = i < nq;
-    wave_bin_add(bins, act ? order_bin(beam_pq[i], max_beam) : 0u, act);
+  // every wave runs the same rounds (wave_bin_add is wave-wide)
+  for (uint32_t c0 = 0; c0 < nq; c0 += 1024u * kOrderPerThread) {
+    uint32_t bin[kOrderPerThread];
+#pragma unroll
+    for (int t = 0; t < kOrderPerThread; t++) {
+      const uint32_t i = c0 + (uint32_t)t * 1024u + threadIdx.x;
+      bin[t] = i < nq ? order_bin(beam_pq[i], max_beam) : 0u;
+    }
+#pragma unroll
+    for (int t = 0; t < kOrderPerThread; t++) {
+      const uint32_t i = c0 + (uint32_t)t * 1024u + threadIdx.x;
+      wave_bin_add(bins, bin[t], i < nq);
+    }
   }
   __syncthreads();
   order_scan(bins, nbins, part);
-  for (uint32_t i0 = 0; i0 < nq; i0 += 1024u) {
-    const uint32_t i = i0 + threadIdx.x;
-    const bool act = i < nq;
-    const uint32_t pos = wave_bin_add(bins, act ? order_bin(beam_pq[i], max_beam) : 0u, act);
-    if (act) order[pos] = i;
+  for (uint32_t c0 = 0; c0 < nq; c0 += 1024u * kOrderPerThread) {
+    uint32_t bin[kOrderPerThread];
+#pragma unroll
+    for (int t = 0; t < kOrderPerThread; t++) {
+      const uint32_t i = c0 + (uint32_t)t * 1024u + threadIdx.x;
+      bin[t] = i < nq ? order_bin(beam_pq[i], max_beam) : 0u;
+    }
+#pragma unroll
+    for (int t = 0; t < kOrderPerThread; t++) {
+      const uint32_t i = c0 + (uint32_t)t * 1024u + threadIdx.x;
+      const bool act = i < nq;
+      const uint32_t pos = wave_bin_add(bins, bin[t], act);
+      if (act) order[pos] = i;
+    }
   }
 }
 
@@ -118,7 +134,6 @@ static int nch_for(int row_bytes) {
 
 hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool sequential,
                               uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, a.work_counter, 1u);
   int nch = nch_for((int)a.row_bytes);
   if (sequential) {
     if (store != ST_F32) return hipErrorInvalidValue;

@@ -129,6 +129,8 @@ hipError_t launch_bs_i8(const SearchArgs& a, int metric, int nch, uint32_t grid,
 hipError_t launch_bs_bf16(const SearchArgs& a, int metric, int nch, uint32_t grid, hipStream_t s);
 hipError_t launch_bs_fp8(const SearchArgs& a, int metric, int nch, uint32_t grid, hipStream_t s);
 hipError_t launch_bs_fp8r(const SearchArgs& a, int metric, int nch, uint32_t grid, hipStream_t s);
+// the search kernel; a.work_counter and a.status must be zero at its start (the
+// library's launch set-up clears both in one kernel before each search)
 hipError_t launch_beam_search(const SearchArgs& a, int store, int metric, bool sequential,
                               uint32_t grid, hipStream_t s);
 

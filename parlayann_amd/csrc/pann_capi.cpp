@@ -280,7 +280,12 @@ pann_status ensure_vis(StreamCtx& C, size_t words) {
   return PANN_OK;
 }
 
-__global__ void k_zero_status(uint32_t* p) { p[threadIdx.x] = 0u; }
+// A launch's status word and work counter start at zero: one kernel clears both
+// (a fill kernel, not a memset node, so a captured search replays it correctly).
+__global__ void k_zero_launch(uint32_t* status, uint32_t* counter) {
+  *status = 0u;
+  *counter = 0u;
+}
 
 // A batch with per-query beams runs longest-first: a.qorder = the queries by
 // descending beam (clamped to max_beam), sorted on the device.  The persistent
@@ -894,8 +899,8 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     C->starts_host.assign(starts, starts + nstarts);
   }
-  hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(1), 0, stream, a.status);
-  HIP_TRY(hipGetLastError(), "status reset");
+  hipLaunchKernelGGL(k_zero_launch, dim3(1), dim3(1), 0, stream, a.status, a.work_counter);
+  HIP_TRY(hipGetLastError(), "status and work counter reset");
   return PANN_OK;
 }
 
