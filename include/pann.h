@@ -113,9 +113,9 @@ typedef struct {
                           against the frontier and pending candidates afterwards -- same
                           ids, distances and visited counts, out_cmps counts evaluations, no
                           capacity limit */
-  int32_t visits_per_round; /* 0 or 2: while a merge is deferred, the next visit's rows are
-                               gathered with the current one's; 3: the same, but only where
-                               the current visit is predicted to defer its merge (rows
+  int32_t visits_per_round; /* 2: while a merge is deferred, the next visit's rows are
+                               gathered with the current one's; 0 or 3: the same, but only
+                               where the current visit is predicted to defer its merge (rows
                                gathered for a visit that a merge cancels are discarded);
                                1: one visit per round.  Results do not depend on it (a
                                tuning knob) */
