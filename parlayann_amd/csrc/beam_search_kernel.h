@@ -493,18 +493,28 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
   // ---- starting points (beamSearch.h:71-93) ----
   // evaluated 64 at a time into F1, then rank-sorted into F0 (equal keys keep
   // their order, so repeated starts stay repeated as in the reference)
-  for (int t = 0; t < ns; t += PANN_WAVE) {
-    const int m = ns - t < PANN_WAVE ? ns - t : PANN_WAVE;
-    if (lane < m) {
-      const uint32_t s = sp ? sp[t + lane] : P.start1;
-      L.NL[lane] = s;
-      if constexpr (EX) lds_insert(L.H, hlog2, s);
+  if (P.pass > 0) {
+    // a later pass has one start, the last pass's nearest id, whose distance
+    // (same store, same query) that pass left in DL[0]: not gathered again
+    if (lane == 0) {
+      if constexpr (EX) lds_insert(L.H, hlog2, P.start1);
+      L.F1[0] = make_key(L.DL[0], P.start1);
     }
     wave_lds_sync();
-    eval_n<ST, MT, NCH, SEQ, CO>(D, a, L, W, m, false, lane);
-    wave_lds_sync();
-    if (lane < m) L.F1[t + lane] = make_key(L.DL[lane], L.NL[lane]);
-    wave_lds_sync();
+  } else {
+    for (int t = 0; t < ns; t += PANN_WAVE) {
+      const int m = ns - t < PANN_WAVE ? ns - t : PANN_WAVE;
+      if (lane < m) {
+        const uint32_t s = sp ? sp[t + lane] : P.start1;
+        L.NL[lane] = s;
+        if constexpr (EX) lds_insert(L.H, hlog2, s);
+      }
+      wave_lds_sync();
+      eval_n<ST, MT, NCH, SEQ, CO>(D, a, L, W, m, false, lane);
+      wave_lds_sync();
+      if (lane < m) L.F1[t + lane] = make_key(L.DL[lane], L.NL[lane]);
+      wave_lds_sync();
+    }
   }
   lds_count = (uint32_t)ns;
   int cur_buf = 0;
@@ -809,8 +819,12 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
 
   const uint64_t* F = cur_buf ? L.F1 : L.F0;
   if (!P.final) {  // an upper HNSW level: W[0], the nearest (HNSW.hpp:1557-1558)
+    // and its distance in DL[0], where the next pass's start takes it
     const uint32_t best = key_id(F[0]);
-    if (lane == 0 && status) atomicOr(a.status, status);
+    if (lane == 0) {
+      L.DL[0] = key_dist(F[0]);
+      if (status) atomicOr(a.status, status);
+    }
     wave_lds_sync();
     return PassOut{best, P.vis0 + nvis, P.cmps0 + cmps, true};
   }
