@@ -430,10 +430,10 @@ struct PassOut {
 };
 
 template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB, bool CO>
-__device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds& L, uint32_t q, int lane,
-                              unsigned long long* ghash, const Pass& P) {
+__device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds& L,
+                                              const GroupDist<ST, MT, NCH>& D, uint32_t q,
+                                              int lane, unsigned long long* ghash, const Pass& P) {
   const uint32_t gtag = a.ghash_tag_base + q * (a.nup + 1u) + P.pass + 1u;
-  using QT = typename QueryElem<ST>::T;
   const int B = P.B;
   const int k = P.k;
   const uint32_t self = a.self_ids ? a.self_ids[q] : PANN_SENTINEL;
@@ -451,15 +451,11 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
     return PassOut{PANN_SENTINEL, P.vis0, P.cmps0, false};
   }
 
-  // ---- query into registers (group mode) and LDS (sequential / rerank) ----
-  GroupDist<ST, MT, NCH> D;
+  // (the kernel loads the query once for all of its passes: into D's
+  // registers, and into QL where the launch gives QL a region of its own)
   const uint64_t qr = a.self_ids ? (uint64_t)self : (uint64_t)q;  // build mode: the point's own row
   const char* qrow = a.queries + qr * a.q_stride;
-  load_query_slices(D, (const QT*)qrow, a.scales, (int)a.dim, lane & 7,
-                    a.lane_words == 3u ? 3 : 4);
   const float* fq = SEQ ? (const float*)qrow : (const float*)(a.fqueries + qr * a.fq_stride);
-  if (a.q_own && (SEQ || a.full))
-    for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
   // ---- clear the visited hash and the cache of evaluated ids ----
   for (uint32_t i = lane * 4; i < hslots; i += PANN_WAVE * 4)
     *(uint4*)(L.H + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
@@ -471,17 +467,6 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
     for (uint32_t i = lane * 4; i < (1u << rcl); i += PANN_WAVE * 4)
       *(uint4*)(L.RC + i) = make_uint4(PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL, PANN_SENTINEL);
   wave_lds_sync();
-  if constexpr (ST == ST_FP8R) {
-    // the query's row of q . c (one per centroid, ncent a multiple of 4),
-    // into LDS once per query (its first pass)
-    if (P.pass == 0) {
-      const float* row = a.qtable + qr * a.ncent;
-      for (uint32_t c0 = 4u * lane; c0 < a.ncent; c0 += 4u * PANN_WAVE)
-        *(float4*)(L.CT + c0) = *(const float4*)(row + c0);
-      wave_lds_sync();
-    }
-    D.T = L.CT;
-  }
 
   const int W = CO ? (int)a.wpq : 1;
 
@@ -913,12 +898,16 @@ __device__ __forceinline__ void coop_helper(const SearchArgs& a, const WaveLds& 
   }
 }
 
-// The one-line FP8 inner-product kernel is held to 128 VGPRs (4 waves per SIMD, what
-// its LDS allows at the beams it serves); left alone, the 8-lane rerank takes it to 132.
+// Inner-product kernels that land just past 128 VGPRs are held to 128 (4 waves per
+// SIMD, what their LDS allows at the beams they serve): FP8 rows of one or two lines
+// (left alone, the 8-lane rerank takes them to 129-132) and BF16 rows of two lines
+// (96-128 dimensions), whose count moves around 128 with small changes elsewhere.
 // CO: workgroups of 2 or 4 waves per query (SearchArgs::wpq); otherwise one.
 template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB, bool CO>
 __global__ __launch_bounds__(CO ? 4 * PANN_WAVE : PANN_WAVE,
-                             (!CO && ST == ST_FP8 && MT == M_IP && NCH == 1 && !EX) ? 4 : 1)
+                             (!CO && MT == M_IP && !EX &&
+                              ((ST == ST_FP8 && NCH == 1) || (ST == ST_BF16 && NCH == 2) ||
+                               (ST == ST_FP8 && NCH == 2 && !FB))) ? 4 : 1)
 void k_beam_search(SearchArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   // (the one-wave kernels keep threadIdx.x itself: its known range [0, 64)
@@ -943,6 +932,28 @@ void k_beam_search(SearchArgs a) {
     if (W > 1) {  // the helpers load the query row into their gather registers
       coop_post(L, lane, CO_QUERY, a.self_ids ? a.self_ids[q] : q, 0u, 0u);
       __syncthreads();
+    }
+    // the query, once for all of its passes: into registers (group mode), into
+    // QL where the launch gives it a region of its own (sequential sums, the
+    // rerank), and for the FP8 residual store its row of q . c into LDS
+    GroupDist<ST, MT, NCH> D;
+    {
+      using QT = typename QueryElem<ST>::T;
+      const uint64_t qr = a.self_ids ? (uint64_t)a.self_ids[q] : (uint64_t)q;
+      const char* qrow = a.queries + qr * a.q_stride;
+      load_query_slices(D, (const QT*)qrow, a.scales, (int)a.dim, lane & 7,
+                        a.lane_words == 3u ? 3 : 4);
+      if (a.q_own && (SEQ || a.full)) {
+        const float* fq = SEQ ? (const float*)qrow : (const float*)(a.fqueries + qr * a.fq_stride);
+        for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
+      }
+      if constexpr (ST == ST_FP8R) {  // one per centroid, ncent a multiple of 4
+        const float* row = a.qtable + qr * a.ncent;
+        for (uint32_t c0 = 4u * lane; c0 < a.ncent; c0 += 4u * PANN_WAVE)
+          *(float4*)(L.CT + c0) = *(const float4*)(row + c0);
+        D.T = L.CT;
+      }
+      wave_lds_sync();
     }
     // passes 0 .. nup-1: the HNSW upper levels at beam 1 (search_layer_to,
     // HNSW.hpp:1545-1566), the first from the start points, each next one
@@ -972,7 +983,7 @@ void k_beam_search(SearchArgs a) {
       P.final = !up;
       P.vis0 = po.vis;
       P.cmps0 = po.cmps;
-      po = search_one<ST, MT, NCH, SEQ, EX, FB, CO>(a, L, q, lane, ghash, P);
+      po = search_one<ST, MT, NCH, SEQ, EX, FB, CO>(a, L, D, q, lane, ghash, P);
       po.best = readfirstlane(po.best);  // the same in every lane: keep it scalar
       po.vis = readfirstlane(po.vis);
       po.cmps = readfirstlane(po.cmps);
