@@ -298,11 +298,21 @@ __device__ __forceinline__ void eval_rerank_rows(const SearchArgs& a, const floa
 // rank is a distinct position.  merge_place returns the lowest new position
 // of an unvisited entry among this thread's entries (INT_MAX if none).
 // ---------------------------------------------------------------------------
+// The rank of C[j]: the candidates smaller than it, read 8 at a time, so that
+// a rank costs c / 8 LDS round trips rather than c.
 __device__ __forceinline__ void merge_rank_sort(const WaveLds& L, int c, int tid, int nt) {
   for (int j = tid; j < c; j += nt) {
     const uint64_t x = L.C[j];
     int r = 0;
-    for (int i = 0; i < c; i++) r += (L.C[i] < x);
+    int i = 0;
+    for (; i + 8 <= c; i += 8) {
+      uint64_t y[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) y[t] = L.C[i + t];
+#pragma unroll
+      for (int t = 0; t < 8; t++) r += (y[t] < x);
+    }
+    for (; i < c; i++) r += (L.C[i] < x);
     L.Cs[r] = x;
   }
 }
@@ -323,14 +333,39 @@ __device__ __forceinline__ int merge_place(const WaveLds& L, int cur_buf, int c,
       first = pos < first ? pos : first;
     }
   }
-  for (int i = tid; i < f; i += nt) {
-    const uint64_t x = F[i];
-    const int pos = i + lower_bound_u64(L.Cs, c, x);
-    if (pos < B) {
-      F2[pos] = x;
-      const uint8_t v = FV[i];
-      FV2[pos] = v;
-      if (!v) first = pos < first ? pos : first;
+  // the frontier: each thread a contiguous run of entries, whose first one a
+  // search places among Cs; the rest follow it by merging, Cs[p] the first
+  // candidate not below the current entry (one LDS round trip per candidate
+  // passed, rather than a search per entry)
+  const int per = (f + nt - 1) / nt;
+  const int i0 = tid * per;
+  const int i1 = i0 + per < f ? i0 + per : f;
+  if (i0 < i1) {
+    int p = lower_bound_u64(L.Cs, c, F[i0]);
+    uint64_t y = p < c ? L.Cs[p] : ~0ull;
+    for (int i = i0; i < i1; i += 4) {
+      uint64_t x[4];
+      uint8_t v[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        x[t] = i + t < i1 ? F[i + t] : 0ull;
+        v[t] = i + t < i1 ? FV[i + t] : (uint8_t)1;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        if (i + t < i1) {
+          while (y < x[t]) {
+            p++;
+            y = p < c ? L.Cs[p] : ~0ull;
+          }
+          const int pos = i + t + p;
+          if (pos < B) {
+            F2[pos] = x[t];
+            FV2[pos] = v[t];
+            if (!v[t]) first = pos < first ? pos : first;
+          }
+        }
+      }
     }
   }
   return first;
