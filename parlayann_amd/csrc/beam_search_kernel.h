@@ -246,6 +246,23 @@ __device__ __forceinline__ bool sorted_contains(const uint64_t* A, int n, uint64
   return lo < n && A[lo] == x;
 }
 
+// Whether sorted A[0..n) holds xa, and xb (n >= 1, the same in every lane):
+// two branch-free searches in step, so that their LDS reads overlap, each
+// ending on the last entry <= its key (or on A[0])
+__device__ __forceinline__ void sorted_contains2(const uint64_t* A, int n, uint64_t xa,
+                                                 uint64_t xb, bool& ha, bool& hb) {
+  int ba = 0, bb = 0;
+  for (int len = n; len > 1;) {
+    const int half = len >> 1;
+    const uint64_t ya = A[ba + half], yb = A[bb + half];
+    ba = ya <= xa ? ba + half : ba;
+    bb = yb <= xb ? bb + half : bb;
+    len -= half;
+  }
+  ha = A[ba] == xa;
+  hb = A[bb] == xb;
+}
+
 // Distances of rows NL[0..ncmp) into DL[0..ncmp) with the search's store.
 // PAD: NL is this wave's whole 128-entry list (not a cooperative share), so
 // the gather may fill its tail (eval_group_t).
@@ -618,23 +635,40 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
   uint32_t pref_id = PANN_SENTINEL;
   uint32_t nb_pref = PANN_SENTINEL;
   int acc_last = 0;  // candidates the last committed visit added to C
-  // Frontier-and-pending seen set: rows [lo, hi) of NL/DL with d < cutoff
-  // join C unless their key is already in the frontier F (binary search on
-  // the exact (dist, id) key: a row's distance is computed the same way every
-  // time) or already pending in C (the small LDS table H, emptied at every
-  // merge).  An id the reference's seen set would also drop is dropped here.
-  auto accept_range = [&](const uint64_t* F, int lo, int hi, float cutoff) {
-    const int r = lo + lane;
-    bool ins = false;
-    uint64_t key = 0;
-    if (r < hi) {
-      const float d = L.DL[r];
-      key = make_key(d, L.NL[r]);
-      // (with the frontier-id filter on no evaluated id is in F; the exact
-      // check stays, at the cost of one binary search per id under the cutoff)
-      if (!(d >= cutoff) && !sorted_contains(F, f, key)) ins = lds_insert(L.H, hlog2, key_id(key));
-      if (rcl) L.RC[rc_slot(key_id(key), rcl)] = key_id(key);  // evaluated and committed
+  // Frontier-and-pending seen set: an evaluated row with d < cutoff joins C
+  // unless its key is already in the frontier F (binary search on the exact
+  // (dist, id) key: a row's distance is computed the same way every time) or
+  // already pending in C (the small LDS table H, emptied at every merge).  An
+  // id the reference's seen set would also drop is dropped here.
+  // test_rows: rows ra = lane of the first visit's n1 and rb = n1 + lane of the
+  // second's nb2 (F and the cutoff stay the same until a merge, so both lists
+  // are searched in F at once); ca / cb: under the cutoff and not in F.
+  auto test_rows = [&](const uint64_t* F, int n1, int nb2, float cutoff, uint64_t& ka, bool& ca,
+                       uint64_t& kb, bool& cb) {
+    float da = FLT_MAX, db = FLT_MAX;
+    ka = kb = 0;
+    if (lane < n1) {
+      da = L.DL[lane];
+      ka = make_key(da, L.NL[lane]);
     }
+    if (lane < nb2) {
+      db = L.DL[n1 + lane];
+      kb = make_key(db, L.NL[n1 + lane]);
+    }
+    const bool ua = lane < n1 && !(da >= cutoff), ub = lane < nb2 && !(db >= cutoff);
+    ca = cb = false;
+    if (ballot(ua || ub) != 0) {
+      bool ha, hb;
+      sorted_contains2(F, f, ka, kb, ha, hb);
+      ca = ua && !ha;
+      cb = ub && !hb;
+    }
+  };
+  // commit_rows: a tested list's candidates join C (in lane order) unless
+  // already pending; every row of the list (valid) enters the evaluated-id cache
+  auto commit_rows = [&](uint64_t key, bool cand, bool valid) {
+    const bool ins = cand && lds_insert(L.H, hlog2, key_id(key));
+    if (rcl && valid) L.RC[rc_slot(key_id(key), rcl)] = key_id(key);  // evaluated and committed
     const uint64_t mi = ballot(ins);
     if (ins) L.C[c + (int)mbcnt(mi)] = key;
     c += __popcll(mi);
@@ -724,7 +758,7 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       }
     } else {
       // Every neighbour is evaluated; the seen test happens after the
-      // distance (accept_range).  Which ids the reference's seen set would
+      // distance (test_rows, commit_rows).  Which ids the reference's seen set would
       // skip, and this kernel evaluates:
       //   * ids rejected earlier: the cutoff never rises once the frontier is
       //     full (and nothing is rejected before), so they are rejected again;
@@ -732,8 +766,8 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       //     leaves the frontier full of B smaller keys, so they now fail the
       //     cutoff too;
       //   * ids in the frontier: skipped before their gather when the
-      //     frontier-id filter is on, else found by accept_range;
-      //   * ids pending in C: found by accept_range.
+      //     frontier-id filter is on, else found by test_rows;
+      //   * ids pending in C: found by commit_rows.
       // Candidates, merges and visits are therefore the reference's; only the
       // count of distance evaluations exceeds its dist_cmps.
       //
@@ -781,8 +815,11 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       wave_lds_sync();
       eval_n<ST, MT, NCH, SEQ, CO>(D, a, L, W, n1 + n2, false, lane);
       wave_lds_sync();
+      uint64_t ka, kb;
+      bool ca, cb;
+      test_rows(F, n1, n2, cutoff, ka, ca, kb, cb);
       const int c0 = c;
-      accept_range(F, 0, n1, cutoff);
+      commit_rows(ka, ca, lane < n1);
       acc_last = c - c0;
       cmps += (uint32_t)n1;
       wave_lds_sync();
@@ -794,7 +831,7 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
         if (lane == 0) FV[nu] = 1;
         record_visit(key2);
         const int c1 = c;
-        accept_range(F, n1, n1 + n2, cutoff);
+        commit_rows(kb, cb, lane < n2);
         acc_last = c - c1;
         cmps += (uint32_t)n2;
         wave_lds_sync();
@@ -933,16 +970,20 @@ __device__ __forceinline__ void coop_helper(const SearchArgs& a, const WaveLds& 
   }
 }
 
-// Inner-product kernels that land just past 128 VGPRs are held to 128 (4 waves per
-// SIMD, what their LDS allows at the beams they serve): FP8 rows of one or two lines
-// (left alone, the 8-lane rerank takes them to 129-132) and BF16 rows of two lines
-// (96-128 dimensions), whose count moves around 128 with small changes elsewhere.
+// Kernels that land just past 128 VGPRs are held to 128 (4 waves per SIMD, what their
+// LDS allows at the beams they serve): FP8 inner-product rows of one or two lines (the
+// 8-lane rerank takes them to 129-132), BF16 inner-product rows of two lines (96-128
+// dimensions), one-line 8-bit rows and three-line FP32 rows, whose counts move
+// around 128 with small changes elsewhere in the search.
 // CO: workgroups of 2 or 4 waves per query (SearchArgs::wpq); otherwise one.
 template <int ST, int MT, int NCH, bool SEQ, bool EX, bool FB, bool CO>
 __global__ __launch_bounds__(CO ? 4 * PANN_WAVE : PANN_WAVE,
-                             (!CO && MT == M_IP && !EX &&
-                              ((ST == ST_FP8 && NCH == 1) || (ST == ST_BF16 && NCH == 2) ||
-                               (ST == ST_FP8 && NCH == 2 && !FB))) ? 4 : 1)
+                             (!CO && !EX &&
+                              ((MT == M_IP && ST == ST_FP8 && NCH == 1) ||
+                               (MT == M_IP && ST == ST_FP8 && NCH == 2 && !FB) ||
+                               (MT == M_IP && ST == ST_BF16 && NCH == 2) ||
+                               ((ST == ST_I8 || ST == ST_U8) && NCH == 1 && !FB) ||
+                               (ST == ST_F32 && !SEQ && NCH == 3))) ? 4 : 1)
 void k_beam_search(SearchArgs a) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   // (the one-wave kernels keep threadIdx.x itself: its known range [0, 64)
