@@ -656,8 +656,10 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       kb = make_key(db, L.NL[n1 + lane]);
     }
     const bool ua = lane < n1 && !(da >= cutoff), ub = lane < nb2 && !(db >= cutoff);
-    ca = cb = false;
-    if (ballot(ua || ub) != 0) {
+    ca = ua;
+    cb = ub;
+    // with the frontier-id filter on, ids in F were not gathered: none to find
+    if (!fbl && ballot(ua || ub) != 0) {
       bool ha, hb;
       sorted_contains2(F, f, ka, kb, ha, hb);
       ca = ua && !ha;
