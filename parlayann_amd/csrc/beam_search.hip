@@ -38,7 +38,9 @@ __device__ __forceinline__ uint32_t wave_bin_add(uint32_t* bins, uint32_t bin, b
 }
 
 // bins[0 .. nbins) to exclusive prefix sums, in one workgroup of 1024
-// threads, each summing a contiguous run of bins
+// threads: each thread sums a contiguous run of bins, the first wave scans the
+// 1024 run sums (16 per lane, then across its lanes), and each thread writes
+// its run's prefixes
 __device__ __forceinline__ void order_scan(uint32_t* bins, uint32_t nbins, uint32_t* part) {
   const uint32_t t = threadIdx.x;
   const uint32_t per = (nbins + 1023u) / 1024u;
@@ -48,13 +50,29 @@ __device__ __forceinline__ void order_scan(uint32_t* bins, uint32_t nbins, uint3
   for (uint32_t i = lo; i < hi; i++) s += bins[i];
   part[t] = s;
   __syncthreads();
-  for (uint32_t o = 1; o < 1024u; o <<= 1) {  // inclusive scan of the runs' sums
-    const uint32_t v = t >= o ? part[t - o] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  if (t < PANN_WAVE) {
+    uint32_t v[16];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      v[i] = part[t * 16u + i];
+      tot += v[i];
+    }
+    uint32_t inc = tot;  // inclusive scan of the lanes' totals
+#pragma unroll
+    for (int o = 1; o < PANN_WAVE; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o);
+      if ((int)t >= o) inc += y;
+    }
+    uint32_t run = inc - tot;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      part[t * 16u + i] = run;
+      run += v[i];
+    }
   }
-  uint32_t base = part[t] - s;
+  __syncthreads();
+  uint32_t base = part[t];
   for (uint32_t i = lo; i < hi; i++) {
     const uint32_t c = bins[i];
     bins[i] = base;
@@ -69,9 +87,14 @@ __device__ __forceinline__ void order_scan(uint32_t* bins, uint32_t nbins, uint3
 constexpr uint32_t kOrderLdsBins = 8192;
 constexpr int kOrderPerThread = 16;
 __global__ __launch_bounds__(1024) void k_order_lds(const int32_t* beam_pq, uint32_t nq,
-                                                    uint32_t max_beam, uint32_t* order) {
+                                                    uint32_t max_beam, uint32_t* order,
+                                                    uint32_t* status, uint32_t* counter) {
   __shared__ uint32_t bins[kOrderLdsBins];
   __shared__ uint32_t part[1024];
+  if (threadIdx.x == 0) {  // the search launch that follows starts from zero
+    *status = 0u;
+    *counter = 0u;
+  }
   const uint32_t nbins = max_beam + 1u;
   for (uint32_t i = threadIdx.x; i < nbins; i += 1024u) bins[i] = 0u;
   __syncthreads();
@@ -109,11 +132,12 @@ __global__ __launch_bounds__(1024) void k_order_lds(const int32_t* beam_pq, uint
 }
 
 hipError_t launch_order_by_beam(const int32_t* beam_pq, uint32_t nq, uint32_t max_beam,
-                                uint32_t* order, hipStream_t s) {
-  if (nq == 0) return hipSuccess;
+                                uint32_t* order, uint32_t* status, uint32_t* counter,
+                                hipStream_t s) {
   // (a beam past the bins would not fit a wave's LDS frontier either)
-  if (max_beam + 1u > kOrderLdsBins) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_order_lds, dim3(1), dim3(1024), 0, s, beam_pq, nq, max_beam, order);
+  if (max_beam + 1u > kOrderLdsBins || !status || !counter) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_order_lds, dim3(1), dim3(1024), 0, s, beam_pq, nq, max_beam, order,
+                     status, counter);
   return hipGetLastError();
 }
 

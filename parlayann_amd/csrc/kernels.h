@@ -223,9 +223,11 @@ hipError_t launch_hnsw_cut(uint32_t* ids, float* dists, uint32_t stride, uint32_
                            const uint32_t* node_ids, hipStream_t s);
 // order[0 .. nq): the queries by descending beam_pq (beams clamped to
 // [0, max_beam], max_beam < 8192), a counting sort in one workgroup; the
-// order inside one beam value is unspecified
+// order inside one beam value is unspecified.  The same launch clears the
+// search launch's status word and work counter (both non-null).
 hipError_t launch_order_by_beam(const int32_t* beam_pq, uint32_t nq, uint32_t max_beam,
-                                uint32_t* order, hipStream_t s);
+                                uint32_t* order, uint32_t* status, uint32_t* counter,
+                                hipStream_t s);
 // acc[0] += v[0] + .. + v[n - 1]
 hipError_t launch_sum_u32(const uint32_t* v, uint32_t n, unsigned long long* acc, hipStream_t s);
 hipError_t launch_update_neighbors(uint32_t* nbrs, uint32_t nbr_stride, uint32_t max_degree,

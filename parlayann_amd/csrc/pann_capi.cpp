@@ -290,10 +290,16 @@ __global__ void k_zero_launch(uint32_t* status, uint32_t* counter) {
 // A batch with per-query beams runs longest-first: a.qorder = the queries by
 // descending beam (clamped to max_beam), sorted on the device.  The persistent
 // grid takes queries in that order, so the long searches do not start last
-// and set the batch's end.  Results do not depend on the order.
+// and set the batch's end.  Results do not depend on the order.  The sort
+// clears the launch's status word and work counter (prepare_search was told
+// to leave them); past its bins they are cleared on their own.
 pann_status order_by_beam(StreamCtx& C, const int32_t* beam_pq, uint32_t nq, int64_t max_beam,
                           pann::SearchArgs& a, hipStream_t stream) {
-  if (max_beam >= 8192) return PANN_OK;  // past the sort's bins: the order given
+  if (max_beam >= 8192) {  // past the sort's bins: the order given
+    hipLaunchKernelGGL(k_zero_launch, dim3(1), dim3(1), 0, stream, a.status, a.work_counter);
+    HIP_TRY(hipGetLastError(), "status and work counter reset");
+    return PANN_OK;
+  }
   const size_t need = nq;
   if (C.order_cap < need) {
     if (capturing(stream))
@@ -310,7 +316,8 @@ pann_status order_by_beam(StreamCtx& C, const int32_t* beam_pq, uint32_t nq, int
     HIP_TRY(dev_malloc(&C.d_order, cap * 4), "hipMalloc(query order)");
     C.order_cap = cap;
   }
-  HIP_TRY(pann::launch_order_by_beam(beam_pq, nq, (uint32_t)max_beam, C.d_order, stream),
+  HIP_TRY(pann::launch_order_by_beam(beam_pq, nq, (uint32_t)max_beam, C.d_order, a.status,
+                                     a.work_counter, stream),
           "query order launch");
   a.qorder = C.d_order;
   return PANN_OK;
@@ -679,7 +686,7 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
                            const pann_search_opts* opts, uint32_t nq, pann::SearchArgs& a,
                            int& store_kind, bool& sequential, uint32_t& grid,
                            hipStream_t stream, StreamCtx*& C, uint32_t passes = 1,
-                           const LaunchMem* fixed = nullptr) {
+                           const LaunchMem* fixed = nullptr, bool zero = true) {
   C = nullptr;
   if (!fixed) {
     pann_status cst = stream_ctx(idx, stream, &C);
@@ -901,8 +908,10 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     C->starts_host.assign(starts, starts + nstarts);
   }
-  hipLaunchKernelGGL(k_zero_launch, dim3(1), dim3(1), 0, stream, a.status, a.work_counter);
-  HIP_TRY(hipGetLastError(), "status and work counter reset");
+  if (zero) {  // else the caller's order_by_beam clears them
+    hipLaunchKernelGGL(k_zero_launch, dim3(1), dim3(1), 0, stream, a.status, a.work_counter);
+    HIP_TRY(hipGetLastError(), "status and work counter reset");
+  }
   return PANN_OK;
 }
 
@@ -990,8 +999,9 @@ pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
   bool sequential;
   uint32_t grid;
   StreamCtx* C = nullptr;
+  const bool will_order = beam_pq && !(opts && opts->query_order);  // order_by_beam below
   pann_status st = prepare_search(idx, 0, qp, starts, nstarts, nullptr, opts, nq, a, store_kind,
-                                  sequential, grid, stream, C);
+                                  sequential, grid, stream, C, 1, nullptr, !will_order);
   if (st != PANN_OK) return st;
   const bool f32_queries = idx->dtype == PANN_DT_F32;
   a.queries = (const char*)queries;
@@ -1018,7 +1028,7 @@ pann_status search_impl(pann_index_t idx, const void* queries, uint32_t nq,
   }
   st = stage_queries(idx, *C, opts ? opts->store : PANN_STORE_NATIVE, queries, nq, a, stream);
   if (st != PANN_OK) return st;
-  if (beam_pq && !(opts && opts->query_order)) {
+  if (will_order) {
     st = order_by_beam(*C, beam_pq, nq, qp->beam_size, a, stream);
     if (st != PANN_OK) return st;
   }
@@ -1690,7 +1700,8 @@ static pann_status hnsw_search_once(pann_index_t idx, const void* queries, uint3
     uint32_t grid;
     StreamCtx* Cl = nullptr;
     pann_status st = prepare_search(idx, 0, &qp, entrance, nentrance, nullptr, &o, nq, a,
-                                    store_kind, sequential, grid, stream, Cl, top_level + 1);
+                                    store_kind, sequential, grid, stream, Cl, top_level + 1,
+                                    nullptr, !(ef_pq && !o.query_order));
     if (st != PANN_OK) return st;
     a.queries = (const char*)queries;
     a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
@@ -1751,7 +1762,8 @@ static pann_status hnsw_search_once(pann_index_t idx, const void* queries, uint3
     StreamCtx* Cl = nullptr;
     pann_status st = prepare_search(idx, (uint32_t)l, &qp, entrance, nentrance,
                                     first ? nullptr : &ss, &lo, nq, a, store_kind, sequential,
-                                    grid, stream, Cl);
+                                    grid, stream, Cl, 1, nullptr,
+                                    !(last && ef_pq && !o.query_order));
     if (st != PANN_OK) return st;
     a.queries = (const char*)queries;
     a.q_stride = (uint64_t)idx->dim * (f32_queries ? 4 : 1);
