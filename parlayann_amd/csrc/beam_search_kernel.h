@@ -6,9 +6,12 @@
 //   * one visit per step: the closest unvisited entry of the (possibly stale)
 //     frontier (beamSearch.h:116-127);
 //   * neighbours G[cur][0 .. min(deg, degree_limit)) not seen before and not
-//     the query's own row are distance-compared (:151-160, :176-182); the seen
-//     set is exact (LDS hash with a per-wave global overflow table), like the
-//     reference's hashset before it ever grows (hashset.h:12-55);
+//     the query's own row are distance-compared (:151-160, :176-182).  With
+//     exact counts the seen set is the reference's (an LDS hash with a per-wave
+//     global overflow table that grows as hashset.h:12-55 does); by default
+//     every neighbour is evaluated and the seen test against the frontier and
+//     the pending candidates follows the distance, which leaves candidates,
+//     merges and visits the reference's (the main loop says why);
 //   * candidates with d >= cutoff (last frontier distance when the frontier is
 //     full) are dropped (:173-181);
 //   * the merge is deferred while |candidates| < batch_factor * beam and more
@@ -229,21 +232,6 @@ __device__ __forceinline__ uint32_t fb_h2(uint32_t id, uint32_t log2n) {
 __device__ __forceinline__ bool fb_test(const uint32_t* FB, uint32_t log2n, uint32_t id) {
   const uint32_t a = fb_h1(id, log2n), b = fb_h2(id, log2n);
   return ((FB[a >> 5] >> (a & 31u)) & (FB[b >> 5] >> (b & 31u)) & 1u) != 0u;
-}
-
-// whether sorted A[0..n) holds x
-__device__ __forceinline__ bool sorted_contains(const uint64_t* A, int n, uint64_t x) {
-  int lo = 0, len = n;
-  while (len > 0) {
-    const int half = len >> 1;
-    if (A[lo + half] < x) {
-      lo += half + 1;
-      len -= half + 1;
-    } else {
-      len = half;
-    }
-  }
-  return lo < n && A[lo] == x;
 }
 
 // Whether sorted A[0..n) holds xa, and xb (n >= 1, the same in every lane):
