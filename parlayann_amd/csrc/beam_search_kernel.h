@@ -887,9 +887,48 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) L.QL[j] = fq[j];
       wave_lds_sync();
     }
+    const uint64_t* S = F;  // the entries re-scored: F[0..nchk), or those kept in R
+    if constexpr (ST == ST_FP8 && MT == M_IP) {
+      if (a.row_bound && nchk > k) {
+        // Entries that cannot reach the first k keep their FP32 rows unread:
+        // |D8 - D32| <= |q| b_x (row_bound), so an entry with D8 - |q| b_x above
+        // U = max over the first k of D8 + |q| b_x has D32 above k smaller D32s.
+        // Each sum carries a margin of 2^-22 of its magnitudes, which covers
+        // its float roundings; |q| is raised by 2^-16 for its own.
+        float qs = 0.0f;
+        for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) qs += L.QL[j] * L.QL[j];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) qs += __shfl_xor(qs, o);
+        const float qn = sqrtf(qs) * (1.0f + 0x1p-16f);
+        float U = -FLT_MAX;
+        for (int i = lane; i < k; i += PANN_WAVE) {
+          const float d = key_dist(F[i]), e = qn * a.row_bound[key_id(F[i])];
+          U = fmaxf(U, d + e + 0x1p-22f * (fabsf(d) + e));
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) U = fmaxf(U, __shfl_xor(U, o));
+        int nk = 0;
+        for (int t = 0; t < nchk; t += PANN_WAVE) {
+          const int i = t + lane;
+          bool keep = false;
+          uint64_t key = 0;
+          if (i < nchk) {
+            key = F[i];
+            const float d = key_dist(key), e = qn * a.row_bound[key_id(key)];
+            keep = i < k || !(d - e - 0x1p-22f * (fabsf(d) + e) > U);
+          }
+          const uint64_t mk = ballot(keep);
+          if (keep) R[nk + (int)mbcnt(mk)] = key;
+          nk += __popcll(mk);
+        }
+        wave_lds_sync();
+        nchk = nk;
+        S = R;
+      }
+    }
     for (int t = 0; t < nchk; t += PANN_WAVE) {
       int m = nchk - t < PANN_WAVE ? nchk - t : PANN_WAVE;
-      if (lane < m) L.NL[lane] = key_id(F[t + lane]);
+      if (lane < m) L.NL[lane] = key_id(S[t + lane]);
       wave_lds_sync();
       eval_n<ST, MT, NCH, SEQ, CO>(D, a, L, W, m, true, lane);
       wave_lds_sync();

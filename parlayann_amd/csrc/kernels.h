@@ -30,6 +30,10 @@ struct SearchArgs {
   // for one-chunk rows of 1-byte elements of at most 96 bytes
   uint32_t lane_words;
   const float* scales;  // FP8 per-dimension scales (nullptr otherwise)
+  // FP8 inner-product rows with the FP32 rerank: per row, b with |FP8 distance -
+  // FP32 distance| <= |q| b for every query q (launch_fp8_row_bound); the rerank
+  // then skips rows that cannot reach the first k.  nullptr: every row is re-scored
+  const float* row_bound;
   // FP8 residual store: q . c for every query and centroid (nq x ncent,
   // computed before the launch); each row's last 4 bytes name its centroid
   const float* qtable;
@@ -145,6 +149,11 @@ hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint3
                             uint32_t* absmax_bits, float* scales, uint8_t* Y, uint64_t stride_b,
                             hipStream_t s, const float* cent = nullptr, uint64_t cstride_f = 0,
                             const uint32_t* assign = nullptr);
+// b[0 .. n): per FP8 inner-product row an upper bound on |FP8 distance - FP32
+// distance| / |q|, over every query q (the store of launch_fp8_store, uncentred)
+hipError_t launch_fp8_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                                const float* scales, const uint8_t* Y, uint64_t stride_b,
+                                float* b, hipStream_t s);
 // brute-force candidates on the matrix cores (bruteforce.hip)
 struct BFArgs {
   const void* X;          // points (f32 / u8 / i8), row stride xs elements

@@ -121,6 +121,9 @@ struct pann_index_s {
   uint64_t fp8_stride = 0;
   float* d_scales = nullptr;
   uint32_t* d_absmax = nullptr;
+  // inner product: per FP8 row, a bound on |FP8 distance - FP32 distance| / |q|
+  // (launch_fp8_row_bound), with which the rerank skips rows that cannot reach k
+  float* d_fp8_bound = nullptr;
   float* d_xnorm = nullptr;        // |x|^2 per point (brute force, L2)
   // FP8 residual store (PANN_STORE_FP8_RESIDUAL): e4m3 codes of x - c with
   // per-dimension scales, c the nearest of ncent sampled points
@@ -645,6 +648,14 @@ pann_status pann_build_store(pann_index_t idx, int32_t store) {
                                    idx->d_absmax, idx->d_scales, (uint8_t*)idx->d_fp8,
                                    idx->fp8_stride, nullptr),
             "fp8 store kernel");
+    if (idx->metric == PANN_METRIC_MIPS) {
+      if (!idx->d_fp8_bound)
+        HIP_TRY(dev_malloc(&idx->d_fp8_bound, (size_t)idx->n * 4), "hipMalloc(fp8 bound)");
+      HIP_TRY(pann::launch_fp8_row_bound((const float*)idx->d_pts, fstride, idx->n, idx->dim,
+                                         idx->d_scales, (const uint8_t*)idx->d_fp8,
+                                         idx->fp8_stride, idx->d_fp8_bound, nullptr),
+              "fp8 row bound kernel");
+    }
   }
   HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
   return PANN_OK;
@@ -773,6 +784,8 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   a.ncent = store_kind == pann::ST_FP8R ? idx->ncent : 0u;
   a.full = o.rerank ? idx->d_pts : nullptr;
   a.full_stride = idx->pts_stride;
+  a.row_bound = (o.rerank && store_kind == pann::ST_FP8 && idx->metric == PANN_METRIC_MIPS)
+                    ? idx->d_fp8_bound : nullptr;
   a.nbrs = G.d_nbrs;
   a.nbr_stride = G.nbr_stride;
   a.max_degree = G.max_degree;
@@ -2031,7 +2044,7 @@ pann_status pann_close(pann_index_t idx) {
   void* ptrs[] = {idx->d_pts,    idx->d_bf16,    idx->d_fp8,   idx->d_scales, idx->d_absmax,
                   idx->d_nbrs,   idx->d_q8,      idx->d_xnorm, idx->d_dummy,  idx->d_pkeys,
                   idx->d_pflags, idx->d_fp8r,    idx->d_rscales, idx->d_rabsmax, idx->d_cent,
-                  idx->d_assign, idx->d_evals};
+                  idx->d_assign, idx->d_evals, idx->d_fp8_bound};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (size_t l = 1; l < idx->levels.size(); l++)

@@ -136,6 +136,49 @@ hipError_t launch_bf16_store(const float* X, uint64_t stride_f, uint32_t n, uint
   return hipGetLastError();
 }
 
+// Per FP8 row, b >= |D8(q, x) - D32(q, x)| / |q| for every query q, where D8 is the
+// search's FP8 inner-product distance and D32 the FP32 rerank's (the reference's
+// float loop).  With x^ = s * code (exact reals), r = |x - x^|:
+//   |<q,x^> - <q,x>| <= |q| r;
+//   D8 rounds q_j s_j to FP32 and then to FP16 after a power-of-two scale 2^e (at
+//   most 2^-11 (1 + 2^-12) relative, or 2^-25 2^-e absolute where the FP16 value
+//   is subnormal), multiplies exactly and sums in FP32 (at most gamma_96 of the
+//   sum of magnitudes, in any order): <= |q| |x^| (4.884e-4 + 5.73e-6) plus
+//   96 * 448 * 2^-39 * max|q_j s_j| <= |q| * 7.9e-8 * max_j s_j;
+//   D32's float loop of 96 products: <= gamma_97 |q| |x| <= |q| |x| 5.79e-6.
+// So b = r + 4.95e-4 |x^| + 5.8e-6 |x| + 8e-8 max_j s_j, computed in double and
+// raised by 1e-6 relative before it is rounded to float (dim <= 96 per the
+// gamma terms; longer rows scale them by dim / 96).
+__global__ void k_fp8_row_bound(const float* __restrict__ X, uint64_t stride_f, uint32_t n,
+                                uint32_t dim, const float* __restrict__ scales,
+                                const uint8_t* __restrict__ Y, uint64_t stride_b,
+                                float* __restrict__ b) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double rr = 0.0, xn = 0.0, xh = 0.0, smax = 0.0;
+  for (uint32_t j = 0; j < dim; j++) {
+    const double s = (double)scales[j];
+    const double x = (double)X[(uint64_t)r * stride_f + j];
+    const double h = s * (double)fp8e4m3_to_f32(Y[(uint64_t)r * stride_b + j]);
+    rr += (x - h) * (x - h);
+    xn += x * x;
+    xh += h * h;
+    smax = s > smax ? s : smax;
+  }
+  const double g = dim > 96u ? (double)dim / 96.0 : 1.0;
+  const double v = sqrt(rr) + (4.884e-4 + 5.73e-6 * g) * sqrt(xh) + 5.8e-6 * g * sqrt(xn) +
+                   8e-8 * g * smax;
+  b[r] = (float)(v * (1.0 + 1e-6));
+}
+
+hipError_t launch_fp8_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                                const float* scales, const uint8_t* Y, uint64_t stride_b,
+                                float* b, hipStream_t s) {
+  hipLaunchKernelGGL(k_fp8_row_bound, dim3((n + 255) / 256), dim3(256), 0, s, X, stride_f, n,
+                     dim, scales, Y, stride_b, b);
+  return hipGetLastError();
+}
+
 hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                             uint32_t* absmax_bits, float* scales, uint8_t* Y, uint64_t stride_b,
                             hipStream_t s, const float* cent, uint64_t cstride_f,
