@@ -123,27 +123,29 @@ def test_reduced_precision_rerank_at_full_beams(oracle, aniso, beam):
     idx.close()
 
 
-@pytest.mark.parametrize("beam,rf", [(64, 100.0), (256, 100.0), (352, 100.0), (256, 3.0)])
-def test_fp8_rerank_equals_rescored_frontier(oracle, aniso, beam, rf):
+@pytest.mark.parametrize("beam,rf,k", [(64, 100.0, 10), (256, 100.0, 10), (352, 100.0, 10),
+                                       (256, 3.0, 10), (128, 100.0, 1), (256, 2.0, 100),
+                                       (352, 1.5, 100)])
+def test_fp8_rerank_equals_rescored_frontier(oracle, aniso, beam, rf, k):
     """FP8 search + FP32 rerank (beam_search_rerank, beamSearch.h:569-587): the
-    result must be the first 10 by (FP32 distance, id) of the FP8 frontier's
-    first min(10 * rerank_factor, |frontier|) entries, whatever rows the
+    result must be the first k by (FP32 distance, id) of the FP8 frontier's
+    first min(k * rerank_factor, |frontier|) entries, whatever rows the
     kernel's per-row error bounds let it leave unread.  The frontier comes from
     the same search at k = beam; the FP32 distances from the oracle."""
     from parlayann_amd import DeviceIndex
     X, Q, adj = aniso
-    _say(f"[fp8 rerank pruning] beam={beam} rerank_factor={rf}")
+    _say(f"[fp8 rerank pruning] beam={beam} rerank_factor={rf} k={k}")
     idx = DeviceIndex(X, adj, "ip")
     idx.build_store("fp8")
     front = idx.search(Q, beam, beam, store="fp8")
-    r = idx.search(Q, 10, beam, store="fp8", rerank=True, rerank_factor=rf)
-    nchk = min(int(10 * rf), beam)
+    r = idx.search(Q, k, beam, store="fp8", rerank=True, rerank_factor=rf)
+    nchk = min(int(k * rf), beam)
     for i in range(len(Q)):
         ids = front["ids"][i][:nchk].astype(np.int64)
         d = np.array([np.float32(oracle.distance(X[a], Q[i], "ip")) for a in ids])
-        order = np.lexsort((ids, d))[:10]
-        assert np.array_equal(r["ids"][i].astype(np.int64), ids[order]), (beam, i)
-        assert np.array_equal(r["dists"][i], d[order]), (beam, i)
+        order = np.lexsort((ids, d))[:k]
+        assert np.array_equal(r["ids"][i].astype(np.int64), ids[order]), (beam, k, i)
+        assert np.array_equal(r["dists"][i], d[order]), (beam, k, i)
     idx.close()
 
 
