@@ -888,21 +888,48 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       wave_lds_sync();
     }
     const uint64_t* S = F;  // the entries re-scored: F[0..nchk), or those kept in R
-    if constexpr (ST == ST_FP8 && MT == M_IP) {
+    if constexpr (MT == M_IP && (ST == ST_FP8 || ST == ST_I8)) {
       if (a.row_bound && nchk > k) {
-        // Entries that cannot reach the first k keep their FP32 rows unread:
-        // |D8 - D32| <= |q| b_x (row_bound), so an entry with D8 - |q| b_x above
-        // U = max over the first k of D8 + |q| b_x has D32 above k smaller D32s.
-        // Each sum carries a margin of 2^-22 of its magnitudes, which covers
-        // its float roundings; |q| is raised by 2^-16 for its own.
-        float qs = 0.0f;
-        for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) qs += L.QL[j] * L.QL[j];
+        // Entries that cannot reach the first k keep their FP32 rows unread.
+        // An entry's search distance d and its FP32 distance D32 satisfy
+        // |d - c D32| <= e (row_bound), c > 0 the same for every row:
+        //   FP8: c = 1, e = |q| b_x;
+        //   the reference's 8-bit codes: d = -<code(q), code(x)> exactly,
+        //   c = s^2, e = |s q| R_x + |s q - code(q)| N_x.
+        // So an entry with d - e above U = max over the first k of d + e has
+        // D32 above k smaller D32s.  Each sum carries a margin of 2^-22 of its
+        // magnitudes, which covers its float roundings; the query norms are
+        // raised by 2^-16 for their own, |s q - code(q)| also by 2^-20 |s q|
+        // for the rounding of s q.
+        float qa = 0.0f, qb = 0.0f;
+        if constexpr (ST == ST_FP8) {
+          for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) qa += L.QL[j] * L.QL[j];
+        } else {
+          const int8_t* cq = (const int8_t*)qrow;
+          for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) {
+            const float v = a.q8_scale * L.QL[j], w = v - (float)cq[j];
+            qa += v * v;
+            qb += w * w;
+          }
+        }
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) qs += __shfl_xor(qs, o);
-        const float qn = sqrtf(qs) * (1.0f + 0x1p-16f);
+        for (int o = 32; o >= 1; o >>= 1) {
+          qa += __shfl_xor(qa, o);
+          if constexpr (ST == ST_I8) qb += __shfl_xor(qb, o);
+        }
+        qa = sqrtf(qa) * (1.0f + 0x1p-16f);
+        qb = (sqrtf(qb) + 0x1p-20f * qa) * (1.0f + 0x1p-16f);
+        auto bound_of = [&](uint32_t id) -> float {
+          if constexpr (ST == ST_FP8) {
+            return qa * a.row_bound[id];
+          } else {
+            const float2 rn = *(const float2*)(a.row_bound + 2 * (uint64_t)id);
+            return qa * rn.x + qb * rn.y;
+          }
+        };
         float U = -FLT_MAX;
         for (int i = lane; i < k; i += PANN_WAVE) {
-          const float d = key_dist(F[i]), e = qn * a.row_bound[key_id(F[i])];
+          const float d = key_dist(F[i]), e = bound_of(key_id(F[i]));
           U = fmaxf(U, d + e + 0x1p-22f * (fabsf(d) + e));
         }
 #pragma unroll
@@ -914,7 +941,7 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
           uint64_t key = 0;
           if (i < nchk) {
             key = F[i];
-            const float d = key_dist(key), e = qn * a.row_bound[key_id(key)];
+            const float d = key_dist(key), e = bound_of(key_id(key));
             keep = i < k || !(d - e - 0x1p-22f * (fabsf(d) + e) > U);
           }
           const uint64_t mk = ballot(keep);

@@ -30,10 +30,16 @@ struct SearchArgs {
   // for one-chunk rows of 1-byte elements of at most 96 bytes
   uint32_t lane_words;
   const float* scales;  // FP8 per-dimension scales (nullptr otherwise)
-  // FP8 inner-product rows with the FP32 rerank: per row, b with |FP8 distance -
-  // FP32 distance| <= |q| b for every query q (launch_fp8_row_bound); the rerank
-  // then skips rows that cannot reach the first k.  nullptr: every row is re-scored
+  // Inner-product rows with the FP32 rerank, per-row error bounds with which the
+  // rerank skips rows that cannot reach the first k (nullptr: every row is
+  // re-scored):
+  //  * FP8: one float b per row, |FP8 distance - FP32 distance| <= |q| b for
+  //    every query q (launch_fp8_row_bound);
+  //  * the reference's 8-bit codes: two floats (R, N) per row,
+  //    |code distance - s^2 FP32 distance| <= |s q| R + |s q - code(q)| N for
+  //    every query q, s = q8_scale (launch_q8_row_bound)
   const float* row_bound;
+  float q8_scale;
   // FP8 residual store: q . c for every query and centroid (nq x ncent,
   // computed before the launch); each row's last 4 bytes name its centroid
   const float* qtable;
@@ -154,6 +160,12 @@ hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint3
 hipError_t launch_fp8_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                                 const float* scales, const uint8_t* Y, uint64_t stride_b,
                                 float* b, hipStream_t s);
+// b[0 .. 2n): per row of the reference's 8-bit inner-product codes Y (scale s),
+// R = s (|x - Y/s| + gamma |x|) and N = |Y|: |-<code(q), Y> - s^2 D32(q, x)| <=
+// |s q| R + |s q - code(q)| N for every query q
+hipError_t launch_q8_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                               float scale, const int8_t* Y, uint64_t stride_b, float* b,
+                               hipStream_t s);
 // brute-force candidates on the matrix cores (bruteforce.hip)
 struct BFArgs {
   const void* X;          // points (f32 / u8 / i8), row stride xs elements

@@ -124,6 +124,8 @@ struct pann_index_s {
   // inner product: per FP8 row, a bound on |FP8 distance - FP32 distance| / |q|
   // (launch_fp8_row_bound), with which the rerank skips rows that cannot reach k
   float* d_fp8_bound = nullptr;
+  // inner product: per Q8 row, (R, N) of launch_q8_row_bound, for the same purpose
+  float* d_q8_bound = nullptr;
   float* d_xnorm = nullptr;        // |x|^2 per point (brute force, L2)
   // FP8 residual store (PANN_STORE_FP8_RESIDUAL): e4m3 codes of x - c with
   // per-dimension scales, c the nearest of ncent sampled points
@@ -538,6 +540,14 @@ pann_status build_q8(pann_index_t idx, uint64_t fstride) {
                                  idx->metric == PANN_METRIC_L2 ? 0 : 1, slope, offset, max_v,
                                  scale, (uint8_t*)idx->d_q8, idx->q8_stride, nullptr),
           "Q8 encode");
+  if (idx->metric == PANN_METRIC_MIPS) {
+    if (!idx->d_q8_bound)
+      HIP_TRY(dev_malloc(&idx->d_q8_bound, (size_t)n * 8), "hipMalloc(q8 bound)");
+    HIP_TRY(pann::launch_q8_row_bound((const float*)idx->d_pts, fstride, n, idx->dim, scale,
+                                      (const int8_t*)idx->d_q8, idx->q8_stride,
+                                      idx->d_q8_bound, nullptr),
+            "Q8 row bound kernel");
+  }
   return PANN_OK;
 }
 
@@ -784,8 +794,11 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   a.ncent = store_kind == pann::ST_FP8R ? idx->ncent : 0u;
   a.full = o.rerank ? idx->d_pts : nullptr;
   a.full_stride = idx->pts_stride;
-  a.row_bound = (o.rerank && store_kind == pann::ST_FP8 && idx->metric == PANN_METRIC_MIPS)
-                    ? idx->d_fp8_bound : nullptr;
+  a.row_bound = !o.rerank || idx->metric != PANN_METRIC_MIPS ? nullptr
+                : store_kind == pann::ST_FP8                    ? idx->d_fp8_bound
+                : o.store == PANN_STORE_Q8                      ? idx->d_q8_bound
+                                                                : nullptr;
+  a.q8_scale = idx->q8_scale;
   a.nbrs = G.d_nbrs;
   a.nbr_stride = G.nbr_stride;
   a.max_degree = G.max_degree;
@@ -2044,7 +2057,7 @@ pann_status pann_close(pann_index_t idx) {
   void* ptrs[] = {idx->d_pts,    idx->d_bf16,    idx->d_fp8,   idx->d_scales, idx->d_absmax,
                   idx->d_nbrs,   idx->d_q8,      idx->d_xnorm, idx->d_dummy,  idx->d_pkeys,
                   idx->d_pflags, idx->d_fp8r,    idx->d_rscales, idx->d_rabsmax, idx->d_cent,
-                  idx->d_assign, idx->d_evals, idx->d_fp8_bound};
+                  idx->d_assign, idx->d_evals, idx->d_fp8_bound, idx->d_q8_bound};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (size_t l = 1; l < idx->levels.size(); l++)

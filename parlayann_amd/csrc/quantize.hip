@@ -171,6 +171,42 @@ __global__ void k_fp8_row_bound(const float* __restrict__ X, uint64_t stride_f, 
   b[r] = (float)(v * (1.0 + 1e-6));
 }
 
+// The reference's 8-bit inner-product codes (scale s, codes clipped to
+// [-127, 127]) give the search the distance -<cq, cx>, an exact integer.  With
+// x^ = cx / s and q^ = cq / s (exact reals):
+//   <q,x> - <q^,x^> = <q, x - x^> + <q - q^, x^>,
+// so |<cq,cx> - s^2 <q,x>| <= |s q| s |x - x^| + |s q - cq| |cx|.  The rerank's
+// float loop of dim products (the reference's) differs from -<q,x> by at most
+// gamma_(dim+1) |q| |x| <= 5.8e-6 g |q| |x|, g = max(1, dim / 96).  Per row:
+// R = s (|x - x^| + 5.8e-6 g |x|) and N = |cx|, in double, raised by 1e-6
+// relative before they are rounded to float.
+__global__ void k_q8_row_bound(const float* __restrict__ X, uint64_t stride_f, uint32_t n,
+                               uint32_t dim, float scale, const int8_t* __restrict__ Y,
+                               uint64_t stride_b, float* __restrict__ b) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double s = (double)scale;
+  double rr = 0.0, xn = 0.0, cn = 0.0;
+  for (uint32_t j = 0; j < dim; j++) {
+    const double x = (double)X[(uint64_t)r * stride_f + j];
+    const double c = (double)Y[(uint64_t)r * stride_b + j];
+    rr += (x - c / s) * (x - c / s);
+    xn += x * x;
+    cn += c * c;
+  }
+  const double g = dim > 96u ? (double)dim / 96.0 : 1.0;
+  b[2 * (uint64_t)r] = (float)(s * (sqrt(rr) + 5.8e-6 * g * sqrt(xn)) * (1.0 + 1e-6));
+  b[2 * (uint64_t)r + 1] = (float)(sqrt(cn) * (1.0 + 1e-6));
+}
+
+hipError_t launch_q8_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                               float scale, const int8_t* Y, uint64_t stride_b, float* b,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_q8_row_bound, dim3((n + 255) / 256), dim3(256), 0, s, X, stride_f, n,
+                     dim, scale, Y, stride_b, b);
+  return hipGetLastError();
+}
+
 hipError_t launch_fp8_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                                 const float* scales, const uint8_t* Y, uint64_t stride_b,
                                 float* b, hipStream_t s) {
