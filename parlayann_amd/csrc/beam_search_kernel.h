@@ -888,23 +888,30 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       wave_lds_sync();
     }
     const uint64_t* S = F;  // the entries re-scored: F[0..nchk), or those kept in R
-    if constexpr (MT == M_IP && (ST == ST_FP8 || ST == ST_I8)) {
+    if constexpr (ST == ST_FP8 || (ST == ST_I8 && MT == M_IP)) {
       if (a.row_bound && nchk > k) {
         // Entries that cannot reach the first k keep their FP32 rows unread.
-        // An entry's search distance d and its FP32 distance D32 satisfy
-        // |d - c D32| <= e (row_bound), c > 0 the same for every row:
-        //   FP8: c = 1, e = |q| b_x;
-        //   the reference's 8-bit codes: d = -<code(q), code(x)> exactly,
-        //   c = s^2, e = |s q| R_x + |s q - code(q)| N_x.
-        // So an entry with d - e above U = max over the first k of d + e has
-        // D32 above k smaller D32s.  Each sum carries a margin of 2^-22 of its
-        // magnitudes, which covers its float roundings; the query norms are
-        // raised by 2^-16 for their own, |s q - code(q)| also by 2^-20 |s q|
-        // for the rounding of s q.
+        // Each entry gets bounds lo <= g(D32) <= hi from its search distance d
+        // and its row's bound, g the same increasing map for every row:
+        //   FP8, inner product: g = id, |d - D32| <= |q| b_x;
+        //   the reference's 8-bit codes (inner product): d = -<code(q),
+        //   code(x)> exactly, g(D) = s^2 D, |d - g(D32)| <= |s q| R_x +
+        //   |s q - code(q)| N_x;
+        //   FP8, L2: g = sqrt; |sqrt(d) - |q - x^|| <= 2^-16 (sqrt(d) + |x^|)
+        //   (the FP8 sum's roundings), ||q - x^| - |q - x|| <= |x - x^| and
+        //   sqrt(D32) is within eta = (dim + 8) 2^-24 relative of |q - x|,
+        //   with b_x = |x - x^| + 2^-16 |x^|.
+        // An entry whose lo lies above U = max over the first k of hi has
+        // D32 above k smaller D32s.  A NaN among the first k's bounds makes U
+        // infinite, and an entry with an infinite or NaN lo is kept: rows or
+        // queries with non-finite values prune nothing.  The inner-product
+        // sums carry a margin of 2^-22 of their magnitudes, which covers their
+        // float roundings; the query norms are raised by 2^-16 for their own,
+        // |s q - code(q)| also by 2^-20 |s q| for the rounding of s q.
         float qa = 0.0f, qb = 0.0f;
-        if constexpr (ST == ST_FP8) {
+        if constexpr (ST == ST_FP8 && MT == M_IP) {
           for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) qa += L.QL[j] * L.QL[j];
-        } else {
+        } else if constexpr (ST == ST_I8) {
           const int8_t* cq = (const int8_t*)qrow;
           for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) {
             const float v = a.q8_scale * L.QL[j], w = v - (float)cq[j];
@@ -912,25 +919,45 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
             qb += w * w;
           }
         }
+        if constexpr (MT == M_IP) {
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-          qa += __shfl_xor(qa, o);
-          if constexpr (ST == ST_I8) qb += __shfl_xor(qb, o);
+          for (int o = 32; o >= 1; o >>= 1) {
+            qa += __shfl_xor(qa, o);
+            if constexpr (ST == ST_I8) qb += __shfl_xor(qb, o);
+          }
+          qa = sqrtf(qa) * (1.0f + 0x1p-16f);
+          qb = (sqrtf(qb) + 0x1p-20f * qa) * (1.0f + 0x1p-16f);
         }
-        qa = sqrtf(qa) * (1.0f + 0x1p-16f);
-        qb = (sqrtf(qb) + 0x1p-20f * qa) * (1.0f + 0x1p-16f);
-        auto bound_of = [&](uint32_t id) -> float {
-          if constexpr (ST == ST_FP8) {
-            return qa * a.row_bound[id];
+        const float eta = (float)(a.dim + 8u) * 0x1p-24f;
+        auto bounds = [&](uint64_t key, float& lo, float& hi) {
+          const float d = key_dist(key);
+          const uint32_t id = key_id(key);
+          if constexpr (MT == M_L2) {
+            // lo <= (sqrt(d) (1 - 2^-16) - b) (1 - eta) and hi >= (sqrt(d)
+            // (1 + 2^-16) + b) (1 + eta): the factors carry 2^-16 and 2^-20
+            // beyond those, more than the float roundings of sqrt, the
+            // products and the one subtraction can take away
+            const float u = d < 0.0f ? 0.0f : sqrtf(d), b = a.row_bound[id];
+            lo = u * ((1.0f - 0x1p-15f) * (1.0f - 2.0f * eta)) - b * (1.0f + 0x1p-20f);
+            hi = (u * (1.0f + 0x1p-15f) + b * (1.0f + 0x1p-20f)) * (1.0f + 2.0f * eta);
           } else {
-            const float2 rn = *(const float2*)(a.row_bound + 2 * (uint64_t)id);
-            return qa * rn.x + qb * rn.y;
+            float e;
+            if constexpr (ST == ST_FP8) {
+              e = qa * a.row_bound[id];
+            } else {
+              const float2 rn = *(const float2*)(a.row_bound + 2 * (uint64_t)id);
+              e = qa * rn.x + qb * rn.y;
+            }
+            const float m = 0x1p-22f * (fabsf(d) + e);
+            lo = d - e - m;
+            hi = d + e + m;
           }
         };
         float U = -FLT_MAX;
         for (int i = lane; i < k; i += PANN_WAVE) {
-          const float d = key_dist(F[i]), e = bound_of(key_id(F[i]));
-          U = fmaxf(U, d + e + 0x1p-22f * (fabsf(d) + e));
+          float lo, hi;
+          bounds(F[i], lo, hi);
+          U = fmaxf(U, hi == hi ? hi : HUGE_VALF);
         }
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) U = fmaxf(U, __shfl_xor(U, o));
@@ -941,8 +968,9 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
           uint64_t key = 0;
           if (i < nchk) {
             key = F[i];
-            const float d = key_dist(key), e = bound_of(key_id(key));
-            keep = i < k || !(d - e - 0x1p-22f * (fabsf(d) + e) > U);
+            float lo, hi;
+            bounds(key, lo, hi);
+            keep = i < k || !(lo > U && lo < HUGE_VALF);
           }
           const uint64_t mk = ballot(keep);
           if (keep) R[nk + (int)mbcnt(mk)] = key;

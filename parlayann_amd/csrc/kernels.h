@@ -30,11 +30,12 @@ struct SearchArgs {
   // for one-chunk rows of 1-byte elements of at most 96 bytes
   uint32_t lane_words;
   const float* scales;  // FP8 per-dimension scales (nullptr otherwise)
-  // Inner-product rows with the FP32 rerank, per-row error bounds with which the
-  // rerank skips rows that cannot reach the first k (nullptr: every row is
-  // re-scored):
-  //  * FP8: one float b per row, |FP8 distance - FP32 distance| <= |q| b for
-  //    every query q (launch_fp8_row_bound);
+  // Rows with the FP32 rerank, per-row error bounds with which the rerank skips
+  // rows that cannot reach the first k (nullptr: every row is re-scored):
+  //  * FP8, inner product: one float b per row, |FP8 distance - FP32 distance|
+  //    <= |q| b for every query q (launch_fp8_row_bound);
+  //  * FP8, L2: one float b per row, |sqrt(FP8 distance) - |q - x|| <= b +
+  //    2^-16 sqrt(FP8 distance) for every query q (launch_fp8_row_bound);
   //  * the reference's 8-bit codes: two floats (R, N) per row,
   //    |code distance - s^2 FP32 distance| <= |s q| R + |s q - code(q)| N for
   //    every query q, s = q8_scale (launch_q8_row_bound)
@@ -155,11 +156,13 @@ hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint3
                             uint32_t* absmax_bits, float* scales, uint8_t* Y, uint64_t stride_b,
                             hipStream_t s, const float* cent = nullptr, uint64_t cstride_f = 0,
                             const uint32_t* assign = nullptr);
-// b[0 .. n): per FP8 inner-product row an upper bound on |FP8 distance - FP32
-// distance| / |q|, over every query q (the store of launch_fp8_store, uncentred)
+// b[0 .. n): per FP8 row (the store of launch_fp8_store, uncentred), over every
+// query q: inner product (metric 1), an upper bound on |FP8 distance - FP32
+// distance| / |q|; L2 (metric 0), one on |sqrt(FP8 distance) - |q - x|| less
+// 2^-16 sqrt(FP8 distance)
 hipError_t launch_fp8_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                                 const float* scales, const uint8_t* Y, uint64_t stride_b,
-                                float* b, hipStream_t s);
+                                float* b, int metric, hipStream_t s);
 // b[0 .. 2n): per row of the reference's 8-bit inner-product codes Y (scale s),
 // R = s (|x - Y/s| + gamma |x|) and N = |Y|: |-<code(q), Y> - s^2 D32(q, x)| <=
 // |s q| R + |s q - code(q)| N for every query q

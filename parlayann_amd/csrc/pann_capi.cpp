@@ -121,8 +121,8 @@ struct pann_index_s {
   uint64_t fp8_stride = 0;
   float* d_scales = nullptr;
   uint32_t* d_absmax = nullptr;
-  // inner product: per FP8 row, a bound on |FP8 distance - FP32 distance| / |q|
-  // (launch_fp8_row_bound), with which the rerank skips rows that cannot reach k
+  // per FP8 row, a bound on its FP8 distance's error (launch_fp8_row_bound), with
+  // which the rerank skips rows that cannot reach k
   float* d_fp8_bound = nullptr;
   // inner product: per Q8 row, (R, N) of launch_q8_row_bound, for the same purpose
   float* d_q8_bound = nullptr;
@@ -658,14 +658,13 @@ pann_status pann_build_store(pann_index_t idx, int32_t store) {
                                    idx->d_absmax, idx->d_scales, (uint8_t*)idx->d_fp8,
                                    idx->fp8_stride, nullptr),
             "fp8 store kernel");
-    if (idx->metric == PANN_METRIC_MIPS) {
-      if (!idx->d_fp8_bound)
-        HIP_TRY(dev_malloc(&idx->d_fp8_bound, (size_t)idx->n * 4), "hipMalloc(fp8 bound)");
-      HIP_TRY(pann::launch_fp8_row_bound((const float*)idx->d_pts, fstride, idx->n, idx->dim,
-                                         idx->d_scales, (const uint8_t*)idx->d_fp8,
-                                         idx->fp8_stride, idx->d_fp8_bound, nullptr),
-              "fp8 row bound kernel");
-    }
+    if (!idx->d_fp8_bound)
+      HIP_TRY(dev_malloc(&idx->d_fp8_bound, (size_t)idx->n * 4), "hipMalloc(fp8 bound)");
+    HIP_TRY(pann::launch_fp8_row_bound((const float*)idx->d_pts, fstride, idx->n, idx->dim,
+                                       idx->d_scales, (const uint8_t*)idx->d_fp8,
+                                       idx->fp8_stride, idx->d_fp8_bound,
+                                       idx->metric == PANN_METRIC_MIPS ? 1 : 0, nullptr),
+            "fp8 row bound kernel");
   }
   HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
   return PANN_OK;
@@ -794,10 +793,10 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   a.ncent = store_kind == pann::ST_FP8R ? idx->ncent : 0u;
   a.full = o.rerank ? idx->d_pts : nullptr;
   a.full_stride = idx->pts_stride;
-  a.row_bound = !o.rerank || idx->metric != PANN_METRIC_MIPS ? nullptr
-                : store_kind == pann::ST_FP8                    ? idx->d_fp8_bound
-                : o.store == PANN_STORE_Q8                      ? idx->d_q8_bound
-                                                                : nullptr;
+  a.row_bound = !o.rerank                   ? nullptr
+                : store_kind == pann::ST_FP8 ? idx->d_fp8_bound
+                : o.store == PANN_STORE_Q8 && idx->metric == PANN_METRIC_MIPS ? idx->d_q8_bound
+                                                                               : nullptr;
   a.q8_scale = idx->q8_scale;
   a.nbrs = G.d_nbrs;
   a.nbr_stride = G.nbr_stride;

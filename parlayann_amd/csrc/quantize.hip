@@ -149,10 +149,15 @@ hipError_t launch_bf16_store(const float* X, uint64_t stride_f, uint32_t n, uint
 // So b = r + 4.95e-4 |x^| + 5.8e-6 |x| + 8e-8 max_j s_j, computed in double and
 // raised by 1e-6 relative before it is rounded to float (dim <= 96 per the
 // gamma terms; longer rows scale them by dim / 96).
+//
+// L2 (metric 0): b = |x - x^| + 2^-16 |x^|, raised the same way.  The search's
+// sum of (q_j - s_j fp8_j)^2 (the product and the difference rounded to FP32,
+// at most 16 NCH + 8 FMA roundings deep, NCH <= 8) is within 2^-16 (|q - x^| +
+// |x^|) of |q - x^| after the square root, and ||q - x^| - |q - x|| <= |x - x^|.
 __global__ void k_fp8_row_bound(const float* __restrict__ X, uint64_t stride_f, uint32_t n,
                                 uint32_t dim, const float* __restrict__ scales,
                                 const uint8_t* __restrict__ Y, uint64_t stride_b,
-                                float* __restrict__ b) {
+                                float* __restrict__ b, int metric) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   double rr = 0.0, xn = 0.0, xh = 0.0, smax = 0.0;
@@ -166,8 +171,9 @@ __global__ void k_fp8_row_bound(const float* __restrict__ X, uint64_t stride_f, 
     smax = s > smax ? s : smax;
   }
   const double g = dim > 96u ? (double)dim / 96.0 : 1.0;
-  const double v = sqrt(rr) + (4.884e-4 + 5.73e-6 * g) * sqrt(xh) + 5.8e-6 * g * sqrt(xn) +
-                   8e-8 * g * smax;
+  const double v = metric == 0 ? sqrt(rr) + 0x1p-16 * sqrt(xh)
+                               : sqrt(rr) + (4.884e-4 + 5.73e-6 * g) * sqrt(xh) +
+                                     5.8e-6 * g * sqrt(xn) + 8e-8 * g * smax;
   b[r] = (float)(v * (1.0 + 1e-6));
 }
 
@@ -209,9 +215,9 @@ hipError_t launch_q8_row_bound(const float* X, uint64_t stride_f, uint32_t n, ui
 
 hipError_t launch_fp8_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                                 const float* scales, const uint8_t* Y, uint64_t stride_b,
-                                float* b, hipStream_t s) {
+                                float* b, int metric, hipStream_t s) {
   hipLaunchKernelGGL(k_fp8_row_bound, dim3((n + 255) / 256), dim3(256), 0, s, X, stride_f, n,
-                     dim, scales, Y, stride_b, b);
+                     dim, scales, Y, stride_b, b, metric);
   return hipGetLastError();
 }
 

@@ -17,6 +17,14 @@ the terms of the bound large: rows with e4m3-subnormal codes, queries along a
 row's quantization error, queries spanning many binades (FP16-subnormal query
 terms).  The worst case comes within 2 % of the bound, and dropping its
 quantization term or its FP16 query term fails.
+
+For L2 the search's FP8 distance is the FP32 sum of (q_j - s_j x8_j)^2, and
+the library keeps b = |x - x^| + 2^-16 |x^| per row; the kernel's bounds on
+sqrt(D32) are lo = sqrt(d) (1 - 2^-15)(1 - 2 eta) - b (1 + 2^-20) and hi =
+(sqrt(d) (1 + 2^-15) + b (1 + 2^-20))(1 + 2 eta), eta = (dim + 8) 2^-24, in
+float.  The L2 tests emulate that sum (fused and unfused differences, two
+summation orders) against the oracle's L2 loop, with queries on the line
+through a row and its decoded row, where ||q - x^| - |q - x|| = |x - x^|.
 """
 import numpy as np
 
@@ -137,3 +145,95 @@ def test_fp8_row_bound_needs_its_terms(oracle):
     assert _worst_ratio(oracle, X, s, tab, codes, Q, drop="quant") > 1.0
     X, s, tab, codes, Q = _case(16, 19)
     assert _worst_ratio(oracle, X, s, tab, codes, Q, drop="fp16") > 1.0
+
+
+def _d8_l2(q, s, codes, tab, order, fused):
+    """The kernel's FP8 L2 distance: t_j = q_j - s_j x8_j in FP32 (one rounding
+    when fused, two otherwise), t_j^2 summed by FMA."""
+    out = np.empty(len(codes), np.float32)
+    q32 = q.astype(np.float32)
+    for r, c in enumerate(codes):
+        p = tab[c]
+        if fused:
+            t = (q32.astype(np.float64) - s.astype(np.float64) * p).astype(np.float32)
+        else:
+            t = (q32 - (s * p.astype(np.float32)).astype(np.float32)).astype(np.float32)
+        sq = t.astype(np.float64) ** 2
+        if order == "sequential":
+            acc = np.float32(0)
+            for v in sq:
+                acc = np.float32(np.float64(acc) + v)
+        else:  # per-lane FMA chains of 16, then pairwise
+            parts = []
+            for i in range(0, len(sq), 16):
+                acc = np.float32(0)
+                for v in sq[i:i + 16]:
+                    acc = np.float32(np.float64(acc) + v)
+                parts.append(acc)
+            a = np.array(parts, np.float32)
+            while len(a) > 1:
+                if len(a) % 2:
+                    a = np.append(a, np.float32(0))
+                a = (a[0::2] + a[1::2]).astype(np.float32)
+            acc = a[0]
+        out[r] = acc
+    return out
+
+
+def _bound_l2(X, s, codes, tab, drop=False):
+    xh = s.astype(np.float64)[None, :] * tab[codes]
+    r = np.linalg.norm(X.astype(np.float64) - xh, axis=1)
+    v = (0.0 if drop else r) + 2.0 ** -16 * np.linalg.norm(xh, axis=1)
+    return (v * (1.0 + 1e-6)).astype(np.float32)
+
+
+def _l2_case(dim, seed):
+    X, s, tab, codes, _ = _case(dim, seed)
+    rng = np.random.default_rng(seed + 1)
+    xh = s.astype(np.float64)[None, :] * tab[codes]
+    Q = [X[r] + rng.standard_normal(dim).astype(np.float32) * 0.1 for r in range(3)]
+    for r in rng.choice(len(X), 8, replace=False):  # on the line through x^ and x
+        e = X[r].astype(np.float64) - xh[r]
+        for t in (0.0, 1.0, 3.0):
+            Q.append(X[r].astype(np.float64) + t * e)
+    Q.append(np.zeros(dim))
+    Q.append(rng.standard_normal(dim) * 10.0 ** rng.uniform(-3, 1, dim))
+    return X, s, tab, codes, [np.asarray(q, np.float32) for q in Q]
+
+
+def _l2_worst(oracle, X, s, tab, codes, Q, drop=False):
+    """Largest |sqrt(d) - sqrt(D32)| / (b + 2^-16 sqrt(d)); asserts lo <= sqrt(D32) <= hi."""
+    b = _bound_l2(X, s, codes, tab, drop).astype(np.float32)
+    eta = np.float32((X.shape[1] + 8) * 2.0 ** -24)
+    worst = 0.0
+    for q in Q:
+        d32 = np.array([oracle.distance(x, q, "l2") for x in X], np.float64)
+        for order in ("sequential", "pairwise"):
+            for fused in (False, True):
+                d8 = _d8_l2(q, s, codes, tab, order, fused)
+                u = np.sqrt(np.maximum(d8, 0)).astype(np.float32)
+                lo = (u * np.float32((1 - 2.0 ** -15) * (1 - 2 * eta))).astype(np.float32) \
+                    - (b * np.float32(1 + 2.0 ** -20)).astype(np.float32)
+                hi = ((u * np.float32(1 + 2.0 ** -15)).astype(np.float32)
+                      + (b * np.float32(1 + 2.0 ** -20)).astype(np.float32)) * np.float32(1 + 2 * eta)
+                t = np.sqrt(d32)
+                if not drop:
+                    assert np.all(lo.astype(np.float64) <= t) and np.all(t <= hi.astype(np.float64))
+                lim = b.astype(np.float64) + 2.0 ** -16 * u
+                ok = lim > 0
+                worst = max(worst, float(np.max(np.abs(u.astype(np.float64) - t)[ok] / lim[ok])))
+    return worst
+
+
+def test_fp8_l2_row_bound_holds(oracle):
+    for dim in (16, 128, 200):
+        X, s, tab, codes, Q = _l2_case(dim, 5 + dim)
+        worst = _l2_worst(oracle, X, s, tab, codes, Q)
+        assert worst <= 1.0, (dim, worst)
+        assert worst > 0.5, (dim, worst)  # the collinear queries reach the bound
+
+
+def test_fp8_l2_row_bound_needs_its_terms(oracle):
+    """Without |x - x^| the FP8 distances leave the bound."""
+    X, s, tab, codes, Q = _l2_case(128, 71)
+    assert _l2_worst(oracle, X, s, tab, codes, Q, drop=True) > 1.0

@@ -152,6 +152,39 @@ def test_pruned_rerank_equals_rescored_frontier(oracle, aniso, store, beam, rf, 
 
 
 @pytest.fixture(scope="module")
+def f32_l2(oracle):
+    from parlayann_amd import datagen
+    X, Q = datagen.workload_f32_l2(n=20000, nq=300, d=128)
+    _say("[f32 l2] oracle Vamana build n=20000 d=128 R=32 L=64 alpha=1.2")
+    adj = oracle.build_vamana(X, "l2", 32, 64, 1.2, datagen.insertion_orders(len(X), 1, 43))
+    return X, Q, adj
+
+
+@pytest.mark.parametrize("beam,rf,k", [(64, 100.0, 10), (256, 100.0, 10), (1000, 100.0, 10),
+                                       (256, 100.0, 1), (512, 2.0, 100)])
+def test_pruned_l2_rerank_equals_rescored_frontier(oracle, f32_l2, beam, rf, k):
+    """FP8 L2 search + FP32 rerank (beamSearch.h:569-587): as for inner
+    product, the first k by (FP32 distance, id) of the frontier's first
+    min(k * rerank_factor, |frontier|) entries, whatever rows the per-row
+    bounds leave unread."""
+    from parlayann_amd import DeviceIndex
+    X, Q, adj = f32_l2
+    _say(f"[l2 rerank pruning] beam={beam} rerank_factor={rf} k={k}")
+    idx = DeviceIndex(X, adj, "l2")
+    idx.build_store("fp8")
+    front = idx.search(Q, beam, beam, store="fp8")
+    r = idx.search(Q, k, beam, store="fp8", rerank=True, rerank_factor=rf)
+    nchk = min(int(k * rf), beam)
+    for i in range(len(Q)):
+        ids = front["ids"][i][:nchk].astype(np.int64)
+        d = np.array([np.float32(oracle.distance(X[a], Q[i], "l2")) for a in ids])
+        order = np.lexsort((ids, d))[:k]
+        assert np.array_equal(r["ids"][i].astype(np.int64), ids[order]), (beam, k, i)
+        assert np.array_equal(r["dists"][i], d[order]), (beam, k, i)
+    idx.close()
+
+
+@pytest.fixture(scope="module")
 def hnsw_ip(oracle):
     from parlayann_amd import datagen, hnsw
     from parlayann_amd.index import _normalize
