@@ -888,17 +888,17 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
       wave_lds_sync();
     }
     const uint64_t* S = F;  // the entries re-scored: F[0..nchk), or those kept in R
-    if constexpr (ST == ST_FP8 || (ST == ST_I8 && MT == M_IP)) {
+    if constexpr (ST == ST_FP8 || ST == ST_BF16 || (ST == ST_I8 && MT == M_IP)) {
       if (a.row_bound && nchk > k) {
         // Entries that cannot reach the first k keep their FP32 rows unread.
         // Each entry gets bounds lo <= g(D32) <= hi from its search distance d
         // and its row's bound, g the same increasing map for every row:
-        //   FP8, inner product: g = id, |d - D32| <= |q| b_x;
+        //   FP8 or BF16, inner product: g = id, |d - D32| <= |q| b_x;
         //   the reference's 8-bit codes (inner product): d = -<code(q),
         //   code(x)> exactly, g(D) = s^2 D, |d - g(D32)| <= |s q| R_x +
         //   |s q - code(q)| N_x;
-        //   FP8, L2: g = sqrt; |sqrt(d) - |q - x^|| <= 2^-16 (sqrt(d) + |x^|)
-        //   (the FP8 sum's roundings), ||q - x^| - |q - x|| <= |x - x^| and
+        //   FP8 or BF16, L2: g = sqrt; |sqrt(d) - |q - x^|| <= 2^-16 (sqrt(d) + |x^|)
+        //   (the search sum's roundings), ||q - x^| - |q - x|| <= |x - x^| and
         //   sqrt(D32) is within eta = (dim + 8) 2^-24 relative of |q - x|,
         //   with b_x = |x - x^| + 2^-16 |x^|.
         // An entry whose lo lies above U = max over the first k of hi has
@@ -909,7 +909,7 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
         // float roundings; the query norms are raised by 2^-16 for their own,
         // |s q - code(q)| also by 2^-20 |s q| for the rounding of s q.
         float qa = 0.0f, qb = 0.0f;
-        if constexpr (ST == ST_FP8 && MT == M_IP) {
+        if constexpr ((ST == ST_FP8 || ST == ST_BF16) && MT == M_IP) {
           for (uint32_t j = lane; j < a.dim; j += PANN_WAVE) qa += L.QL[j] * L.QL[j];
         } else if constexpr (ST == ST_I8) {
           const int8_t* cq = (const int8_t*)qrow;
@@ -942,7 +942,7 @@ __device__ __forceinline__ PassOut search_one(const SearchArgs& a, const WaveLds
             hi = (u * (1.0f + 0x1p-15f) + b * (1.0f + 0x1p-20f)) * (1.0f + 2.0f * eta);
           } else {
             float e;
-            if constexpr (ST == ST_FP8) {
+            if constexpr (ST == ST_FP8 || ST == ST_BF16) {
               e = qa * a.row_bound[id];
             } else {
               const float2 rn = *(const float2*)(a.row_bound + 2 * (uint64_t)id);

@@ -36,6 +36,7 @@ struct SearchArgs {
   //    <= |q| b for every query q (launch_fp8_row_bound);
   //  * FP8, L2: one float b per row, |sqrt(FP8 distance) - |q - x|| <= b +
   //    2^-16 sqrt(FP8 distance) for every query q (launch_fp8_row_bound);
+  //  * BF16: the same two forms (launch_bf16_row_bound);
   //  * the reference's 8-bit codes: two floats (R, N) per row,
   //    |code distance - s^2 FP32 distance| <= |s q| R + |s q - code(q)| N for
   //    every query q, s = q8_scale (launch_q8_row_bound)
@@ -163,6 +164,11 @@ hipError_t launch_fp8_store(const float* X, uint64_t stride_f, uint32_t n, uint3
 hipError_t launch_fp8_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
                                 const float* scales, const uint8_t* Y, uint64_t stride_b,
                                 float* b, int metric, hipStream_t s);
+// b[0 .. n): per BF16 row Y, the same bounds as launch_fp8_row_bound's for the
+// BF16 distance (inner product: |BF16 distance - FP32 distance| <= |q| b)
+hipError_t launch_bf16_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                                 const uint16_t* Y, uint64_t stride_h, float* b, int metric,
+                                 hipStream_t s);
 // b[0 .. 2n): per row of the reference's 8-bit inner-product codes Y (scale s),
 // R = s (|x - Y/s| + gamma |x|) and N = |Y|: |-<code(q), Y> - s^2 D32(q, x)| <=
 // |s q| R + |s q - code(q)| N for every query q

@@ -124,6 +124,8 @@ struct pann_index_s {
   // per FP8 row, a bound on its FP8 distance's error (launch_fp8_row_bound), with
   // which the rerank skips rows that cannot reach k
   float* d_fp8_bound = nullptr;
+  // per BF16 row, the same (launch_bf16_row_bound)
+  float* d_bf16_bound = nullptr;
   // inner product: per Q8 row, (R, N) of launch_q8_row_bound, for the same purpose
   float* d_q8_bound = nullptr;
   float* d_xnorm = nullptr;        // |x|^2 per point (brute force, L2)
@@ -645,6 +647,13 @@ pann_status pann_build_store(pann_index_t idx, int32_t store) {
     HIP_TRY(pann::launch_bf16_store((const float*)idx->d_pts, fstride, idx->n, idx->dim,
                                     (uint16_t*)idx->d_bf16, idx->bf16_stride / 2, nullptr),
             "bf16 store kernel");
+    if (!idx->d_bf16_bound)
+      HIP_TRY(dev_malloc(&idx->d_bf16_bound, (size_t)idx->n * 4), "hipMalloc(bf16 bound)");
+    HIP_TRY(pann::launch_bf16_row_bound((const float*)idx->d_pts, fstride, idx->n, idx->dim,
+                                        (const uint16_t*)idx->d_bf16, idx->bf16_stride / 2,
+                                        idx->d_bf16_bound,
+                                        idx->metric == PANN_METRIC_MIPS ? 1 : 0, nullptr),
+            "bf16 row bound kernel");
   } else {
     if (!idx->d_fp8) {
       idx->fp8_stride = round_up((uint64_t)idx->dim, 128);
@@ -795,6 +804,7 @@ pann_status prepare_search(pann_index_t idx, uint32_t level, const pann_qparams*
   a.full_stride = idx->pts_stride;
   a.row_bound = !o.rerank                   ? nullptr
                 : store_kind == pann::ST_FP8 ? idx->d_fp8_bound
+                : store_kind == pann::ST_BF16 ? idx->d_bf16_bound
                 : o.store == PANN_STORE_Q8 && idx->metric == PANN_METRIC_MIPS ? idx->d_q8_bound
                                                                                : nullptr;
   a.q8_scale = idx->q8_scale;
@@ -2056,7 +2066,8 @@ pann_status pann_close(pann_index_t idx) {
   void* ptrs[] = {idx->d_pts,    idx->d_bf16,    idx->d_fp8,   idx->d_scales, idx->d_absmax,
                   idx->d_nbrs,   idx->d_q8,      idx->d_xnorm, idx->d_dummy,  idx->d_pkeys,
                   idx->d_pflags, idx->d_fp8r,    idx->d_rscales, idx->d_rabsmax, idx->d_cent,
-                  idx->d_assign, idx->d_evals, idx->d_fp8_bound, idx->d_q8_bound};
+                  idx->d_assign, idx->d_evals, idx->d_fp8_bound, idx->d_q8_bound,
+                  idx->d_bf16_bound};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (size_t l = 1; l < idx->levels.size(); l++)

@@ -136,6 +136,42 @@ hipError_t launch_bf16_store(const float* X, uint64_t stride_f, uint32_t n, uint
   return hipGetLastError();
 }
 
+// Per BF16 row x^ (exact in FP32), for every query q.  The search's sums take
+// the FP32 query against x^ with FMA, at most 8 NCH + 8 roundings deep:
+//  * inner product (metric 1): |<q,x^> - <q,x>| <= |q| |x - x^|, that sum is
+//    within gamma |q| |x^| of -<q,x^> and the rerank's float loop within
+//    gamma |q| |x| of -<q,x>, so b = |x - x^| + 5.8e-6 g (|x^| + |x|) with
+//    |D16 - D32| <= |q| b (gamma <= 5.8e-6 g, g = max(1, dim / 96), as below);
+//  * L2 (metric 0): b = |x - x^| + 2^-16 |x^|, as for the FP8 L2 rows below
+//    (the difference q_j - x^_j rounds once, the rest is the same sum).
+// Computed in double, raised by 1e-6 relative before the float rounding.
+__global__ void k_bf16_row_bound(const float* __restrict__ X, uint64_t stride_f, uint32_t n,
+                                 uint32_t dim, const uint16_t* __restrict__ Y,
+                                 uint64_t stride_h, float* __restrict__ b, int metric) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double rr = 0.0, xn = 0.0, xh = 0.0;
+  for (uint32_t j = 0; j < dim; j++) {
+    const double x = (double)X[(uint64_t)r * stride_f + j];
+    const double h = (double)__uint_as_float((uint32_t)Y[(uint64_t)r * stride_h + j] << 16);
+    rr += (x - h) * (x - h);
+    xn += x * x;
+    xh += h * h;
+  }
+  const double g = dim > 96u ? (double)dim / 96.0 : 1.0;
+  const double v = metric == 0 ? sqrt(rr) + 0x1p-16 * sqrt(xh)
+                               : sqrt(rr) + 5.8e-6 * g * (sqrt(xh) + sqrt(xn));
+  b[r] = (float)(v * (1.0 + 1e-6));
+}
+
+hipError_t launch_bf16_row_bound(const float* X, uint64_t stride_f, uint32_t n, uint32_t dim,
+                                 const uint16_t* Y, uint64_t stride_h, float* b, int metric,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(k_bf16_row_bound, dim3((n + 255) / 256), dim3(256), 0, s, X, stride_f, n,
+                     dim, Y, stride_h, b, metric);
+  return hipGetLastError();
+}
+
 // Per FP8 row, b >= |D8(q, x) - D32(q, x)| / |q| for every query q, where D8 is the
 // search's FP8 inner-product distance and D32 the FP32 rerank's (the reference's
 // float loop).  With x^ = s * code (exact reals), r = |x - x^|:

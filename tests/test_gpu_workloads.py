@@ -123,12 +123,12 @@ def test_reduced_precision_rerank_at_full_beams(oracle, aniso, beam):
     idx.close()
 
 
-@pytest.mark.parametrize("store", ["fp8", "q8"])
+@pytest.mark.parametrize("store", ["fp8", "q8", "bf16"])
 @pytest.mark.parametrize("beam,rf,k", [(64, 100.0, 10), (256, 100.0, 10), (352, 100.0, 10),
                                        (256, 3.0, 10), (128, 100.0, 1), (256, 2.0, 100),
                                        (352, 1.5, 100)])
 def test_pruned_rerank_equals_rescored_frontier(oracle, aniso, store, beam, rf, k):
-    """FP8 or 8-bit-code search + FP32 rerank (beam_search_rerank,
+    """FP8, BF16 or 8-bit-code search + FP32 rerank (beam_search_rerank,
     beamSearch.h:569-587): the result must be the first k by (FP32 distance,
     id) of the search frontier's first min(k * rerank_factor, |frontier|)
     entries, whatever rows the kernel's per-row error bounds let it leave
@@ -160,20 +160,21 @@ def f32_l2(oracle):
     return X, Q, adj
 
 
+@pytest.mark.parametrize("store", ["fp8", "bf16"])
 @pytest.mark.parametrize("beam,rf,k", [(64, 100.0, 10), (256, 100.0, 10), (1000, 100.0, 10),
                                        (256, 100.0, 1), (512, 2.0, 100)])
-def test_pruned_l2_rerank_equals_rescored_frontier(oracle, f32_l2, beam, rf, k):
-    """FP8 L2 search + FP32 rerank (beamSearch.h:569-587): as for inner
+def test_pruned_l2_rerank_equals_rescored_frontier(oracle, f32_l2, store, beam, rf, k):
+    """FP8 or BF16 L2 search + FP32 rerank (beamSearch.h:569-587): as for inner
     product, the first k by (FP32 distance, id) of the frontier's first
     min(k * rerank_factor, |frontier|) entries, whatever rows the per-row
     bounds leave unread."""
     from parlayann_amd import DeviceIndex
     X, Q, adj = f32_l2
-    _say(f"[l2 rerank pruning] beam={beam} rerank_factor={rf} k={k}")
+    _say(f"[l2 rerank pruning] store={store} beam={beam} rerank_factor={rf} k={k}")
     idx = DeviceIndex(X, adj, "l2")
-    idx.build_store("fp8")
-    front = idx.search(Q, beam, beam, store="fp8")
-    r = idx.search(Q, k, beam, store="fp8", rerank=True, rerank_factor=rf)
+    idx.build_store(store)
+    front = idx.search(Q, beam, beam, store=store)
+    r = idx.search(Q, k, beam, store=store, rerank=True, rerank_factor=rf)
     nchk = min(int(k * rf), beam)
     for i in range(len(Q)):
         ids = front["ids"][i][:nchk].astype(np.int64)
